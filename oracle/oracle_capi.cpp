@@ -1,0 +1,305 @@
+// oracle_capi.cpp -- C entry points of the CPU restatement.
+//
+// TEST INFRASTRUCTURE (oracle).  Built twice by oracle/Makefile: with
+// ORC_REAL=double (liboracle_f64.so) and ORC_REAL=float (liboracle_f32.so).
+// Only tests/, the smoke check and the CPU baseline leg of bench.py load it.
+//
+// scene_run() follows the same SceneDesc/SceneOut contract as the scene
+// driver (oracle/scene_driver.cpp), so the restatement, the reference build
+// and the MI355X library are interchangeable behind one harness.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "orc_world.hpp"
+#include "scene.h"
+
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL R;
+using namespace orc;
+
+namespace {
+
+inline bool has(double v) { return !std::isnan(v); }
+
+void apply_limot_params(Joint<R> &j, const double *p, bool hinge2) {
+  auto set = [](LimMot<R> &lm, int which, double v) {
+    if (!has(v)) return;
+    R x = (R)v;
+    switch (which) {  // dxJointLimitMotor::set, joint.cpp:472-509
+      case SCJ_LO: lm.lostop = x; break;
+      case SCJ_HI: lm.histop = x; break;
+      case SCJ_VEL: lm.vel = x; break;
+      case SCJ_FMAX: if (x >= 0) lm.fmax = x; break;
+      case SCJ_FUDGE: if (x >= 0 && x <= 1) lm.fudge = x; break;
+      case SCJ_BOUNCE: lm.bounce = x; break;
+      case SCJ_CFM: lm.normal_cfm = x; break;
+      case SCJ_STOP_ERP: lm.stop_erp = x; break;
+      case SCJ_STOP_CFM: lm.stop_cfm = x; break;
+    }
+  };
+  for (int k : {SCJ_LO, SCJ_HI, SCJ_VEL, SCJ_FMAX, SCJ_FUDGE, SCJ_BOUNCE, SCJ_CFM, SCJ_STOP_ERP, SCJ_STOP_CFM})
+    set(j.limot, k, p[k]);
+  if (hinge2) {
+    if (has(p[SCJ_SUSP_ERP])) j.susp_erp = (R)p[SCJ_SUSP_ERP];
+    if (has(p[SCJ_SUSP_CFM])) j.susp_cfm = (R)p[SCJ_SUSP_CFM];
+    set(j.limot2, SCJ_VEL, p[SCJ_VEL2]);
+    set(j.limot2, SCJ_FMAX, p[SCJ_FMAX2]);
+  }
+}
+
+void record(const World<R> &w, double *dst) {
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    const Body<R> &b = w.bodies[i];
+    double *r = dst + 13 * i;
+    for (int k = 0; k < 3; k++) r[k] = b.pos[k];
+    for (int k = 0; k < 4; k++) r[3 + k] = b.q[k];
+    for (int k = 0; k < 3; k++) r[7 + k] = b.lvel[k];
+    for (int k = 0; k < 3; k++) r[10 + k] = b.avel[k];
+  }
+}
+
+}  // namespace
+
+// Builds the world exactly as the scene driver does through the ODE API.
+static void build_world(const SceneDesc *s, World<R> &w) {
+  for (int k = 0; k < 3; k++) w.prm.gravity[k] = (R)s->gravity[k];
+  w.prm.erp = (R)s->erp;
+  w.prm.cfm = (R)s->cfm;
+  w.prm.iterations = s->iterations;
+  w.prm.w = (R)s->sor_w;
+  w.prm.max_vel = (R)s->max_vel;
+  w.prm.min_depth = (R)s->min_depth;
+  w.prm.hash_min = s->hash_minlevel;
+  w.prm.hash_max = s->hash_maxlevel;
+  for (int i = 0; i < s->nb; i++) {
+    int b = w.add_body();
+    R I9[9];
+    for (int k = 0; k < 9; k++) I9[k] = (R)s->body_I[9 * (size_t)i + k];
+    w.set_mass(b, (R)s->body_mass[i], I9);
+    for (int k = 0; k < 3; k++) w.bodies[b].pos[k] = (R)s->body_pos[3 * (size_t)i + k];
+    R q[4];
+    for (int k = 0; k < 4; k++) q[k] = (R)s->body_q[4 * (size_t)i + k];
+    w.set_quat(b, q);
+    for (int k = 0; k < 3; k++) w.bodies[b].lvel[k] = (R)s->body_lvel[3 * (size_t)i + k];
+    for (int k = 0; k < 3; k++) w.bodies[b].avel[k] = (R)s->body_avel[3 * (size_t)i + k];
+  }
+  for (int i = 0; i < s->ng; i++) {
+    R p[4];
+    for (int k = 0; k < 4; k++) p[k] = (R)s->geom_param[4 * (size_t)i + k];
+    int g = w.add_geom(s->geom_type[i], p);
+    if (s->geom_type[i] != SC_PLANE) {
+      if (s->geom_body[i] >= 0) w.geom_set_body(g, s->geom_body[i]);
+      else {
+        for (int k = 0; k < 3; k++) w.geoms[g].pos[k] = (R)s->geom_pos[3 * (size_t)i + k];
+        R q[4];
+        for (int k = 0; k < 4; k++) q[k] = (R)s->geom_q[4 * (size_t)i + k];
+        quat_to_rot(w.geoms[g].Rm, q);  // dGeomSetQuaternion, collision_kernel.cpp:599-613
+      }
+    }
+  }
+  for (int i = 0; i < s->nj; i++) {
+    Joint<R> j;
+    j.type = s->joint_type[i];
+    j.erp = w.prm.erp;
+    j.cfm = w.prm.cfm;
+    j.limot.init(w.prm.erp, w.prm.cfm);
+    j.limot2.init(w.prm.erp, w.prm.cfm);
+    j.susp_erp = w.prm.erp;
+    j.susp_cfm = w.prm.cfm;
+    if (j.type == SC_HINGE2) { j.axis1[0] = 1; j.axis1[1] = 0; j.axis2[0] = 0; j.axis2[1] = 1; }
+    w.joints.push_back(j);
+    int ji = (int)w.joints.size() - 1;
+    w.attach(ji, s->joint_b1[i], s->joint_b2[i]);
+    Joint<R> &jr = w.joints[ji];
+    const double *a = s->joint_anchor + 3 * (size_t)i;
+    const double *x1 = s->joint_axis1 + 3 * (size_t)i;
+    const double *x2 = s->joint_axis2 + 3 * (size_t)i;
+    const double *prm = s->joint_param + SC_NJP * (size_t)i;
+    if (jr.type == SC_BALL) {
+      w.set_anchors(jr, (R)a[0], (R)a[1], (R)a[2]);
+      if (has(prm[SCJ_ERP])) jr.erp = (R)prm[SCJ_ERP];
+      if (has(prm[SCJ_CFM])) jr.cfm = (R)prm[SCJ_CFM];
+    } else if (jr.type == SC_HINGE) {
+      w.set_anchors(jr, (R)a[0], (R)a[1], (R)a[2]);
+      w.hinge_qrel(jr);
+      w.set_axes(jr, (R)x1[0], (R)x1[1], (R)x1[2], true, true);
+      w.hinge_qrel(jr);
+      apply_limot_params(jr, prm, false);
+    } else {
+      // hinge2 setters: hinge2.cpp:214-256 (the final axes decide s0, c0, v1, v2)
+      w.set_anchors(jr, (R)a[0], (R)a[1], (R)a[2]);
+      w.hinge2_make_v1v2(jr);
+      R ax1[4], ax2[4], axc[4];
+      w.set_axes(jr, (R)x1[0], (R)x1[1], (R)x1[2], true, false);
+      w.hinge2_axis_info(jr, ax1, ax2, axc, jr.s0, jr.c0);
+      w.hinge2_make_v1v2(jr);
+      w.set_axes(jr, (R)x2[0], (R)x2[1], (R)x2[2], false, true);
+      w.hinge2_axis_info(jr, ax1, ax2, axc, jr.s0, jr.c0);
+      w.hinge2_make_v1v2(jr);
+      apply_limot_params(jr, prm, true);
+    }
+  }
+  w.n_perm = (int)w.joints.size();
+}
+
+extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
+  using clk = std::chrono::steady_clock;
+  World<R> w;
+  build_world(s, w);
+  Surface<R> surf;
+  std::memset(&surf, 0, sizeof(surf));
+  surf.mode = s->surf_mode;
+  surf.mu = (R)s->mu;
+  surf.mu2 = (R)s->mu2;
+  surf.bounce = (R)s->bounce;
+  surf.bounce_vel = (R)s->bounce_vel;
+  surf.soft_erp = (R)s->soft_erp;
+  surf.soft_cfm = (R)s->soft_cfm;
+  o->total_pairs = 0;
+  o->total_contacts = 0;
+  o->sec_collide = 0;
+  o->sec_step = 0;
+  o->status = 0;
+  if (o->feedback) w.fbout.assign(12 * (size_t)o->max_contacts, 0);
+  long long npairs_diag = 0, ncont_diag = 0;
+  int rec = 0;
+  if (o->traj) record(w, o->traj);
+  rec++;
+  w.rng.seed = s->seed;
+  std::vector<std::pair<int, int>> pairs;
+  std::vector<ContactGeom<R>> cbuf(s->max_contacts > 0 ? s->max_contacts : 1);
+  for (int step = 0; step < o->nsteps; step++) {
+    bool diag = step < o->diag_steps;
+    if (s->step_ctrl) {
+      const double *ctl = s->step_ctrl + (size_t)step * s->nj * 4;
+      for (int i = 0; i < s->nj; i++) {
+        const double *u = ctl + 4 * (size_t)i;
+        Joint<R> &j = w.joints[i];
+        if (j.type == SC_HINGE2 || j.type == SC_HINGE) {
+          if (has(u[0])) j.limot.lostop = (R)u[0];
+          if (has(u[1])) j.limot.histop = (R)u[1];
+        }
+        if (j.type == SC_HINGE2 && (u[2] != 0 || u[3] != 0)) w.hinge2_add_torques(i, (R)u[2], (R)u[3]);
+      }
+    }
+    auto t0 = clk::now();
+    w.hash_pairs(pairs);
+    int step_contacts = 0;
+    for (auto &pr : pairs) {
+      int g1 = pr.first, g2 = pr.second;
+      if (diag && npairs_diag < o->max_pairs) {
+        o->pairs[2 * npairs_diag] = g1;
+        o->pairs[2 * npairs_diag + 1] = g2;
+        npairs_diag++;
+      }
+      int b1 = w.geoms[g1].body, b2 = w.geoms[g2].body;
+      if (s->skip_connected && b1 >= 0 && b2 >= 0 && w.connected_non_contact(b1, b2)) continue;
+      int n = collide_geoms(w.shape(g1), g1, w.shape(g2), g2, s->max_contacts, cbuf.data());
+      for (int i = 0; i < n; i++) {
+        int ji = w.add_contact_joint(surf, cbuf[i], b1, b2);
+        if (diag && ncont_diag < o->max_contacts) {
+          double *r = o->contacts + 9 * ncont_diag;
+          for (int k = 0; k < 3; k++) r[k] = cbuf[i].pos[k];
+          for (int k = 0; k < 3; k++) r[3 + k] = cbuf[i].normal[k];
+          r[6] = cbuf[i].depth;
+          r[7] = cbuf[i].g1;
+          r[8] = cbuf[i].g2;
+          if (o->feedback) w.joints[ji].fb = ncont_diag;
+          ncont_diag++;
+        }
+        step_contacts++;
+      }
+    }
+    auto t1 = clk::now();
+    w.quickstep((R)s->dt);
+    w.empty_contacts();
+    auto t2 = clk::now();
+    o->sec_collide += std::chrono::duration<double>(t1 - t0).count();
+    o->sec_step += std::chrono::duration<double>(t2 - t1).count();
+    if (diag) {
+      if (o->pair_step_count) o->pair_step_count[step] = (int)pairs.size();
+      if (o->contact_step_count) o->contact_step_count[step] = step_contacts;
+    }
+    o->total_pairs += (long long)pairs.size();
+    o->total_contacts += step_contacts;
+    if (o->traj && o->record_every > 0 && (step + 1) % o->record_every == 0) {
+      record(w, o->traj + (size_t)rec * s->nb * 13);
+      rec++;
+    }
+  }
+  if (o->feedback)
+    for (size_t k = 0; k < 12 * (size_t)o->max_contacts; k++) o->feedback[k] = (double)w.fbout[k];
+  return o->status;
+}
+
+extern "C" int scene_dreal_size(void) { return (int)sizeof(R); }
+
+// ---------------------------------------------------------------- components
+
+// dTestRand known answer (ode/src/misc.cpp:52-62)
+extern "C" int orc_rand_test(void) {
+  Rng r;
+  r.seed = 0;
+  const unsigned long want[5] = {0x3c6ef35fUL, 0x47502932UL, 0xd1ccf6e9UL, 0xaaf95334UL, 0x6252e503UL};
+  for (int i = 0; i < 5; i++)
+    if (r.next() != want[i]) return 0;
+  return 1;
+}
+
+// n draws of dRandInt(bound[i]) from `seed`; writes the results and returns
+// the final seed.
+extern "C" unsigned long orc_rand_ints(unsigned long seed, int n, const int *bound, int *out) {
+  Rng r;
+  r.seed = seed;
+  for (int i = 0; i < n; i++) out[i] = r.below(bound[i]);
+  return r.seed;
+}
+
+// One dCollide call between two geoms given their type, pose and shape.
+// pos: 3 doubles, rot: 12 doubles (3x4), param: 4 doubles.  Writes up to
+// (flags & 0xffff) contacts of 7 doubles (pos3 normal3 depth).
+extern "C" int orc_collide(int t1, const double *pos1, const double *rot1, const double *prm1, int t2,
+                           const double *pos2, const double *rot2, const double *prm2, int flags, double *out) {
+  R p1[4], p2[4], r1[12], r2[12];
+  GeomShape<R> a, b;
+  a.type = t1; b.type = t2;
+  for (int k = 0; k < 3; k++) { p1[k] = (R)pos1[k]; p2[k] = (R)pos2[k]; }
+  for (int k = 0; k < 12; k++) { r1[k] = (R)rot1[k]; r2[k] = (R)rot2[k]; }
+  for (int k = 0; k < 4; k++) { a.param[k] = (R)prm1[k]; b.param[k] = (R)prm2[k]; }
+  a.pos = p1; a.Rm = r1; b.pos = p2; b.Rm = r2;
+  int maxc = flags & 0xffff;
+  std::vector<ContactGeom<R>> c(maxc > 0 ? maxc : 1);
+  int n = collide_geoms(a, 0, b, 1, flags, c.data());
+  for (int i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) out[7 * i + k] = c[i].pos[k];
+    for (int k = 0; k < 3; k++) out[7 * i + 3 + k] = c[i].normal[k];
+    out[7 * i + 6] = c[i].depth;
+  }
+  return n;
+}
+
+// AABB of one geom (6 doubles).
+extern "C" void orc_aabb(int t, const double *pos, const double *rot, const double *prm, double *out) {
+  R p[4], r[12];
+  GeomShape<R> g;
+  g.type = t;
+  for (int k = 0; k < 3; k++) p[k] = (R)pos[k];
+  for (int k = 0; k < 12; k++) r[k] = (R)rot[k];
+  for (int k = 0; k < 4; k++) g.param[k] = (R)prm[k];
+  g.pos = p;
+  g.Rm = r;
+  R a[6];
+  geom_aabb(g, a);
+  for (int k = 0; k < 6; k++) out[k] = a[k];
+}
+
+// dPlaneSpace of a unit normal.
+extern "C" void orc_plane_space(const double *n, double *p, double *q) {
+  R nn[3] = {(R)n[0], (R)n[1], (R)n[2]}, pp[3], qq[3];
+  plane_space(nn, pp, qq);
+  for (int k = 0; k < 3; k++) { p[k] = pp[k]; q[k] = qq[k]; }
+}
