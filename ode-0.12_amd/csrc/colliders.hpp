@@ -1,0 +1,457 @@
+// colliders.hpp -- narrowphase for the path's five geom pairs, one pair per
+// lane.  Each collider writes up to `maxc` contacts (position, normal into
+// body 1, depth) and returns their count; the contact normal convention and
+// the contact selection are those of the reference colliders cited below.
+#pragma once
+#include "devmath.hpp"
+#include "layout.hpp"
+
+namespace odemi {
+
+struct GeomView {
+  int type;
+  float3 p;      // world position (unused for planes)
+  Mat3 R;        // world rotation
+  float4 prm;    // radius | lx,ly,lz | plane a,b,c,d (unit normal)
+};
+
+struct CPoint {
+  float3 pos;
+  float3 n;
+  float depth;
+};
+
+// world AABB (minx,maxx,miny,maxy,minz,maxz): box.cpp:60-77, sphere.cpp:59-67,
+// plane.cpp:80-107 (an axis-aligned plane bounds a half space, others are
+// unbounded)
+__device__ inline void geom_aabb(const GeomView &g, float *a) {
+  if (g.type == GEOM_SPHERE) {
+    float r = g.prm.x;
+    a[0] = g.p.x - r; a[1] = g.p.x + r;
+    a[2] = g.p.y - r; a[3] = g.p.y + r;
+    a[4] = g.p.z - r; a[5] = g.p.z + r;
+  } else if (g.type == GEOM_BOX) {
+    float3 s = mk3(g.prm.x, g.prm.y, g.prm.z);
+    float ex = 0.5f * (fabsf(g.R.r0.x * s.x) + fabsf(g.R.r0.y * s.y) + fabsf(g.R.r0.z * s.z));
+    float ey = 0.5f * (fabsf(g.R.r1.x * s.x) + fabsf(g.R.r1.y * s.y) + fabsf(g.R.r1.z * s.z));
+    float ez = 0.5f * (fabsf(g.R.r2.x * s.x) + fabsf(g.R.r2.y * s.y) + fabsf(g.R.r2.z * s.z));
+    a[0] = g.p.x - ex; a[1] = g.p.x + ex;
+    a[2] = g.p.y - ey; a[3] = g.p.y + ey;
+    a[4] = g.p.z - ez; a[5] = g.p.z + ez;
+  } else {
+    const float inf = __int_as_float(0x7f800000);
+    float4 p = g.prm;
+    for (int k = 0; k < 6; k++) a[k] = (k & 1) ? inf : -inf;
+    int ax = -1;
+    if (p.y == 0.f && p.z == 0.f) ax = 0;
+    else if (p.x == 0.f && p.z == 0.f) ax = 1;
+    else if (p.x == 0.f && p.y == 0.f) ax = 2;
+    if (ax >= 0) {
+      float c = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+      a[2 * ax] = (c > 0.f) ? -inf : -p.w;
+      a[2 * ax + 1] = (c > 0.f) ? p.w : inf;
+    }
+  }
+}
+
+// sphere-sphere: collision_util.cpp:38-67
+__device__ inline int coll_sphere_sphere(float3 p1, float r1, float3 p2, float r2, CPoint *c) {
+  float3 d = p1 - p2;
+  float dist = sqrtf(dot(d, d));
+  if (dist > r1 + r2) return 0;
+  if (dist <= 0.f) {
+    c->pos = p1;
+    c->n = mk3(1.f, 0.f, 0.f);
+    c->depth = r1 + r2;
+  } else {
+    c->n = (1.f / dist) * d;
+    c->pos = p1 + (0.5f * (r2 - r1 - dist)) * c->n;
+    c->depth = r1 + r2 - dist;
+  }
+  return 1;
+}
+
+// sphere-box: sphere.cpp:131-219
+__device__ inline int coll_sphere_box(float3 ps, float rad, float3 pb, const Mat3 &R, float3 side, CPoint *c) {
+  float3 p = ps - pb;
+  float3 l = 0.5f * side;
+  float t[3];
+  bool border = false;
+  for (int k = 0; k < 3; k++) {
+    float3 ax = R.col(k);
+    float v = dot(p, ax);
+    float lk = comp(l, k);
+    if (v < -lk) { v = -lk; border = true; }
+    if (v > lk) { v = lk; border = true; }
+    t[k] = v;
+  }
+  if (!border) {
+    float best = l.x - fabsf(t[0]);
+    int bi = 0;
+    float f1 = l.y - fabsf(t[1]);
+    if (f1 < best) { best = f1; bi = 1; }
+    float f2 = l.z - fabsf(t[2]);
+    if (f2 < best) { best = f2; bi = 2; }
+    c->pos = ps;
+    float sgn = (t[bi] > 0.f) ? 1.f : -1.f;
+    c->n = sgn * R.col(bi);
+    c->depth = best + rad;
+    return 1;
+  }
+  float3 q = R.mul(mk3(t[0], t[1], t[2]));
+  float3 r = p - q;
+  float depth = rad - sqrtf(dot(r, r));
+  if (depth < 0.f) return 0;
+  c->pos = q + pb;
+  c->n = safe_normalize(r);
+  c->depth = depth;
+  return 1;
+}
+
+// sphere-plane: sphere.cpp:222-251
+__device__ inline int coll_sphere_plane(float3 ps, float rad, float4 pl, CPoint *c) {
+  float3 n = mk3(pl.x, pl.y, pl.z);
+  float depth = pl.w - dot(ps, n) + rad;
+  if (depth < 0.f) return 0;
+  c->n = n;
+  c->pos = ps - rad * n;
+  c->depth = depth;
+  return 1;
+}
+
+// box-plane: box.cpp:745-878 -- deepest vertex, then the vertices along the
+// two box edges with the smallest projections on the normal, then (for 4
+// requested contacts) the fourth vertex of that face.
+__device__ inline int coll_box_plane(float3 pb, const Mat3 &R, float3 side, float4 pl, int maxc, CPoint *c) {
+  float3 n = mk3(pl.x, pl.y, pl.z);
+  float3 ax0 = R.col(0), ax1 = R.col(1), ax2 = R.col(2);
+  float A[3] = {side.x * dot(n, ax0), side.y * dot(n, ax1), side.z * dot(n, ax2)};
+  float B[3] = {fabsf(A[0]), fabsf(A[1]), fabsf(A[2])};
+  float depth = pl.w + 0.5f * (B[0] + B[1] + B[2]) - dot(n, pb);
+  if (depth < 0.f) return 0;
+  if (maxc > 4) maxc = 4;
+  float3 p = pb;
+  p = (A[0] > 0.f) ? p - (0.5f * side.x) * ax0 : p + (0.5f * side.x) * ax0;
+  p = (A[1] > 0.f) ? p - (0.5f * side.y) * ax1 : p + (0.5f * side.y) * ax1;
+  p = (A[2] > 0.f) ? p - (0.5f * side.z) * ax2 : p + (0.5f * side.z) * ax2;
+  c[0].pos = p;
+  c[0].depth = depth;
+  int ret = 1;
+  int first, second;
+  if (B[0] < B[1]) {
+    if (B[2] < B[0]) { first = 2; second = 0; }
+    else { first = 0; second = (B[1] < B[2]) ? 1 : 2; }
+  } else {
+    if (B[2] < B[1]) { first = 2; second = (B[0] < B[1]) ? 0 : 1; }
+    else { first = 1; second = (B[0] < B[2]) ? 0 : 2; }
+  }
+  if (maxc > 1) {
+    for (int s = 0; s < 2; s++) {
+      if (s == 1 && maxc <= 2) break;
+      int ax = s == 0 ? first : second;
+      float Bk = B[ax];
+      if (depth - Bk < 0.f) break;
+      float3 axv = sel3(ax, ax0, ax1, ax2);
+      float sd = comp(side, ax);
+      c[1 + s].pos = (A[ax] > 0.f) ? p + sd * axv : p - sd * axv;
+      c[1 + s].depth = depth - Bk;
+      ret++;
+    }
+  }
+  if (maxc == 4 && ret == 3) {
+    float d4 = c[1].depth + c[2].depth - depth;
+    if (d4 > 0.f) {
+      c[3].pos = c[1].pos + c[2].pos - p;
+      c[3].depth = d4;
+      ret++;
+    }
+  }
+  for (int j = 0; j < ret; j++) c[j].n = n;
+  return ret;
+}
+
+// closest approach of two lines (collision_util.cpp:70-92)
+__device__ inline void line_closest(float3 pa, float3 ua, float3 pb, float3 ub, float &al, float &be) {
+  float3 p = pb - pa;
+  float uaub = dot(ua, ub);
+  float q1 = dot(ua, p);
+  float q2 = -dot(ub, p);
+  float d = 1.f - uaub * uaub;
+  if (d <= 0.0001f) { al = 0.f; be = 0.f; return; }
+  d = 1.f / d;
+  al = (q1 + uaub * q2) * d;
+  be = (uaub * q1 + q2) * d;
+}
+
+// clip a quadrilateral against the rectangle |x|<h0, |y|<h1 with four
+// half-plane chops; stops once a chop has produced 8 points (box.cpp:187-238)
+__device__ inline int clip_rect_quad(float h0, float h1, const float *quad, float *ret) {
+  float bufA[16], bufB[16];
+  for (int i = 0; i < 8; i++) bufA[i] = quad[i];
+  float *q = bufA, *r = bufB;
+  int nq = 4, nr = 0;
+  for (int chop = 0; chop < 4; chop++) {
+    int dir = chop >> 1;
+    float sign = (chop & 1) ? 1.f : -1.f;
+    float h = dir ? h1 : h0;
+    nr = 0;
+    bool full = false;
+    for (int i = 0; i < nq && !full; i++) {
+      float ax = q[2 * i], ay = q[2 * i + 1];
+      int j = (i + 1 < nq) ? i + 1 : 0;
+      float bx = q[2 * j], by = q[2 * j + 1];
+      float ad = dir ? ay : ax, bd = dir ? by : bx;
+      bool ain = sign * ad < h, bin = sign * bd < h;
+      if (ain) {
+        r[2 * nr] = ax; r[2 * nr + 1] = ay; nr++;
+        if (nr & 8) { full = true; break; }
+      }
+      if (ain != bin) {
+        float ao = dir ? ax : ay, bo = dir ? bx : by;
+        float o = ao + (bo - ao) / (bd - ad) * (sign * h - ad);
+        float d = sign * h;
+        r[2 * nr] = dir ? o : d;
+        r[2 * nr + 1] = dir ? d : o;
+        nr++;
+        if (nr & 8) { full = true; break; }
+      }
+    }
+    float *t = q; q = r; r = t;
+    nq = nr;
+    if (full) break;
+  }
+  for (int i = 0; i < 2 * nq; i++) ret[i] = q[i];
+  return nq;
+}
+
+// choose m of n (<=8) points spread around the centroid, starting at i0
+// (box.cpp:249-311)
+__device__ inline void cull_points(int n, const float *p, int m, int i0, int *iret) {
+  float cx, cy;
+  if (n == 1) { cx = p[0]; cy = p[1]; }
+  else if (n == 2) { cx = 0.5f * (p[0] + p[2]); cy = 0.5f * (p[1] + p[3]); }
+  else {
+    float a = 0.f, q;
+    cx = 0.f; cy = 0.f;
+    for (int i = 0; i < n - 1; i++) {
+      q = p[2 * i] * p[2 * i + 3] - p[2 * i + 2] * p[2 * i + 1];
+      a += q;
+      cx += q * (p[2 * i] + p[2 * i + 2]);
+      cy += q * (p[2 * i + 1] + p[2 * i + 3]);
+    }
+    q = p[2 * n - 2] * p[1] - p[0] * p[2 * n - 1];
+    a = 1.f / (3.f * (a + q));
+    cx = a * (cx + q * (p[2 * n - 2] + p[0]));
+    cy = a * (cy + q * (p[2 * n - 1] + p[1]));
+  }
+  float ang[8];
+  for (int i = 0; i < n; i++) ang[i] = atan2f(p[2 * i + 1] - cy, p[2 * i] - cx);
+  unsigned avail = (1u << n) - 1u;
+  avail &= ~(1u << i0);
+  iret[0] = i0;
+  const float PI = 3.14159265358979323846f;
+  for (int j = 1; j < m; j++) {
+    float a = (float)j * (2.f * PI / (float)m) + ang[i0];
+    if (a > PI) a -= 2.f * PI;
+    float best = 1e9f;
+    int bi = i0;
+    for (int i = 0; i < n; i++) {
+      if (!(avail & (1u << i))) continue;
+      float diff = fabsf(ang[i] - a);
+      if (diff > PI) diff = 2.f * PI - diff;
+      if (diff < best) { best = diff; bi = i; }
+    }
+    iret[j] = bi;
+    avail &= ~(1u << bi);
+  }
+}
+
+// box-box: separating axis test over the 15 axes with the 1.05 preference for
+// face axes, then one edge-edge contact or face clipping with culling to
+// maxc points.  box.cpp:331-712, normal negated as in 716-742.
+__device__ inline int coll_box_box(float3 p1, const Mat3 &R1, float3 side1, float3 p2, const Mat3 &R2, float3 side2,
+                                   int maxc, CPoint *out) {
+  const float fudge = 1.05f;
+  float3 u0 = R1.col(0), u1 = R1.col(1), u2 = R1.col(2);
+  float3 v0 = R2.col(0), v1 = R2.col(1), v2 = R2.col(2);
+  float3 p = p2 - p1;
+  float pp0 = dot(u0, p), pp1 = dot(u1, p), pp2 = dot(u2, p);
+  float3 A = 0.5f * side1, B = 0.5f * side2;
+  float R11 = dot(u0, v0), R12 = dot(u0, v1), R13 = dot(u0, v2);
+  float R21 = dot(u1, v0), R22 = dot(u1, v1), R23 = dot(u1, v2);
+  float R31 = dot(u2, v0), R32 = dot(u2, v1), R33 = dot(u2, v2);
+  float Q11 = fabsf(R11), Q12 = fabsf(R12), Q13 = fabsf(R13);
+  float Q21 = fabsf(R21), Q22 = fabsf(R22), Q23 = fabsf(R23);
+  float Q31 = fabsf(R31), Q32 = fabsf(R32), Q33 = fabsf(R33);
+  float s = -__int_as_float(0x7f800000);
+  bool invert = false;
+  int code = 0;
+  float3 nrm = mk3(0.f, 0.f, 0.f);   // face axis (world) or edge axis (box-1 frame)
+  // face axes of box 1 and box 2
+  auto face = [&](float e1, float e2, float3 ax, int cc) -> bool {
+    float s2 = fabsf(e1) - e2;
+    if (s2 > 0.f) return false;
+    if (s2 > s) { s = s2; nrm = ax; invert = e1 < 0.f; code = cc; }
+    return true;
+  };
+  auto edge = [&](float e1, float e2, float n1, float n2, float n3, int cc) -> bool {
+    float s2 = fabsf(e1) - e2;
+    if (s2 > 0.f) return false;
+    float l = sqrtf(n1 * n1 + n2 * n2 + n3 * n3);
+    if (l > 0.f) {
+      s2 /= l;
+      if (s2 * fudge > s) { s = s2; nrm = mk3(n1 / l, n2 / l, n3 / l); invert = e1 < 0.f; code = cc; }
+    }
+    return true;
+  };
+  if (!face(pp0, A.x + B.x * Q11 + B.y * Q12 + B.z * Q13, u0, 1)) return 0;
+  if (!face(pp1, A.y + B.x * Q21 + B.y * Q22 + B.z * Q23, u1, 2)) return 0;
+  if (!face(pp2, A.z + B.x * Q31 + B.y * Q32 + B.z * Q33, u2, 3)) return 0;
+  if (!face(dot(v0, p), A.x * Q11 + A.y * Q21 + A.z * Q31 + B.x, v0, 4)) return 0;
+  if (!face(dot(v1, p), A.x * Q12 + A.y * Q22 + A.z * Q32 + B.y, v1, 5)) return 0;
+  if (!face(dot(v2, p), A.x * Q13 + A.y * Q23 + A.z * Q33 + B.z, v2, 6)) return 0;
+  if (!edge(pp2 * R21 - pp1 * R31, A.y * Q31 + A.z * Q21 + B.y * Q13 + B.z * Q12, 0.f, -R31, R21, 7)) return 0;
+  if (!edge(pp2 * R22 - pp1 * R32, A.y * Q32 + A.z * Q22 + B.x * Q13 + B.z * Q11, 0.f, -R32, R22, 8)) return 0;
+  if (!edge(pp2 * R23 - pp1 * R33, A.y * Q33 + A.z * Q23 + B.x * Q12 + B.y * Q11, 0.f, -R33, R23, 9)) return 0;
+  if (!edge(pp0 * R31 - pp2 * R11, A.x * Q31 + A.z * Q11 + B.y * Q23 + B.z * Q22, R31, 0.f, -R11, 10)) return 0;
+  if (!edge(pp0 * R32 - pp2 * R12, A.x * Q32 + A.z * Q12 + B.x * Q23 + B.z * Q21, R32, 0.f, -R12, 11)) return 0;
+  if (!edge(pp0 * R33 - pp2 * R13, A.x * Q33 + A.z * Q13 + B.x * Q22 + B.y * Q21, R33, 0.f, -R13, 12)) return 0;
+  if (!edge(pp1 * R11 - pp0 * R21, A.x * Q21 + A.y * Q11 + B.y * Q33 + B.z * Q32, -R21, R11, 0.f, 13)) return 0;
+  if (!edge(pp1 * R12 - pp0 * R22, A.x * Q22 + A.y * Q12 + B.x * Q33 + B.z * Q31, -R22, R12, 0.f, 14)) return 0;
+  if (!edge(pp1 * R13 - pp0 * R23, A.x * Q23 + A.y * Q13 + B.x * Q32 + B.y * Q31, -R23, R13, 0.f, 15)) return 0;
+  if (!code) return 0;
+  float3 normal = (code <= 6) ? nrm : R1.mul(nrm);
+  if (invert) normal = -normal;
+  float depth = -s;
+  int n_out;
+  if (code > 6) {
+    // edge-edge: halfway between the closest points of the two edges
+    float3 pa = p1;
+    pa = pa + ((dot(normal, u0) > 0.f) ? 1.f : -1.f) * A.x * u0;
+    pa = pa + ((dot(normal, u1) > 0.f) ? 1.f : -1.f) * A.y * u1;
+    pa = pa + ((dot(normal, u2) > 0.f) ? 1.f : -1.f) * A.z * u2;
+    float3 pb = p2;
+    pb = pb + ((dot(normal, v0) > 0.f) ? -1.f : 1.f) * B.x * v0;
+    pb = pb + ((dot(normal, v1) > 0.f) ? -1.f : 1.f) * B.y * v1;
+    pb = pb + ((dot(normal, v2) > 0.f) ? -1.f : 1.f) * B.z * v2;
+    float3 ua = sel3((code - 7) / 3, u0, u1, u2);
+    float3 ub = sel3((code - 7) % 3, v0, v1, v2);
+    float al, be;
+    line_closest(pa, ua, pb, ub, al, be);
+    pa = pa + al * ua;
+    pb = pb + be * ub;
+    out[0].pos = 0.5f * (pa + pb);
+    out[0].depth = depth;
+    n_out = 1;
+  } else {
+    // face-something: clip the incident face of box b against the reference
+    // face of box a
+    bool aIs1 = code <= 3;
+    float3 ra0 = aIs1 ? u0 : v0, ra1 = aIs1 ? u1 : v1, ra2 = aIs1 ? u2 : v2;
+    float3 rb0 = aIs1 ? v0 : u0, rb1 = aIs1 ? v1 : u1, rb2 = aIs1 ? v2 : u2;
+    float3 pa = aIs1 ? p1 : p2, pb = aIs1 ? p2 : p1;
+    float3 Sa = aIs1 ? A : B, Sb = aIs1 ? B : A;
+    float3 n2 = aIs1 ? normal : -normal;
+    float nr0 = dot(rb0, n2), nr1 = dot(rb1, n2), nr2 = dot(rb2, n2);
+    float an0 = fabsf(nr0), an1 = fabsf(nr1), an2 = fabsf(nr2);
+    int lanr, a1, a2;
+    if (an1 > an0) {
+      if (an1 > an2) { a1 = 0; lanr = 1; a2 = 2; }
+      else { a1 = 0; a2 = 1; lanr = 2; }
+    } else {
+      if (an0 > an2) { lanr = 0; a1 = 1; a2 = 2; }
+      else { a1 = 0; a2 = 1; lanr = 2; }
+    }
+    float nrl = lanr == 0 ? nr0 : (lanr == 1 ? nr1 : nr2);
+    float3 rbl = sel3(lanr, rb0, rb1, rb2);
+    float sbl = comp(Sb, lanr);
+    float3 center = (nrl < 0.f) ? (pb - pa) + sbl * rbl : (pb - pa) - sbl * rbl;
+    int codeN = aIs1 ? code - 1 : code - 4;
+    int code1 = (codeN == 0) ? 1 : 0;
+    int code2 = (codeN == 2) ? 1 : 2;
+    float3 rc1 = sel3(code1, ra0, ra1, ra2), rc2 = sel3(code2, ra0, ra1, ra2);
+    float3 rba1 = sel3(a1, rb0, rb1, rb2), rba2 = sel3(a2, rb0, rb1, rb2);
+    float c1 = dot(center, rc1), c2 = dot(center, rc2);
+    float m11 = dot(rc1, rba1), m12 = dot(rc1, rba2), m21 = dot(rc2, rba1), m22 = dot(rc2, rba2);
+    float quad[8];
+    {
+      float sa1 = comp(Sb, a1), sa2 = comp(Sb, a2);
+      float k1 = m11 * sa1, k2 = m21 * sa1, k3 = m12 * sa2, k4 = m22 * sa2;
+      quad[0] = c1 - k1 - k3; quad[1] = c2 - k2 - k4;
+      quad[2] = c1 - k1 + k3; quad[3] = c2 - k2 + k4;
+      quad[4] = c1 + k1 + k3; quad[5] = c2 + k2 + k4;
+      quad[6] = c1 + k1 - k3; quad[7] = c2 + k2 - k4;
+    }
+    float ret[16];
+    int n = clip_rect_quad(comp(Sa, code1), comp(Sa, code2), quad, ret);
+    if (n < 1) return 0;
+    float det1 = 1.f / (m11 * m22 - m12 * m21);
+    m11 *= det1; m12 *= det1; m21 *= det1; m22 *= det1;
+    float3 point[8];
+    float dep[8];
+    float san = comp(Sa, codeN);
+    int cnum = 0;
+    for (int j = 0; j < n; j++) {
+      float k1 = m22 * (ret[2 * j] - c1) - m12 * (ret[2 * j + 1] - c2);
+      float k2 = -m21 * (ret[2 * j] - c1) + m11 * (ret[2 * j + 1] - c2);
+      float3 pt = center + k1 * rba1 + k2 * rba2;
+      float d = san - dot(n2, pt);
+      if (d >= 0.f) {
+        point[cnum] = pt;
+        dep[cnum] = d;
+        ret[2 * cnum] = ret[2 * j];
+        ret[2 * cnum + 1] = ret[2 * j + 1];
+        cnum++;
+      }
+    }
+    if (cnum < 1) return 0;
+    int mc = maxc;
+    if (mc > cnum) mc = cnum;
+    if (mc < 1) mc = 1;
+    if (cnum <= mc) {
+      for (int j = 0; j < cnum; j++) {
+        out[j].pos = point[j] + pa;
+        out[j].depth = dep[j];
+      }
+      n_out = cnum;
+    } else {
+      int i1 = 0;
+      float md = dep[0];
+      for (int i = 1; i < cnum; i++)
+        if (dep[i] > md) { md = dep[i]; i1 = i; }
+      int iret[8];
+      cull_points(cnum, ret, mc, i1, iret);
+      for (int j = 0; j < mc; j++) {
+        out[j].pos = point[iret[j]] + pa;
+        out[j].depth = dep[iret[j]];
+      }
+      n_out = mc;
+    }
+  }
+  for (int j = 0; j < n_out; j++) out[j].n = -normal;
+  return n_out;
+}
+
+// dCollide dispatch (collision_kernel.cpp:292-339): sphere/box come first in
+// the collider argument order; a reversed call negates the normals.
+__device__ inline int collide_pair(const GeomView &a, const GeomView &b, int maxc, CPoint *c) {
+  if (maxc <= 0) return 0;
+  bool rev = a.type > b.type;
+  const GeomView &x = rev ? b : a;
+  const GeomView &y = rev ? a : b;
+  int n = 0;
+  if (x.type == GEOM_SPHERE) {
+    if (y.type == GEOM_SPHERE) n = coll_sphere_sphere(x.p, x.prm.x, y.p, y.prm.x, c);
+    else if (y.type == GEOM_BOX) n = coll_sphere_box(x.p, x.prm.x, y.p, y.R, mk3(y.prm.x, y.prm.y, y.prm.z), c);
+    else n = coll_sphere_plane(x.p, x.prm.x, y.prm, c);
+  } else if (x.type == GEOM_BOX) {
+    if (y.type == GEOM_BOX)
+      n = coll_box_box(x.p, x.R, mk3(x.prm.x, x.prm.y, x.prm.z), y.p, y.R, mk3(y.prm.x, y.prm.y, y.prm.z), maxc, c);
+    else n = coll_box_plane(x.p, x.R, mk3(x.prm.x, x.prm.y, x.prm.z), y.prm, maxc, c);
+  }
+  if (rev)
+    for (int i = 0; i < n; i++) c[i].n = -c[i].n;
+  return n;
+}
+
+}  // namespace odemi

@@ -1,0 +1,215 @@
+// layout.hpp -- data layout shared by the host library and the HIP kernels.
+//
+// Device state is structure-of-arrays in HBM and stays resident across steps.
+// Worlds are contiguous ranges of the global body / geom / joint arrays; all
+// indices stored inside a world (geom->body, joint->bodies) are local to it.
+// Every world also owns a private scratch region whose layout is computed by
+// scratch_layout() identically on host and device.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define OD_HD __host__ __device__
+#else
+#define OD_HD
+#endif
+
+namespace odemi {
+
+enum { GEOM_SPHERE = 0, GEOM_BOX = 1, GEOM_PLANE = 2 };
+enum { JT_CONTACT = 0, JT_BALL = 1, JT_HINGE = 2, JT_HINGE2 = 3 };
+enum { ORDER_REFERENCE = 0, ORDER_COLORED = 1 };
+enum { JF_REVERSE = 2 };
+// contact surface mode bits (include/ode/contact.h:33-49)
+enum {
+  SM_MU2 = 0x001, SM_FDIR1 = 0x002, SM_BOUNCE = 0x004, SM_SOFT_ERP = 0x008, SM_SOFT_CFM = 0x010,
+  SM_MOTION1 = 0x020, SM_MOTION2 = 0x040, SM_MOTIONN = 0x080, SM_SLIP1 = 0x100, SM_SLIP2 = 0x200,
+  SM_APPROX1_1 = 0x1000, SM_APPROX1_2 = 0x2000
+};
+// per-world counters written by the step kernels
+enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_N };
+enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8 };
+
+static const int MAX_SLOTS = 8;   // most contacts a collider produces for one pair
+
+struct WorldInfo {
+  int b0, nb;        // bodies
+  int g0, ng;        // geoms
+  int j0, nj;        // permanent joints
+  int pcap;          // pair capacity
+  int kcap;          // contact capacity
+  int rcap;          // row capacity
+  int adj0;          // offset into the permanent-joint adjacency arrays
+  long long scratch; // byte offset of the world's scratch region
+};
+
+struct WorldParams {
+  float gravity[3];
+  float erp, cfm;
+  int iterations;
+  float sor_w;
+  float max_vel, min_depth;
+  float dt;
+  int surf_mode;
+  float mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm;
+  float motion1, motion2, motionN, slip1, slip2;
+  int max_contacts;
+  int skip_connected;
+  int hash_min, hash_max;
+  unsigned int seed;       // SOR shuffle RNG state (advanced by the kernel)
+  float cell_scale;        // uniform-grid cell = cell_scale * largest AABB extent
+};
+
+struct Limot {             // ode/src/joints/joint.h:172-190
+  float vel, fmax, lostop, histop, fudge, normal_cfm, stop_erp, stop_cfm, bounce;
+  int limit;
+  float limit_err;
+  float pad;
+};
+
+struct JointDev {          // a permanent joint
+  int type, b1, b2, flags; // b1/b2: node order after dJointAttach (b2 may be -1)
+  float anchor1[4], anchor2[4], axis1[4], axis2[4], qrel[4], v1[4], v2[4];
+  float erp, cfm, c0, s0, susp_erp, susp_cfm;
+  int seq;                 // attach sequence number (body joint lists are newest first)
+  int fb;                  // feedback slot or -1
+  Limot lm1, lm2;
+};
+
+struct ContactIn {         // a contact joint created on the host (ODE API path)
+  float pos[4], normal[4], fdir1[4];
+  float depth;
+  int b1, b2, flags;       // node order after dJointAttach
+  int seq, mode, fb, pad;
+  float mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm, motion1, motion2, motionN, slip1, slip2, pad2;
+};
+
+// a candidate pair; key orders pairs as the reference's hash space reports them
+struct PairRec {
+  int g1, g2;
+  unsigned long long key;
+};
+
+// a contact (world-local geom/body indices)
+struct ContactRec {
+  float pos[3], depth;
+  float normal[3];
+  int g1, g2;
+  int b1, b2;              // node order after attach (b1 >= 0 unless both static)
+  int flags;               // JF_REVERSE
+};
+
+// one constraint row, 144 bytes.  J and iMJ are unscaled; the solver
+// applies ad = w / (J inv(M) J^T + cfm) (ode/src/quickstep.cpp:370-402
+// scales J and b by it instead).
+struct Row {
+  float J[12];             // l1 a1 l2 a2
+  float iMJ[12];           // inv(M) J^T
+  float rhs;               // c until the rhs pass, then the LCP right hand side
+  float cfm;               // divided by h in the rhs pass
+  float lo, hi;
+  float ad;
+  float lambda;
+  int findex;              // absolute row index of the normal row, or -1
+  int b1, b2;              // world-local bodies, b2 may be -1
+  int pad[3];
+};
+
+// one constraint: a contact (up to 3 rows) or a permanent joint (up to 6)
+struct Cons {
+  int kind;                // JT_*
+  int src;                 // contact index or joint index
+  int b1, b2;
+  int row0, m;
+  int seq;                 // attach sequence (body joint list order)
+  int fb;                  // feedback slot or -1
+};
+
+struct ScratchLayout {
+  long long aabb, gbig, gidx, glevel, gdb, cells, ecount, pairs, pkeys, pcount, contacts, cons, rows, btmp,
+      order, adjs, adj, island, color, misc, total;
+};
+
+OD_HD inline long long od_align(long long x) { return (x + 255) & ~255LL; }
+OD_HD inline long long od_pow2(long long n) {
+  long long p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// Per-world scratch layout.  nb/ng/nj: world sizes; pcap (a power of two),
+// kcap, rcap: pair, contact and row capacities.  Every array the kernels
+// index is sized here, including the power-of-two padding of the sorts.
+OD_HD inline ScratchLayout scratch_layout(int nb, int ng, int nj, int pcap, int kcap, int rcap) {
+  ScratchLayout L;
+  long long o = 0;
+  long long ncell = od_pow2(8LL * ng + 8);
+  long long ncons = (long long)kcap + nj + 1;
+  L.aabb = o;     o = od_align(o + 24LL * ng);
+  L.gbig = o;     o = od_align(o + 4LL * ng);
+  L.gidx = o;     o = od_align(o + 4LL * ng);
+  L.glevel = o;   o = od_align(o + 4LL * ng);
+  L.gdb = o;      o = od_align(o + 24LL * ng);
+  L.cells = o;    o = od_align(o + 8LL * ncell);
+  L.ecount = o;   o = od_align(o + 4LL * (ncell + 1));
+  L.pairs = o;    o = od_align(o + (long long)sizeof(PairRec) * pcap);
+  L.pkeys = o;    o = od_align(o + 8LL * pcap);
+  L.pcount = o;   o = od_align(o + 4LL * (pcap + 1));
+  L.contacts = o; o = od_align(o + (long long)sizeof(ContactRec) * kcap);
+  L.cons = o;     o = od_align(o + (long long)sizeof(Cons) * ncons);
+  L.rows = o;     o = od_align(o + (long long)sizeof(Row) * rcap);
+  L.btmp = o;     o = od_align(o + 4LL * 32 * nb);
+  L.order = o;    o = od_align(o + 4LL * (rcap + ncons + nb + 2LL * ng + 8));
+  L.adjs = o;     o = od_align(o + 4LL * (nb + 1));
+  L.adj = o;      o = od_align(o + 4LL * 2 * ncons);
+  L.island = o;   o = od_align(o + 4LL * (6LL * nb + 2 * ncons + 3LL * ng + 16));
+  L.color = o;    o = od_align(o + 4LL * (ncons + nb));
+  L.misc = o;     o = od_align(o + 4LL * 64);
+  L.total = o;
+  return L;
+}
+
+// All device pointers the kernels need.
+struct DevBatch {
+  int nworlds;
+  int order_mode;
+  int track_residual;
+  const WorldInfo *wi;
+  WorldParams *wp;
+  // bodies: pos, q, lvel, avel as float4 each; R as 3 float4 rows
+  float4 *pos, *q, *lvel, *avel, *facc, *tacc;
+  float4 *R;
+  float4 *I, *invI;        // body frame, 3 rows of float4
+  float *mass, *invMass;
+  // geoms
+  int *gtype, *gbody;
+  float4 *gparam, *gpos, *gR;
+  int *grank;              // position of the geom in its space list
+  unsigned int *gcat, *gcol;
+  // permanent joints
+  JointDev *joints;
+  const int *adj_start, *adj_other;   // per-world body -> bodies joined by a permanent joint
+  const int *bgeom_start, *bgeom;     // global body -> its geoms (world-local ids, newest first)
+  // host-created contacts (ODE API path)
+  const ContactIn *cin;
+  const int *cin_start, *cin_count;   // per world
+  // per-step joint controls (lostop, histop, torque1, torque2), may be null
+  const float4 *ctrl;
+  // collide-phase outputs for the host (ODE API path)
+  PairRec *pairs_out;
+  ContactRec *contacts_out;
+  int *pcount_out;         // contacts per pair (pairs_out order)
+  // feedback out: 12 floats per slot (f1 t1 f2 t2)
+  float *feedback;
+  // counters and diagnostics
+  int *counters;           // CNT_N per world
+  float *residual;         // per world
+  int *work_counter;       // persistent-kernel world queue
+  const int *world_order;  // processing order of worlds (largest first)
+  char *scratch;
+  // tape for the adjoint (null when not recording)
+  char *tape;
+  const long long *tape_off;   // per world byte offset into the tape
+};
+
+}  // namespace odemi

@@ -1,0 +1,1084 @@
+// ode_api.cpp -- the ODE 0.12 C API subset on top of the device pipeline.
+//
+// The host keeps the object model the API exposes (worlds, bodies, spaces,
+// geoms, joints, joint groups) with the reference's list semantics; each world
+// owns a device mirror (a one-world batch).  dSpaceCollide runs the broadphase
+// and a speculative narrowphase on the GPU, then invokes the user's near
+// callback for every pair in the reference's report order; dCollide inside the
+// callback returns the contacts the GPU computed for that pair (or launches a
+// one-pair collide for a different request).  dWorldQuickStep uploads the
+// contact joints the callback created and runs the step kernel in the
+// reference order (islands, row order, seeded shuffle).
+#include <algorithm>
+#include <cstdarg>
+#include <cstdlib>
+
+#include "colliders.hpp"
+#include "host_common.hpp"
+#include "odemi.h"
+#include "odemi_internal.hpp"
+
+namespace odemi {
+unsigned long &global_seed();
+void report_debug(const char *fmt, ...);
+hipError_t launch_collide_one(const GeomView &a, const GeomView &b, int maxc, CPoint *out, int *n, hipStream_t st);
+}  // namespace odemi
+
+using namespace odemi;
+
+enum { BODY_GYRO = 256 };
+
+struct dxJointGroup;
+struct dxSpace;
+
+struct dxWorld {
+  float gravity[3] = {0, 0, 0};
+  float erp = 0.2f, cfm = 1e-5f;           // ode.cpp:1550-1556 (dSINGLE)
+  int iterations = 20;
+  float sor_w = 1.3f;
+  float max_vel = INFINITY, min_depth = 0;
+  std::vector<dxBody *> bodies;            // creation order
+  std::vector<dxJoint *> joints;           // creation order
+  int attach_seq = 0;
+  // device mirror
+  BatchImpl *mir = nullptr;
+  dxSpace *mir_space = nullptr;
+  std::vector<dxGeom *> mgeoms;            // mirror geom index -> geom
+  std::vector<dxJoint *> mjoints;          // mirror joint index -> joint
+  bool struct_dirty = true, state_dirty = true, joints_dirty = true, rank_dirty = true;
+  bool device_newer = false;
+  bool tape = false;
+  DevBuf cin, cin_start, cin_count, feedback, pairs_out, pcount_out, contacts_out, one_out, one_n;
+  int pcap = 0, kcap = 0;
+};
+
+struct dxBody {
+  dxWorld *world;
+  float pos[4] = {0, 0, 0, 0}, q[4] = {1, 0, 0, 0}, R[12], lvel[4] = {0, 0, 0, 0}, avel[4] = {0, 0, 0, 0};
+  float facc[4] = {0, 0, 0, 0}, tacc[4] = {0, 0, 0, 0};
+  dMass mass;
+  float invI[12];
+  float invMass = 1;
+  void *data = nullptr;
+  std::vector<dxGeom *> geoms;             // newest first
+  std::vector<dxJoint *> jlist;            // newest attach first
+  int mindex = -1;
+};
+
+struct dxSpace {
+  int cls = dHashSpaceClass;
+  int minlevel = -3, maxlevel = 10;
+  std::vector<dxGeom *> list;              // space list, front first
+  int cached_maxc = 4;
+  dxWorld *world = nullptr;
+};
+
+struct dxGeom {
+  int cls;
+  dxBody *body = nullptr;
+  dxSpace *space = nullptr;
+  float param[4] = {0, 0, 0, 0};
+  float pos[4] = {0, 0, 0, 0}, R[12];
+  unsigned long cat = ~0UL, col = ~0UL;
+  void *data = nullptr;
+  bool dirty = true;
+  int mindex = -1;
+};
+
+struct dxJoint {
+  int type;                                // dJointType*
+  dxWorld *world;
+  dxJointGroup *group = nullptr;
+  dxBody *node[2] = {nullptr, nullptr};
+  int flags = 0;
+  JointDev dev;                            // ball / hinge / hinge2 record
+  dContact contact;                        // contact joints
+  dJointFeedback *fb = nullptr;
+  int seq = 0;
+  void *data = nullptr;
+};
+
+struct dxJointGroup {
+  std::vector<dxJoint *> joints;
+};
+
+namespace {
+
+struct CollideCtx {
+  bool active = false;
+  dxSpace *space = nullptr;
+  dxGeom *g1 = nullptr, *g2 = nullptr;
+  const ContactRec *con = nullptr;
+  int n = 0, maxc = 0;
+} g_ctx;
+
+int g_init = 0;
+
+void body_rot_from_q(dxBody *b) { h_quat_to_rot(b->q, b->R); }
+
+void mark_state(dxWorld *w) { if (w) w->state_dirty = true; }
+void mark_struct(dxWorld *w) { if (w) w->struct_dirty = true; }
+
+// geoms move to the front of their space list when first marked dirty
+// (collision_space.cpp:47-75, 209-213)
+void geom_moved(dxGeom *g) {
+  if (!g->space || g->dirty) return;
+  auto &L = g->space->list;
+  auto it = std::find(L.begin(), L.end(), g);
+  if (it != L.end()) L.erase(it);
+  L.insert(L.begin(), g);
+  g->dirty = true;
+  if (g->space->world) g->space->world->rank_dirty = true;
+}
+void body_moved(dxBody *b) {
+  for (dxGeom *g : b->geoms) geom_moved(g);
+  mark_state(b->world);
+}
+
+// device state -> host copies
+int pull_state(dxWorld *w) {
+  if (!w->device_newer || !w->mir) return 1;
+  BatchImpl *m = w->mir;
+  std::vector<float> st(13 * (size_t)m->nb);
+  if (!batch_get_state(m, st.data())) return 0;
+  for (dxBody *b : w->bodies) {
+    if (b->mindex < 0) continue;
+    const float *r = &st[13 * (size_t)b->mindex];
+    for (int k = 0; k < 3; k++) b->pos[k] = r[k];
+    for (int k = 0; k < 4; k++) b->q[k] = r[3 + k];
+    for (int k = 0; k < 3; k++) b->lvel[k] = r[7 + k];
+    for (int k = 0; k < 3; k++) b->avel[k] = r[10 + k];
+    body_rot_from_q(b);
+  }
+  w->device_newer = false;
+  return 1;
+}
+
+JointDev joint_record(const dxJoint *j) {
+  JointDev d = j->dev;
+  d.b1 = j->node[0] ? j->node[0]->mindex : -1;
+  d.b2 = j->node[1] ? j->node[1]->mindex : -1;
+  d.seq = j->seq;
+  d.flags = j->flags;
+  return d;
+}
+
+bool is_permanent(const dxJoint *j) { return j->type != dJointTypeContact; }
+
+// (re)creates the device mirror of a world with its space
+int build_mirror(dxWorld *w, dxSpace *sp) {
+  if (w->mir) {
+    pull_state(w);
+    batch_destroy(w->mir);
+    w->mir = nullptr;
+  }
+  BatchArrays A;
+  WorldInfo wi{};
+  int nb = (int)w->bodies.size();
+  wi.b0 = 0; wi.nb = nb; wi.g0 = 0; wi.j0 = 0;
+  w->mgeoms.clear();
+  if (sp) {
+    // mirror index: creation order within the space (reverse of the list
+    // order when nothing has moved); ranks carry the actual list order
+    std::vector<dxGeom *> gl = sp->list;
+    w->mgeoms = gl;
+  }
+  for (int i = 0; i < nb; i++) w->bodies[i]->mindex = i;
+  w->mjoints.clear();
+  for (dxJoint *j : w->joints)
+    if (is_permanent(j) && j->node[0]) w->mjoints.push_back(j);
+  wi.ng = (int)w->mgeoms.size();
+  wi.nj = (int)w->mjoints.size();
+  world_capacities(nb, wi.ng, wi.nj, 1.0, wi.pcap, wi.kcap, wi.rcap);
+  wi.kcap = std::max(wi.kcap, 4096);
+  wi.rcap = 3 * wi.kcap + 6 * wi.nj + 8;
+  A.wi.push_back(wi);
+  WorldParams P;
+  std::memset(&P, 0, sizeof(P));
+  for (int k = 0; k < 3; k++) P.gravity[k] = w->gravity[k];
+  P.erp = w->erp; P.cfm = w->cfm; P.iterations = w->iterations; P.sor_w = w->sor_w;
+  P.max_vel = w->max_vel; P.min_depth = w->min_depth; P.dt = 0.01f;
+  P.hash_min = sp ? sp->minlevel : -3;
+  P.hash_max = sp ? sp->maxlevel : 10;
+  P.max_contacts = sp ? sp->cached_maxc : 4;
+  P.cell_scale = 1.0f;
+  A.wp.push_back(P);
+  for (dxBody *b : w->bodies) {
+    A.pos.push_back(make_float4(b->pos[0], b->pos[1], b->pos[2], 0));
+    A.q.push_back(make_float4(b->q[0], b->q[1], b->q[2], b->q[3]));
+    A.lvel.push_back(make_float4(b->lvel[0], b->lvel[1], b->lvel[2], 0));
+    A.avel.push_back(make_float4(b->avel[0], b->avel[1], b->avel[2], 0));
+    A.facc.push_back(make_float4(b->facc[0], b->facc[1], b->facc[2], 0));
+    A.tacc.push_back(make_float4(b->tacc[0], b->tacc[1], b->tacc[2], 0));
+    for (int r = 0; r < 3; r++) {
+      A.I.push_back(make_float4(b->mass.I[4 * r], b->mass.I[4 * r + 1], b->mass.I[4 * r + 2], 0));
+      A.invI.push_back(make_float4(b->invI[4 * r], b->invI[4 * r + 1], b->invI[4 * r + 2], 0));
+    }
+    A.mass.push_back(b->mass.mass);
+    A.invMass.push_back(b->invMass);
+  }
+  for (size_t i = 0; i < w->mgeoms.size(); i++) {
+    dxGeom *g = w->mgeoms[i];
+    g->mindex = (int)i;
+    int t = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
+    A.gtype.push_back(t);
+    A.gbody.push_back(g->body ? g->body->mindex : -1);
+    A.gparam.push_back(make_float4(g->param[0], g->param[1], g->param[2], g->param[3]));
+    A.gpos.push_back(make_float4(g->pos[0], g->pos[1], g->pos[2], 0));
+    for (int r = 0; r < 3; r++) A.gR.push_back(make_float4(g->R[4 * r], g->R[4 * r + 1], g->R[4 * r + 2], 0));
+    A.gcat.push_back((unsigned)g->cat);
+    A.gcol.push_back((unsigned)g->col);
+  }
+  A.grank.assign(w->mgeoms.size(), 0);
+  if (sp)
+    for (size_t r = 0; r < sp->list.size(); r++) A.grank[sp->list[r]->mindex] = (int)r;
+  for (dxJoint *j : w->mjoints) A.joints.push_back(joint_record(j));
+  w->mir = batch_create_arrays(A, ODEMI_ORDER_REFERENCE);
+  if (!w->mir) return 0;
+  w->mir_space = sp;
+  w->pcap = wi.pcap;
+  w->kcap = wi.kcap;
+  if (!w->pairs_out.alloc(sizeof(PairRec) * wi.pcap) || !w->pcount_out.alloc(sizeof(int) * (wi.pcap + 1)) ||
+      !w->contacts_out.alloc(sizeof(ContactRec) * wi.kcap) || !w->cin_start.alloc(sizeof(int)) ||
+      !w->cin_count.alloc(sizeof(int)) || !w->one_out.alloc(sizeof(CPoint) * MAX_SLOTS) || !w->one_n.alloc(16)) {
+    report_error("device allocation failed");
+    return 0;
+  }
+  w->struct_dirty = false;
+  w->state_dirty = false;
+  w->joints_dirty = false;
+  w->rank_dirty = false;
+  w->device_newer = false;
+  return 1;
+}
+
+// brings the device mirror up to date with the host objects
+int sync_mirror(dxWorld *w, dxSpace *sp) {
+  if (!sp) sp = w->mir_space;
+  if (w->struct_dirty || !w->mir || (sp && sp != w->mir_space)) return build_mirror(w, sp);
+  BatchImpl *m = w->mir;
+  if (w->state_dirty) {
+    pull_state(w);
+    std::vector<float> st(13 * (size_t)m->nb);
+    std::vector<float4> fa(m->nb), ta(m->nb);
+    for (dxBody *b : w->bodies) {
+      float *r = &st[13 * (size_t)b->mindex];
+      for (int k = 0; k < 3; k++) r[k] = b->pos[k];
+      for (int k = 0; k < 4; k++) r[3 + k] = b->q[k];
+      for (int k = 0; k < 3; k++) r[7 + k] = b->lvel[k];
+      for (int k = 0; k < 3; k++) r[10 + k] = b->avel[k];
+      fa[b->mindex] = make_float4(b->facc[0], b->facc[1], b->facc[2], 0);
+      ta[b->mindex] = make_float4(b->tacc[0], b->tacc[1], b->tacc[2], 0);
+    }
+    if (!batch_set_state(m, st.data())) return 0;
+    if (m->nb) {
+      OD_HIP(hipMemcpy(m->d.facc.p, fa.data(), sizeof(float4) * m->nb, hipMemcpyHostToDevice));
+      OD_HIP(hipMemcpy(m->d.tacc.p, ta.data(), sizeof(float4) * m->nb, hipMemcpyHostToDevice));
+    }
+    w->state_dirty = false;
+  }
+  if (w->joints_dirty) {
+    std::vector<JointDev> J;
+    for (dxJoint *j : w->mjoints) J.push_back(joint_record(j));
+    if (!J.empty()) OD_HIP(hipMemcpy(m->d.joints.p, J.data(), sizeof(JointDev) * J.size(), hipMemcpyHostToDevice));
+    w->joints_dirty = false;
+  }
+  if (w->rank_dirty && sp) {
+    std::vector<int> rk(w->mgeoms.size());
+    for (size_t r = 0; r < sp->list.size(); r++) rk[sp->list[r]->mindex] = (int)r;
+    if (!rk.empty()) OD_HIP(hipMemcpy(m->d.grank.p, rk.data(), sizeof(int) * rk.size(), hipMemcpyHostToDevice));
+    w->rank_dirty = false;
+  }
+  return 1;
+}
+
+int world_params_upload(dxWorld *w, float dt, int maxc) {
+  BatchImpl *m = w->mir;
+  WorldParams P;
+  OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 3; k++) P.gravity[k] = w->gravity[k];
+  P.erp = w->erp; P.cfm = w->cfm; P.iterations = w->iterations; P.sor_w = w->sor_w;
+  P.max_vel = w->max_vel; P.min_depth = w->min_depth;
+  if (dt > 0) P.dt = dt;
+  if (maxc > 0) P.max_contacts = maxc;
+  P.seed = (unsigned)global_seed();
+  if (w->mir_space) { P.hash_min = w->mir_space->minlevel; P.hash_max = w->mir_space->maxlevel; }
+  OD_HIP(hipMemcpy(m->d.wp.p, &P, sizeof(P), hipMemcpyHostToDevice));
+  return 1;
+}
+
+dxWorld *space_world(dxSpace *sp) {
+  if (sp->world) return sp->world;
+  for (dxGeom *g : sp->list)
+    if (g->body) return sp->world = g->body->world;
+  return nullptr;
+}
+
+void detach(dxJoint *j) {
+  for (int i = 0; i < 2; i++) {
+    dxBody *b = j->node[i];
+    if (!b) continue;
+    auto it = std::find(b->jlist.begin(), b->jlist.end(), j);
+    if (it != b->jlist.end()) b->jlist.erase(it);
+  }
+  j->node[0] = j->node[1] = nullptr;
+}
+
+void destroy_joint(dxJoint *j) {
+  detach(j);
+  dxWorld *w = j->world;
+  if (w) {
+    auto it = std::find(w->joints.begin(), w->joints.end(), j);
+    if (it != w->joints.end()) w->joints.erase(it);
+    if (is_permanent(j)) mark_struct(w);
+  }
+  delete j;
+}
+
+HPose pose_of(dxBody *b) {
+  HPose p;
+  std::memcpy(p.pos, b->pos, sizeof(p.pos));
+  std::memcpy(p.q, b->q, sizeof(p.q));
+  std::memcpy(p.R, b->R, sizeof(p.R));
+  return p;
+}
+
+void joint_changed(dxJoint *j) {
+  if (j->world) j->world->joints_dirty = true;
+}
+
+void write_contact(dContactGeom *c, const float *pos, const float *n, float depth, dGeomID g1, dGeomID g2) {
+  for (int k = 0; k < 3; k++) { c->pos[k] = pos[k]; c->normal[k] = n[k]; }
+  c->pos[3] = 0;
+  c->normal[3] = 0;
+  c->depth = depth;
+  c->g1 = g1;
+  c->g2 = g2;
+  c->side1 = -1;
+  c->side2 = -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- init
+int dInitODE2(unsigned int) { g_init++; return 1; }
+void dInitODE(void) { g_init++; }
+void dCloseODE(void) { if (g_init > 0) g_init--; }
+
+// ---------------------------------------------------------------- world
+dWorldID dWorldCreate(void) { return new dxWorld(); }
+void dWorldDestroy(dWorldID w) {
+  if (!w) return;
+  if (w->mir) batch_destroy(w->mir);
+  for (dxJoint *j : w->joints) {
+    j->world = nullptr;
+    detach(j);
+    if (!j->group) delete j;
+  }
+  for (dxBody *b : w->bodies) delete b;
+  delete w;
+}
+void dWorldSetGravity(dWorldID w, dReal x, dReal y, dReal z) { w->gravity[0] = x; w->gravity[1] = y; w->gravity[2] = z; }
+void dWorldGetGravity(dWorldID w, dVector3 g) { for (int k = 0; k < 3; k++) g[k] = w->gravity[k]; }
+void dWorldSetERP(dWorldID w, dReal erp) { w->erp = erp; }
+dReal dWorldGetERP(dWorldID w) { return w->erp; }
+void dWorldSetCFM(dWorldID w, dReal cfm) { w->cfm = cfm; }
+dReal dWorldGetCFM(dWorldID w) { return w->cfm; }
+void dWorldSetQuickStepNumIterations(dWorldID w, int n) { w->iterations = n; }
+int dWorldGetQuickStepNumIterations(dWorldID w) { return w->iterations; }
+void dWorldSetQuickStepW(dWorldID w, dReal v) { w->sor_w = v; }
+dReal dWorldGetQuickStepW(dWorldID w) { return w->sor_w; }
+void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal v) { w->max_vel = v; }
+void dWorldSetContactSurfaceLayer(dWorldID w, dReal d) { w->min_depth = d; }
+
+// dWorldQuickStep (ode.cpp:1807-1825): uploads the contact joints and runs
+// the step kernel on the world's mirror.
+int dWorldQuickStep(dWorldID w, dReal stepsize) {
+  if (!w || !(stepsize > 0)) return 0;
+  if (!sync_mirror(w, nullptr)) return 0;
+  BatchImpl *m = w->mir;
+  // contact joints in creation order
+  std::vector<ContactIn> cin;
+  std::vector<dJointFeedback *> fbs;
+  for (dxJoint *j : w->joints) {
+    if (j->type != dJointTypeContact || !j->node[0]) continue;
+    ContactIn c;
+    std::memset(&c, 0, sizeof(c));
+    const dContact &k = j->contact;
+    for (int i = 0; i < 3; i++) { c.pos[i] = k.geom.pos[i]; c.normal[i] = k.geom.normal[i]; c.fdir1[i] = k.fdir1[i]; }
+    c.depth = k.geom.depth;
+    c.b1 = j->node[0]->mindex;
+    c.b2 = j->node[1] ? j->node[1]->mindex : -1;
+    c.flags = j->flags;
+    c.seq = j->seq;
+    c.mode = k.surface.mode;
+    c.mu = k.surface.mu; c.mu2 = k.surface.mu2; c.bounce = k.surface.bounce; c.bounce_vel = k.surface.bounce_vel;
+    c.soft_erp = k.surface.soft_erp; c.soft_cfm = k.surface.soft_cfm; c.motion1 = k.surface.motion1;
+    c.motion2 = k.surface.motion2; c.motionN = k.surface.motionN; c.slip1 = k.surface.slip1;
+    c.slip2 = k.surface.slip2;
+    c.fb = -1;
+    if (j->fb) { c.fb = (int)fbs.size(); fbs.push_back(j->fb); }
+    cin.push_back(c);
+  }
+  if ((int)cin.size() > m->wi[0].kcap) {
+    report_error("dWorldQuickStep: %zu contact joints exceed the capacity %d", cin.size(), m->wi[0].kcap);
+    return 0;
+  }
+  // feedback slots of permanent joints
+  std::vector<JointDev> J;
+  bool jfb = false;
+  for (dxJoint *j : w->mjoints) {
+    JointDev d = joint_record(j);
+    d.fb = -1;
+    if (j->fb) { d.fb = (int)fbs.size(); fbs.push_back(j->fb); jfb = true; }
+    J.push_back(d);
+  }
+  if (jfb || w->joints_dirty) {
+    if (!J.empty()) OD_HIP(hipMemcpy(m->d.joints.p, J.data(), sizeof(JointDev) * J.size(), hipMemcpyHostToDevice));
+    w->joints_dirty = false;
+  }
+  if (!w->cin.alloc(sizeof(ContactIn) * (cin.size() + 1))) return 0;
+  if (!cin.empty()) OD_HIP(hipMemcpy(w->cin.p, cin.data(), sizeof(ContactIn) * cin.size(), hipMemcpyHostToDevice));
+  int zero = 0, nk = (int)cin.size();
+  OD_HIP(hipMemcpy(w->cin_start.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+  OD_HIP(hipMemcpy(w->cin_count.p, &nk, sizeof(int), hipMemcpyHostToDevice));
+  if (!w->feedback.alloc(sizeof(float) * 12 * (fbs.size() + 1))) return 0;
+  if (!world_params_upload(w, stepsize, 0)) return 0;
+  refresh_dev(m);
+  DevBatch D = m->dev;
+  D.cin = w->cin.as<ContactIn>();
+  D.cin_start = w->cin_start.as<int>();
+  D.cin_count = w->cin_count.as<int>();
+  D.feedback = fbs.empty() ? nullptr : w->feedback.as<float>();
+  D.ctrl = nullptr;
+  OD_HIP(launch_world_kernel(2, D, 0, 1, 0, m->wg, 0, m->stream));
+  OD_HIP(hipStreamSynchronize(m->stream));
+  long long cnt[5];
+  batch_counters(m, cnt);
+  if (cnt[4]) {
+    report_error("dWorldQuickStep: device capacity exceeded (flags %lld)", cnt[4]);
+    return 0;
+  }
+  // RNG state, feedback, joint limit state, space order
+  WorldParams P;
+  OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
+  global_seed() = P.seed;
+  if (!fbs.empty()) {
+    std::vector<float> f(12 * fbs.size());
+    OD_HIP(hipMemcpy(f.data(), w->feedback.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < fbs.size(); i++) {
+      dJointFeedback *o = fbs[i];
+      for (int k = 0; k < 3; k++) {
+        o->f1[k] = f[12 * i + k]; o->t1[k] = f[12 * i + 3 + k];
+        o->f2[k] = f[12 * i + 6 + k]; o->t2[k] = f[12 * i + 9 + k];
+      }
+    }
+  }
+  if (!w->mjoints.empty()) {
+    std::vector<JointDev> Jd(w->mjoints.size());
+    OD_HIP(hipMemcpy(Jd.data(), m->d.joints.p, sizeof(JointDev) * Jd.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < Jd.size(); i++) {
+      w->mjoints[i]->dev.lm1 = Jd[i].lm1;
+      w->mjoints[i]->dev.lm2 = Jd[i].lm2;
+    }
+  }
+  if (w->mir_space && !w->mgeoms.empty()) {
+    std::vector<int> rk(w->mgeoms.size());
+    OD_HIP(hipMemcpy(rk.data(), m->d.grank.p, sizeof(int) * rk.size(), hipMemcpyDeviceToHost));
+    std::vector<dxGeom *> L(rk.size());
+    for (size_t g = 0; g < rk.size(); g++) L[rk[g]] = w->mgeoms[g];
+    w->mir_space->list = L;
+    for (dxGeom *g : w->mgeoms)
+      if (g->body) g->dirty = true;
+  }
+  for (dxBody *b : w->bodies) {
+    for (int k = 0; k < 3; k++) { b->facc[k] = 0; b->tacc[k] = 0; }
+  }
+  w->device_newer = true;
+  return 1;
+}
+
+// ---------------------------------------------------------------- bodies
+dBodyID dBodyCreate(dWorldID w) {
+  dxBody *b = new dxBody();
+  b->world = w;
+  dMassSetParameters(&b->mass, 1, 0, 0, 0, 1, 1, 1, 0, 0, 0);
+  std::memset(b->invI, 0, sizeof(b->invI));
+  b->invI[0] = b->invI[5] = b->invI[10] = 1;
+  body_rot_from_q(b);
+  w->bodies.push_back(b);
+  mark_struct(w);
+  return b;
+}
+void dBodyDestroy(dBodyID b) {
+  dxWorld *w = b->world;
+  pull_state(w);
+  for (dxGeom *g : b->geoms) g->body = nullptr;
+  std::vector<dxJoint *> js = b->jlist;
+  for (dxJoint *j : js) detach(j);
+  w->bodies.erase(std::find(w->bodies.begin(), w->bodies.end(), b));
+  mark_struct(w);
+  delete b;
+}
+void dBodySetData(dBodyID b, void *d) { b->data = d; }
+void *dBodyGetData(dBodyID b) { return b->data; }
+dWorldID dBodyGetWorld(dBodyID b) { return b->world; }
+void dBodySetPosition(dBodyID b, dReal x, dReal y, dReal z) {
+  pull_state(b->world);
+  b->pos[0] = x; b->pos[1] = y; b->pos[2] = z;
+  body_moved(b);
+}
+void dBodySetRotation(dBodyID b, const dMatrix3 R) {
+  // quaternion of R (rotation.cpp:262-307), normalised; R follows from it
+  pull_state(b->world);
+  float tr = R[0] + R[5] + R[10], s, q[4];
+  if (tr >= 0) {
+    s = std::sqrt(tr + 1);
+    q[0] = 0.5f * s;
+    s = 0.5f / s;
+    q[1] = (R[9] - R[6]) * s; q[2] = (R[2] - R[8]) * s; q[3] = (R[4] - R[1]) * s;
+  } else if (R[5] > R[0] && R[5] >= R[10]) {
+    s = std::sqrt((R[5] - (R[10] + R[0])) + 1);
+    q[2] = 0.5f * s;
+    s = 0.5f / s;
+    q[3] = (R[6] + R[9]) * s; q[1] = (R[1] + R[4]) * s; q[0] = (R[2] - R[8]) * s;
+  } else if (R[10] > R[0]) {
+    s = std::sqrt((R[10] - (R[0] + R[5])) + 1);
+    q[3] = 0.5f * s;
+    s = 0.5f / s;
+    q[1] = (R[8] + R[2]) * s; q[2] = (R[6] + R[9]) * s; q[0] = (R[4] - R[1]) * s;
+  } else {
+    s = std::sqrt((R[0] - (R[5] + R[10])) + 1);
+    q[1] = 0.5f * s;
+    s = 0.5f / s;
+    q[2] = (R[1] + R[4]) * s; q[3] = (R[8] + R[2]) * s; q[0] = (R[9] - R[6]) * s;
+  }
+  h_normalize4(q);
+  std::memcpy(b->q, q, sizeof(q));
+  body_rot_from_q(b);
+  body_moved(b);
+}
+void dBodySetQuaternion(dBodyID b, const dQuaternion q) {
+  pull_state(b->world);
+  for (int k = 0; k < 4; k++) b->q[k] = q[k];
+  h_normalize4(b->q);
+  body_rot_from_q(b);
+  body_moved(b);
+}
+void dBodySetLinearVel(dBodyID b, dReal x, dReal y, dReal z) {
+  pull_state(b->world);
+  b->lvel[0] = x; b->lvel[1] = y; b->lvel[2] = z;
+  mark_state(b->world);
+}
+void dBodySetAngularVel(dBodyID b, dReal x, dReal y, dReal z) {
+  pull_state(b->world);
+  b->avel[0] = x; b->avel[1] = y; b->avel[2] = z;
+  mark_state(b->world);
+}
+const dReal *dBodyGetPosition(dBodyID b) { pull_state(b->world); return b->pos; }
+const dReal *dBodyGetRotation(dBodyID b) { pull_state(b->world); return b->R; }
+const dReal *dBodyGetQuaternion(dBodyID b) { pull_state(b->world); return b->q; }
+const dReal *dBodyGetLinearVel(dBodyID b) { pull_state(b->world); return b->lvel; }
+const dReal *dBodyGetAngularVel(dBodyID b) { pull_state(b->world); return b->avel; }
+void dBodySetMass(dBodyID b, const dMass *m) {
+  // dBodySetMass (ode.cpp:492-509): inverse inertia by Cholesky
+  b->mass = *m;
+  if (!h_invert_pd3(b->mass.I, b->invI)) {
+    report_debug("inertia must be positive definite!");
+    std::memset(b->invI, 0, sizeof(b->invI));
+    b->invI[0] = b->invI[5] = b->invI[10] = 1;
+  }
+  b->invMass = 1.0f / m->mass;
+  mark_struct(b->world);
+}
+void dBodyGetMass(dBodyID b, dMass *m) { *m = b->mass; }
+void dBodyAddForce(dBodyID b, dReal x, dReal y, dReal z) {
+  b->facc[0] += x; b->facc[1] += y; b->facc[2] += z;
+  mark_state(b->world);
+}
+void dBodyAddTorque(dBodyID b, dReal x, dReal y, dReal z) {
+  b->tacc[0] += x; b->tacc[1] += y; b->tacc[2] += z;
+  mark_state(b->world);
+}
+const dReal *dBodyGetForce(dBodyID b) { return b->facc; }
+const dReal *dBodyGetTorque(dBodyID b) { return b->tacc; }
+
+// ---------------------------------------------------------------- mass
+void dMassSetZero(dMass *m) { std::memset(m, 0, sizeof(*m)); }
+void dMassSetParameters(dMass *m, dReal mass, dReal cx, dReal cy, dReal cz, dReal I11, dReal I22, dReal I33,
+                        dReal I12, dReal I13, dReal I23) {  // mass.cpp:94-113
+  dMassSetZero(m);
+  m->mass = mass;
+  m->c[0] = cx; m->c[1] = cy; m->c[2] = cz;
+  m->I[0] = I11; m->I[5] = I22; m->I[10] = I33;
+  m->I[1] = I12; m->I[2] = I13; m->I[6] = I23;
+  m->I[4] = I12; m->I[8] = I13; m->I[9] = I23;
+}
+void dMassSetSphereTotal(dMass *m, dReal total, dReal r) {  // mass.cpp:125-140
+  dMassSetZero(m);
+  m->mass = total;
+  dReal II = 0.4f * total * r * r;
+  m->I[0] = m->I[5] = m->I[10] = II;
+}
+void dMassSetSphere(dMass *m, dReal density, dReal r) {
+  dMassSetSphereTotal(m, (dReal)((4.0f / 3.0f) * 3.14159265358979323846 * r * r * r * density), r);
+}
+void dMassSetBoxTotal(dMass *m, dReal total, dReal lx, dReal ly, dReal lz) {  // mass.cpp:208-221
+  dMassSetZero(m);
+  m->mass = total;
+  m->I[0] = total / 12.0f * (ly * ly + lz * lz);
+  m->I[5] = total / 12.0f * (lx * lx + lz * lz);
+  m->I[10] = total / 12.0f * (lx * lx + ly * ly);
+}
+void dMassSetBox(dMass *m, dReal density, dReal lx, dReal ly, dReal lz) {
+  dMassSetBoxTotal(m, lx * ly * lz * density, lx, ly, lz);
+}
+void dMassAdjust(dMass *m, dReal newmass) {  // mass.cpp:441-452
+  dReal s = newmass / m->mass;
+  m->mass = newmass;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m->I[4 * i + j] *= s;
+}
+
+// ---------------------------------------------------------------- joints
+dJointGroupID dJointGroupCreate(int) { return new dxJointGroup(); }
+void dJointGroupEmpty(dJointGroupID g) {
+  // newest first (ode.cpp:1305-1332)
+  for (auto it = g->joints.rbegin(); it != g->joints.rend(); ++it) destroy_joint(*it);
+  g->joints.clear();
+}
+void dJointGroupDestroy(dJointGroupID g) {
+  dJointGroupEmpty(g);
+  delete g;
+}
+static dxJoint *make_joint(dWorldID w, dJointGroupID g, int type) {
+  dxJoint *j = new dxJoint();
+  j->type = type;
+  j->world = w;
+  j->group = g;
+  int jt = type == dJointTypeBall ? JT_BALL : (type == dJointTypeHinge ? JT_HINGE : (type == dJointTypeHinge2 ? JT_HINGE2 : JT_CONTACT));
+  h_joint_defaults(j->dev, jt, w->erp, w->cfm);
+  w->joints.push_back(j);
+  if (g) g->joints.push_back(j);
+  if (type != dJointTypeContact) mark_struct(w);
+  return j;
+}
+dJointID dJointCreateBall(dWorldID w, dJointGroupID g) { return make_joint(w, g, dJointTypeBall); }
+dJointID dJointCreateHinge(dWorldID w, dJointGroupID g) { return make_joint(w, g, dJointTypeHinge); }
+dJointID dJointCreateHinge2(dWorldID w, dJointGroupID g) { return make_joint(w, g, dJointTypeHinge2); }
+dJointID dJointCreateContact(dWorldID w, dJointGroupID g, const dContact *c) {
+  dxJoint *j = make_joint(w, g, dJointTypeContact);
+  j->contact = *c;
+  return j;
+}
+void dJointDestroy(dJointID j) {
+  if (!j || j->group) return;   // grouped joints are destroyed by their group (ode.cpp:1276-1287)
+  destroy_joint(j);
+}
+// dJointAttach (ode.cpp:1348-1399)
+void dJointAttach(dJointID j, dBodyID b1, dBodyID b2) {
+  if (j->node[0] || j->node[1]) detach(j);
+  if (!b1) { b1 = b2; b2 = nullptr; j->flags |= JF_REVERSE; }
+  else j->flags &= ~JF_REVERSE;
+  j->node[0] = b1;
+  j->node[1] = b2;
+  if (b1) b1->jlist.insert(b1->jlist.begin(), j);
+  if (b2) b2->jlist.insert(b2->jlist.begin(), j);
+  j->seq = j->world->attach_seq++;
+  if (is_permanent(j)) {
+    // relative values are (re)computed by the anchor/axis setters
+    mark_struct(j->world);
+  }
+}
+int dJointGetType(dJointID j) { return j->type; }
+dBodyID dJointGetBody(dJointID j, int i) {
+  if (i < 0 || i > 1) return nullptr;
+  if (j->flags & JF_REVERSE) return j->node[1 - i];
+  return j->node[i];
+}
+void dJointSetFeedback(dJointID j, dJointFeedback *fb) { j->fb = fb; }
+dJointFeedback *dJointGetFeedback(dJointID j) { return j->fb; }
+
+void dJointSetBallAnchor(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  HPose a, b;
+  if (j->node[0]) a = pose_of(j->node[0]);
+  if (j->node[1]) b = pose_of(j->node[1]);
+  h_set_anchors(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr, x, y, z);
+  joint_changed(j);
+}
+void dJointSetBallParam(dJointID j, int p, dReal v) {
+  if (p == dParamCFM) j->dev.cfm = v;
+  else if (p == dParamERP) j->dev.erp = v;
+  joint_changed(j);
+}
+void dJointSetHingeAnchor(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  HPose a, b;
+  if (j->node[0]) a = pose_of(j->node[0]);
+  if (j->node[1]) b = pose_of(j->node[1]);
+  h_set_anchors(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr, x, y, z);
+  h_hinge_qrel(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr);
+  joint_changed(j);
+}
+void dJointSetHingeAxis(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  HPose a, b;
+  if (j->node[0]) a = pose_of(j->node[0]);
+  if (j->node[1]) b = pose_of(j->node[1]);
+  h_set_axes(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr, x, y, z, true, true);
+  h_hinge_qrel(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr);
+  joint_changed(j);
+}
+void dJointSetHingeParam(dJointID j, int p, dReal v) {
+  h_limot_set(j->dev.lm1, p, v);
+  joint_changed(j);
+}
+void dJointSetHinge2Anchor(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  HPose a, b;
+  if (j->node[0]) a = pose_of(j->node[0]);
+  if (j->node[1]) b = pose_of(j->node[1]);
+  h_set_anchors(j->dev, j->node[0] ? &a : nullptr, j->node[1] ? &b : nullptr, x, y, z);
+  if (j->node[0] && j->node[1]) h_hinge2_v1v2(j->dev, &a, &b);
+  joint_changed(j);
+}
+void dJointSetHinge2Axis1(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  if (!j->node[0]) return;
+  HPose a = pose_of(j->node[0]), b;
+  if (j->node[1]) b = pose_of(j->node[1]);
+  h_set_axes(j->dev, &a, j->node[1] ? &b : nullptr, x, y, z, true, false);
+  if (j->node[1]) {
+    h_hinge2_axis_info(j->dev, &a, &b, j->dev.s0, j->dev.c0);
+    h_hinge2_v1v2(j->dev, &a, &b);
+  }
+  joint_changed(j);
+}
+void dJointSetHinge2Axis2(dJointID j, dReal x, dReal y, dReal z) {
+  pull_state(j->world);
+  if (!j->node[0] || !j->node[1]) return;
+  HPose a = pose_of(j->node[0]), b = pose_of(j->node[1]);
+  h_set_axes(j->dev, &a, &b, x, y, z, false, true);
+  h_hinge2_axis_info(j->dev, &a, &b, j->dev.s0, j->dev.c0);
+  h_hinge2_v1v2(j->dev, &a, &b);
+  joint_changed(j);
+}
+void dJointSetHinge2Param(dJointID j, int p, dReal v) {  // hinge2.cpp:259-274
+  if ((p & 0xff00) == 0x100) h_limot_set(j->dev.lm2, p & 0xff, v);
+  else if (p == dParamSuspensionERP) j->dev.susp_erp = v;
+  else if (p == dParamSuspensionCFM) j->dev.susp_cfm = v;
+  else h_limot_set(j->dev.lm1, p, v);
+  joint_changed(j);
+}
+void dJointAddHinge2Torques(dJointID j, dReal t1, dReal t2) {  // hinge2.cpp:393-410
+  if (!j->node[0] || !j->node[1]) return;
+  pull_state(j->world);
+  float a1[3], a2[3], t[3];
+  h_mv(a1, j->node[0]->R, j->dev.axis1);
+  h_mv(a2, j->node[1]->R, j->dev.axis2);
+  for (int k = 0; k < 3; k++) t[k] = a1[k] * t1 + a2[k] * t2;
+  dBodyAddTorque(j->node[0], t[0], t[1], t[2]);
+  dBodyAddTorque(j->node[1], -t[0], -t[1], -t[2]);
+}
+// ode.cpp:1518-1536
+int dAreConnected(dBodyID a, dBodyID b) {
+  for (dxJoint *j : a->jlist) {
+    dxBody *other = (j->node[0] == a) ? j->node[1] : j->node[0];
+    if (other == b) return 1;
+  }
+  return 0;
+}
+int dAreConnectedExcluding(dBodyID a, dBodyID b, int joint_type) {
+  for (dxJoint *j : a->jlist) {
+    dxBody *other = (j->node[0] == a) ? j->node[1] : j->node[0];
+    if (j->type != joint_type && other == b) return 1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- spaces
+dSpaceID dSimpleSpaceCreate(dSpaceID) {
+  dxSpace *s = new dxSpace();
+  s->cls = dSimpleSpaceClass;
+  return s;
+}
+dSpaceID dHashSpaceCreate(dSpaceID) { return new dxSpace(); }
+void dHashSpaceSetLevels(dSpaceID s, int lo, int hi) {
+  s->minlevel = lo;
+  s->maxlevel = hi;
+}
+void dSpaceDestroy(dSpaceID s) {
+  // the space owns its geoms by default (collision_space.cpp:86-104)
+  std::vector<dxGeom *> gl = s->list;
+  for (dxGeom *g : gl) dGeomDestroy(g);
+  for (dxWorld *w : std::vector<dxWorld *>{s->world})
+    if (w && w->mir_space == s) { w->mir_space = nullptr; mark_struct(w); }
+  delete s;
+}
+void dSpaceAdd(dSpaceID s, dGeomID g) {
+  if (g->space) return;
+  g->space = s;
+  s->list.insert(s->list.begin(), g);
+  g->dirty = true;
+  dxWorld *w = space_world(s);
+  if (w) mark_struct(w);
+}
+void dSpaceRemove(dSpaceID s, dGeomID g) {
+  auto it = std::find(s->list.begin(), s->list.end(), g);
+  if (it != s->list.end()) s->list.erase(it);
+  g->space = nullptr;
+  if (s->world) mark_struct(s->world);
+}
+int dSpaceGetNumGeoms(dSpaceID s) { return (int)s->list.size(); }
+
+// ---------------------------------------------------------------- geoms
+static dxGeom *make_geom(dSpaceID s, int cls) {
+  dxGeom *g = new dxGeom();
+  g->cls = cls;
+  std::memset(g->R, 0, sizeof(g->R));
+  g->R[0] = g->R[5] = g->R[10] = 1;
+  if (s) dSpaceAdd(s, g);
+  return g;
+}
+dGeomID dCreateSphere(dSpaceID s, dReal r) {
+  dxGeom *g = make_geom(s, dSphereClass);
+  g->param[0] = r;
+  return g;
+}
+dGeomID dCreateBox(dSpaceID s, dReal lx, dReal ly, dReal lz) {
+  dxGeom *g = make_geom(s, dBoxClass);
+  g->param[0] = lx; g->param[1] = ly; g->param[2] = lz;
+  return g;
+}
+dGeomID dCreatePlane(dSpaceID s, dReal a, dReal b, dReal c, dReal d) {
+  dxGeom *g = make_geom(s, dPlaneClass);
+  float l = a * a + b * b + c * c;   // plane.cpp:49-76
+  if (l > 0) {
+    l = 1.0f / std::sqrt(l);
+    g->param[0] = a * l; g->param[1] = b * l; g->param[2] = c * l; g->param[3] = d * l;
+  } else {
+    g->param[0] = 1;
+  }
+  return g;
+}
+void dGeomDestroy(dGeomID g) {
+  if (g->space) dSpaceRemove(g->space, g);
+  if (g->body) {
+    auto &L = g->body->geoms;
+    L.erase(std::find(L.begin(), L.end(), g));
+    mark_struct(g->body->world);
+  }
+  delete g;
+}
+void dGeomSetData(dGeomID g, void *d) { g->data = d; }
+void *dGeomGetData(dGeomID g) { return g->data; }
+void dGeomSetBody(dGeomID g, dBodyID b) {  // collision_kernel.cpp:507-556
+  if (b) {
+    if (g->body != b) {
+      if (g->body) {
+        auto &L = g->body->geoms;
+        L.erase(std::find(L.begin(), L.end(), g));
+      }
+      g->body = b;
+      b->geoms.insert(b->geoms.begin(), g);
+    }
+    geom_moved(g);
+    if (g->space && !g->space->world) g->space->world = b->world;
+    mark_struct(b->world);
+  } else if (g->body) {
+    pull_state(g->body->world);
+    std::memcpy(g->pos, g->body->pos, sizeof(g->pos));
+    std::memcpy(g->R, g->body->R, sizeof(g->R));
+    auto &L = g->body->geoms;
+    L.erase(std::find(L.begin(), L.end(), g));
+    mark_struct(g->body->world);
+    g->body = nullptr;
+  }
+}
+dBodyID dGeomGetBody(dGeomID g) { return g->body; }
+void dGeomSetPosition(dGeomID g, dReal x, dReal y, dReal z) {
+  if (g->body) { dBodySetPosition(g->body, x, y, z); return; }
+  g->pos[0] = x; g->pos[1] = y; g->pos[2] = z;
+  geom_moved(g);
+  if (g->space && g->space->world) mark_struct(g->space->world);
+}
+void dGeomSetRotation(dGeomID g, const dMatrix3 R) {
+  if (g->body) { dBodySetRotation(g->body, R); return; }
+  std::memcpy(g->R, R, sizeof(g->R));
+  geom_moved(g);
+  if (g->space && g->space->world) mark_struct(g->space->world);
+}
+void dGeomSetQuaternion(dGeomID g, const dQuaternion q) {
+  if (g->body) { dBodySetQuaternion(g->body, q); return; }
+  h_quat_to_rot(q, g->R);
+  geom_moved(g);
+  if (g->space && g->space->world) mark_struct(g->space->world);
+}
+const dReal *dGeomGetPosition(dGeomID g) {
+  if (g->body) return dBodyGetPosition(g->body);
+  return g->pos;
+}
+const dReal *dGeomGetRotation(dGeomID g) {
+  if (g->body) return dBodyGetRotation(g->body);
+  return g->R;
+}
+int dGeomGetClass(dGeomID g) { return g->cls; }
+void dGeomSetCategoryBits(dGeomID g, unsigned long bits) {
+  g->cat = bits;
+  if (g->space && g->space->world) mark_struct(g->space->world);
+}
+void dGeomSetCollideBits(dGeomID g, unsigned long bits) {
+  g->col = bits;
+  if (g->space && g->space->world) mark_struct(g->space->world);
+}
+dReal dGeomSphereGetRadius(dGeomID g) { return g->param[0]; }
+void dGeomBoxGetLengths(dGeomID g, dVector3 r) { for (int k = 0; k < 3; k++) r[k] = g->param[k]; }
+void dGeomPlaneGetParams(dGeomID g, dVector4 r) { for (int k = 0; k < 4; k++) r[k] = g->param[k]; }
+void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
+  // box.cpp:60-77, sphere.cpp:59-67, plane.cpp:80-107 on the host copies
+  const float *p = dGeomGetPosition(g);
+  const float *R = dGeomGetRotation(g);
+  if (g->cls == dSphereClass) {
+    for (int k = 0; k < 3; k++) { aabb[2 * k] = p[k] - g->param[0]; aabb[2 * k + 1] = p[k] + g->param[0]; }
+  } else if (g->cls == dBoxClass) {
+    for (int k = 0; k < 3; k++) {
+      float e = 0.5f * (std::fabs(R[4 * k] * g->param[0]) + std::fabs(R[4 * k + 1] * g->param[1]) +
+                        std::fabs(R[4 * k + 2] * g->param[2]));
+      aabb[2 * k] = p[k] - e;
+      aabb[2 * k + 1] = p[k] + e;
+    }
+  } else {
+    for (int k = 0; k < 6; k++) aabb[k] = (k & 1) ? INFINITY : -INFINITY;
+    const float *n = g->param;
+    int ax = (n[1] == 0 && n[2] == 0) ? 0 : ((n[0] == 0 && n[2] == 0) ? 1 : ((n[0] == 0 && n[1] == 0) ? 2 : -1));
+    if (ax >= 0) {
+      aabb[2 * ax] = n[ax] > 0 ? -INFINITY : -n[3];
+      aabb[2 * ax + 1] = n[ax] > 0 ? n[3] : INFINITY;
+    }
+  }
+}
+
+// dSpaceCollide (collision.h:825): GPU broadphase + speculative narrowphase,
+// then the user's callback for every pair in the reference's report order.
+void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
+  if (!s || !cb || s->list.size() < 2) {
+    if (s) for (dxGeom *g : s->list) g->dirty = false;
+    return;
+  }
+  dxWorld *w = space_world(s);
+  if (!w) {
+    report_error("dSpaceCollide: the space holds no geom attached to a body");
+    return;
+  }
+  if (!sync_mirror(w, s)) return;
+  BatchImpl *m = w->mir;
+  if (!world_params_upload(w, 0, s->cached_maxc)) return;
+  refresh_dev(m);
+  DevBatch D = m->dev;
+  D.pairs_out = w->pairs_out.as<PairRec>();
+  D.pcount_out = w->pcount_out.as<int>();
+  D.contacts_out = w->contacts_out.as<ContactRec>();
+  D.ctrl = nullptr;
+  if (launch_world_kernel(1, D, 0, 1, 0, m->wg, s->cached_maxc, m->stream) != hipSuccess ||
+      hipStreamSynchronize(m->stream) != hipSuccess) {
+    report_error("dSpaceCollide: collide kernel failed");
+    return;
+  }
+  long long cnt[5];
+  batch_counters(m, cnt);
+  if (cnt[4]) report_error("dSpaceCollide: device capacity exceeded (flags %lld)", cnt[4]);
+  int np = (int)cnt[0], nk = (int)cnt[1];
+  std::vector<PairRec> pr(np);
+  std::vector<int> pc(np);
+  std::vector<ContactRec> cr(nk);
+  if (np) {
+    (void)hipMemcpy(pr.data(), w->pairs_out.p, sizeof(PairRec) * np, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(pc.data(), w->pcount_out.p, sizeof(int) * np, hipMemcpyDeviceToHost);
+  }
+  if (nk) (void)hipMemcpy(cr.data(), w->contacts_out.p, sizeof(ContactRec) * nk, hipMemcpyDeviceToHost);
+  for (dxGeom *g : s->list) g->dirty = false;
+  std::vector<dxGeom *> mg = w->mgeoms;
+  int off = 0;
+  for (int i = 0; i < np; i++) {
+    CollideCtx saved = g_ctx;
+    g_ctx.active = true;
+    g_ctx.space = s;
+    g_ctx.g1 = mg[pr[i].g1];
+    g_ctx.g2 = mg[pr[i].g2];
+    g_ctx.con = cr.data() + off;
+    g_ctx.n = pc[i];
+    g_ctx.maxc = s->cached_maxc;
+    cb(data, g_ctx.g1, g_ctx.g2);
+    g_ctx = saved;
+    off += pc[i];
+  }
+}
+
+// dCollide (collision_kernel.cpp:292-339)
+int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip) {
+  int maxc = flags & 0xffff;
+  if (maxc <= 0 || o1 == o2) return 0;
+  if (o1->body && o1->body == o2->body) return 0;
+  if (g_ctx.active && g_ctx.g1 == o1 && g_ctx.g2 == o2 && g_ctx.space) {
+    int want = maxc > MAX_SLOTS ? MAX_SLOTS : maxc;
+    int have = g_ctx.maxc > MAX_SLOTS ? MAX_SLOTS : g_ctx.maxc;
+    g_ctx.space->cached_maxc = maxc;
+    if (want == have) {
+      for (int i = 0; i < g_ctx.n; i++) {
+        const ContactRec &c = g_ctx.con[i];
+        write_contact((dContactGeom *)((char *)contact + (size_t)skip * i), c.pos, c.normal, c.depth, o1, o2);
+      }
+      return g_ctx.n;
+    }
+  }
+  // one pair on the device with the current poses
+  dxWorld *w = o1->body ? o1->body->world : (o2->body ? o2->body->world : nullptr);
+  if (w) pull_state(w);
+  auto view = [](dxGeom *g) {
+    GeomView v;
+    v.type = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
+    const float *p = g->body ? g->body->pos : g->pos;
+    const float *R = g->body ? g->body->R : g->R;
+    v.p = make_float3(p[0], p[1], p[2]);
+    v.R.r0 = make_float3(R[0], R[1], R[2]);
+    v.R.r1 = make_float3(R[4], R[5], R[6]);
+    v.R.r2 = make_float3(R[8], R[9], R[10]);
+    v.prm = make_float4(g->param[0], g->param[1], g->param[2], g->param[3]);
+    return v;
+  };
+  static DevBuf out, nout;
+  static hipStream_t st = nullptr;
+  if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return 0;
+  if (!out.alloc(sizeof(CPoint) * MAX_SLOTS) || !nout.alloc(16)) return 0;
+  if (launch_collide_one(view(o1), view(o2), maxc, out.as<CPoint>(), nout.as<int>(), st) != hipSuccess) return 0;
+  CPoint pts[MAX_SLOTS];
+  int n = 0;
+  if (hipMemcpyAsync(&n, nout.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(pts, out.p, sizeof(pts), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return 0;
+  if (n > maxc) n = maxc;
+  for (int i = 0; i < n; i++) {
+    float p[3] = {pts[i].pos.x, pts[i].pos.y, pts[i].pos.z}, nn[3] = {pts[i].n.x, pts[i].n.y, pts[i].n.z};
+    write_contact((dContactGeom *)((char *)contact + (size_t)skip * i), p, nn, pts[i].depth, o1, o2);
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------- adjoint (ODE API form)
+int dWorldSetTape(dWorldID w, int on) {
+  if (!w) return 0;
+  w->tape = on != 0;
+  return 1;
+}
+int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in) {
+  (void)grad_out; (void)grad_in;
+  if (!w) return 0;
+  report_error("dWorldQuickStepBackward: use the batch form (dBatchStepBackward) in this build");
+  return 0;
+}
+
+}  // extern "C"

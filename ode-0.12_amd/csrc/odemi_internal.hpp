@@ -1,0 +1,81 @@
+// odemi_internal.hpp -- declarations shared by the host-side translation
+// units of the library (batch API, ODE API layer, adjoint).
+#pragma once
+#include "host_common.hpp"
+#include "odemi.h"
+
+namespace odemi {
+
+// Forward tape of recorded steps for the adjoint: per step a copy of the
+// pre-step body state and of the rows, lambdas and constraint records the
+// step produced.
+struct Tape {
+  bool recording = false;
+  int max_steps = 0;
+  int nsteps = 0;
+  DevBuf buf;                    // per step: state + per-world scratch copies
+  std::vector<long long> step_off;
+  long long state_bytes = 0;
+  long long scratch_bytes = 0;
+};
+
+// device arrays of a batch
+struct BatchBufs {
+  DevBuf wi, wp, pos, q, lvel, avel, facc, tacc, R, I, invI, mass, invMass;
+  DevBuf gtype, gbody, gparam, gpos, gR, grank, gcat, gcol;
+  DevBuf joints, adj_start, adj_other, bgeom_start, bgeom;
+  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback;
+};
+
+struct BatchImpl {
+  int nworlds = 0;
+  int nb = 0, ng = 0, nj = 0;
+  int order_mode = ORDER_COLORED;
+  std::vector<WorldInfo> wi;
+  std::vector<WorldParams> wp;
+  BatchBufs d;
+  DevBatch dev{};
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int wg = 256;
+  int grid = 0;           // persistent grid size (0: one workgroup per world)
+  float cell_scale = 1.0f;
+  bool ctrl_set = false;
+  int cu_count = 256;
+  Tape tape;              // forward tape for the adjoint
+};
+
+// host arrays of a batch before upload
+struct BatchArrays {
+  std::vector<WorldInfo> wi;
+  std::vector<WorldParams> wp;
+  std::vector<float4> pos, q, lvel, avel, facc, tacc, I, invI, gparam, gpos, gR;
+  std::vector<float> mass, invMass;
+  std::vector<int> gtype, gbody, grank, adj_start, adj_other, bgeom_start, bgeom;
+  std::vector<unsigned> gcat, gcol;
+  std::vector<JointDev> joints;
+};
+
+void destroy_graph(BatchImpl *b);
+void refresh_dev(BatchImpl *b);
+BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode);
+BatchImpl *batch_create(const dBatchDesc *D, int order_mode, double cap_scale);
+void batch_destroy(BatchImpl *b);
+int batch_step(BatchImpl *b, int nsteps, bool sync);
+int batch_step_plain(BatchImpl *b, int nsteps, bool sync);
+int batch_launch_once(BatchImpl *b);
+int batch_get_state(BatchImpl *b, float *out);
+int batch_set_state(BatchImpl *b, const float *in);
+int batch_counters(BatchImpl *b, long long *out5);
+int batch_set_controls(BatchImpl *b, const double *ctrl);
+int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid);
+int batch_autotune(BatchImpl *b, int trial_steps);
+
+int tape_begin(BatchImpl *b, int max_steps);
+int tape_end(BatchImpl *b);
+int tape_step(BatchImpl *b, bool sync);
+int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_in, float *grad_mu,
+                  float *grad_bounce);
+
+}  // namespace odemi

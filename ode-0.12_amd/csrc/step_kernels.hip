@@ -1,0 +1,1203 @@
+// step_kernels.hip -- the ODE 0.12 step path on MI355X, one workgroup per
+// world.
+//
+//   collide:  geom AABBs -> uniform-grid broadphase (cell keys, bitonic sort,
+//             run scan with a home-cell rule) -> narrowphase, one pair per lane
+//   step:     body preprocessing -> constraint rows (contact, ball, hinge,
+//             hinge2) -> SOR-LCP -> velocity update -> integration
+//
+// The SOR-LCP runs in one of two orders:
+//   ORDER_REFERENCE  islands found by the reference's depth-first walk, rows in
+//                    island/joint order, non-friction rows first and friction
+//                    rows reversed at the tail, shuffled every 8 iterations by
+//                    the reference's LCG (each island's RNG state is obtained by
+//                    skip-ahead so islands relax in parallel); candidate pairs
+//                    are sorted into the reference hash space's callback order.
+//   ORDER_COLORED    constraints coloured so that one colour shares no body;
+//                    a colour's constraints relax in parallel, one per lane.
+//
+// Reference: ode/src/collision_space.cpp:420-583 (hash space),
+// ode/src/quickstep.cpp:342-1025 (SOR-LCP, dxQuickStepper),
+// ode/src/util.cpp:255-568 (integrator, islands).
+#include <hip/hip_runtime.h>
+
+#include "colliders.hpp"
+#include "devmath.hpp"
+#include "joints_dev.hpp"
+#include "layout.hpp"
+
+namespace odemi {
+
+constexpr int WG = 256;
+constexpr int SORT_LDS = 4096;   // u64 keys sorted in LDS
+
+struct Shared {
+  int wsum[WG / 64];
+  float wmax[WG / 64];
+  int ival[16];
+  float fval[8];
+  unsigned long long keys[SORT_LDS];
+};
+
+// ------------------------------------------------------------------ block helpers
+
+__device__ inline int block_excl_scan(int v, int &total, Shared &sh) {
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh.wsum[wid] = x;
+  __syncthreads();
+  int nw = (blockDim.x + 63) >> 6, off = 0, tot = 0;
+  for (int k = 0; k < nw; k++) {
+    int s = sh.wsum[k];
+    if (k < wid) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+__device__ inline float block_max(float v, Shared &sh) {
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if (lane == 0) sh.wmax[wid] = v;
+  __syncthreads();
+  int nw = (blockDim.x + 63) >> 6;
+  float m = sh.wmax[0];
+  for (int k = 1; k < nw; k++) m = fmaxf(m, sh.wmax[k]);
+  __syncthreads();
+  return m;
+}
+
+__device__ inline int block_sum(int v, Shared &sh) {
+  int t;
+  block_excl_scan(v, t, sh);
+  return t;
+}
+
+// ascending bitonic sort of n (power of two) u64 keys; in LDS when it fits
+__device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh) {
+  bool lds = n <= SORT_LDS;
+  unsigned long long *k = lds ? sh.keys : a;
+  if (lds) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sh.keys[i] = a[i];
+    __syncthreads();
+  }
+  for (int size = 2; size <= n; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int j = i ^ stride;
+        if (j > i) {
+          bool asc = (i & size) == 0;
+          unsigned long long x = k[i], y = k[j];
+          if ((x > y) == asc) { k[i] = y; k[j] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (lds) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = sh.keys[i];
+    __syncthreads();
+  }
+}
+
+__device__ inline int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// ------------------------------------------------------------------ world context
+
+struct Ctx {
+  int w;
+  WorldInfo W;
+  ScratchLayout L;
+  char *S;
+  template <class T> __device__ T *at(long long off) const { return (T *)(S + off); }
+};
+
+__device__ inline GeomView load_geom(const DevBatch &B, const Ctx &c, int g) {
+  int gg = c.W.g0 + g;
+  GeomView v;
+  v.type = B.gtype[gg];
+  v.prm = B.gparam[gg];
+  int b = B.gbody[gg];
+  if (b >= 0) {
+    int gb = c.W.b0 + b;
+    v.p = xyz(B.pos[gb]);
+    v.R = load_mat(B.R + 3 * (size_t)gb);
+  } else {
+    v.p = xyz(B.gpos[gg]);
+    v.R = load_mat(B.gR + 3 * (size_t)gg);
+  }
+  return v;
+}
+
+__device__ inline bool aabb_overlap(const float *a, const float *b) {
+  return !(a[0] > b[1] || a[1] < b[0] || a[2] > b[3] || a[3] < b[2] || a[4] > b[5] || a[5] < b[4]);
+}
+
+// collideAABBs filters (collision_space_internal.h:48-82)
+__device__ inline bool pair_passes(const DevBatch &B, const Ctx &c, int g1, int g2, const float *aabb) {
+  int b1 = B.gbody[c.W.g0 + g1], b2 = B.gbody[c.W.g0 + g2];
+  if (b1 == b2 && b1 >= 0) return false;
+  unsigned c1 = B.gcat[c.W.g0 + g1], c2 = B.gcat[c.W.g0 + g2];
+  unsigned k1 = B.gcol[c.W.g0 + g1], k2 = B.gcol[c.W.g0 + g2];
+  if (((c1 & k2) || (c2 & k1)) == 0) return false;
+  return aabb_overlap(aabb + 6 * g1, aabb + 6 * g2);
+}
+
+// the reference's near-callback filter: skip bodies joined by a permanent joint
+__device__ inline bool connected(const DevBatch &B, const Ctx &c, int b1, int b2) {
+  if (b1 < 0 || b2 < 0 || !B.adj_start) return false;
+  int s = B.adj_start[c.W.b0 + b1], e = B.adj_start[c.W.b0 + b1 + 1];
+  for (int k = s; k < e; k++)
+    if (B.adj_other[k] == b2) return true;
+  return false;
+}
+
+__device__ inline unsigned cell_hash(int x, int y, int z) {
+  return ((unsigned)x * 73856093u) ^ ((unsigned)y * 19349663u) ^ ((unsigned)z * 83492791u);
+}
+
+// ------------------------------------------------------------------ collide
+
+// AABBs, big-geom flags, reference hash levels and discrete bounds
+// (collision_space.cpp:331-351, 444-476).
+__device__ void phase_aabb(const DevBatch &B, const Ctx &c, const WorldParams &P, Shared &sh) {
+  float *aabb = c.at<float>(c.L.aabb);
+  int *gbig = c.at<int>(c.L.gbig);
+  int *glevel = c.at<int>(c.L.glevel);
+  int *gdb = c.at<int>(c.L.gdb);
+  for (int g = threadIdx.x; g < c.W.ng; g += blockDim.x) {
+    GeomView v = load_geom(B, c, g);
+    float a[6];
+    geom_aabb(v, a);
+    for (int k = 0; k < 6; k++) aabb[6 * g + k] = a[k];
+    bool inf = false;
+    for (int k = 0; k < 6; k++) inf |= isinf(a[k]);
+    int lvl = 0x7fffffff;
+    if (!inf) {
+      float q = fmaxf(a[1] - a[0], fmaxf(a[3] - a[2], a[5] - a[4]));
+      int e;
+      frexpf(q, &e);
+      lvl = e < P.hash_min ? P.hash_min : e;
+    }
+    bool big = inf || lvl > P.hash_max;
+    gbig[g] = big ? 1 : 0;
+    glevel[g] = lvl;
+    if (!big) {
+      float cell = ldexpf(1.f, lvl);
+      for (int k = 0; k < 6; k++) gdb[6 * g + k] = (int)floorf(a[k] / cell);
+    }
+  }
+  __syncthreads();
+}
+
+// reference hash index of every non-big geom (position in the space list
+// among non-big geoms) and the big-geom list in reverse space order.
+__device__ void phase_ref_index(const DevBatch &B, const Ctx &c, Shared &sh) {
+  int *gbig = c.at<int>(c.L.gbig);
+  int *gidx = c.at<int>(c.L.gidx);
+  int *inv = c.at<int>(c.L.order);   // rank -> geom, then big list
+  for (int g = threadIdx.x; g < c.W.ng; g += blockDim.x) inv[B.grank[c.W.g0 + g]] = g;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0, nbig = 0;
+    for (int r = 0; r < c.W.ng; r++) {
+      int g = inv[r];
+      if (gbig[g]) { gidx[g] = -1; nbig++; }
+      else gidx[g] = n++;
+    }
+    // big geoms: reverse space order
+    int *big = inv + c.W.ng;
+    int k = 0;
+    for (int r = c.W.ng - 1; r >= 0; r--)
+      if (gbig[inv[r]]) big[k++] = inv[r];
+    for (int r = 0; r < nbig; r++) gbig[big[r]] = 1 + r;   // 1 + big rank
+    sh.ival[0] = n;
+    sh.ival[1] = nbig;
+  }
+  __syncthreads();
+}
+
+// key placing pair (a,b) of non-big geoms at the reference's first encounter
+// in the hash-space traversal; also returns the emitted orientation.
+__device__ inline unsigned long long ref_pair_key(const Ctx &c, int a, int b, int n, int &g1, int &g2) {
+  const int *gidx = c.at<int>(c.L.gidx);
+  const int *glevel = c.at<int>(c.L.glevel);
+  const int *gdb = c.at<int>(c.L.gdb);
+  int X = gidx[a] > gidx[b] ? a : b;    // larger index: earlier in the traversal
+  int Y = X == a ? b : a;
+  int outer, part;
+  if (glevel[Y] >= glevel[X]) { outer = X; part = Y; }
+  else { outer = Y; part = X; }
+  int L = glevel[part];
+  int sh = L - glevel[outer];
+  int d[3];
+  for (int k = 0; k < 3; k++) {
+    int o0 = gdb[6 * outer + 2 * k] >> sh;
+    int p0 = gdb[6 * part + 2 * k];
+    int c0 = o0 > p0 ? o0 : p0;
+    d[k] = c0 - o0;
+  }
+  g1 = outer;
+  g2 = part;
+  unsigned long long rank = (unsigned long long)(n - 1 - gidx[outer]);
+  return (rank << 40) | ((unsigned long long)(sh & 31) << 35) | ((unsigned long long)(d[0] & 1) << 34) |
+         ((unsigned long long)(d[1] & 1) << 33) | ((unsigned long long)(d[2] & 1) << 32) |
+         (unsigned long long)gidx[part];
+}
+
+// Uniform-grid broadphase.  Each non-big geom enters the (at most 8) cells
+// its AABB covers; entries are sorted by cell key; a pair is reported from the
+// run of its home cell (the low corner of the two cell ranges' overlap), so
+// every overlapping pair appears exactly once.  Big geoms (unbounded AABBs)
+// are tested against everything.  Returns the pair count.
+__device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldParams &P, Shared &sh, bool ref) {
+  const float *aabb = c.at<float>(c.L.aabb);
+  const int *gbig = c.at<int>(c.L.gbig);
+  unsigned long long *cells = c.at<unsigned long long>(c.L.cells);
+  int *ecount = c.at<int>(c.L.ecount);
+  PairRec *pairs = c.at<PairRec>(c.L.pairs);
+  unsigned long long *pkeys = c.at<unsigned long long>(c.L.pkeys);
+  const int ng = c.W.ng;
+  int nref = 0;
+  if (ref) {
+    phase_ref_index(B, c, sh);
+    nref = sh.ival[0];
+  }
+  // cell size from the largest extent
+  float ext = 0.f;
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+    if (gbig[g]) continue;
+    const float *a = aabb + 6 * g;
+    ext = fmaxf(ext, fmaxf(a[1] - a[0], fmaxf(a[3] - a[2], a[5] - a[4])));
+  }
+  ext = block_max(ext, sh);
+  float cell = fmaxf(ext * P.cell_scale * 1.0001f, 1e-6f);
+  float inv = 1.f / cell;
+  // cell entries: key = hash << 32 | geom << 3 | subcell
+  int base = 0;
+  for (int start = 0; start < ng; start += blockDim.x) {
+    int g = start + threadIdx.x;
+    int cnt = 0, lo[3] = {0, 0, 0}, span[3] = {1, 1, 1};
+    if (g < ng && !gbig[g]) {
+      const float *a = aabb + 6 * g;
+      for (int k = 0; k < 3; k++) {
+        lo[k] = (int)floorf(a[2 * k] * inv);
+        int hi = (int)floorf(a[2 * k + 1] * inv);
+        span[k] = hi > lo[k] ? 2 : 1;
+      }
+      cnt = span[0] * span[1] * span[2];
+    }
+    int tot;
+    int off = block_excl_scan(cnt, tot, sh);
+    if (cnt) {
+      int e = base + off;
+      for (int dz = 0; dz < span[2]; dz++)
+        for (int dy = 0; dy < span[1]; dy++)
+          for (int dx = 0; dx < span[0]; dx++) {
+            unsigned h = cell_hash(lo[0] + dx, lo[1] + dy, lo[2] + dz);
+            cells[e++] = ((unsigned long long)h << 32) | ((unsigned long long)g << 3) | (unsigned)(dx | dy << 1 | dz << 2);
+          }
+    }
+    base += tot;
+  }
+  const int ne = base;
+  const int np2 = pow2_at_least(ne < 2 ? 2 : ne);
+  for (int i = ne + threadIdx.x; i < np2; i += blockDim.x) cells[i] = ~0ULL;
+  __syncthreads();
+  bitonic_sort(cells, np2, sh);
+  // pair scan, counting pass then writing pass
+  auto visit = [&](int i, bool write, int &cnt, int out0) {
+    unsigned long long ei = cells[i];
+    unsigned hi = (unsigned)(ei >> 32);
+    int gi = (int)((ei & 0xffffffffu) >> 3), si = (int)(ei & 7);
+    const float *ai = aabb + 6 * gi;
+    int li[3];
+    for (int k = 0; k < 3; k++) li[k] = (int)floorf(ai[2 * k] * inv) + ((si >> k) & 1);
+    for (int j = i + 1; j < ne; j++) {
+      unsigned long long ej = cells[j];
+      if ((unsigned)(ej >> 32) != hi) break;
+      int gj = (int)((ej & 0xffffffffu) >> 3), sj = (int)(ej & 7);
+      if (gj == gi) continue;
+      const float *aj = aabb + 6 * gj;
+      bool same = true, home = true;
+      for (int k = 0; k < 3; k++) {
+        int ljk0 = (int)floorf(aj[2 * k] * inv);
+        int lj = ljk0 + ((sj >> k) & 1);
+        int lik0 = li[k] - ((si >> k) & 1);
+        int hc = lik0 > ljk0 ? lik0 : ljk0;
+        same &= lj == li[k];
+        home &= hc == li[k];
+      }
+      if (!same || !home) continue;
+      if (!pair_passes(B, c, gi, gj, aabb)) continue;
+      if (write) {
+        int o = out0 + cnt;
+        if (o < c.W.pcap) {
+          if (ref) {
+            int a, b;
+            pkeys[o] = ref_pair_key(c, gi, gj, nref, a, b);
+          } else {
+            pairs[o].g1 = gi < gj ? gi : gj;
+            pairs[o].g2 = gi < gj ? gj : gi;
+            pairs[o].key = 0;
+          }
+        }
+      }
+      cnt++;
+    }
+  };
+  int npairs = 0;
+  for (int start = 0; start < ne; start += blockDim.x) {
+    int i = start + threadIdx.x;
+    int cnt = 0;
+    if (i < ne) visit(i, false, cnt, 0);
+    int tot;
+    int off = block_excl_scan(cnt, tot, sh);
+    if (i < ne && cnt) {
+      int c2 = 0;
+      visit(i, true, c2, npairs + off);
+    }
+    npairs += tot;
+  }
+  // big geoms against the rest, then big against big
+  for (int start = 0; start < ng; start += blockDim.x) {
+    int g = start + threadIdx.x;
+    int cnt = 0;
+    if (g < ng)
+      for (int b = 0; b < ng; b++) {
+        if (!gbig[b] || b == g) continue;
+        if (gbig[g] && !(b > g)) continue;  // big-big once
+        if (pair_passes(B, c, g, b, aabb)) cnt++;
+      }
+    int tot;
+    int off = block_excl_scan(cnt, tot, sh);
+    if (cnt) {
+      int o = npairs + off;
+      for (int b = 0; b < ng; b++) {
+        if (!gbig[b] || b == g) continue;
+        if (gbig[g] && !(b > g)) continue;
+        if (!pair_passes(B, c, g, b, aabb)) continue;
+        if (o < c.W.pcap) {
+          if (ref) {
+            const int *gidx = c.at<int>(c.L.gidx);
+            int bb = gbig[b] - 1;
+            if (!gbig[g]) {
+              pkeys[o] = (1ULL << 63) | ((unsigned long long)(nref - 1 - gidx[g]) << 32) | (unsigned)bb;
+            } else {
+              int ga = gbig[g] - 1;
+              int r1 = ga < bb ? ga : bb, r2 = ga < bb ? bb : ga;
+              pkeys[o] = (3ULL << 62) | ((unsigned long long)r1 << 24) | (unsigned)r2;
+            }
+          } else {
+            pairs[o].g1 = g;
+            pairs[o].g2 = b;
+            pairs[o].key = 0;
+          }
+        }
+        o++;
+      }
+    }
+    npairs += tot;
+  }
+  if (npairs > c.W.pcap) {
+    if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_PAIRS);
+    npairs = c.W.pcap;
+  }
+  if (ref) {
+    // sort the keys into the callback order and decode them
+    int np = pow2_at_least(npairs < 2 ? 2 : npairs);
+    if (np > c.W.pcap) np = c.W.pcap;  // pcap is a power of two
+    for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
+    __syncthreads();
+    bitonic_sort(pkeys, np, sh);
+    const int *gidx = c.at<int>(c.L.gidx);
+    // inverse maps: hash index -> geom, big rank -> geom
+    int *inv = c.at<int>(c.L.order);
+    int *byidx = c.at<int>(c.L.island);
+    for (int g = threadIdx.x; g < ng; g += blockDim.x)
+      if (gidx[g] >= 0) byidx[gidx[g]] = g;
+    __syncthreads();
+    const int *big = inv + ng;
+    for (int i = threadIdx.x; i < npairs; i += blockDim.x) {
+      unsigned long long k = pkeys[i];
+      int a, b;
+      if ((k >> 62) == 3ULL) { a = big[(k >> 24) & 0xffffff]; b = big[k & 0xffffff]; }
+      else if (k >> 63) { a = byidx[nref - 1 - (int)((k >> 32) & 0x3fffffff)]; b = big[k & 0xffffffff]; }
+      else { a = byidx[nref - 1 - (int)(k >> 40)]; b = byidx[k & 0xffffffff]; }
+      pairs[i].g1 = a;
+      pairs[i].g2 = b;
+      pairs[i].key = k;
+    }
+  }
+  __syncthreads();
+  return npairs;
+}
+
+__device__ inline void store_contacts(const DevBatch &B, const Ctx &c, int g1, int g2, const CPoint *pts, int n,
+                                      ContactRec *dst, int room) {
+  int b1 = B.gbody[c.W.g0 + g1], b2 = B.gbody[c.W.g0 + g2];
+  int flags = 0;
+  if (b1 < 0) { b1 = b2; b2 = -1; flags = JF_REVERSE; }
+  for (int i = 0; i < n && i < room; i++) {
+    ContactRec &r = dst[i];
+    r.pos[0] = pts[i].pos.x; r.pos[1] = pts[i].pos.y; r.pos[2] = pts[i].pos.z;
+    r.depth = pts[i].depth;
+    r.normal[0] = pts[i].n.x; r.normal[1] = pts[i].n.y; r.normal[2] = pts[i].n.z;
+    r.g1 = g1; r.g2 = g2; r.b1 = b1; r.b2 = b2; r.flags = flags;
+  }
+}
+
+// narrowphase over the pair list, one pair per lane; contacts keep pair order.
+__device__ int phase_narrowphase(const DevBatch &B, const Ctx &c, const WorldParams &P, int npairs, Shared &sh,
+                                 int maxc, bool apply_policy) {
+  const PairRec *pairs = c.at<PairRec>(c.L.pairs);
+  int *pcount = c.at<int>(c.L.pcount);
+  ContactRec *con = c.at<ContactRec>(c.L.contacts);
+  int nk = 0;
+  if (maxc > MAX_SLOTS) maxc = MAX_SLOTS;
+  for (int start = 0; start < npairs; start += blockDim.x) {
+    int i = start + threadIdx.x;
+    CPoint pts[MAX_SLOTS];
+    int n = 0;
+    if (i < npairs) {
+      int g1 = pairs[i].g1, g2 = pairs[i].g2;
+      bool skip = false;
+      if (apply_policy && P.skip_connected) {
+        int b1 = B.gbody[c.W.g0 + g1], b2 = B.gbody[c.W.g0 + g2];
+        skip = connected(B, c, b1, b2);
+      }
+      if (!skip) {
+        GeomView a = load_geom(B, c, g1), b = load_geom(B, c, g2);
+        n = collide_pair(a, b, maxc, pts);
+      }
+      pcount[i] = n;
+    }
+    int tot;
+    int off = block_excl_scan(n, tot, sh);
+    if (n) {
+      int o = nk + off;
+      int room = c.W.kcap - o;
+      if (room > 0) store_contacts(B, c, pairs[i].g1, pairs[i].g2, pts, n, con + o, room);
+    }
+    nk += tot;
+  }
+  if (nk > c.W.kcap) {
+    if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_CONTACTS);
+    nk = c.W.kcap;
+  }
+  __syncthreads();
+  return nk;
+}
+
+// ------------------------------------------------------------------ step
+
+struct StepIn {
+  int nk;               // contacts
+  bool from_host;       // contacts are ContactIn records of the host
+  int seq_base;         // attach sequence of the first fused contact
+};
+
+__device__ inline Surf world_surface(const WorldParams &P) {
+  Surf s;
+  s.mode = P.surf_mode; s.mu = P.mu; s.mu2 = P.mu2; s.bounce = P.bounce; s.bounce_vel = P.bounce_vel;
+  s.soft_erp = P.soft_erp; s.soft_cfm = P.soft_cfm; s.motion1 = P.motion1; s.motion2 = P.motion2;
+  s.motionN = P.motionN; s.slip1 = P.slip1; s.slip2 = P.slip2; s.fdir1 = mk3(0.f, 0.f, 0.f);
+  return s;
+}
+
+__device__ inline Surf host_surface(const ContactIn &ci) {
+  Surf s;
+  s.mode = ci.mode; s.mu = ci.mu; s.mu2 = ci.mu2; s.bounce = ci.bounce; s.bounce_vel = ci.bounce_vel;
+  s.soft_erp = ci.soft_erp; s.soft_cfm = ci.soft_cfm; s.motion1 = ci.motion1; s.motion2 = ci.motion2;
+  s.motionN = ci.motionN; s.slip1 = ci.slip1; s.slip2 = ci.slip2;
+  s.fdir1 = mk3(ci.fdir1[0], ci.fdir1[1], ci.fdir1[2]);
+  return s;
+}
+
+// relax one row: ode/src/quickstep.cpp:484-582 (J and b scaled by ad there)
+__device__ inline void relax_row(Row *rows, int r, float *fc) {
+  Row &R = rows[r];
+  float old = R.lambda;
+  float *f1 = fc + 32 * R.b1 + 9;
+  float s = R.J[0] * f1[0] + R.J[1] * f1[1] + R.J[2] * f1[2] + R.J[3] * f1[3] + R.J[4] * f1[4] + R.J[5] * f1[5];
+  float *f2 = nullptr;
+  if (R.b2 >= 0) {
+    f2 = fc + 32 * R.b2 + 9;
+    s += R.J[6] * f2[0] + R.J[7] * f2[1] + R.J[8] * f2[2] + R.J[9] * f2[3] + R.J[10] * f2[4] + R.J[11] * f2[5];
+  }
+  float delta = R.ad * (R.rhs - old * R.cfm - s);
+  float lo, hi;
+  if (R.findex >= 0) {
+    hi = fabsf(R.hi * rows[R.findex].lambda);
+    lo = -hi;
+  } else {
+    lo = R.lo;
+    hi = R.hi;
+  }
+  float nl = old + delta;
+  if (nl < lo) { delta = lo - old; R.lambda = lo; }
+  else if (nl > hi) { delta = hi - old; R.lambda = hi; }
+  else R.lambda = nl;
+  for (int k = 0; k < 6; k++) f1[k] += delta * R.iMJ[k];
+  if (f2)
+    for (int k = 0; k < 6; k++) f2[k] += delta * R.iMJ[6 + k];
+}
+
+// complementarity residual of one row: |lambda - clamp(lambda - (A lambda - b)/A_ii)|
+__device__ inline float row_residual(const Row *rows, int r, const float *fc) {
+  const Row &R = rows[r];
+  const float *f1 = fc + 32 * R.b1 + 9;
+  float s = 0.f, d = 0.f;
+  for (int k = 0; k < 6; k++) { s += R.J[k] * f1[k]; d += R.J[k] * R.iMJ[k]; }
+  if (R.b2 >= 0) {
+    const float *f2 = fc + 32 * R.b2 + 9;
+    for (int k = 0; k < 6; k++) { s += R.J[6 + k] * f2[k]; d += R.J[6 + k] * R.iMJ[6 + k]; }
+  }
+  float Aii = d + R.cfm;
+  float y = s + R.cfm * R.lambda - R.rhs;
+  float lo = R.lo, hi = R.hi;
+  if (R.findex >= 0) { hi = fabsf(R.hi * rows[R.findex].lambda); lo = -hi; }
+  float p = R.lambda - y / Aii;
+  p = fminf(fmaxf(p, lo), hi);
+  return fabsf(R.lambda - p);
+}
+
+__device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, const StepIn &in, Shared &sh,
+                           bool ref) {
+  const int nb = c.W.nb, nj = c.W.nj, nk = in.nk;
+  const float h = P.dt, hinv = 1.f / P.dt;
+  float *bt = c.at<float>(c.L.btmp);   // per body: invIw 0..8, fc 9..14, tmp 15..20, extra torque 21..23
+  Cons *cons = c.at<Cons>(c.L.cons);
+  Row *rows = c.at<Row>(c.L.rows);
+  const ContactRec *con = c.at<ContactRec>(c.L.contacts);
+  int *misc = c.at<int>(c.L.misc);
+  // --- bodies: world inverse inertia, gyroscopic torque, gravity
+  //     (quickstep.cpp:609-665)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int gb = c.W.b0 + b;
+    Mat3 R = load_mat(B.R + 3 * (size_t)gb);
+    Mat3 iI = rotate_tensor(R, load_mat(B.invI + 3 * (size_t)gb));
+    float *t = bt + 32 * b;
+    t[0] = iI.r0.x; t[1] = iI.r0.y; t[2] = iI.r0.z;
+    t[3] = iI.r1.x; t[4] = iI.r1.y; t[5] = iI.r1.z;
+    t[6] = iI.r2.x; t[7] = iI.r2.y; t[8] = iI.r2.z;
+    for (int k = 9; k < 24; k++) t[k] = 0.f;
+    float3 w = xyz(B.avel[gb]);
+    float4 ta = B.tacc[gb];
+    Mat3 Iw = rotate_tensor(R, load_mat(B.I + 3 * (size_t)gb));
+    float3 gyro = cross(w, Iw.mul(w));
+    ta.x -= gyro.x; ta.y -= gyro.y; ta.z -= gyro.z;
+    B.tacc[gb] = ta;
+    float4 fa = B.facc[gb];
+    float m = B.mass[gb];
+    if (P.gravity[0] != 0.f) fa.x += m * P.gravity[0];
+    if (P.gravity[1] != 0.f) fa.y += m * P.gravity[1];
+    if (P.gravity[2] != 0.f) fa.z += m * P.gravity[2];
+    B.facc[gb] = fa;
+  }
+  // --- constraints: contacts (k < nk) then permanent joints
+  const int ncons = nk + nj;
+  for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+    Cons &q = cons[i];
+    if (i < nk) {
+      q.kind = JT_CONTACT;
+      q.src = i;
+      if (in.from_host) {
+        const ContactIn &ci = B.cin[B.cin_start[c.w] + i];
+        q.b1 = ci.b1; q.b2 = ci.b2; q.seq = ci.seq; q.fb = ci.fb;
+      } else {
+        q.b1 = con[i].b1; q.b2 = con[i].b2; q.seq = in.seq_base + i; q.fb = -1;
+      }
+    } else {
+      const JointDev &j = B.joints[c.W.j0 + i - nk];
+      q.kind = j.type; q.src = i - nk; q.b1 = j.b1; q.b2 = j.b2; q.seq = j.seq; q.fb = j.fb;
+    }
+    q.m = 0;
+    q.row0 = 0;
+  }
+  __syncthreads();
+  // --- getInfo1 (quickstep.cpp:670-688); joints update their limit state
+  for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+    Cons &q = cons[i];
+    int m = 0;
+    if (q.b1 >= 0) {
+      if (q.kind == JT_CONTACT) {
+        Surf s = in.from_host ? host_surface(B.cin[B.cin_start[c.w] + i]) : world_surface(P);
+        m = contact_info1(s);
+      } else {
+        JointDev &j = B.joints[c.W.j0 + q.src];
+        BodyPose s1 = load_pose(B, c.W.b0 + q.b1);
+        BodyPose s2;
+        if (q.b2 >= 0) s2 = load_pose(B, c.W.b0 + q.b2);
+        if (!(j.type == JT_HINGE2 && q.b2 < 0)) m = joint_info1(j, s1, q.b2 >= 0 ? &s2 : nullptr);
+      }
+    }
+    q.m = m;
+  }
+  __syncthreads();
+  // --- constraint order, row offsets, groups
+  int *corder = c.at<int>(c.L.order);          // constraint order (island order when ref)
+  int *isl = c.at<int>(c.L.island);            // ref: [cons0, ncons, row0, nrows, seed] per island
+  int *color = c.at<int>(c.L.color);
+  int nrows = 0, ngroups = 0;
+  if (ref) {
+    if (threadIdx.x == 0) {
+      // adjacency, newest attach first: body joint lists prepend (ode.cpp:1382-1389)
+      int *adjs = c.at<int>(c.L.adjs);
+      int *adj = c.at<int>(c.L.adj);
+      for (int b = 0; b <= nb; b++) adjs[b] = 0;
+      for (int i = 0; i < ncons; i++) {
+        if (cons[i].m <= 0 && cons[i].kind != JT_CONTACT) continue;
+        if (cons[i].b1 < 0) continue;
+        adjs[cons[i].b1 + 1]++;
+        if (cons[i].b2 >= 0) adjs[cons[i].b2 + 1]++;
+      }
+      for (int b = 0; b < nb; b++) adjs[b + 1] += adjs[b];
+      // visit constraints in descending attach sequence: contacts carry
+      // increasing sequence numbers in order; joints are merged in by seq
+      int *jord = color;   // joints sorted by seq (insertion sort, few joints)
+      int nj2 = 0;
+      for (int i = nk; i < ncons; i++) {
+        if (cons[i].b1 < 0) continue;
+        int s = cons[i].seq, p = nj2++;
+        while (p > 0 && cons[jord[p - 1]].seq > s) { jord[p] = jord[p - 1]; p--; }
+        jord[p] = i;
+      }
+      int *fill = isl;     // per-body fill cursor (scratch reuse)
+      for (int b = 0; b < nb; b++) fill[b] = adjs[b];
+      int ci = nk - 1, jj = nj2 - 1;
+      while (ci >= 0 || jj >= 0) {
+        int pick;
+        if (jj < 0) pick = ci--;
+        else if (ci < 0) pick = jord[jj--];
+        else if (cons[ci].seq > cons[jord[jj]].seq) pick = ci--;
+        else pick = jord[jj--];
+        const Cons &q = cons[pick];
+        if (q.b1 < 0) continue;
+        adj[fill[q.b1]++] = pick;
+        if (q.b2 >= 0) adj[fill[q.b2]++] = pick;
+      }
+      // islands: depth-first walk from bodies in world-list order, newest
+      // body first (util.cpp:420-487)
+      int *tag = color;           // body tags, then constraint tags
+      int *ctag = color + nb;
+      for (int b = 0; b < nb; b++) tag[b] = 0;
+      for (int i = 0; i < ncons; i++) ctag[i] = 0;
+      int *border = corder + ncons;  // body visit order
+      int *stack = isl + 5 * (nb + 1);  // island table first, then the stack
+      int nic = 0, nib = 0, nisl = 0;
+      for (int bb = nb - 1; bb >= 0; bb--) {
+        if (tag[bb]) continue;
+        tag[bb] = 1;
+        int c0 = nic;
+        border[nib++] = bb;
+        int sp = 0, b = bb;
+        while (true) {
+          for (int k = adjs[b]; k < adjs[b + 1]; k++) {
+            int qi = adj[k];
+            if (ctag[qi]) continue;
+            const Cons &q = cons[qi];
+            bool enabled = B.invMass[c.W.b0 + q.b1] > 0.f || (q.b2 >= 0 && B.invMass[c.W.b0 + q.b2] > 0.f);
+            if (!enabled) { ctag[qi] = 2; continue; }
+            ctag[qi] = 1;
+            corder[nic++] = qi;
+            int other = (q.b1 == b) ? q.b2 : q.b1;
+            if (other >= 0 && tag[other] <= 0) {
+              tag[other] = 1;
+              stack[sp++] = other;
+            }
+          }
+          if (sp == 0) break;
+          b = stack[--sp];
+          border[nib++] = b;
+        }
+        isl[5 * nisl + 0] = c0;
+        isl[5 * nisl + 1] = nic - c0;
+        nisl++;
+      }
+      for (int i = 0; i < ncons; i++)
+        if (ctag[i] != 1) cons[i].m = 0;
+      // rows in island order; the RNG state of every island by skip-ahead:
+      // an island with m rows makes ceil(iterations/8) * (m-1) draws
+      // (quickstep.cpp:473-482)
+      unsigned seed = P.seed;
+      unsigned long long shuffles = (unsigned long long)((P.iterations + 7) / 8);
+      int r = 0;
+      float erp = P.erp;
+      for (int k = 0; k < nisl; k++) {
+        int c0 = isl[5 * k], cn = isl[5 * k + 1];
+        int r0 = r;
+        erp = P.erp;   // Info2 erp is reset per island (quickstep.cpp:777-779)
+        for (int x = 0; x < cn; x++) {
+          Cons &q = cons[corder[c0 + x]];
+          q.row0 = r;
+          r += q.m;
+          // a ball joint sets the Info2 erp seen by the joints after it
+          if (q.kind == JT_BALL) erp = B.joints[c.W.j0 + q.src].erp;
+          ((float *)ctag)[corder[c0 + x]] = erp;
+        }
+        int m = r - r0;
+        isl[5 * k + 2] = r0;
+        isl[5 * k + 3] = m;
+        isl[5 * k + 4] = (int)seed;
+        if (m > 0 && m - 1 > 0) seed = lcg_skip(seed, shuffles * (unsigned long long)(m - 1));
+      }
+      P.seed = seed;
+      misc[0] = r;
+      misc[1] = nisl;
+      misc[2] = nic;
+      misc[3] = nib;
+    }
+    __syncthreads();
+    nrows = misc[0];
+    ngroups = misc[1];
+  } else {
+    // natural order; rows by a block scan
+    int r0 = 0;
+    for (int start = 0; start < ncons; start += blockDim.x) {
+      int i = start + threadIdx.x;
+      int m = (i < ncons) ? cons[i].m : 0;
+      int tot;
+      int off = block_excl_scan(m, tot, sh);
+      if (i < ncons) { cons[i].row0 = r0 + off; corder[i] = i; }
+      r0 += tot;
+    }
+    nrows = r0;
+    // colouring: each round takes the constraints whose priority is the
+    // largest among the still uncoloured constraints at both their bodies
+    int *bmax = c.at<int>(c.L.adjs);   // per body (nb+1 ints)
+    for (int i = threadIdx.x; i < ncons; i += blockDim.x) color[i] = (cons[i].m > 0) ? -1 : -2;
+    __syncthreads();
+    int round = 0;
+    while (true) {
+      for (int b = threadIdx.x; b < nb; b += blockDim.x) bmax[b] = -1;
+      __syncthreads();
+      int left = 0;
+      for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+        if (color[i] != -1) continue;
+        left++;
+        int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);   // unique hashed priority
+        atomicMax(&bmax[cons[i].b1], pr);
+        if (cons[i].b2 >= 0) atomicMax(&bmax[cons[i].b2], pr);
+      }
+      int nleft = block_sum(left, sh);
+      if (nleft == 0) break;
+      for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+        if (color[i] != -1) continue;
+        int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);
+        bool win = bmax[cons[i].b1] == pr && (cons[i].b2 < 0 || bmax[cons[i].b2] == pr);
+        if (win) color[i] = round;
+      }
+      __syncthreads();
+      round++;
+    }
+    ngroups = round;
+    // bucket constraints by colour: corder sorted by (colour, index)
+    int *cstart = c.at<int>(c.L.island);   // ngroups+1 starts
+    if (threadIdx.x == 0) {
+      for (int k = 0; k <= ngroups; k++) cstart[k] = 0;
+      for (int i = 0; i < ncons; i++)
+        if (color[i] >= 0) cstart[color[i] + 1]++;
+      for (int k = 0; k < ngroups; k++) cstart[k + 1] += cstart[k];
+      int *cur = cstart + ngroups + 1;
+      for (int k = 0; k < ngroups; k++) cur[k] = cstart[k];
+      for (int i = 0; i < ncons; i++)
+        if (color[i] >= 0) corder[cur[color[i]]++] = i;
+    }
+    __syncthreads();
+  }
+  if (nrows > c.W.rcap) {
+    // not enough row storage: leave the world unchanged for this step
+    if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_ROWS);
+    __syncthreads();
+    return;
+  }
+  // --- getInfo2 (quickstep.cpp:764-830): one constraint per lane
+  const float *erp_of = ref ? (const float *)(color + nb) : nullptr;
+  for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+    Cons &q = cons[i];
+    if (q.m <= 0) continue;
+    Row *r = rows + q.row0;
+    float cfm0 = P.cfm;
+    for (int k = 0; k < q.m; k++) {
+      row_clear(r[k], cfm0);
+      r[k].b1 = q.b1;
+      r[k].b2 = q.b2;
+    }
+    BodyPose s1 = load_pose(B, c.W.b0 + q.b1);
+    BodyPose s2;
+    if (q.b2 >= 0) s2 = load_pose(B, c.W.b0 + q.b2);
+    const BodyPose *p2 = q.b2 >= 0 ? &s2 : nullptr;
+    float erp = erp_of ? erp_of[i] : P.erp;
+    if (q.kind == JT_CONTACT) {
+      float3 cpos, cn;
+      float cd;
+      bool rev;
+      Surf s;
+      if (in.from_host) {
+        const ContactIn &ci = B.cin[B.cin_start[c.w] + q.src];
+        cpos = mk3(ci.pos[0], ci.pos[1], ci.pos[2]);
+        cn = mk3(ci.normal[0], ci.normal[1], ci.normal[2]);
+        cd = ci.depth;
+        rev = (ci.flags & JF_REVERSE) != 0;
+        s = host_surface(ci);
+      } else {
+        const ContactRec &cr = con[q.src];
+        cpos = mk3(cr.pos[0], cr.pos[1], cr.pos[2]);
+        cn = mk3(cr.normal[0], cr.normal[1], cr.normal[2]);
+        cd = cr.depth;
+        rev = (cr.flags & JF_REVERSE) != 0;
+        s = world_surface(P);
+      }
+      int m = contact_info1(s);
+      contact_info2(r, m, s, cpos, cn, cd, rev, s1, p2, hinv, erp, P.min_depth, P.max_vel);
+    } else {
+      JointDev &j = B.joints[c.W.j0 + q.src];
+      float3 e1 = mk3(0.f, 0.f, 0.f), e2 = mk3(0.f, 0.f, 0.f);
+      float e = erp;
+      joint_info2(j, r, s1, p2, hinv, e, e1, e2);
+      float *t1 = bt + 32 * q.b1 + 21;
+      if (e1.x != 0.f || e1.y != 0.f || e1.z != 0.f) {
+        // a motor pushing against its limit: torque applied below in order
+        atomicAdd(&t1[0], e1.x); atomicAdd(&t1[1], e1.y); atomicAdd(&t1[2], e1.z);
+        if (q.b2 >= 0) {
+          float *t2 = bt + 32 * q.b2 + 21;
+          atomicAdd(&t2[0], e2.x); atomicAdd(&t2[1], e2.y); atomicAdd(&t2[2], e2.z);
+        }
+      }
+    }
+    for (int k = 0; k < q.m; k++)
+      if (r[k].findex >= 0) r[k].findex += q.row0;
+  }
+  __syncthreads();
+  // --- rhs = c/h - J (v/h + inv(M) f_ext); cfm /= h (quickstep.cpp:832-857)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int gb = c.W.b0 + b;
+    float *t = bt + 32 * b;
+    float4 ta = B.tacc[gb];
+    ta.x += t[21]; ta.y += t[22]; ta.z += t[23];
+    B.tacc[gb] = ta;
+    float im = B.invMass[gb];
+    float4 fa = B.facc[gb];
+    float3 v = xyz(B.lvel[gb]), w = xyz(B.avel[gb]);
+    t[15] = fa.x * im + v.x * hinv;
+    t[16] = fa.y * im + v.y * hinv;
+    t[17] = fa.z * im + v.z * hinv;
+    float3 it = mk3(t[0] * ta.x + t[1] * ta.y + t[2] * ta.z, t[3] * ta.x + t[4] * ta.y + t[5] * ta.z,
+                    t[6] * ta.x + t[7] * ta.y + t[8] * ta.z);
+    t[18] = it.x + w.x * hinv;
+    t[19] = it.y + w.y * hinv;
+    t[20] = it.z + w.z * hinv;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nrows; i += blockDim.x) {
+    Row &R = rows[i];
+    const float *t1 = bt + 32 * R.b1;
+    float s = 0.f;
+    for (int k = 0; k < 6; k++) s += R.J[k] * t1[15 + k];
+    float d = 0.f;
+    float im1 = B.invMass[c.W.b0 + R.b1];
+    for (int k = 0; k < 3; k++) R.iMJ[k] = im1 * R.J[k];
+    for (int k = 0; k < 3; k++) R.iMJ[3 + k] = t1[3 * k] * R.J[3] + t1[3 * k + 1] * R.J[4] + t1[3 * k + 2] * R.J[5];
+    for (int k = 0; k < 6; k++) d += R.iMJ[k] * R.J[k];
+    if (R.b2 >= 0) {
+      const float *t2 = bt + 32 * R.b2;
+      for (int k = 0; k < 6; k++) s += R.J[6 + k] * t2[15 + k];
+      float im2 = B.invMass[c.W.b0 + R.b2];
+      for (int k = 0; k < 3; k++) R.iMJ[6 + k] = im2 * R.J[6 + k];
+      for (int k = 0; k < 3; k++) R.iMJ[9 + k] = t2[3 * k] * R.J[9] + t2[3 * k + 1] * R.J[10] + t2[3 * k + 2] * R.J[11];
+      for (int k = 6; k < 12; k++) d += R.iMJ[k] * R.J[k];
+    } else {
+      for (int k = 6; k < 12; k++) R.iMJ[k] = 0.f;
+    }
+    R.rhs = R.rhs * hinv - s;
+    R.cfm = R.cfm * hinv;
+    R.ad = P.sor_w / (d + R.cfm);
+    R.lambda = 0.f;
+  }
+  __syncthreads();
+  // --- SOR-LCP
+  const int iters = P.iterations;
+  if (ref) {
+    // one island per lane, rows in the reference order with its shuffle
+    int *ord = corder + ncons + nb;
+    for (int k = threadIdx.x; k < ngroups; k += blockDim.x) {
+      int r0 = isl[5 * k + 2], m = isl[5 * k + 3];
+      if (m <= 0) continue;
+      unsigned seed = (unsigned)isl[5 * k + 4];
+      int *o = ord + r0;
+      int head = 0, tail = m - 1;
+      for (int i = 0; i < m; i++) {
+        if (rows[r0 + i].findex == -1) o[head++] = r0 + i;
+        else o[tail--] = r0 + i;
+      }
+      for (int it = 0; it < iters; it++) {
+        if ((it & 7) == 0) {
+          for (int i = 1; i < m; i++) {
+            int sw = lcg_below(seed, i + 1);
+            int t = o[i]; o[i] = o[sw]; o[sw] = t;
+          }
+        }
+        for (int i = 0; i < m; i++) relax_row(rows, o[i], bt);
+      }
+    }
+  } else {
+    const int *cstart = c.at<int>(c.L.island);
+    for (int it = 0; it < iters; it++) {
+      for (int g = 0; g < ngroups; g++) {
+        int s0 = cstart[g], s1 = cstart[g + 1];
+        for (int x = s0 + threadIdx.x; x < s1; x += blockDim.x) {
+          const Cons &q = cons[corder[x]];
+          for (int k = 0; k < q.m; k++)
+            if (rows[q.row0 + k].findex < 0) relax_row(rows, q.row0 + k, bt);
+          for (int k = 0; k < q.m; k++)
+            if (rows[q.row0 + k].findex >= 0) relax_row(rows, q.row0 + k, bt);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  // --- residual (optional), feedback
+  if (B.track_residual) {
+    float mr = 0.f;
+    for (int i = threadIdx.x; i < nrows; i += blockDim.x) mr = fmaxf(mr, row_residual(rows, i, bt));
+    mr = block_max(mr, sh);
+    if (threadIdx.x == 0) B.residual[c.w] = mr;
+  }
+  if (B.feedback) {
+    for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+      const Cons &q = cons[i];
+      if (q.fb < 0 || q.m <= 0) continue;
+      float d[12];
+      for (int k = 0; k < 12; k++) d[k] = 0.f;
+      for (int r = 0; r < q.m; r++) {
+        const Row &R = rows[q.row0 + r];
+        for (int k = 0; k < 12; k++) d[k] += R.J[k] * R.lambda;
+      }
+      float *o = B.feedback + 12 * (size_t)q.fb;
+      for (int k = 0; k < 6; k++) o[k] = d[k];
+      if (q.b2 >= 0)
+        for (int k = 0; k < 6; k++) o[6 + k] = d[6 + k];
+    }
+  }
+  // --- velocity update, integration (quickstep.cpp:905-1021, util.cpp:255-360)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int gb = c.W.b0 + b;
+    float *t = bt + 32 * b;
+    float3 v = xyz(B.lvel[gb]), w = xyz(B.avel[gb]);
+    v = v + h * mk3(t[9], t[10], t[11]);
+    w = w + h * mk3(t[12], t[13], t[14]);
+    float im = B.invMass[gb] * h;
+    float4 fa = B.facc[gb], ta = B.tacc[gb];
+    v = v + im * xyz(fa);
+    float3 th = h * xyz(ta);
+    w = w + mk3(t[0] * th.x + t[1] * th.y + t[2] * th.z, t[3] * th.x + t[4] * th.y + t[5] * th.z,
+                t[6] * th.x + t[7] * th.y + t[8] * th.z);
+    float3 p = xyz(B.pos[gb]) + h * v;
+    float4 q = B.q[gb];
+    float4 dq = make_float4(0.5f * (-w.x * q.y - w.y * q.z - w.z * q.w), 0.5f * (w.x * q.x + w.y * q.w - w.z * q.z),
+                            0.5f * (-w.x * q.w + w.y * q.x + w.z * q.y), 0.5f * (w.x * q.z - w.y * q.y + w.z * q.x));
+    q = make_float4(q.x + h * dq.x, q.y + h * dq.y, q.z + h * dq.z, q.w + h * dq.w);
+    q = quat_normalize(q);
+    B.pos[gb] = f4(p);
+    B.q[gb] = q;
+    store_mat(B.R + 3 * (size_t)gb, quat_to_mat(q));
+    B.lvel[gb] = f4(v);
+    B.avel[gb] = f4(w);
+    B.facc[gb] = make_float4(0.f, 0.f, 0.f, 0.f);
+    B.tacc[gb] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // --- space list after dGeomMoved of every stepped body: moved geoms move
+  //     to the front in reverse move order (collision_space.cpp:47-75, 209-213)
+  if (ref && threadIdx.x == 0) {
+    int nib = misc[3];
+    const int *border = corder + ncons;
+    int *old = isl + 5 * (ngroups + 1);
+    const int ng = c.W.ng;
+    int *inv = old;                     // old rank -> geom
+    for (int g = 0; g < ng; g++) inv[B.grank[c.W.g0 + g]] = g;
+    int *moved = old + ng;
+    int M = 0;
+    for (int x = 0; x < nib; x++) {
+      int gb = c.W.b0 + border[x];
+      for (int k = B.bgeom_start[gb]; k < B.bgeom_start[gb + 1]; k++) moved[M++] = B.bgeom[k];
+    }
+    int *flag = moved + M;
+    for (int g = 0; g < ng; g++) flag[g] = 0;
+    for (int x = 0; x < M; x++) {
+      B.grank[c.W.g0 + moved[x]] = M - 1 - x;
+      flag[moved[x]] = 1;
+    }
+    int r = M;
+    for (int k = 0; k < ng; k++) {
+      int g = inv[k];
+      if (!flag[g]) B.grank[c.W.g0 + g] = r++;
+    }
+  }
+  if (threadIdx.x == 0) {
+    B.counters[c.w * CNT_N + CNT_ROWS] = nrows;
+    B.counters[c.w * CNT_N + CNT_GROUPS] = ngroups;
+  }
+  __syncthreads();
+}
+
+// joint controls (steer stops, wheel torques) at the start of a step
+__device__ void phase_controls(const DevBatch &B, const Ctx &c) {
+  if (!B.ctrl) return;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < c.W.nj; i++) {
+      JointDev &j = B.joints[c.W.j0 + i];
+      float4 u = B.ctrl[c.W.j0 + i];
+      if (!isnan(u.x)) j.lm1.lostop = u.x;
+      if (!isnan(u.y)) j.lm1.histop = u.y;
+      if (j.type == JT_HINGE2 && (u.z != 0.f || u.w != 0.f) && j.b1 >= 0 && j.b2 >= 0) {
+        // dJointAddHinge2Torques (hinge2.cpp:393-410)
+        int g1 = c.W.b0 + j.b1, g2 = c.W.b0 + j.b2;
+        float3 a1 = load_mat(B.R + 3 * (size_t)g1).mul(mk3(j.axis1[0], j.axis1[1], j.axis1[2]));
+        float3 a2 = load_mat(B.R + 3 * (size_t)g2).mul(mk3(j.axis2[0], j.axis2[1], j.axis2[2]));
+        float3 t = u.z * a1 + u.w * a2;
+        float4 t1 = B.tacc[g1], t2 = B.tacc[g2];
+        t1.x += t.x; t1.y += t.y; t1.z += t.z;
+        t2.x -= t.x; t2.y -= t.y; t2.z -= t.z;
+        B.tacc[g1] = t1;
+        B.tacc[g2] = t2;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ kernels
+
+enum { MODE_COLLIDE = 1, MODE_STEP = 2, MODE_FUSED = 3 };
+
+template <int MODE>
+__device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
+  Ctx c;
+  c.w = w;
+  c.W = B.wi[w];
+  c.L = scratch_layout(c.W.nb, c.W.ng, c.W.nj, c.W.pcap, c.W.kcap, c.W.rcap);
+  c.S = B.scratch + c.W.scratch;
+  WorldParams &P = B.wp[w];
+  const bool ref = B.order_mode == ORDER_REFERENCE;
+  if (threadIdx.x == 0) B.counters[w * CNT_N + CNT_OVERFLOW] = 0;
+  if (MODE == MODE_FUSED) phase_controls(B, c);
+  int nk = 0;
+  if (MODE & MODE_COLLIDE) {
+    phase_aabb(B, c, P, sh);
+    int np = phase_broadphase(B, c, P, sh, ref);
+    int maxc = (MODE == MODE_FUSED) ? P.max_contacts : api_maxc;
+    nk = phase_narrowphase(B, c, P, np, sh, maxc, MODE == MODE_FUSED);
+    if (threadIdx.x == 0) {
+      B.counters[w * CNT_N + CNT_PAIRS] = np;
+      B.counters[w * CNT_N + CNT_CONTACTS] = nk;
+    }
+    if (MODE == MODE_COLLIDE) {
+      // hand pairs and contacts to the host (one world per launch)
+      const PairRec *pr = c.at<PairRec>(c.L.pairs);
+      const int *pc = c.at<int>(c.L.pcount);
+      const ContactRec *cr = c.at<ContactRec>(c.L.contacts);
+      for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        B.pairs_out[i] = pr[i];
+        B.pcount_out[i] = pc[i];
+      }
+      for (int i = threadIdx.x; i < nk; i += blockDim.x) B.contacts_out[i] = cr[i];
+    }
+  }
+  if (MODE & MODE_STEP) {
+    StepIn in;
+    in.from_host = (MODE == MODE_STEP);
+    in.nk = in.from_host ? B.cin_count[w] : nk;
+    in.seq_base = 1 << 20;
+    if (in.from_host && in.nk > c.W.kcap) {
+      if (threadIdx.x == 0) atomicOr(&B.counters[w * CNT_N + CNT_OVERFLOW], OVF_CONTACTS);
+      in.nk = c.W.kcap;
+    }
+    phase_step(B, c, P, in, sh, ref);
+  }
+  __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(WG) void world_kernel(DevBatch B, int first_world, int persistent, int api_maxc) {
+  __shared__ Shared sh;
+  if (!persistent) {
+    int w = first_world + blockIdx.x;
+    if (w < B.nworlds) run_world<MODE>(B, B.world_order ? B.world_order[w] : w, sh, api_maxc);
+    return;
+  }
+  // persistent: workgroups pull worlds (largest first) from a queue
+  while (true) {
+    if (threadIdx.x == 0) sh.ival[15] = atomicAdd(B.work_counter, 1);
+    __syncthreads();
+    int k = sh.ival[15];
+    __syncthreads();
+    if (k >= B.nworlds) break;
+    run_world<MODE>(B, B.world_order ? B.world_order[k] : k, sh, api_maxc);
+  }
+}
+
+__global__ void reset_counter_kernel(int *ctr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0;
+}
+
+// one dCollide call on the device (ODE API path outside a cached batch)
+__global__ void collide_one_kernel(GeomView a, GeomView b, int maxc, CPoint *out, int *n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    CPoint pts[MAX_SLOTS];
+    int k = collide_pair(a, b, maxc > MAX_SLOTS ? MAX_SLOTS : maxc, pts);
+    for (int i = 0; i < k; i++) out[i] = pts[i];
+    *n = k;
+  }
+}
+
+// body-frame setup: inverse inertia is computed on the host; here the
+// rotation matrices are derived from the quaternions
+__global__ void init_rot_kernel(DevBatch B, int nbodies) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbodies) store_mat(B.R + 3 * (size_t)b, quat_to_mat(B.q[b]));
+}
+
+}  // namespace odemi
+
+// ------------------------------------------------------------------ launch API
+namespace odemi {
+
+hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid,
+                               int wg, int api_maxc, hipStream_t st) {
+  int persistent = grid > 0 ? 1 : 0;
+  int blocks = persistent ? grid : nworlds_launch;
+  if (blocks <= 0) return hipSuccess;
+  if (wg > WG || wg < 64 || (wg & 63)) return hipErrorInvalidValue;
+  if (persistent) reset_counter_kernel<<<1, 64, 0, st>>>(B.work_counter);
+  switch (mode) {
+    case MODE_COLLIDE: world_kernel<MODE_COLLIDE><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
+    case MODE_STEP: world_kernel<MODE_STEP><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
+    default: world_kernel<MODE_FUSED><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_collide_one(const GeomView &a, const GeomView &b, int maxc, CPoint *out, int *n, hipStream_t st) {
+  collide_one_kernel<<<1, 64, 0, st>>>(a, b, maxc, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_rot(const DevBatch &B, int nbodies, hipStream_t st) {
+  if (nbodies <= 0) return hipSuccess;
+  init_rot_kernel<<<(nbodies + 255) / 256, 256, 0, st>>>(B, nbodies);
+  return hipGetLastError();
+}
+
+}  // namespace odemi

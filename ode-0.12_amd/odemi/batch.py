@@ -1,0 +1,147 @@
+"""Batch of independent worlds on the GPU (dBatch* entry points)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .capi import BatchDesc, BatchWorldDesc, load_library
+
+ORDER_REFERENCE = 0
+ORDER_COLORED = 1
+
+
+def _cat(scenes, name, width, dtype=np.float64):
+    parts = [np.asarray(getattr(s, name), dtype).reshape(-1, width) if width else
+             np.asarray(getattr(s, name), dtype).reshape(-1) for s in scenes]
+    a = np.concatenate(parts) if parts else np.zeros((0, width) if width else (0,), dtype)
+    return np.ascontiguousarray(a)
+
+
+class Batch:
+    """Device state of a list of `odemi.scenes.Scene` worlds.
+
+    `step(n)` runs the batched path (one persistent launch per step from a
+    HIP graph); `step_plain(n)` runs one world per launch.
+    """
+
+    def __init__(self, scenes, order: int = ORDER_COLORED):
+        self.L = load_library()
+        self.scenes = list(scenes)
+        self.nworlds = len(self.scenes)
+        self.nb = sum(s.nb for s in self.scenes)
+        self.nj = sum(s.nj for s in self.scenes)
+        self.body_offsets = np.cumsum([0] + [s.nb for s in self.scenes])
+        self.joint_offsets = np.cumsum([0] + [s.nj for s in self.scenes])
+        W = (BatchWorldDesc * self.nworlds)()
+        for w, s in zip(W, self.scenes):
+            w.gravity[:] = list(s.gravity)
+            for k in ("erp", "cfm", "iterations", "sor_w", "max_vel", "min_depth", "dt", "seed", "hash_minlevel",
+                      "hash_maxlevel", "surf_mode", "mu", "mu2", "bounce", "bounce_vel", "soft_erp", "soft_cfm",
+                      "max_contacts", "skip_connected"):
+                setattr(w, k, getattr(s, k))
+            w.nb, w.ng, w.nj = s.nb, s.ng, s.nj
+        keep = [W]
+        D = BatchDesc()
+        D.nworlds = self.nworlds
+        D.worlds = C.cast(W, C.POINTER(BatchWorldDesc))
+        for name, width in (("body_pos", 3), ("body_q", 4), ("body_lvel", 3), ("body_avel", 3), ("body_mass", 0),
+                            ("body_I", 9), ("geom_param", 4), ("geom_pos", 3), ("geom_q", 4),
+                            ("joint_anchor", 3), ("joint_axis1", 3), ("joint_axis2", 3), ("joint_param", 14)):
+            a = _cat(self.scenes, name, width)
+            keep.append(a)
+            setattr(D, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+        for name in ("geom_type", "geom_body", "joint_type", "joint_b1", "joint_b2"):
+            a = _cat(self.scenes, name, 0, np.int32)
+            keep.append(a)
+            setattr(D, name, a.ctypes.data_as(C.POINTER(C.c_int)))
+        self.h = self.L.dBatchCreate(C.byref(D), order)
+        del keep
+        if not self.h:
+            raise RuntimeError("dBatchCreate failed")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dBatchDestroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def step(self, n: int = 1):
+        if not self.L.dBatchStep(self.h, n):
+            raise RuntimeError("dBatchStep failed")
+
+    def step_timed(self, n: int = 1) -> float:
+        """n batched steps; returns their device time in ms (HIP events)."""
+        ms = C.c_float()
+        if not self.L.dBatchStepTimed(self.h, n, C.byref(ms)):
+            raise RuntimeError("dBatchStepTimed failed")
+        return ms.value
+
+    def step_plain(self, n: int = 1):
+        if not self.L.dBatchStepPlain(self.h, n):
+            raise RuntimeError("dBatchStepPlain failed")
+
+    def set_controls(self, ctrl):
+        if ctrl is None:
+            self.L.dBatchSetControls(self.h, None)
+            return
+        a = np.ascontiguousarray(ctrl, np.float64).reshape(-1)
+        assert a.size == 4 * self.nj
+        if not self.L.dBatchSetControls(self.h, a.ctypes.data_as(C.POINTER(C.c_double))):
+            raise RuntimeError("dBatchSetControls failed")
+
+    def state(self) -> np.ndarray:
+        out = np.zeros((self.nb, 13), np.float32)
+        if not self.L.dBatchGetState(self.h, out.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError("dBatchGetState failed")
+        return out
+
+    def set_state(self, st):
+        a = np.ascontiguousarray(st, np.float32).reshape(self.nb, 13)
+        if not self.L.dBatchSetState(self.h, a.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError("dBatchSetState failed")
+
+    def counters(self) -> dict:
+        c = (C.c_longlong * 5)()
+        self.L.dBatchGetCounters(self.h, c)
+        return dict(pairs=c[0], contacts=c[1], rows=c[2], groups=c[3], overflow=c[4])
+
+    def track_residual(self, on: bool = True):
+        self.L.dBatchSetResidualTracking(self.h, 1 if on else 0)
+
+    def residual(self) -> np.ndarray:
+        out = np.zeros(self.nworlds, np.float32)
+        self.L.dBatchGetResidual(self.h, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def autotune(self, trial_steps: int = 3) -> int:
+        return self.L.dBatchAutotune(self.h, trial_steps)
+
+    def launch_config(self):
+        wg = C.c_int()
+        sc = C.c_float()
+        grid = self.L.dBatchGetLaunchConfig(self.h, C.byref(wg), C.byref(sc))
+        return dict(wg=wg.value, cell_scale=sc.value, grid=grid)
+
+    def set_launch_config(self, wg: int, cell_scale: float = 1.0):
+        if not self.L.dBatchSetLaunchConfig(self.h, wg, cell_scale):
+            raise RuntimeError("bad launch config")
+
+    def pairs(self, w: int, maxn: int = 1 << 20) -> np.ndarray:
+        out = np.zeros((maxn, 2), np.int32)
+        n = self.L.dBatchGetPairs(self.h, w, maxn, out.ctypes.data_as(C.POINTER(C.c_int)))
+        if n < 0:
+            raise IndexError(w)
+        return out[:min(n, maxn)].copy()
+
+    def contacts(self, w: int, maxn: int = 1 << 18) -> np.ndarray:
+        """Last step's contacts of world w: pos 3, normal 3, depth, g1, g2, lambda 3."""
+        out = np.zeros((maxn, 12), np.float32)
+        n = self.L.dBatchGetContacts(self.h, w, maxn, out.ctypes.data_as(C.POINTER(C.c_float)))
+        if n < 0:
+            raise IndexError(w)
+        return out[:min(n, maxn)].copy()
+
+    def world_state(self, st, w):
+        return st[self.body_offsets[w]:self.body_offsets[w + 1]]

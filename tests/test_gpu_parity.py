@@ -1,0 +1,311 @@
+"""Parity of the HIP path against the reference (golden fixtures written from
+the reference double build) and the CPU restatement, through the C ABI.
+
+Tolerances (FP32 device arithmetic against the FP64 reference):
+* pair sets, pair order and contact counts: exact;
+* contact position/normal/depth: |a-b| <= 2e-5 * max(1, |b|);
+* contact impulses of one step from the same state: |f-g| <= 2e-3 * max(1, |g|);
+* trajectories: within 4x the distance between the reference's own
+  single-precision build and its double build at the same step, plus 1e-3 m.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import ROOT, golden
+from odemi import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def odemi_mod():
+    import torch
+    assert torch.cuda.is_available()
+    import odemi
+    return odemi
+
+
+# ------------------------------------------------------------------ narrowphase
+
+class Api:
+    """ctypes view of the ODE API subset of libodemi.so."""
+
+    def __init__(self):
+        import odemi
+        L = odemi.lib()
+        V, F = C.c_void_p, C.c_float
+        for name, res, args in [
+            ("dCreateBox", V, [V, F, F, F]), ("dCreateSphere", V, [V, F]), ("dCreatePlane", V, [V, F, F, F, F]),
+            ("dGeomSetPosition", None, [V, F, F, F]), ("dGeomSetRotation", None, [V, C.POINTER(F)]),
+            ("dCollide", C.c_int, [V, V, C.c_int, V, C.c_int]), ("dGeomDestroy", None, [V])]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+
+    def geom(self, t, prm, p, R4):
+        L = self.L
+        if t == 0:
+            g = L.dCreateSphere(None, prm[0])
+        elif t == 1:
+            g = L.dCreateBox(None, prm[0], prm[1], prm[2])
+        else:
+            return L.dCreatePlane(None, *prm)
+        L.dGeomSetPosition(g, *p)
+        R = (C.c_float * 12)(*R4)
+        L.dGeomSetRotation(g, R)
+        return g
+
+    def collide(self, g1, g2, maxc):
+        buf = (C.c_byte * (64 * maxc))()
+        n = self.L.dCollide(g1, g2, maxc, buf, 64)
+        raw = np.frombuffer(bytes(buf), dtype=np.float32).reshape(maxc, 16)[:n]
+        return np.concatenate([raw[:, 0:3], raw[:, 4:7], raw[:, 8:9]], axis=1)
+
+
+def _rot(q):
+    w, x, y, z = q / np.linalg.norm(q)
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    R4 = np.zeros((3, 4))
+    R4[:, :3] = R
+    return R4.reshape(12)
+
+
+def test_narrowphase_pairs_vs_oracle(odemi_mod):
+    from test_oracle_pins import collide as orc_collide
+    api = Api()
+    rng = np.random.default_rng(11)
+    combos = [(0, 0), (0, 1), (1, 0), (0, 2), (2, 0), (1, 1), (1, 2), (2, 1)]
+    n_cmp = n_disc = 0
+    for t1, t2 in combos:
+        say("narrowphase case", t1, t2)
+        for trial in range(60):
+            def shape(t):
+                if t == 0:
+                    return [rng.uniform(0.1, 0.6), 0, 0, 0]
+                if t == 1:
+                    return [*rng.uniform(0.2, 1.2, 3), 0]
+                n = rng.normal(size=3)
+                n /= np.linalg.norm(n)
+                return [*n, rng.uniform(-0.3, 0.3)]
+            s1, s2 = shape(t1), shape(t2)
+            q1, q2 = rng.normal(size=4), rng.normal(size=4)
+            p1, p2 = rng.uniform(-0.4, 0.4, 3), rng.uniform(-0.4, 0.4, 3)
+            if t1 == 2:
+                p1 = np.zeros(3); q1 = np.array([1.0, 0, 0, 0])
+            if t2 == 2:
+                p2 = np.zeros(3); q2 = np.array([1.0, 0, 0, 0])
+            maxc = 4
+            g1 = api.geom(t1, s1, p1, _rot(q1))
+            g2 = api.geom(t2, s2, p2, _rot(q2))
+            got = api.collide(g1, g2, maxc)
+            api.L.dGeomDestroy(g1)
+            api.L.dGeomDestroy(g2)
+            ref32 = orc_collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc, prec="f32")
+            ref64 = orc_collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc, prec="f64")
+            if len(ref32) != len(ref64):
+                n_disc += 1      # a precision tie: the decision flips between f32 and f64
+                continue
+            assert len(got) == len(ref64), (t1, t2, trial, got, ref64)
+            if len(got):
+                err = np.abs(got - ref64) / np.maximum(1.0, np.abs(ref64))
+                assert err.max() <= 2e-5, (t1, t2, trial, err.max())
+                n_cmp += 1
+    say("narrowphase compared", n_cmp, "precision ties skipped", n_disc)
+    assert n_cmp > 150 and n_disc < 10
+
+
+# ------------------------------------------------------------------ broadphase + contacts
+
+
+def _batch(odemi_mod, scs, order=None):
+    return odemi_mod.Batch(scs, odemi_mod.ORDER_REFERENCE if order is None else order)
+
+
+@pytest.mark.parametrize("name", ["pile4096", "pile256_dense", "chains", "vehicle"])
+def test_first_step_pairs_and_contacts_match_reference(odemi_mod, name):
+    say("first-step parity", name)
+    from test_golden import CASES
+    sc = CASES[name][0]()
+    g = golden(name)
+    b = _batch(odemi_mod, [sc])
+    if sc.step_ctrl is not None:
+        b.set_controls(sc.step_ctrl[0])
+    b.step(1)
+    n0 = int(g["pair_step_count"][0])
+    k0 = int(g["contact_step_count"][0])
+    pairs = b.pairs(0)
+    assert np.array_equal(pairs, g["pairs"][:n0]), "pair order/set differs from the reference"
+    con = b.contacts(0)
+    assert len(con) == k0
+    ref = g["contacts"][:k0]
+    if k0:
+        assert np.array_equal(con[:, 7:9].astype(int), ref[:, 7:9].astype(int))
+        err = np.abs(con[:, :7] - ref[:, :7]) / np.maximum(1.0, np.abs(ref[:, :7]))
+        say(name, "pairs", n0, "contacts", k0, "max rel err", float(err.max()))
+        assert err.max() <= 2e-5
+    assert b.counters()["overflow"] == 0
+
+
+def _plane_space(n):
+    if abs(n[2]) > math.sqrt(0.5):
+        a = n[1] ** 2 + n[2] ** 2
+        k = 1 / math.sqrt(a)
+        p = np.array([0, -n[2] * k, n[1] * k])
+        q = np.array([a * k, -n[0] * p[2], n[0] * p[1]])
+    else:
+        a = n[0] ** 2 + n[1] ** 2
+        k = 1 / math.sqrt(a)
+        p = np.array([-n[1] * k, n[0] * k, 0])
+        q = np.array([-n[2] * p[1], n[2] * p[0], a * k])
+    return p, q
+
+
+@pytest.mark.parametrize("name", ["pile4096", "chains"])
+def test_first_step_contact_impulses_match_reference(odemi_mod, name):
+    say("impulse parity", name)
+    from test_golden import CASES
+    sc = CASES[name][0]()
+    g = golden(name)
+    b = _batch(odemi_mod, [sc])
+    b.step(1)
+    con = b.contacts(0)
+    k0 = int(g["contact_step_count"][0])
+    fb = g["feedback"][:k0]
+    if k0 == 0:
+        pytest.skip("no contacts in the first step")
+    # force on body 1 of each contact joint: J_l1^T lambda = n l0 + t1 l1 + t2 l2
+    f = np.zeros((k0, 3))
+    bodies = sc.geom_body
+    for i, c in enumerate(con):
+        n = c[3:6].astype(np.float64)
+        g1 = int(c[7])
+        if bodies[g1] < 0:
+            n = -n   # contact attached with the static geom first: reversed
+        t1, t2 = _plane_space(n)
+        f[i] = n * c[9] + t1 * c[10] + t2 * c[11]
+    scale = np.maximum(1.0, np.abs(fb[:, :3]).max())
+    err = np.abs(f - fb[:, :3]).max() / scale
+    say(name, "contacts", k0, "max impulse err (rel to max force)", float(err))
+    assert err <= 2e-3
+
+
+def _envelope(sc, n, every):
+    """Distance of the reference's own FP32 build from its FP64 build."""
+    a = po.run(po.oracle_lib("f64"), sc, n, every)["traj"]
+    b = po.run(po.oracle_lib("f32"), sc, n, every)["traj"]
+    return np.abs(a[:, :, :3] - b[:, :, :3]).max(axis=(1, 2))
+
+
+@pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle"])
+def test_trajectory_within_fp32_envelope(odemi_mod, name):
+    say("trajectory", name)
+    from test_golden import CASES
+    fac, n, every, _ = CASES[name]
+    sc = fac()
+    g = golden(name)
+    env = _envelope(sc, n, every)
+    b = _batch(odemi_mod, [sc])
+    worst = 0.0
+    for k in range(n // every):
+        for s in range(every):
+            if sc.step_ctrl is not None:
+                b.set_controls(sc.step_ctrl[k * every + s])
+            b.step(1)
+        st = b.state()
+        err = float(np.abs(st[:, :3] - g["traj"][k + 1][:, :3]).max())
+        tol = 4 * env[k + 1] + 1e-3
+        worst = max(worst, err / tol)
+        assert err <= tol, (name, k, err, env[k + 1])
+    say(name, "worst err/tol", worst)
+
+
+def test_dropin_driver_runs_the_reference_frame_loop(odemi_mod):
+    """The unchanged ODE-API application (oracle/scene_driver.cpp) linked
+    against libodemi.so: callback pair order, contacts and trajectories."""
+    say("drop-in driver")
+    lib = po.load(os.path.join(ROOT, "tests", "_build", "libscene_odemi.so"))
+    assert lib.scene_dreal_size() == 4
+    for name in ("box_stack", "vehicle"):
+        from test_golden import CASES
+        fac, n, every, diag = CASES[name]
+        sc = fac()
+        n = min(n, 60)
+        g = golden(name)
+        r = po.run(lib, sc, n, every, diag_steps=min(diag, 5), max_pairs=100000, max_contacts=100000,
+                   feedback=True)
+        np_ = int(g["pair_step_count"][:5].sum())
+        assert np.array_equal(r["pairs"], g["pairs"][:np_]), name
+        env = _envelope(sc, n, every)
+        err = np.abs(r["traj"][:, :, :3] - g["traj"][: n // every + 1, :, :3]).max(axis=(1, 2))
+        say(name, "drop-in traj err", err.max())
+        assert np.all(err <= 4 * env + 1e-3)
+
+
+def test_plain_path_equals_batched_path(odemi_mod):
+    say("plain vs batched")
+    ws = scenes.uneven_worlds(24, seed=3)
+    a = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    a.step(15)
+    b.step_plain(15)
+    assert np.array_equal(a.state(), b.state())
+    r1 = _batch(odemi_mod, ws[:6])
+    r2 = _batch(odemi_mod, ws[:6])
+    r1.step(10)
+    r2.step_plain(10)
+    assert np.array_equal(r1.state(), r2.state())
+
+
+def test_reference_order_independent_of_launch_shape(odemi_mod):
+    say("launch shapes")
+    ws = scenes.uneven_worlds(8, seed=5)
+    outs = []
+    for wg, cs in ((256, 1.0), (64, 2.0), (128, 1.5)):
+        b = _batch(odemi_mod, ws)
+        b.set_launch_config(wg, cs)
+        b.step(12)
+        outs.append(b.state())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_colored_solver_converges_with_iterations(odemi_mod):
+    say("coloured convergence")
+    res = {}
+    for iters in (20, 400):
+        sc = scenes.pile_world(np.random.default_rng(8), 48, 2, seed=2)
+        sc.iterations = iters
+        b = _batch(odemi_mod, [sc], odemi_mod.ORDER_COLORED)
+        b.step(60)
+        b.track_residual(True)
+        b.step(1)
+        res[iters] = float(b.residual().max())
+    say("residual", res)
+    assert res[400] < 0.5 * res[20]
+
+
+def test_autotune_keeps_state(odemi_mod):
+    say("autotune")
+    ws = scenes.uneven_worlds(16, seed=9)
+    a = _batch(odemi_mod, ws)
+    a.step(3)
+    st = a.state()
+    wg = a.autotune(1)
+    assert wg in (64, 128, 256)
+    assert np.array_equal(a.state(), st)
+    b = _batch(odemi_mod, ws)
+    b.step(3)
+    a.step(5)
+    b.step(5)
+    assert np.array_equal(a.state(), b.state())

@@ -1,0 +1,266 @@
+"""Pins the CPU restatement (oracle/) before it is trusted as the checker:
+known-answer tests of the reference, closed forms, invariants and brute force
+on small inputs, each chosen so that a plausible mistake fails it."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+DP = C.POINTER(C.c_double)
+
+
+def lib(p="f64"):
+    L = po.oracle_lib(p)
+    L.orc_rand_test.restype = C.c_int
+    L.orc_rand_ints.restype = C.c_ulong
+    L.orc_rand_ints.argtypes = [C.c_ulong, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.orc_collide.restype = C.c_int
+    L.orc_collide.argtypes = [C.c_int, DP, DP, DP, C.c_int, DP, DP, DP, C.c_int, DP]
+    L.orc_aabb.argtypes = [C.c_int, DP, DP, DP, DP]
+    L.orc_plane_space.argtypes = [DP, DP, DP]
+    return L
+
+
+def d(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return a, a.ctypes.data_as(DP)
+
+
+def rot(q):
+    w, x, y, z = q / np.linalg.norm(q)
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    R4 = np.zeros((3, 4))
+    R4[:, :3] = R
+    return R4.reshape(12), R
+
+
+def collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc=4, prec="f64"):
+    L = lib(prec)
+    R1, _ = rot(np.asarray(q1, float))
+    R2, _ = rot(np.asarray(q2, float))
+    bufs = [d(p1), d(R1), d(s1), d(p2), d(R2), d(s2)]
+    out = np.zeros((max(maxc, 1), 7))
+    n = L.orc_collide(t1, bufs[0][1], bufs[1][1], bufs[2][1], t2, bufs[3][1], bufs[4][1], bufs[5][1], maxc,
+                      out.ctypes.data_as(DP))
+    return out[:n]
+
+
+SPHERE, BOX, PLANE = 0, 1, 2
+ID = [1.0, 0, 0, 0]
+
+
+def test_rand_known_answer():
+    # dTestRand: seed 0 -> 0x3c6ef35f, 0x47502932, 0xd1ccf6e9, 0xaaf95334, 0x6252e503 (ode/src/misc.cpp:52-62)
+    assert lib().orc_rand_test() == 1
+
+
+def test_rand_int_matches_lcg_closed_form():
+    # dRandInt(n) = xor-folded LCG value mod n (misc.cpp:66-117), restated here independently
+    bounds = np.array([1, 2, 3, 4, 5, 16, 17, 200, 256, 257, 5000, 65536, 70000], np.int32)
+    out = np.zeros_like(bounds)
+    lib().orc_rand_ints(12345, len(bounds), bounds.ctypes.data_as(C.POINTER(C.c_int)),
+                        out.ctypes.data_as(C.POINTER(C.c_int)))
+    s = 12345
+    for n, got in zip(bounds.tolist(), out.tolist()):
+        s = (1664525 * s + 1013904223) & 0xFFFFFFFF
+        r = s
+        if n <= 16:
+            r ^= r >> 16; r ^= r >> 8; r ^= r >> 4
+            if n <= 2:
+                r ^= r >> 2; r ^= r >> 1
+            elif n <= 4:
+                r ^= r >> 2
+        elif n <= 256:
+            r ^= r >> 16; r ^= r >> 8
+        elif n <= 65536:
+            r ^= r >> 16
+        assert got == r % n
+
+
+def test_plane_space_orthonormal():
+    # testPlaneSpace invariant (ode/demo/demo_ode.cpp:219-235)
+    rng = np.random.default_rng(0)
+    L = lib()
+    for _ in range(200):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p = np.zeros(3); q = np.zeros(3)
+        L.orc_plane_space(d(n)[1], p.ctypes.data_as(DP), q.ctypes.data_as(DP))
+        M = np.stack([n, p, q])
+        assert np.allclose(M @ M.T, np.eye(3), atol=1e-12)
+        assert np.allclose(np.cross(n, p), q, atol=1e-12)
+
+
+def test_sphere_sphere_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        p1, p2 = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        r1, r2 = rng.uniform(0.1, 1, 2)
+        c = collide(SPHERE, p1, ID, [r1, 0, 0, 0], SPHERE, p2, ID, [r2, 0, 0, 0])
+        dist = np.linalg.norm(p1 - p2)
+        if dist > r1 + r2:
+            assert len(c) == 0
+            continue
+        n = (p1 - p2) / dist
+        assert np.allclose(c[0, 3:6], n) and math.isclose(c[0, 6], r1 + r2 - dist, abs_tol=1e-12)
+        assert np.allclose(c[0, :3], p1 + n * 0.5 * (r2 - r1 - dist))
+
+
+def test_sphere_plane_and_reversed_dispatch():
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        off = rng.uniform(-1, 1)
+        p = rng.uniform(-2, 2, 3); r = rng.uniform(0.1, 1)
+        c = collide(SPHERE, p, ID, [r, 0, 0, 0], PLANE, [0, 0, 0], ID, [*n, off])
+        depth = off - n @ p + r
+        if depth < 0:
+            assert len(c) == 0
+            continue
+        assert np.allclose(c[0, 3:6], n) and math.isclose(c[0, 6], depth, abs_tol=1e-12)
+        assert np.allclose(c[0, :3], p - r * n)
+        # plane first: the normal is negated (collision_kernel.cpp:319-333)
+        c2 = collide(PLANE, [0, 0, 0], ID, [*n, off], SPHERE, p, ID, [r, 0, 0, 0])
+        assert np.allclose(c2[0, 3:6], -n) and np.allclose(c2[0, :3], c[0, :3])
+
+
+def test_box_plane_against_vertex_enumeration():
+    # the deepest contact is the deepest box vertex; every contact is a box
+    # vertex whose depth is its distance below the plane
+    rng = np.random.default_rng(3)
+    n_checked = 0
+    for _ in range(300):
+        q = rng.normal(size=4)
+        side = rng.uniform(0.2, 2, 3)
+        p = np.array([0, 0, rng.uniform(-0.5, 1.0)])
+        c = collide(BOX, p, q, [*side, 0], PLANE, [0, 0, 0], ID, [0, 0, 1, 0], maxc=4)
+        _, R = rot(q)
+        verts = np.array([p + R @ (np.array(s) * side / 2) for s in np.ndindex(2, 2, 2)]) if False else \
+            np.array([p + R @ (np.array([sx, sy, sz]) * side / 2) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+        depths = -verts[:, 2]
+        if depths.max() < 0:
+            assert len(c) == 0
+            continue
+        n_checked += 1
+        assert math.isclose(c[0, 6], depths.max(), abs_tol=1e-9)
+        for row in c:
+            k = np.argmin(np.linalg.norm(verts - row[:3], axis=1))
+            assert np.linalg.norm(verts[k] - row[:3]) < 1e-9
+            assert math.isclose(row[6], depths[k], abs_tol=1e-9) and row[6] >= -1e-12
+            assert np.allclose(row[3:6], [0, 0, 1])
+    assert n_checked > 50
+
+
+def test_sphere_box_closed_form():
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        q = rng.normal(size=4)
+        side = rng.uniform(0.3, 2, 3)
+        _, R = rot(q)
+        ps = rng.uniform(-1.5, 1.5, 3)
+        r = rng.uniform(0.05, 0.6)
+        c = collide(SPHERE, ps, ID, [r, 0, 0, 0], BOX, [0, 0, 0], q, [*side, 0])
+        local = R.T @ ps
+        clamped = np.clip(local, -side / 2, side / 2)
+        if np.all(np.abs(local) < side / 2):
+            face = np.argmin(side / 2 - np.abs(local))
+            assert len(c) == 1 and math.isclose(c[0, 6], side[face] / 2 - abs(local[face]) + r, abs_tol=1e-9)
+            assert np.allclose(c[0, 3:6], np.sign(local[face]) * R[:, face])
+            continue
+        dist = np.linalg.norm(local - clamped)
+        if dist > r:
+            assert len(c) == 0
+            continue
+        assert math.isclose(c[0, 6], r - dist, abs_tol=1e-9)
+        assert np.allclose(c[0, :3], R @ clamped, atol=1e-9)
+
+
+def _sat_depth(p1, R1, s1, p2, R2, s2):
+    """Brute-force minimum penetration over the 15 separating axes."""
+    axes = [R1[:, i] for i in range(3)] + [R2[:, i] for i in range(3)]
+    axes += [np.cross(R1[:, i], R2[:, j]) for i in range(3) for j in range(3)]
+    best = np.inf
+    for a in axes:
+        l = np.linalg.norm(a)
+        if l < 1e-9:
+            continue
+        a = a / l
+        r1 = sum(abs(a @ R1[:, i]) * s1[i] / 2 for i in range(3))
+        r2 = sum(abs(a @ R2[:, i]) * s2[i] / 2 for i in range(3))
+        pen = r1 + r2 - abs(a @ (p2 - p1))
+        if pen < 0:
+            return -1.0
+        best = min(best, pen)
+    return best
+
+
+def test_box_box_against_brute_force_sat():
+    rng = np.random.default_rng(5)
+    hits = 0
+    for _ in range(400):
+        q1, q2 = rng.normal(size=4), rng.normal(size=4)
+        s1, s2 = rng.uniform(0.3, 1.5, 3), rng.uniform(0.3, 1.5, 3)
+        p1 = np.zeros(3); p2 = rng.uniform(-1.2, 1.2, 3)
+        c = collide(BOX, p1, q1, [*s1, 0], BOX, p2, q2, [*s2, 0], maxc=8)
+        _, R1 = rot(q1)
+        _, R2 = rot(q2)
+        pen = _sat_depth(p1, R1, s1, p2, R2, s2)
+        if pen < 0:
+            assert len(c) == 0
+            continue
+        if len(c) == 0:
+            continue  # the face clipping may cull every point for grazing contacts
+        hits += 1
+        assert 1 <= len(c) <= 8
+        assert np.allclose(np.linalg.norm(c[:, 3:6], axis=1), 1)
+        assert np.all(c[:, 6] >= -1e-9)
+        # the reported maximum depth is at least the true minimum penetration
+        # along the chosen axis (edge axes are preferred only within 1.05x)
+        assert c[:, 6].max() <= pen * 1.06 + 1e-9 or len(c) > 1
+        # the normal points into box 1: moving box 1 along it separates
+        n = c[0, 3:6]
+        assert n @ (p1 - p2) >= -1e-9 or len(c) > 1
+    assert hits > 100
+
+
+def test_aabb_closed_form():
+    rng = np.random.default_rng(6)
+    L = lib()
+    for _ in range(100):
+        q = rng.normal(size=4)
+        side = rng.uniform(0.1, 2, 3)
+        p = rng.uniform(-5, 5, 3)
+        R4, R = rot(q)
+        out = np.zeros(6)
+        L.orc_aabb(BOX, d(p)[1], d(R4)[1], d([*side, 0])[1], out.ctypes.data_as(DP))
+        verts = np.array([p + R @ (np.array([sx, sy, sz]) * side / 2) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+        assert np.allclose(out[0::2], verts.min(0)) and np.allclose(out[1::2], verts.max(0))
+
+
+def test_tiny_lcp_contact_against_closed_form():
+    # one unit box resting on the plane: after a step the normal impulses
+    # carry its weight, the velocity normal to the plane stays ~0
+    from odemi import scenes
+    sc = scenes.box_stack(1, gap=-1e-4)
+    r = po.run(lib(), sc, 1, 1, diag_steps=1, max_pairs=16, max_contacts=16, feedback=True)
+    fz = r["feedback"][:, 2].sum()
+    assert math.isclose(fz, 9.81 * 1.0, rel_tol=0.05), fz
+    assert abs(r["traj"][1][0, 9]) < 0.05
+
+
+def test_free_fall_closed_form():
+    # no contacts: v = g t, x = x0 + sum of h v (semi-implicit Euler)
+    from odemi import scenes
+    sc = scenes.box_stack(1)
+    sc.body_pos[0, 2] = 50.0
+    h, n = sc.dt, 25
+    r = po.run(lib(), sc, n, n)
+    vz = -9.81 * h * n
+    z = 50.0 + sum(-9.81 * h * (k + 1) * h for k in range(n))
+    assert math.isclose(r["traj"][1][0, 9], vz, rel_tol=1e-12)
+    assert math.isclose(r["traj"][1][0, 2], z, rel_tol=1e-12)
