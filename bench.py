@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=256, help="worlds in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch config 'wg,cell_scale,lds,split,grid' instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -80,7 +82,11 @@ def main():
           file=sys.stderr, flush=True)
 
     b = odemi.Batch(worlds, odemi.ORDER_COLORED)
-    wg = b.autotune(2) if args.autotune else 256
+    if args.launch:
+        wg, cs, lds, split, grid = args.launch.split(",")
+        b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
+    elif args.autotune:
+        b.autotune(2)
     cfg = b.launch_config()
     print(f"# launch config {cfg}", file=sys.stderr, flush=True)
     if args.warmup:
@@ -96,6 +102,11 @@ def main():
     b.step(1)
     res = b.residual()
     b.track_residual(False)
+    # per-phase workgroup time (in-kernel clocks) over a few more steps
+    b.profile(True)
+    b.step(5)
+    ph = {k: v / 5 for k, v in b.phase_times().items()}
+    b.profile(False)
     ms_step = wall / args.steps * 1e3
     value = nb * args.steps / wall
     kernel_ms = dev_ms / args.steps
@@ -108,7 +119,7 @@ def main():
     plain = None
     if args.plain_steps > 0:
         bp = odemi.Batch(worlds, odemi.ORDER_COLORED)
-        bp.set_launch_config(cfg["wg"], cfg["cell_scale"])
+        bp.set_launch_config(cfg["wg"], cfg["cell_scale"], cfg["lds"], cfg["split"])
         bp.step_plain(1)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
@@ -159,6 +170,7 @@ def main():
         "batched_over_plain": (value / plain) if plain else None,
         "lcp_residual": {"max": float(np.max(res)), "median": float(np.median(res))},
         "counters_per_step": cnt,
+        "phase_wg_ms_per_step": ph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": B, "kernel_ms": kernel_ms},

@@ -346,6 +346,9 @@ ODE_API int dBatchGetState(dBatchID b, float *out);
 ODE_API int dBatchSetState(dBatchID b, const float *in);
 /* Totals of the last step: [pairs, contacts, rows, islands_or_colours, overflow]. */
 ODE_API int dBatchGetCounters(dBatchID b, long long *out5);
+/* The same five counters for every world (5 ints per world; overflow bits:
+ * 1 grid cells, 2 pairs, 4 contacts, 8 rows). */
+ODE_API int dBatchGetWorldCounters(dBatchID b, int *out);
 /* Enables the LCP complementarity residual of the last step per world
  * (max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|). */
 ODE_API int dBatchSetResidualTracking(dBatchID b, int on);
@@ -355,6 +358,25 @@ ODE_API int dBatchGetResidual(dBatchID b, float *out_per_world);
 ODE_API int dBatchAutotune(dBatchID b, int trial_steps);
 ODE_API int dBatchGetLaunchConfig(dBatchID b, int *wg_size, float *cell_scale);
 ODE_API int dBatchSetLaunchConfig(dBatchID b, int wg_size, float cell_scale);
+/* LDS pool per workgroup in bytes (sort keys, colouring state and the
+ * coloured solver's working set live there when they fit; larger worlds
+ * use their device scratch).  Set returns 0 for a size above the device's
+ * per-workgroup limit. */
+ODE_API int dBatchGetLds(dBatchID b);
+/* 1: a step runs as two kernels (collision, then quickstep), each with its
+ * own register budget; 0: one fused kernel.  The autotuner picks one. */
+ODE_API int dBatchGetSplit(dBatchID b);
+/* Persistent grid size (workgroups pulling worlds from a queue); 0 launches
+ * one workgroup per world.  dBatchGetLaunchConfig returns the current one. */
+ODE_API int dBatchSetPersistentGrid(dBatchID b, int grid);
+ODE_API int dBatchSetSplit(dBatchID b, int on);
+ODE_API int dBatchSetLds(dBatchID b, int bytes);
+/* In-kernel phase clocks: per phase (AABB, broadphase, narrowphase, body
+ * setup + getInfo1, getInfo2 + rhs, ordering/colouring, SOR, integration)
+ * the workgroup time summed over worlds since the last read, in ms (8
+ * doubles).  Reading resets the clocks. */
+ODE_API int dBatchSetProfiling(dBatchID b, int on);
+ODE_API int dBatchGetPhaseTimes(dBatchID b, double *ms8);
 /* Device pointer of the body positions (one float4 per body, w unused) and
  * the HIP stream the batch runs on, for zero-copy use from other GPU code. */
 /* Diagnostics of the last step of one world.  dBatchGetPairs writes the

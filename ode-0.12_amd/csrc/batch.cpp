@@ -50,6 +50,32 @@ void refresh_dev(BatchImpl *b) {
   D.scratch = b->d.scratch.as<char>();
   D.tape = b->tape.recording ? b->tape.buf.as<char>() : nullptr;
   D.tape_off = nullptr;
+  D.prof = b->prof_on ? b->d.prof.as<unsigned long long>() : nullptr;
+  D.lds_bytes = b->lds;
+}
+
+// LDS limits of the current device and the default pool: half a CU's LDS
+// less the kernel's static block, so that two workgroups share a CU.
+static void init_lds(BatchImpl *b, int dev) {
+  int per_cu = 0, per_block = 0, khz = 0;
+  if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 65536;
+  if (hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+      per_block <= 0)
+    per_block = 65536;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0) b->wall_khz = khz;
+  int st = world_kernel_static_lds();
+  b->lds_cu = per_cu;
+  b->lds_max = ((per_block < per_cu ? per_block : per_cu) - st) & ~255;
+  if (b->lds_max < 0) b->lds_max = 0;
+  if (b->lds_max > 0 && allow_world_kernel_lds(b->lds_max) != hipSuccess) {
+    (void)hipGetLastError();
+    b->lds_max = ((per_block < 65536 ? per_block : 65536) - st) & ~255;
+  }
+  int half = (per_cu / 2 - st - 256) & ~255;
+  b->lds = half < b->lds_max ? half : b->lds_max;
+  if (b->lds < 0) b->lds = 0;
 }
 
 template <class T>
@@ -247,6 +273,7 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) b->cu_count = prop.multiProcessorCount;
+  init_lds(b, dev);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     report_error("hipStreamCreate failed");
     delete b;
@@ -281,8 +308,9 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
             upload(d.adj_other, A.adj_other) && upload(d.bgeom_start, A.bgeom_start) && upload(d.bgeom, A.bgeom) &&
             upload(d.world_order, order) && d.R.alloc(sizeof(float4) * 3 * (size_t)(nb ? nb : 1)) &&
             d.counters.alloc(sizeof(int) * CNT_N * (size_t)b->nworlds) &&
-            d.residual.alloc(sizeof(float) * (size_t)b->nworlds) && d.work_counter.alloc(64) &&
-            d.scratch.alloc((size_t)scratch) && d.ctrl.alloc(sizeof(float4) * (size_t)(nj ? nj : 1));
+            d.residual.alloc(sizeof(float) * (size_t)b->nworlds) && d.work_counter.alloc(64 * WORK_QUEUES) &&
+            d.scratch.alloc((size_t)scratch) && d.ctrl.alloc(sizeof(float4) * (size_t)(nj ? nj : 1)) &&
+            d.prof.alloc(sizeof(unsigned long long) * PH_N * (size_t)b->nworlds);
   if (!ok) {
     report_error("batch setup failed (%lld scratch bytes)", scratch);
     batch_destroy(b);
@@ -290,6 +318,7 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   }
   (void)hipMemset(d.counters.p, 0, d.counters.n);
   (void)hipMemset(d.residual.p, 0, d.residual.n);
+  (void)hipMemset(d.prof.p, 0, d.prof.n);
   refresh_dev(b);
   if (launch_init_rot(b->dev, nb, b->stream) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
     report_error("batch setup: rotation init failed");
@@ -318,11 +347,19 @@ static int persistent_grid(BatchImpl *b) {
   return 0;
 }
 
+// one step of every world: one fused kernel, or collide and step kernels
+static hipError_t launch_step(BatchImpl *b, int first_world, int nworlds, int grid) {
+  if (!b->split) return launch_world_kernel(KM_FUSED, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
+  hipError_t e = launch_world_kernel(KM_BATCH_COLLIDE, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
+  if (e != hipSuccess) return e;
+  return launch_world_kernel(KM_BATCH_STEP, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 1);
+}
+
 static int build_graph(BatchImpl *b) {
   destroy_graph(b);
   refresh_dev(b);
   OD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_world_kernel(3, b->dev, 0, b->nworlds, persistent_grid(b), b->wg, 0, b->stream);
+  hipError_t e = launch_step(b, 0, b->nworlds, persistent_grid(b));
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess || !g) {
@@ -371,7 +408,7 @@ int batch_step_plain(BatchImpl *b, int nsteps, bool sync) {
   refresh_dev(b);
   for (int s = 0; s < nsteps; s++)
     for (int w = 0; w < b->nworlds; w++)
-      OD_HIP(launch_world_kernel(3, b->dev, w, 1, 0, b->wg, 0, b->stream));
+      OD_HIP(launch_step(b, w, 1, 0));
   if (sync) OD_HIP(hipStreamSynchronize(b->stream));
   return 1;
 }
@@ -379,7 +416,7 @@ int batch_step_plain(BatchImpl *b, int nsteps, bool sync) {
 // one fused step launched directly (used by the tape recorder and autotuner)
 int batch_launch_once(BatchImpl *b) {
   refresh_dev(b);
-  OD_HIP(launch_world_kernel(3, b->dev, 0, b->nworlds, persistent_grid(b), b->wg, 0, b->stream));
+  OD_HIP(launch_step(b, 0, b->nworlds, persistent_grid(b)));
   return 1;
 }
 
@@ -547,30 +584,84 @@ int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid) {
   return 1;
 }
 
+int batch_set_lds(BatchImpl *b, int bytes) {
+  if (bytes < 0 || bytes > b->lds_max) return 0;
+  b->lds = bytes & ~255;
+  destroy_graph(b);
+  return 1;
+}
+
+int batch_set_profiling(BatchImpl *b, bool on) {
+  OD_HIP(hipStreamSynchronize(b->stream));
+  OD_HIP(hipMemset(b->d.prof.p, 0, b->d.prof.n));
+  if (b->prof_on != on) {
+    b->prof_on = on;
+    destroy_graph(b);
+  }
+  return 1;
+}
+
+// per-phase workgroup time summed over worlds since the last call (ms)
+int batch_phase_times(BatchImpl *b, double *ms) {
+  OD_HIP(hipStreamSynchronize(b->stream));
+  std::vector<unsigned long long> t((size_t)PH_N * b->nworlds);
+  OD_HIP(hipMemcpy(t.data(), b->d.prof.p, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+  OD_HIP(hipMemset(b->d.prof.p, 0, b->d.prof.n));
+  for (int k = 0; k < PH_N; k++) ms[k] = 0;
+  for (int w = 0; w < b->nworlds; w++)
+    for (int k = 0; k < PH_N; k++) ms[k] += (double)t[(size_t)PH_N * w + k];
+  for (int k = 0; k < PH_N; k++) ms[k] /= (double)b->wall_khz;
+  return 1;
+}
+
+// Launch-shape search on the live batch (state saved and restored around
+// the trials): broadphase cell scale first, then workgroup size x LDS pool x
+// grid (one workgroup per world, or a persistent grid filling the CUs at the
+// kernel's occupancy for that shape).
 int batch_autotune(BatchImpl *b, int trial_steps) {
   if (trial_steps <= 0) trial_steps = 3;
   Snapshot snap;
   if (!snapshot(b, snap, true)) return 0;
-  const int wgs[] = {64, 128, 256};
-  const float scales[] = {1.0f, 1.5f, 2.0f};
-  const int grids[] = {0, 4, 8};   // 0: one workgroup per world; k: k workgroups per CU
+  auto trial = [&](double &t) -> int {
+    if (!batch_step(b, 1, true)) return 0;   // warm-up and graph build
+    auto t0 = std::chrono::steady_clock::now();
+    if (!batch_step(b, trial_steps, true)) return 0;
+    t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return snapshot(b, snap, false);
+  };
   double best = 1e30;
-  int bw = b->wg, bg = b->grid;
+  int bw = b->wg, bg = b->grid, bl = b->lds;
   float bs = b->cell_scale;
-  for (int wg : wgs)
-    for (float sc : scales)
-      for (int gk : grids) {
-        int grid = gk ? gk * b->cu_count : 0;
-        if (!batch_set_launch(b, wg, sc, grid)) continue;
-        if (!batch_step(b, 1, true)) continue;   // warm-up and graph build
-        auto t0 = std::chrono::steady_clock::now();
-        if (!batch_step(b, trial_steps, true)) continue;
-        double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (t < best) { best = t; bw = wg; bs = sc; bg = grid; }
-        if (!snapshot(b, snap, false)) return 0;
+  const float scales[] = {1.0f, 1.5f, 2.0f};
+  for (float sc : scales) {
+    double t;
+    if (!batch_set_launch(b, bw, sc, 0) || !trial(t)) continue;
+    if (t < best) { best = t; bs = sc; bg = 0; }
+  }
+  const int wgs[] = {64, 128, 256};
+  const int lds_div[] = {2, 3, 4, 8};   // pool = LDS per CU / k
+  int st = world_kernel_static_lds();
+  bool bsplit = b->split;
+  for (int split = 0; split < 2; split++)
+    for (int wg : wgs)
+      for (int k : lds_div) {
+        int lds = (b->lds_cu / k - st - 256) & ~255;
+        if (lds < 0) lds = 0;
+        if (lds > b->lds_max) lds = b->lds_max;
+        int occ = world_kernel_occupancy(split ? KM_BATCH_STEP : KM_FUSED, wg, lds);
+        if (occ <= 0) continue;
+        const int grids[] = {0, occ * b->cu_count};
+        for (int grid : grids) {
+          double t;
+          b->split = split != 0;
+          if (!batch_set_launch(b, wg, bs, grid) || !batch_set_lds(b, lds) || !trial(t)) continue;
+          if (t < best) { best = t; bw = wg; bg = grid; bl = lds; bsplit = b->split; }
+        }
       }
   if (!snapshot(b, snap, false)) return 0;
+  b->split = bsplit;
   batch_set_launch(b, bw, bs, bg);
+  batch_set_lds(b, bl);
   return bw;
 }
 
@@ -639,6 +730,31 @@ int dBatchSetLaunchConfig(dBatchID b, int wg, float cell_scale) {
 void *dBatchDeviceState(dBatchID b) { return b ? b->impl->d.pos.p : nullptr; }
 void *dBatchStream(dBatchID b) { return b ? (void *)b->impl->stream : nullptr; }
 int dBatchNumBodies(dBatchID b) { return b ? b->impl->nb : 0; }
+
+int dBatchGetLds(dBatchID b) { return b ? b->impl->lds : -1; }
+int dBatchGetWorldCounters(dBatchID b, int *out) {
+  if (!b || !out) return 0;
+  BatchImpl *i = b->impl;
+  if (hipStreamSynchronize(i->stream) != hipSuccess) return 0;
+  return hipMemcpy(out, i->d.counters.p, sizeof(int) * CNT_N * (size_t)i->nworlds, hipMemcpyDeviceToHost) ==
+         hipSuccess;
+}
+int dBatchSetPersistentGrid(dBatchID b, int grid) {
+  if (!b || grid < 0) return 0;
+  b->impl->grid = grid;
+  destroy_graph(b->impl);
+  return 1;
+}
+int dBatchGetSplit(dBatchID b) { return b ? (b->impl->split ? 1 : 0) : -1; }
+int dBatchSetSplit(dBatchID b, int on) {
+  if (!b) return 0;
+  b->impl->split = on != 0;
+  destroy_graph(b->impl);
+  return 1;
+}
+int dBatchSetLds(dBatchID b, int bytes) { return b ? batch_set_lds(b->impl, bytes) : 0; }
+int dBatchSetProfiling(dBatchID b, int on) { return b ? batch_set_profiling(b->impl, on != 0) : 0; }
+int dBatchGetPhaseTimes(dBatchID b, double *ms) { return (b && ms) ? batch_phase_times(b->impl, ms) : 0; }
 
 int dBatchGetPairs(dBatchID b, int world, int maxn, int *out) {
   return b ? batch_get_pairs(b->impl, world, maxn, out) : -1;

@@ -269,7 +269,7 @@ __device__ inline void cull_points(int n, const float *p, int m, int i0, int *ir
 // box-box: separating axis test over the 15 axes with the 1.05 preference for
 // face axes, then one edge-edge contact or face clipping with culling to
 // maxc points.  box.cpp:331-712, normal negated as in 716-742.
-__device__ inline int coll_box_box(float3 p1, const Mat3 &R1, float3 side1, float3 p2, const Mat3 &R2, float3 side2,
+__device__ __noinline__ int coll_box_box(float3 p1, const Mat3 &R1, float3 side1, float3 p2, const Mat3 &R2, float3 side2,
                                    int maxc, CPoint *out) {
   const float fudge = 1.05f;
   float3 u0 = R1.col(0), u1 = R1.col(1), u2 = R1.col(2);

@@ -251,7 +251,7 @@ inline void h_joint_defaults(JointDev &j, int type, float erp, float cfm) {
 
 // capacities of a world's scratch buffers
 inline void world_capacities(int nb, int ng, int nj, double scale, int &pcap, int &kcap, int &rcap) {
-  long long p = (long long)(scale * (8.0 * ng + 64));
+  long long p = (long long)(scale * (16.0 * ng + 64));
   pcap = (int)od_pow2(p);
   kcap = (int)(scale * (6.0 * ng + 64));
   rcap = 3 * kcap + 6 * nj + 8;
@@ -260,8 +260,19 @@ inline void world_capacities(int nb, int ng, int nj, double scale, int &pcap, in
 // kernel launch entry points (step_kernels.hip)
 struct GeomView;
 struct CPoint;
+// world kernel modes (step_kernels.hip)
+enum {
+  KM_BATCH_COLLIDE = 1,   // AABB + broadphase + narrowphase of every world
+  KM_BATCH_STEP = 2,      // quickstep on the contacts of KM_BATCH_COLLIDE
+  KM_FUSED = 3,           // both in one kernel
+  KM_API_COLLIDE = 5,     // one world, pairs and contacts copied out (dSpaceCollide)
+  KM_API_STEP = 10        // one world, contact joints from the host (dWorldQuickStep)
+};
 hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid, int wg,
-                               int api_maxc, hipStream_t st);
+                               int api_maxc, hipStream_t st, int queue = 0);
 hipError_t launch_init_rot(const DevBatch &B, int nbodies, hipStream_t st);
+int world_kernel_static_lds();
+hipError_t allow_world_kernel_lds(int bytes);
+int world_kernel_occupancy(int mode, int wg, int lds_bytes);
 
 }  // namespace odemi

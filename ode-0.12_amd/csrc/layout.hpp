@@ -125,9 +125,40 @@ struct Cons {
   int fb;                  // feedback slot or -1
 };
 
+// Compact SOR row (64 bytes) used by the coloured solver.  J_l2 = -J_l1
+// for every row the supported joints produce, and inv(M) J^T is recomputed
+// from the per-body inverse mass and world inverse inertia, so a row needs
+// only J_l1, J_a1, J_a2 and its scalars.
+struct alignas(16) SRow {
+  float l[3], a1[3], a2[3];
+  float rhs, cfm, ad, lo, hi;
+  float lambda;
+  int findex;   // row (within its constraint) of the normal impulse, or -1
+};
+
+// per-body block of the coloured solver: fc (6), symmetric world inverse
+// inertia (xx xy xz yy yz zz), inverse mass, pad -- 64 bytes
+struct alignas(16) SBody {
+  float fc[6];
+  float iI[6];
+  float im;
+  float pad[3];
+};
+
+// per-constraint record of the coloured solver, constraints sorted by colour
+// and their rows stored contiguously in that order
+struct alignas(16) SCons {
+  int row0, m, b1, b2;
+};
+
+constexpr int WORK_QUEUES = 2;   // persistent-queue counters per batch, 16 ints apart
+
+// phases timed by the optional in-kernel profiler (wall-clock ticks)
+enum { PH_AABB = 0, PH_BROAD, PH_NARROW, PH_BODIES, PH_ROWS, PH_ORDER, PH_SOR, PH_INTEGRATE, PH_N };
+
 struct ScratchLayout {
   long long aabb, gbig, gidx, glevel, gdb, cells, ecount, pairs, pkeys, pcount, contacts, cons, rows, btmp,
-      order, adjs, adj, island, color, misc, total;
+      order, adjs, adj, island, color, misc, sor, total;
 };
 
 OD_HD inline long long od_align(long long x) { return (x + 255) & ~255LL; }
@@ -135,6 +166,13 @@ OD_HD inline long long od_pow2(long long n) {
   long long p = 1;
   while (p < n) p <<= 1;
   return p;
+}
+
+// bytes of the coloured solver's working set: bodies, rows, constraints,
+// colour order and colour starts
+OD_HD inline long long sor_bytes(int nb, int nrows, int ncons) {
+  return (long long)sizeof(SBody) * nb + (long long)sizeof(SRow) * nrows + (long long)sizeof(SCons) * ncons +
+         4LL * (2LL * ncons + 8);
 }
 
 // Per-world scratch layout.  nb/ng/nj: world sizes; pcap (a power of two),
@@ -165,6 +203,7 @@ OD_HD inline ScratchLayout scratch_layout(int nb, int ng, int nj, int pcap, int 
   L.island = o;   o = od_align(o + 4LL * (6LL * nb + 2 * ncons + 3LL * ng + 16));
   L.color = o;    o = od_align(o + 4LL * (ncons + nb));
   L.misc = o;     o = od_align(o + 4LL * 64);
+  L.sor = o;      o = od_align(o + sor_bytes(nb, rcap, (int)ncons));
   L.total = o;
   return L;
 }
@@ -205,6 +244,8 @@ struct DevBatch {
   int *counters;           // CNT_N per world
   float *residual;         // per world
   int *work_counter;       // persistent-kernel world queue
+  unsigned long long *prof; // optional: PH_N wall-clock tick counters per world
+  int lds_bytes;           // dynamic LDS pool per workgroup
   const int *world_order;  // processing order of worlds (largest first)
   char *scratch;
   // tape for the adjoint (null when not recording)

@@ -453,7 +453,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   D.cin_count = w->cin_count.as<int>();
   D.feedback = fbs.empty() ? nullptr : w->feedback.as<float>();
   D.ctrl = nullptr;
-  OD_HIP(launch_world_kernel(2, D, 0, 1, 0, m->wg, 0, m->stream));
+  OD_HIP(launch_world_kernel(KM_API_STEP, D, 0, 1, 0, m->wg, 0, m->stream));
   OD_HIP(hipStreamSynchronize(m->stream));
   long long cnt[5];
   batch_counters(m, cnt);
@@ -982,7 +982,7 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
   D.pcount_out = w->pcount_out.as<int>();
   D.contacts_out = w->contacts_out.as<ContactRec>();
   D.ctrl = nullptr;
-  if (launch_world_kernel(1, D, 0, 1, 0, m->wg, s->cached_maxc, m->stream) != hipSuccess ||
+  if (launch_world_kernel(KM_API_COLLIDE, D, 0, 1, 0, m->wg, s->cached_maxc, m->stream) != hipSuccess ||
       hipStreamSynchronize(m->stream) != hipSuccess) {
     report_error("dSpaceCollide: collide kernel failed");
     return;

@@ -24,7 +24,7 @@ struct BatchBufs {
   DevBuf wi, wp, pos, q, lvel, avel, facc, tacc, R, I, invI, mass, invMass;
   DevBuf gtype, gbody, gparam, gpos, gR, grank, gcat, gcol;
   DevBuf joints, adj_start, adj_other, bgeom_start, bgeom;
-  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback;
+  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback, prof;
 };
 
 struct BatchImpl {
@@ -43,6 +43,12 @@ struct BatchImpl {
   float cell_scale = 1.0f;
   bool ctrl_set = false;
   int cu_count = 256;
+  int lds = 0;            // dynamic LDS pool per workgroup (bytes)
+  int lds_cu = 65536;     // LDS per CU
+  int lds_max = 65536;    // largest pool one workgroup may request
+  bool prof_on = false;   // per-phase wall-clock counters
+  bool split = false;     // collide and quickstep as two kernels per step
+  int wall_khz = 100000;  // wall clock rate
   Tape tape;              // forward tape for the adjoint
 };
 
@@ -70,6 +76,9 @@ int batch_set_state(BatchImpl *b, const float *in);
 int batch_counters(BatchImpl *b, long long *out5);
 int batch_set_controls(BatchImpl *b, const double *ctrl);
 int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid);
+int batch_set_lds(BatchImpl *b, int bytes);
+int batch_set_profiling(BatchImpl *b, bool on);
+int batch_phase_times(BatchImpl *b, double *ms);
 int batch_autotune(BatchImpl *b, int trial_steps);
 
 int tape_begin(BatchImpl *b, int max_steps);

@@ -29,14 +29,36 @@
 namespace odemi {
 
 constexpr int WG = 256;
-constexpr int SORT_LDS = 4096;   // u64 keys sorted in LDS
+constexpr int MAX_COLORS_LDS = 256;   // colour buckets counted in LDS
 
 struct Shared {
   int wsum[WG / 64];
   float wmax[WG / 64];
   int ival[16];
   float fval[8];
-  unsigned long long keys[SORT_LDS];
+  int ccount[MAX_COLORS_LDS + 1];
+  int ccur[MAX_COLORS_LDS];
+};
+
+// dynamic LDS pool of the workgroup (size DevBatch::lds_bytes): sort keys,
+// colouring state and the coloured solver's working set, one after another
+extern __shared__ __align__(16) unsigned char od_pool[];
+
+// optional per-phase wall-clock ticks (thread 0)
+struct PhaseClock {
+  unsigned long long *dst;
+  unsigned long long t;
+  __device__ void start(const DevBatch &B, int w) {
+    dst = B.prof ? B.prof + (size_t)w * PH_N : nullptr;
+    if (dst && threadIdx.x == 0) t = wall_clock64();
+  }
+  __device__ void lap(int ph) {
+    if (dst && threadIdx.x == 0) {
+      unsigned long long n = wall_clock64();
+      dst[ph] += n - t;
+      t = n;
+    }
+  }
 };
 
 // ------------------------------------------------------------------ block helpers
@@ -80,11 +102,12 @@ __device__ inline int block_sum(int v, Shared &sh) {
 }
 
 // ascending bitonic sort of n (power of two) u64 keys; in LDS when it fits
-__device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh) {
-  bool lds = n <= SORT_LDS;
-  unsigned long long *k = lds ? sh.keys : a;
+__device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh, int lds_bytes) {
+  bool lds = (long long)n * 8 <= lds_bytes;
+  unsigned long long *pool = (unsigned long long *)od_pool;
+  unsigned long long *k = lds ? pool : a;
   if (lds) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sh.keys[i] = a[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) pool[i] = a[i];
     __syncthreads();
   }
   for (int size = 2; size <= n; size <<= 1) {
@@ -101,7 +124,7 @@ __device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh) {
     }
   }
   if (lds) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = sh.keys[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = pool[i];
     __syncthreads();
   }
 }
@@ -119,6 +142,7 @@ struct Ctx {
   WorldInfo W;
   ScratchLayout L;
   char *S;
+  mutable PhaseClock clk;
   template <class T> __device__ T *at(long long off) const { return (T *)(S + off); }
 };
 
@@ -314,7 +338,7 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
   const int np2 = pow2_at_least(ne < 2 ? 2 : ne);
   for (int i = ne + threadIdx.x; i < np2; i += blockDim.x) cells[i] = ~0ULL;
   __syncthreads();
-  bitonic_sort(cells, np2, sh);
+  bitonic_sort(cells, np2, sh, B.lds_bytes);
   // pair scan, counting pass then writing pass
   auto visit = [&](int i, bool write, int &cnt, int out0) {
     unsigned long long ei = cells[i];
@@ -419,7 +443,7 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     if (np > c.W.pcap) np = c.W.pcap;  // pcap is a power of two
     for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
     __syncthreads();
-    bitonic_sort(pkeys, np, sh);
+    bitonic_sort(pkeys, np, sh, B.lds_bytes);
     const int *gidx = c.at<int>(c.L.gidx);
     // inverse maps: hash index -> geom, big rank -> geom
     int *inv = c.at<int>(c.L.order);
@@ -572,6 +596,133 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
   return fabsf(R.lambda - p);
 }
 
+// Coloured SOR-LCP.  The working set -- per-body fc, world inverse inertia
+// (symmetric) and inverse mass; the constraints sorted by colour with their
+// rows stored contiguously in that order -- is copied into the LDS pool when
+// it fits, else into the world's scratch.  One lane relaxes all rows of one
+// constraint with both bodies' fc in registers; the constraints of a colour
+// share no body, so a colour is one parallel pass.  Each row follows the
+// reference's update (quickstep.cpp:484-582) with inv(M) J^T rebuilt from
+// the body data (every supported row has J_lin2 = -J_lin1).
+constexpr int SOR_MAX_ROWS = 6;   // rows of one constraint (contact 3, ball 3, hinge 6, hinge2 6)
+
+__device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
+                            float *bt, const int *corder, const int *cstart, int nrows, int ngroups, Shared &sh) {
+  const int nb = c.W.nb;
+  const int nact = cstart[ngroups];   // constraints with rows, in colour order
+  const long long need = sor_bytes(nb, nrows, nact);
+  char *base = (need <= B.lds_bytes) ? (char *)od_pool : c.at<char>(c.L.sor);
+  SBody *sb = (SBody *)base;
+  SRow *sr = (SRow *)(sb + nb);
+  SCons *sc = (SCons *)(sr + nrows);
+  int *sstart = (int *)(sc + nact);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const float *t = bt + 32 * b;
+    SBody q;
+    for (int k = 0; k < 6; k++) q.fc[k] = 0.f;
+    q.iI[0] = t[0]; q.iI[1] = t[1]; q.iI[2] = t[2]; q.iI[3] = t[4]; q.iI[4] = t[5]; q.iI[5] = t[8];
+    q.im = B.invMass[c.W.b0 + b];
+    q.pad[0] = q.pad[1] = q.pad[2] = 0.f;
+    sb[b] = q;
+  }
+  for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sstart[k] = cstart[k];
+  // constraints and rows in colour order
+  int rbase = 0;
+  for (int x0 = 0; x0 < nact; x0 += blockDim.x) {
+    int x = x0 + threadIdx.x;
+    const Cons *q = x < nact ? &cons[corder[x]] : nullptr;
+    int m = q ? q->m : 0;
+    int tot;
+    int off = block_excl_scan(m, tot, sh);
+    if (q) {
+      SCons o;
+      o.row0 = rbase + off; o.m = m; o.b1 = q->b1; o.b2 = q->b2;
+      sc[x] = o;
+      for (int k = 0; k < m; k++) {
+        const Row &R = rows[q->row0 + k];
+        SRow w;
+        w.l[0] = R.J[0]; w.l[1] = R.J[1]; w.l[2] = R.J[2];
+        w.a1[0] = R.J[3]; w.a1[1] = R.J[4]; w.a1[2] = R.J[5];
+        w.a2[0] = R.J[9]; w.a2[1] = R.J[10]; w.a2[2] = R.J[11];
+        w.rhs = R.rhs; w.cfm = R.cfm; w.ad = R.ad; w.lo = R.lo; w.hi = R.hi;
+        w.lambda = 0.f;
+        w.findex = R.findex >= 0 ? R.findex - q->row0 : -1;
+        sr[o.row0 + k] = w;
+      }
+    }
+    rbase += tot;
+  }
+  __syncthreads();
+  for (int it = 0; it < P.iterations; it++) {
+    for (int g = 0; g < ngroups; g++) {
+      const int x1 = sstart[g + 1];
+      for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
+        const SCons q = sc[x];
+        const SBody B1 = sb[q.b1];
+        float f1[6], iI1[6];
+        for (int k = 0; k < 6; k++) { f1[k] = B1.fc[k]; iI1[k] = B1.iI[k]; }
+        const float im1 = B1.im;
+        const bool two = q.b2 >= 0;
+        float f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
+        if (two) {
+          const SBody B2 = sb[q.b2];
+          for (int k = 0; k < 6; k++) { f2[k] = B2.fc[k]; iI2[k] = B2.iI[k]; }
+          im2 = B2.im;
+        }
+        float lam0 = 0.f;
+        for (int k = 0; k < SOR_MAX_ROWS; k++) {
+          if (k >= q.m) break;
+          SRow &rr = sr[q.row0 + k];
+          const SRow r = rr;
+          float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] +
+                    r.a1[2] * f1[5];
+          if (two)
+            s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] +
+                 r.a2[2] * f2[5];
+          float old = r.lambda;
+          float delta = r.ad * (r.rhs - old * r.cfm - s);
+          float lo = r.lo, hi = r.hi;
+          if (r.findex >= 0) {
+            float ln = (r.findex == 0) ? lam0 : sr[q.row0 + r.findex].lambda;
+            hi = fabsf(r.hi * ln);
+            lo = -hi;
+          }
+          float nl = old + delta;
+          if (nl < lo) { delta = lo - old; nl = lo; }
+          else if (nl > hi) { delta = hi - old; nl = hi; }
+          rr.lambda = nl;
+          if (k == 0) lam0 = nl;
+          float d1 = delta * im1;
+          f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
+          f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
+          f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
+          f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
+          if (two) {
+            float d2 = delta * im2;
+            f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
+            f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
+            f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
+            f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
+          }
+        }
+        for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
+        if (two)
+          for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
+      }
+      __syncthreads();
+    }
+  }
+  // impulses back to the rows, fc back to the per-body scratch
+  for (int x = threadIdx.x; x < nact; x += blockDim.x) {
+    const Cons &q = cons[corder[x]];
+    const SCons o = sc[x];
+    for (int k = 0; k < o.m; k++) rows[q.row0 + k].lambda = sr[o.row0 + k].lambda;
+  }
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = sb[b].fc[k];
+  __syncthreads();
+}
+
 __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, const StepIn &in, Shared &sh,
                            bool ref) {
   const int nb = c.W.nb, nj = c.W.nj, nk = in.nk;
@@ -645,6 +796,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     q.m = m;
   }
   __syncthreads();
+  c.clk.lap(PH_BODIES);
   // --- constraint order, row offsets, groups
   int *corder = c.at<int>(c.L.order);          // constraint order (island order when ref)
   int *isl = c.at<int>(c.L.island);            // ref: [cons0, ncons, row0, nrows, seed] per island
@@ -774,9 +926,19 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     }
     nrows = r0;
     // colouring: each round takes the constraints whose priority is the
-    // largest among the still uncoloured constraints at both their bodies
-    int *bmax = c.at<int>(c.L.adjs);   // per body (nb+1 ints)
-    for (int i = threadIdx.x; i < ncons; i += blockDim.x) color[i] = (cons[i].m > 0) ? -1 : -2;
+    // largest among the still uncoloured constraints at both their bodies;
+    // the round's winners share no body.  Working set in the LDS pool when
+    // it fits.
+    const bool in_lds = 4LL * (nb + 3LL * ncons) <= B.lds_bytes;
+    int *bmax = in_lds ? (int *)od_pool : c.at<int>(c.L.adjs);
+    int *col = in_lds ? bmax + nb : color;
+    int *cb1 = in_lds ? col + ncons : c.at<int>(c.L.adj);
+    int *cb2 = cb1 + ncons;
+    for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
+      col[i] = (cons[i].m > 0) ? -1 : -2;
+      cb1[i] = cons[i].b1;
+      cb2[i] = cons[i].b2;
+    }
     __syncthreads();
     int round = 0;
     while (true) {
@@ -784,38 +946,55 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
       __syncthreads();
       int left = 0;
       for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
-        if (color[i] != -1) continue;
+        if (col[i] != -1) continue;
         left++;
         int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);   // unique hashed priority
-        atomicMax(&bmax[cons[i].b1], pr);
-        if (cons[i].b2 >= 0) atomicMax(&bmax[cons[i].b2], pr);
+        atomicMax(&bmax[cb1[i]], pr);
+        if (cb2[i] >= 0) atomicMax(&bmax[cb2[i]], pr);
       }
       int nleft = block_sum(left, sh);
       if (nleft == 0) break;
       for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
-        if (color[i] != -1) continue;
+        if (col[i] != -1) continue;
         int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);
-        bool win = bmax[cons[i].b1] == pr && (cons[i].b2 < 0 || bmax[cons[i].b2] == pr);
-        if (win) color[i] = round;
+        bool win = bmax[cb1[i]] == pr && (cb2[i] < 0 || bmax[cb2[i]] == pr);
+        if (win) col[i] = round;
       }
       __syncthreads();
       round++;
     }
     ngroups = round;
-    // bucket constraints by colour: corder sorted by (colour, index)
+    // bucket constraints by colour (order inside a colour is immaterial:
+    // its constraints touch disjoint bodies)
     int *cstart = c.at<int>(c.L.island);   // ngroups+1 starts
-    if (threadIdx.x == 0) {
+    if (ngroups <= MAX_COLORS_LDS) {
+      for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sh.ccount[k] = 0;
+      __syncthreads();
+      for (int i = threadIdx.x; i < ncons; i += blockDim.x)
+        if (col[i] >= 0) atomicAdd(&sh.ccount[col[i] + 1], 1);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int k = 0; k < ngroups; k++) sh.ccount[k + 1] += sh.ccount[k];
+        for (int k = 0; k <= ngroups; k++) cstart[k] = sh.ccount[k];
+      }
+      __syncthreads();
+      for (int k = threadIdx.x; k < ngroups; k += blockDim.x) sh.ccur[k] = sh.ccount[k];
+      __syncthreads();
+      for (int i = threadIdx.x; i < ncons; i += blockDim.x)
+        if (col[i] >= 0) corder[atomicAdd(&sh.ccur[col[i]], 1)] = i;
+    } else if (threadIdx.x == 0) {
       for (int k = 0; k <= ngroups; k++) cstart[k] = 0;
       for (int i = 0; i < ncons; i++)
-        if (color[i] >= 0) cstart[color[i] + 1]++;
+        if (col[i] >= 0) cstart[col[i] + 1]++;
       for (int k = 0; k < ngroups; k++) cstart[k + 1] += cstart[k];
       int *cur = cstart + ngroups + 1;
       for (int k = 0; k < ngroups; k++) cur[k] = cstart[k];
       for (int i = 0; i < ncons; i++)
-        if (color[i] >= 0) corder[cur[color[i]]++] = i;
+        if (col[i] >= 0) corder[cur[col[i]]++] = i;
     }
     __syncthreads();
   }
+  c.clk.lap(PH_ORDER);
   if (nrows > c.W.rcap) {
     // not enough row storage: leave the world unchanged for this step
     if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_ROWS);
@@ -926,6 +1105,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     R.lambda = 0.f;
   }
   __syncthreads();
+  c.clk.lap(PH_ROWS);
   // --- SOR-LCP
   const int iters = P.iterations;
   if (ref) {
@@ -952,22 +1132,9 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
       }
     }
   } else {
-    const int *cstart = c.at<int>(c.L.island);
-    for (int it = 0; it < iters; it++) {
-      for (int g = 0; g < ngroups; g++) {
-        int s0 = cstart[g], s1 = cstart[g + 1];
-        for (int x = s0 + threadIdx.x; x < s1; x += blockDim.x) {
-          const Cons &q = cons[corder[x]];
-          for (int k = 0; k < q.m; k++)
-            if (rows[q.row0 + k].findex < 0) relax_row(rows, q.row0 + k, bt);
-          for (int k = 0; k < q.m; k++)
-            if (rows[q.row0 + k].findex >= 0) relax_row(rows, q.row0 + k, bt);
-        }
-        __syncthreads();
-      }
-    }
+    sor_colored(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
   }
-  __syncthreads();
+  c.clk.lap(PH_SOR);
   // --- residual (optional), feedback
   if (B.track_residual) {
     float mr = 0.f;
@@ -1050,6 +1217,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     B.counters[c.w * CNT_N + CNT_GROUPS] = ngroups;
   }
   __syncthreads();
+  c.clk.lap(PH_INTEGRATE);
 }
 
 // joint controls (steer stops, wheel torques) at the start of a step
@@ -1080,7 +1248,20 @@ __device__ void phase_controls(const DevBatch &B, const Ctx &c) {
 
 // ------------------------------------------------------------------ kernels
 
-enum { MODE_COLLIDE = 1, MODE_STEP = 2, MODE_FUSED = 3 };
+// Kernel modes (bit set): COLLIDE runs AABBs, broadphase and narrowphase;
+// STEP runs the quickstep.  API_OUT hands pairs and contacts of one world to
+// the host (dSpaceCollide); HOST_CONTACTS steps with contact joints the host
+// created (dWorldQuickStep).  Batch steps run COLLIDE|STEP in one kernel, or
+// COLLIDE then STEP as two kernels with the contact count passed through
+// the world's counters.
+enum { MF_COLLIDE = 1, MF_STEP = 2, MF_API_OUT = 4, MF_HOST_CONTACTS = 8 };
+enum {
+  MODE_BATCH_COLLIDE = MF_COLLIDE,
+  MODE_BATCH_STEP = MF_STEP,
+  MODE_FUSED = MF_COLLIDE | MF_STEP,
+  MODE_API_COLLIDE = MF_COLLIDE | MF_API_OUT,
+  MODE_API_STEP = MF_STEP | MF_HOST_CONTACTS
+};
 
 template <int MODE>
 __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
@@ -1089,21 +1270,27 @@ __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
   c.W = B.wi[w];
   c.L = scratch_layout(c.W.nb, c.W.ng, c.W.nj, c.W.pcap, c.W.kcap, c.W.rcap);
   c.S = B.scratch + c.W.scratch;
+  c.clk.start(B, w);
   WorldParams &P = B.wp[w];
   const bool ref = B.order_mode == ORDER_REFERENCE;
-  if (threadIdx.x == 0) B.counters[w * CNT_N + CNT_OVERFLOW] = 0;
-  if (MODE == MODE_FUSED) phase_controls(B, c);
+  constexpr bool api = (MODE & (MF_API_OUT | MF_HOST_CONTACTS)) != 0;
+  // the first kernel of a step clears the overflow flags
+  if (((MODE & MF_COLLIDE) || api) && threadIdx.x == 0) B.counters[w * CNT_N + CNT_OVERFLOW] = 0;
+  if ((MODE & MF_COLLIDE) && !api) phase_controls(B, c);
   int nk = 0;
-  if (MODE & MODE_COLLIDE) {
+  if (MODE & MF_COLLIDE) {
     phase_aabb(B, c, P, sh);
+    c.clk.lap(PH_AABB);
     int np = phase_broadphase(B, c, P, sh, ref);
-    int maxc = (MODE == MODE_FUSED) ? P.max_contacts : api_maxc;
-    nk = phase_narrowphase(B, c, P, np, sh, maxc, MODE == MODE_FUSED);
+    c.clk.lap(PH_BROAD);
+    int maxc = api ? api_maxc : P.max_contacts;
+    nk = phase_narrowphase(B, c, P, np, sh, maxc, !api);
+    c.clk.lap(PH_NARROW);
     if (threadIdx.x == 0) {
       B.counters[w * CNT_N + CNT_PAIRS] = np;
       B.counters[w * CNT_N + CNT_CONTACTS] = nk;
     }
-    if (MODE == MODE_COLLIDE) {
+    if (MODE & MF_API_OUT) {
       // hand pairs and contacts to the host (one world per launch)
       const PairRec *pr = c.at<PairRec>(c.L.pairs);
       const int *pc = c.at<int>(c.L.pcount);
@@ -1115,10 +1302,12 @@ __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
       for (int i = threadIdx.x; i < nk; i += blockDim.x) B.contacts_out[i] = cr[i];
     }
   }
-  if (MODE & MODE_STEP) {
+  if (MODE & MF_STEP) {
     StepIn in;
-    in.from_host = (MODE == MODE_STEP);
-    in.nk = in.from_host ? B.cin_count[w] : nk;
+    in.from_host = (MODE & MF_HOST_CONTACTS) != 0;
+    if (in.from_host) in.nk = B.cin_count[w];
+    else if (MODE & MF_COLLIDE) in.nk = nk;
+    else in.nk = B.counters[w * CNT_N + CNT_CONTACTS];
     in.seq_base = 1 << 20;
     if (in.from_host && in.nk > c.W.kcap) {
       if (threadIdx.x == 0) atomicOr(&B.counters[w * CNT_N + CNT_OVERFLOW], OVF_CONTACTS);
@@ -1174,17 +1363,73 @@ __global__ void init_rot_kernel(DevBatch B, int nbodies) {
 // ------------------------------------------------------------------ launch API
 namespace odemi {
 
-hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid,
-                               int wg, int api_maxc, hipStream_t st) {
+int world_kernel_static_lds() { return (int)sizeof(Shared); }
+
+template <int MODE>
+static const void *world_kernel_fn() { return (const void *)world_kernel<MODE>; }
+
+static const void *kernel_of(int mode) {
+  switch (mode) {
+    case MODE_BATCH_COLLIDE: return world_kernel_fn<MODE_BATCH_COLLIDE>();
+    case MODE_BATCH_STEP: return world_kernel_fn<MODE_BATCH_STEP>();
+    case MODE_FUSED: return world_kernel_fn<MODE_FUSED>();
+    case MODE_API_COLLIDE: return world_kernel_fn<MODE_API_COLLIDE>();
+    case MODE_API_STEP: return world_kernel_fn<MODE_API_STEP>();
+    default: return nullptr;
+  }
+}
+
+hipError_t allow_world_kernel_lds(int bytes) {
+  const int modes[] = {MODE_BATCH_COLLIDE, MODE_BATCH_STEP, MODE_FUSED, MODE_API_COLLIDE, MODE_API_STEP};
+  for (int m : modes) {
+    hipError_t e = hipFuncSetAttribute(kernel_of(m), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int world_kernel_occupancy(int mode, int wg, int lds_bytes) {
+  int n = 0;
+  const void *f = kernel_of(mode);
+  if (!f) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, wg, (size_t)lds_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+// queue: which of the batch's work counters a persistent launch pulls from
+// (kernels of one step use different counters)
+hipError_t launch_world_kernel(int mode, const DevBatch &B0, int first_world, int nworlds_launch, int grid,
+                               int wg, int api_maxc, hipStream_t st, int queue) {
   int persistent = grid > 0 ? 1 : 0;
   int blocks = persistent ? grid : nworlds_launch;
   if (blocks <= 0) return hipSuccess;
   if (wg > WG || wg < 64 || (wg & 63)) return hipErrorInvalidValue;
-  if (persistent) reset_counter_kernel<<<1, 64, 0, st>>>(B.work_counter);
+  if (B0.lds_bytes < 0 || (B0.lds_bytes & 15)) return hipErrorInvalidValue;
+  if (queue < 0 || queue >= WORK_QUEUES) return hipErrorInvalidValue;
+  DevBatch B = B0;
+  if (persistent) {
+    B.work_counter += 16 * queue;
+    reset_counter_kernel<<<1, 64, 0, st>>>(B.work_counter);
+  }
+  size_t lds = (size_t)B.lds_bytes;
   switch (mode) {
-    case MODE_COLLIDE: world_kernel<MODE_COLLIDE><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
-    case MODE_STEP: world_kernel<MODE_STEP><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
-    default: world_kernel<MODE_FUSED><<<blocks, wg, 0, st>>>(B, first_world, persistent, api_maxc); break;
+    case MODE_BATCH_COLLIDE:
+      world_kernel<MODE_BATCH_COLLIDE><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_BATCH_STEP:
+      world_kernel<MODE_BATCH_STEP><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_FUSED: world_kernel<MODE_FUSED><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc); break;
+    case MODE_API_COLLIDE:
+      world_kernel<MODE_API_COLLIDE><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_API_STEP:
+      world_kernel<MODE_API_STEP><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -107,6 +107,13 @@ class Batch:
         self.L.dBatchGetCounters(self.h, c)
         return dict(pairs=c[0], contacts=c[1], rows=c[2], groups=c[3], overflow=c[4])
 
+    def world_counters(self) -> np.ndarray:
+        """(nworlds, 5) int32: pairs, contacts, rows, islands/colours, overflow bits."""
+        out = np.zeros((self.nworlds, 5), np.int32)
+        if not self.L.dBatchGetWorldCounters(self.h, out.ctypes.data_as(C.POINTER(C.c_int))):
+            raise RuntimeError("dBatchGetWorldCounters failed")
+        return out
+
     def track_residual(self, on: bool = True):
         self.L.dBatchSetResidualTracking(self.h, 1 if on else 0)
 
@@ -122,11 +129,33 @@ class Batch:
         wg = C.c_int()
         sc = C.c_float()
         grid = self.L.dBatchGetLaunchConfig(self.h, C.byref(wg), C.byref(sc))
-        return dict(wg=wg.value, cell_scale=sc.value, grid=grid)
+        return dict(wg=wg.value, cell_scale=sc.value, grid=grid, lds=self.L.dBatchGetLds(self.h),
+                    split=self.L.dBatchGetSplit(self.h))
 
-    def set_launch_config(self, wg: int, cell_scale: float = 1.0):
+    def set_launch_config(self, wg: int, cell_scale: float = 1.0, lds: int | None = None,
+                          split: int | None = None, grid: int | None = None):
         if not self.L.dBatchSetLaunchConfig(self.h, wg, cell_scale):
             raise RuntimeError("bad launch config")
+        if lds is not None and not self.L.dBatchSetLds(self.h, lds):
+            raise RuntimeError(f"bad LDS pool size {lds}")
+        if split is not None:
+            self.L.dBatchSetSplit(self.h, int(split))
+        if grid is not None and not self.L.dBatchSetPersistentGrid(self.h, int(grid)):
+            raise RuntimeError(f"bad grid {grid}")
+
+    PHASES = ("aabb", "broadphase", "narrowphase", "bodies_info1", "info2_rhs", "order", "sor", "integrate")
+
+    def profile(self, on: bool = True):
+        """Turns the in-kernel phase clocks on or off (resets them)."""
+        if not self.L.dBatchSetProfiling(self.h, 1 if on else 0):
+            raise RuntimeError("dBatchSetProfiling failed")
+
+    def phase_times(self) -> dict:
+        """Workgroup-ms per phase summed over worlds since the last read."""
+        out = (C.c_double * 8)()
+        if not self.L.dBatchGetPhaseTimes(self.h, out):
+            raise RuntimeError("dBatchGetPhaseTimes failed")
+        return dict(zip(self.PHASES, list(out)))
 
     def pairs(self, w: int, maxn: int = 1 << 20) -> np.ndarray:
         out = np.zeros((maxn, 2), np.int32)

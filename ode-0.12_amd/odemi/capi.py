@@ -59,6 +59,14 @@ def load_library(path: str = LIB_PATH):
         "dBatchAutotune": (C.c_int, [V, C.c_int]),
         "dBatchGetLaunchConfig": (C.c_int, [V, _ip, _fp]),
         "dBatchSetLaunchConfig": (C.c_int, [V, C.c_int, C.c_float]),
+        "dBatchGetLds": (C.c_int, [V]),
+        "dBatchGetWorldCounters": (C.c_int, [V, _ip]),
+        "dBatchGetSplit": (C.c_int, [V]),
+        "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),
+        "dBatchSetSplit": (C.c_int, [V, C.c_int]),
+        "dBatchSetLds": (C.c_int, [V, C.c_int]),
+        "dBatchSetProfiling": (C.c_int, [V, C.c_int]),
+        "dBatchGetPhaseTimes": (C.c_int, [V, C.POINTER(C.c_double)]),
         "dBatchGetPairs": (C.c_int, [V, C.c_int, C.c_int, _ip]),
         "dBatchGetContacts": (C.c_int, [V, C.c_int, C.c_int, _fp]),
         "dBatchDeviceState": (V, [V]),

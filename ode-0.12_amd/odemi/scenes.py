@@ -194,15 +194,18 @@ def random_pile(nb: int = 4096, seed: int = 12345) -> Scene:
     return b.scene(dt=0.005, mu=0.6, seed=seed)
 
 
-def _chain(b: _Builder, rng: np.random.Generator, n_links: int, start, link_len=0.3, link_w=0.1):
+def _chain(b: _Builder, rng: np.random.Generator, n_links: int, start, link_len=0.3, link_w=0.1, pitch=0.27):
     """Horizontal chain of boxes along x, joined alternately by ball and hinge
-    joints at the shared link ends (hinge axis y)."""
+    joints at the middle of the overlap of neighbouring links (hinge axis y).
+    Neighbours overlap by link_len - pitch so that their bounding boxes
+    intersect by a clear margin rather than touching exactly (an exact touch
+    is decided by the last bit of the arithmetic)."""
     x0, y0, z0 = start
     links = []
     for k in range(n_links):
-        links.append(b.box_body((x0 + link_len * (k + 0.5), y0, z0), (link_len, link_w, link_w), 1.0))
+        links.append(b.box_body((x0 + pitch * (k + 0.5), y0, z0), (link_len, link_w, link_w), 1.0))
     for k in range(n_links - 1):
-        anchor = (x0 + link_len * (k + 1), y0, z0)
+        anchor = (x0 + pitch * (k + 1), y0, z0)
         if k % 2 == 0:
             b.joint(BALL, links[k], links[k + 1], anchor)
         else:

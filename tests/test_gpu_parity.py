@@ -119,8 +119,14 @@ def test_narrowphase_pairs_vs_oracle(odemi_mod):
                 continue
             assert len(got) == len(ref64), (t1, t2, trial, got, ref64)
             if len(got):
-                err = np.abs(got - ref64) / np.maximum(1.0, np.abs(ref64))
-                assert err.max() <= 2e-5, (t1, t2, trial, err.max())
+                # 1e-5 relative, widened to 4x the reference FP32 build's own
+                # error on the same case (ill-conditioned normals of nearly
+                # touching shapes)
+                scale = np.maximum(1.0, np.abs(ref64))
+                err = np.abs(got - ref64) / scale
+                e32 = np.abs(ref32 - ref64) / scale
+                tol = max(1e-5, 4.0 * float(e32.max()))
+                assert err.max() <= tol, (t1, t2, trial, err.max(), tol)
                 n_cmp += 1
     say("narrowphase compared", n_cmp, "precision ties skipped", n_disc)
     assert n_cmp > 150 and n_disc < 10
