@@ -2,14 +2,16 @@
 
 Workload: 4096 independent worlds of uneven size (body count per world
 clamp(Poisson(64), 8, 256) boxes/spheres plus 0-16 ball/hinge chains of 4-12
-links), h = 0.01, 20 SOR iterations, synthetic seeded scenes.  One step =
+links), h = 0.005, mu = 0.6, synthetic seeded scenes.  One step =
 broadphase + narrowphase + quickstep (rows, SOR-LCP, integration) of every
-world.
+world.  Steps 0..warmup settle the piles; the timed steps follow.
 
 metric: body-steps/s of the batched persistent path (all worlds in one launch
-per step, HIP-graph replay).  Also reported: the plain one-world-per-launch
-path on the same worlds, the LCP complementarity residual, device counters,
-the HBM roofline of the step kernel and the reference CPU build on the host.
+per step, HIP-graph replay) with every world's SOR iterated until its LCP
+residual is <= 1e-4.  Also reported: the plain one-world-per-launch path on
+the same worlds and state, the same two paths at the quickstep's fixed 20
+iterations, device counters, per-phase clocks, the HBM roofline of the step
+and the reference CPU build on the host.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds N]
 """
@@ -34,21 +36,62 @@ def algorithmic_bytes(nb, ng, pairs, contacts, rows):
     return 180 * nb + 40 * ng + 16 * pairs + 128 * pairs + 36 * contacts + 4 * rows
 
 
-def cpu_baseline(worlds, steps):
-    """Reference ODE 0.12 single-precision build (oracle/_ref) on one host core,
-    on the first worlds of the workload."""
+def _cpu_worker(job):
+    """One host process: the reference single-precision build over a slice of
+    worlds; returns (body-steps, seconds)."""
+    scenes_, steps = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    if not po.have_ref("single"):
-        return None
     lib = po.ref_lib("single")
     t0 = time.perf_counter()
     bodies = 0
-    for sc in worlds:
+    for sc in scenes_:
         po.run(lib, sc, steps, 0)
         bodies += sc.nb
+    return bodies * steps, time.perf_counter() - t0
+
+
+def cpu_baseline(worlds, steps, procs):
+    """Reference ODE 0.12 single-precision build (oracle/_ref) on `procs` host
+    processes (ODE keeps its RNG and collider table in process globals, so
+    worlds are split over processes, not threads), each stepping a slice of
+    the sample.  Runs before the GPU is touched."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import multiprocessing as mp
+    import pyoracle as po
+    if not po.have_ref("single"):
+        return None
+    jobs = [(worlds[k::procs], steps) for k in range(procs)]
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    work = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    return work / wall, wall
+
+
+def _timed(b, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = b.step_timed(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, dev_ms
+
+
+def _plain_rate(odemi, worlds, cfg, state, steps, tol, max_iters, check_every):
+    import torch
+    bp = odemi.Batch(worlds, odemi.ORDER_COLORED)
+    bp.set_launch_config(cfg["wg"], cfg["cell_scale"], cfg["lds"], cfg["split"])
+    bp.set_state(state)
+    if tol > 0:
+        bp.set_tolerance(tol, max_iters, check_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bp.step_plain(steps)
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return bodies * steps / dt, dt
+    bp.close()
+    return dt / steps
 
 
 def main():
@@ -59,7 +102,11 @@ def main():
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--autotune", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=256, help="worlds in the CPU baseline sample")
+    ap.add_argument("--tol", type=float, default=1e-4, help="LCP residual the solver iterates down to")
+    ap.add_argument("--max-iters", type=int, default=1000)
+    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--cpu-sample", type=int, default=1024, help="worlds in the CPU baseline sample")
+    ap.add_argument("--cpu-procs", type=int, default=16, help="host processes of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--launch", default=None,
                     help="fixed launch config 'wg,cell_scale,lds,split,grid' instead of autotuning")
@@ -69,17 +116,33 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import odemi
     from odemi import scenes
 
-    torch.cuda.set_device(0)
     t0 = time.perf_counter()
     worlds = scenes.uneven_worlds(args.worlds, seed=7)
     nb = sum(w.nb for w in worlds)
     ng = sum(w.ng for w in worlds)
-    print(f"# workload: {len(worlds)} worlds, {nb} bodies, {ng} geoms (built in {time.perf_counter() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    log(f"workload: {len(worlds)} worlds, {nb} bodies, {ng} geoms (built in {time.perf_counter() - t0:.1f}s)")
+
+    # reference CPU baseline first, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        procs = max(1, min(args.cpu_procs, os.cpu_count() or 1))
+        sample = worlds[: args.cpu_sample]
+        csteps = args.warmup + args.steps
+        r = cpu_baseline(sample, csteps, procs)
+        if r:
+            cpu = dict(value=r[0], unit="body-steps/s", cores=procs, kind="reference",
+                       sample=f"reference ODE 0.12 dSINGLE build (-O2), dSpaceCollide + dWorldQuickStep "
+                              f"(20 iterations) + dJointGroupEmpty, first {len(sample)} worlds "
+                              f"({sum(w.nb for w in sample)} bodies) x steps 0-{csteps}, split over {procs} "
+                              f"host processes, {r[1]:.1f}s")
+            log(f"cpu baseline: {r[0]:.3g} body-steps/s on {procs} processes ({r[1]:.1f}s)")
+
+    import torch
+    torch.cuda.set_device(0)
 
     b = odemi.Batch(worlds, odemi.ORDER_COLORED)
     if args.launch:
@@ -88,58 +151,54 @@ def main():
     elif args.autotune:
         b.autotune(2)
     cfg = b.launch_config()
-    print(f"# launch config {cfg}", file=sys.stderr, flush=True)
+    log(f"launch config {cfg}")
     if args.warmup:
         b.step(args.warmup)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    dev_ms = b.step_timed(args.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
+    s0 = b.state()
+
+    # (1) the quickstep's fixed iteration count (20, as dWorldQuickStep)
+    wall20, dev20 = _timed(b, args.steps)
+    cnt20 = b.counters()
+    b.track_residual(True)
+    b.step(1)
+    res20 = b.residual()
+    b.track_residual(False)
+    log(f"fixed 20 iterations: {wall20 / args.steps * 1e3:.3f} ms/step, residual median {np.median(res20):.3g} "
+        f"max {res20.max():.3g}, counters {cnt20}")
+
+    # (2) the metric's operating point: every world iterates to residual <= tol
+    b.set_state(s0)
+    b.set_tolerance(args.tol, args.max_iters, args.check_every)
+    wall, dev_ms = _timed(b, args.steps)
     cnt = b.counters()
-    # residual of one more step
+    wc = b.world_counters()
     b.track_residual(True)
     b.step(1)
     res = b.residual()
     b.track_residual(False)
+    iters = wc[:, 5]
+    conv = float(np.mean(res <= args.tol))
+    log(f"tolerance {args.tol:g}: {wall / args.steps * 1e3:.3f} ms/step, iterations mean {iters.mean():.1f} "
+        f"max {iters.max()}, converged {conv:.4f}, residual max {res.max():.3g}")
     # per-phase workgroup time (in-kernel clocks) over a few more steps
     b.profile(True)
     b.step(5)
     ph = {k: v / 5 for k, v in b.phase_times().items()}
     b.profile(False)
+    b.close()
+
     ms_step = wall / args.steps * 1e3
     value = nb * args.steps / wall
     kernel_ms = dev_ms / args.steps
     B = algorithmic_bytes(nb, ng, cnt["pairs"], cnt["contacts"], cnt["rows"])
     achieved = B / (kernel_ms * 1e-3) / 1e9
-    print(f"# batched: {ms_step:.3f} ms/step wall, {kernel_ms:.3f} ms/step device, counters {cnt}", file=sys.stderr,
-          flush=True)
 
-    # plain path: one world per launch
-    plain = None
+    plain = plain20 = None
     if args.plain_steps > 0:
-        bp = odemi.Batch(worlds, odemi.ORDER_COLORED)
-        bp.set_launch_config(cfg["wg"], cfg["cell_scale"], cfg["lds"], cfg["split"])
-        bp.step_plain(1)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        bp.step_plain(args.plain_steps)
-        torch.cuda.synchronize()
-        pw = time.perf_counter() - t2
-        plain = nb * args.plain_steps / pw
-        print(f"# plain: {pw / args.plain_steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-        bp.close()
-
-    cpu = None
-    if not args.no_cpu:
-        sample = worlds[: args.cpu_sample]
-        csteps = args.warmup + args.steps
-        r = cpu_baseline(sample, csteps)
-        if r:
-            cpu = dict(value=r[0], unit="body-steps/s", cores=1, kind="reference",
-                       sample=f"reference ODE 0.12 dSINGLE build, dSpaceCollide+dWorldQuickStep+dJointGroupEmpty, "
-                              f"first {len(sample)} worlds ({sum(w.nb for w in sample)} bodies) x steps 0-{csteps}, "
-                              f"{r[1]:.1f}s on 1 host core")
+        t_tol = _plain_rate(odemi, worlds, cfg, s0, args.plain_steps, args.tol, args.max_iters, args.check_every)
+        t_20 = _plain_rate(odemi, worlds, cfg, s0, args.plain_steps, 0.0, 0, 0)
+        plain, plain20 = nb / t_tol, nb / t_20
+        log(f"plain: {t_tol * 1e3:.1f} ms/step at tolerance, {t_20 * 1e3:.1f} ms/step at 20 iterations")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_uneven_worlds.json")
@@ -163,12 +222,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scenes)",
         "config": {"workload": f"batched uneven worlds: {len(worlds)} worlds, body count clamp(Poisson(64),8,256) "
-                               f"box/sphere piles + 0-16 ball/hinge chains, h=0.01, 20 iterations, seed 7",
+                               f"box/sphere piles + 0-16 ball/hinge chains of 4-12 links, h=0.005, mu=0.6, "
+                               f"steps {args.warmup}-{args.warmup + args.steps}, seed 7",
                    "worlds": len(worlds), "bodies": nb, "geoms": ng, "solver_order": "colored",
+                   "solver": f"SOR until LCP residual <= {args.tol:g} (checked every {args.check_every}, "
+                             f"cap {args.max_iters})",
                    "launch": cfg},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
-        "lcp_residual": {"max": float(np.max(res)), "median": float(np.median(res))},
+        "lcp_residual": {"tol": args.tol, "converged_fraction": conv, "max": float(np.max(res)),
+                         "median": float(np.median(res)), "iterations_mean": float(iters.mean()),
+                         "iterations_max": int(iters.max())},
+        "fixed_20_iterations": {"value": nb * args.steps / wall20, "ms_per_step": wall20 / args.steps * 1e3,
+                                "plain_value": plain20,
+                                "batched_over_plain": (nb * args.steps / wall20 / plain20) if plain20 else None,
+                                "lcp_residual_median": float(np.median(res20)),
+                                "lcp_residual_max": float(np.max(res20))},
         "counters_per_step": cnt,
         "phase_wg_ms_per_step": ph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -346,9 +346,15 @@ ODE_API int dBatchGetState(dBatchID b, float *out);
 ODE_API int dBatchSetState(dBatchID b, const float *in);
 /* Totals of the last step: [pairs, contacts, rows, islands_or_colours, overflow]. */
 ODE_API int dBatchGetCounters(dBatchID b, long long *out5);
-/* The same five counters for every world (5 ints per world; overflow bits:
- * 1 grid cells, 2 pairs, 4 contacts, 8 rows). */
+/* Counters of the last step for every world, 6 ints per world: pairs,
+ * contacts, rows, islands or colours, overflow bits (1 grid cells, 2 pairs,
+ * 4 contacts, 8 rows), SOR iterations run. */
 ODE_API int dBatchGetWorldCounters(dBatchID b, int *out);
+/* Coloured order only: iterate each world's SOR until its LCP residual is
+ * at most tol (checked every check_every iterations, at most max_iterations);
+ * tol = 0 restores the fixed iteration count of the world.  The reference
+ * order always runs the fixed count, as dWorldQuickStep does. */
+ODE_API int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, int check_every);
 /* Enables the LCP complementarity residual of the last step per world
  * (max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|). */
 ODE_API int dBatchSetResidualTracking(dBatchID b, int on);
@@ -393,17 +399,29 @@ ODE_API int dBatchNumBodies(dBatchID b);
 
 /* ------------------------------------------------------------------ 3. adjoint */
 
-/* Records the forward tape (contacts, active set, lambdas) of every step
- * run by dBatchStep from now on, keeping at most `max_steps` steps. */
+/* Starts recording the steps run by dBatchStep (at most `max_steps`): the
+ * state is checkpointed every checkpoint_interval steps (default 10); the
+ * backward pass replays each segment from its checkpoint.  Worlds must be
+ * contact-only (spheres, boxes, planes; no permanent joints) with
+ * dContactApprox1 friction (or mu = 0) and optional dContactBounce.  The
+ * derivative is that of the converged LCP: run the coloured order with a
+ * solver tolerance (dBatchSetSolverTolerance) for accurate gradients. */
 ODE_API int dBatchTapeBegin(dBatchID b, int max_steps);
 ODE_API int dBatchTapeEnd(dBatchID b);
+/* Checkpoint interval, active-set threshold on impulses, relative residual
+ * and iteration cap of the adjoint linear solve (not while recording). */
+ODE_API int dBatchTapeSetOptions(dBatchID b, int checkpoint_interval, float act_tol, float lin_tol, int lin_maxit);
+/* Totals of the last backward pass: adjoint-solve iterations, and contact
+ * pairs whose dual-number collider disagreed with the tape (0 expected). */
+ODE_API int dBatchTapeStats(dBatchID b, long long *out2);
 /* Reverse-mode derivative of the last `nsteps` recorded steps.
  * grad_out: dL/d(state after), 13 floats per body (pos, q, lvel, avel).
  * grad_in:  receives dL/d(state before the first of those steps).
  * grad_mu:  receives dL/d(mu) per world (may be NULL).
  * grad_bounce: receives dL/d(bounce) per world (may be NULL).
- * Contacts are differentiated through the converged LCP on its active set;
- * the collision geometry (contact points, normals, depths) is held fixed. */
+ * The LCP is differentiated implicitly on its active set; contact points,
+ * normals and depths are differentiated through the colliders.  The batch
+ * state is left at the end of the tape. */
 ODE_API int dBatchStepBackward(dBatchID b, int nsteps, const float *grad_out, float *grad_in,
                                float *grad_mu, float *grad_bounce);
 /* Single-world form of the adjoint for the ODE API path: derivative of the

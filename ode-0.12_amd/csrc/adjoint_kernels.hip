@@ -1,0 +1,154 @@
+// adjoint_kernels.hip -- device side of the step adjoint: capture of a
+// replayed step's contacts and row impulses into the segment tape, and the
+// reverse step (adj_step.hpp) with one workgroup per world.
+#include <hip/hip_runtime.h>
+
+#include "adj_step.hpp"
+#include "layout.hpp"
+
+namespace odemi {
+
+using adj::TapeContact;
+
+struct AdjShared {
+  float wsum[16];
+};
+
+__device__ inline float adj_block_sum(float v, AdjShared &sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh.wsum[wid] = v;
+  __syncthreads();
+  float s = 0.f;
+  int nw = (blockDim.x + 63) >> 6;
+  for (int k = 0; k < nw; k++) s += sh.wsum[k];
+  __syncthreads();
+  return s;
+}
+
+// workgroup context of adj::adjoint_step
+struct ParWG {
+  AdjShared *sh;
+  __device__ int tid() const { return threadIdx.x; }
+  __device__ int nt() const { return blockDim.x; }
+  __device__ void sync() { __syncthreads(); }
+  __device__ float sum(float v) { return adj_block_sum(v, *sh); }
+  __device__ void add(float *p, float v) { atomicAdd(p, v); }
+};
+
+// settings of the reverse step
+struct AdjSettings {
+  float act_tol;    // impulses at most this count as at their bound
+  float lin_tol;    // relative residual of the adjoint linear solve
+  int lin_maxit;
+};
+
+// After a replayed forward step: the step's contacts (in creation order)
+// with the impulses of their rows, from the world's scratch.
+__global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const long long *con_off, int *nk_out,
+                                    int *ovf) {
+  const int w = blockIdx.x;
+  if (w >= B.nworlds) return;
+  const WorldInfo W = B.wi[w];
+  const ScratchLayout L = scratch_layout(W.nb, W.ng, W.nj, W.pcap, W.kcap, W.rcap);
+  const char *S = B.scratch + W.scratch;
+  const ContactRec *cr = (const ContactRec *)(S + L.contacts);
+  const Cons *cons = (const Cons *)(S + L.cons);
+  const Row *rows = (const Row *)(S + L.rows);
+  int nk = B.counters[w * CNT_N + CNT_CONTACTS];
+  if (nk > W.kcap) nk = W.kcap;
+  TapeContact<float> *dst = con + con_off[w];
+  for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+    const ContactRec &c = cr[i];
+    TapeContact<float> t;
+    for (int k = 0; k < 3; k++) { t.pos[k] = c.pos[k]; t.normal[k] = c.normal[k]; }
+    t.depth = c.depth;
+    t.g1 = c.g1; t.g2 = c.g2; t.b1 = c.b1; t.b2 = c.b2; t.flags = c.flags;
+    int m = cons[i].m;
+    t.m = m;
+    for (int r = 0; r < 3; r++) t.lam[r] = (r < m) ? rows[cons[i].row0 + r].lambda : 0.f;
+    dst[i] = t;
+  }
+  if (threadIdx.x == 0) {
+    nk_out[w] = nk;
+    if (B.counters[w * CNT_N + CNT_OVERFLOW] & (OVF_PAIRS | OVF_CONTACTS | OVF_ROWS)) atomicOr(ovf, 1);
+  }
+}
+
+// One reverse step for every world.  st_*: the step's pre-step state;
+// gin_*: adjoint of the post-step state; gout_*: adjoint of the pre-step
+// state (written).  Per-world parameter adjoints accumulate into gmu,
+// gbounce; stats: per world [iterations, collider mismatches].
+__global__ __launch_bounds__(256) void adjoint_step_kernel(
+    DevBatch B, const float4 *st_x, const float4 *st_q, const float4 *st_v, const float4 *st_w,
+    const TapeContact<float> *con, const long long *con_off, const int *nk, const float4 *gin_x,
+    const float4 *gin_q, const float4 *gin_v, const float4 *gin_w, float4 *gout_x, float4 *gout_q, float4 *gout_v,
+    float4 *gout_w, float *gmu, float *gbounce, int *stats, float *scr, const long long *scr_off, AdjSettings S) {
+  __shared__ AdjShared sh;
+  const int w = blockIdx.x;
+  if (w >= B.nworlds) return;
+  const WorldInfo W = B.wi[w];
+  const WorldParams &P = B.wp[w];
+  adj::StepData<float> D;
+  D.nb = W.nb;
+  D.nk = nk[w];
+  D.ng = W.ng;
+  D.x = (const float *)(st_x + W.b0);
+  D.q = (const float *)(st_q + W.b0);
+  D.v = (const float *)(st_v + W.b0);
+  D.w = (const float *)(st_w + W.b0);
+  D.mass = B.mass + W.b0;
+  D.invMass = B.invMass + W.b0;
+  D.I = (const float *)(B.I + 3 * (size_t)W.b0);
+  D.invI = (const float *)(B.invI + 3 * (size_t)W.b0);
+  D.gtype = B.gtype + W.g0;
+  D.gbody = B.gbody + W.g0;
+  D.gprm = (const float *)(B.gparam + W.g0);
+  D.gpos = (const float *)(B.gpos + W.g0);
+  D.gR = (const float *)(B.gR + 3 * (size_t)W.g0);
+  D.con = con + con_off[w];
+  adj::StepParams<float> Q;
+  Q.h = P.dt; Q.erp = P.erp; Q.cfm = P.cfm; Q.min_depth = P.min_depth; Q.max_vel = P.max_vel;
+  for (int k = 0; k < 3; k++) Q.gravity[k] = P.gravity[k];
+  Q.mu = P.mu; Q.bounce = P.bounce; Q.bounce_vel = P.bounce_vel;
+  Q.surf_bounce = (P.surf_mode & SM_BOUNCE) ? 1 : 0;
+  Q.maxc = P.max_contacts;
+  Q.act_tol = S.act_tol; Q.lin_tol = S.lin_tol; Q.lin_maxit = S.lin_maxit;
+  adj::StepAdj<float> G;
+  G.gx_in = (const float *)(gin_x + W.b0);
+  G.gq_in = (const float *)(gin_q + W.b0);
+  G.gv_in = (const float *)(gin_v + W.b0);
+  G.gw_in = (const float *)(gin_w + W.b0);
+  G.gx = (float *)(gout_x + W.b0);
+  G.gq = (float *)(gout_q + W.b0);
+  G.gv = (float *)(gout_v + W.b0);
+  G.gw = (float *)(gout_w + W.b0);
+  G.gmu = gmu + w;
+  G.gbounce = gbounce + w;
+  G.stats = stats + 2 * w;
+  ParWG par{&sh};
+  adj::adjoint_step<float>(par, D, Q, G, scr + scr_off[w]);
+}
+
+hipError_t launch_tape_capture(const DevBatch &B, TapeContact<float> *con, const long long *con_off, int *nk,
+                               int *ovf, hipStream_t st) {
+  if (B.nworlds <= 0) return hipSuccess;
+  tape_capture_kernel<<<B.nworlds, 128, 0, st>>>(B, con, con_off, nk, ovf);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], const TapeContact<float> *con,
+                               const long long *con_off, const int *nk, const float4 *const gin[4],
+                               float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
+                               const long long *scr_off, float act_tol, float lin_tol, int lin_maxit, int wg,
+                               hipStream_t st_) {
+  if (B.nworlds <= 0) return hipSuccess;
+  if (wg < 64 || wg > 256 || (wg & 63)) return hipErrorInvalidValue;
+  AdjSettings S{act_tol, lin_tol, lin_maxit};
+  adjoint_step_kernel<<<B.nworlds, wg, 0, st_>>>(B, st[0], st[1], st[2], st[3], con, con_off, nk, gin[0], gin[1],
+                                                  gin[2], gin[3], gout[0], gout[1], gout[2], gout[3], gmu, gbounce,
+                                                  stats, scr, scr_off, S);
+  return hipGetLastError();
+}
+
+}  // namespace odemi

@@ -48,7 +48,7 @@ void refresh_dev(BatchImpl *b) {
   D.work_counter = b->d.work_counter.as<int>();
   D.world_order = b->d.world_order.as<int>();
   D.scratch = b->d.scratch.as<char>();
-  D.tape = b->tape.recording ? b->tape.buf.as<char>() : nullptr;
+  D.tape = nullptr;
   D.tape_off = nullptr;
   D.prof = b->prof_on ? b->d.prof.as<unsigned long long>() : nullptr;
   D.lds_bytes = b->lds;
@@ -542,11 +542,7 @@ int batch_get_contacts(BatchImpl *b, int w, int maxn, float *out) {
   return cnt[CNT_CONTACTS];
 }
 
-// state that one step mutates, saved and restored around autotuning trials
-struct Snapshot {
-  DevBuf pos, q, lvel, avel, facc, tacc, R, grank, joints, wp;
-};
-static int snapshot(BatchImpl *b, Snapshot &s, bool save) {
+int snapshot(BatchImpl *b, Snapshot &s, bool save) {
   struct P { DevBuf *src, *dst; };
   P items[] = {{&b->d.pos, &s.pos}, {&b->d.q, &s.q}, {&b->d.lvel, &s.lvel}, {&b->d.avel, &s.avel},
                {&b->d.facc, &s.facc}, {&b->d.tacc, &s.tacc}, {&b->d.R, &s.R}, {&b->d.grank, &s.grank},
@@ -561,6 +557,28 @@ static int snapshot(BatchImpl *b, Snapshot &s, bool save) {
   }
   OD_HIP(hipStreamSynchronize(b->stream));
   return 1;
+}
+
+// applies f to the world parameters on the host and on the device (the
+// device copy holds live RNG seeds, so it is read back and patched)
+template <class F>
+static int patch_params(BatchImpl *b, F f) {
+  for (auto &p : b->wp) f(p);
+  std::vector<WorldParams> cur(b->nworlds);
+  OD_HIP(hipStreamSynchronize(b->stream));
+  OD_HIP(hipMemcpy(cur.data(), b->d.wp.p, sizeof(WorldParams) * b->nworlds, hipMemcpyDeviceToHost));
+  for (auto &p : cur) f(p);
+  OD_HIP(hipMemcpy(b->d.wp.p, cur.data(), sizeof(WorldParams) * b->nworlds, hipMemcpyHostToDevice));
+  return 1;
+}
+
+int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every) {
+  if (!(tol >= 0.f) || (tol > 0.f && (max_iterations <= 0 || check_every <= 0))) return 0;
+  return patch_params(b, [&](WorldParams &p) {
+    p.tol = tol;
+    p.max_iterations = max_iterations;
+    p.check_every = check_every;
+  });
 }
 
 static int set_cell_scale(BatchImpl *b, float s) {
@@ -732,6 +750,9 @@ void *dBatchStream(dBatchID b) { return b ? (void *)b->impl->stream : nullptr; }
 int dBatchNumBodies(dBatchID b) { return b ? b->impl->nb : 0; }
 
 int dBatchGetLds(dBatchID b) { return b ? b->impl->lds : -1; }
+int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, int check_every) {
+  return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every) : 0;
+}
 int dBatchGetWorldCounters(dBatchID b, int *out) {
   if (!b || !out) return 0;
   BatchImpl *i = b->impl;
@@ -764,6 +785,22 @@ int dBatchGetContacts(dBatchID b, int world, int maxn, float *out) {
 }
 
 int dBatchTapeBegin(dBatchID b, int max_steps) { return b ? tape_begin(b->impl, max_steps) : 0; }
+int dBatchTapeSetOptions(dBatchID b, int checkpoint_interval, float act_tol, float lin_tol, int lin_maxit) {
+  if (!b || checkpoint_interval <= 0 || !(act_tol >= 0.f) || !(lin_tol > 0.f) || lin_maxit <= 0) return 0;
+  Tape &t = b->impl->tape;
+  if (t.recording) return 0;
+  t.interval = checkpoint_interval;
+  t.act_tol = act_tol;
+  t.lin_tol = lin_tol;
+  t.lin_maxit = lin_maxit;
+  return 1;
+}
+int dBatchTapeStats(dBatchID b, long long *out2) {
+  if (!b || !out2) return 0;
+  out2[0] = b->impl->tape.iters;
+  out2[1] = b->impl->tape.mismatches;
+  return 1;
+}
 int dBatchTapeEnd(dBatchID b) { return b ? tape_end(b->impl) : 0; }
 int dBatchStepBackward(dBatchID b, int nsteps, const float *grad_out, float *grad_in, float *grad_mu,
                        float *grad_bounce) {

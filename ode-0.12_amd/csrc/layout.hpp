@@ -9,9 +9,14 @@
 #include <stdint.h>
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define OD_HD __host__ __device__
 #else
 #define OD_HD
+// host-only builds (the adjoint's CPU checks) see the device vector types as
+// plain structs
+struct float3 { float x, y, z; };
+struct float4 { float x, y, z, w; };
 #endif
 
 namespace odemi {
@@ -27,7 +32,7 @@ enum {
   SM_APPROX1_1 = 0x1000, SM_APPROX1_2 = 0x2000
 };
 // per-world counters written by the step kernels
-enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_N };
+enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_ITERS, CNT_N };
 enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8 };
 
 static const int MAX_SLOTS = 8;   // most contacts a collider produces for one pair
@@ -58,6 +63,9 @@ struct WorldParams {
   int hash_min, hash_max;
   unsigned int seed;       // SOR shuffle RNG state (advanced by the kernel)
   float cell_scale;        // uniform-grid cell = cell_scale * largest AABB extent
+  float tol;               // coloured solver: stop once the LCP residual <= tol (0: fixed iterations)
+  int max_iterations;      // iteration cap when tol > 0
+  int check_every;         // iterations between residual checks when tol > 0
 };
 
 struct Limot {             // ode/src/joints/joint.h:172-190

@@ -1,22 +1,32 @@
 // odemi_internal.hpp -- declarations shared by the host-side translation
 // units of the library (batch API, ODE API layer, adjoint).
 #pragma once
+#include <memory>
+
 #include "host_common.hpp"
 #include "odemi.h"
 
 namespace odemi {
 
-// Forward tape of recorded steps for the adjoint: per step a copy of the
-// pre-step body state and of the rows, lambdas and constraint records the
-// step produced.
+// device state that one step mutates; saved and restored around autotuning
+// trials and as the adjoint's checkpoints
+struct Snapshot {
+  DevBuf pos, q, lvel, avel, facc, tacc, R, grank, joints, wp;
+};
+
+// Forward tape for the adjoint: checkpoints of the state every `interval`
+// recorded steps; the backward pass replays each segment from its
+// checkpoint and differentiates it step by step in reverse.
 struct Tape {
   bool recording = false;
+  int interval = 10;
   int max_steps = 0;
-  int nsteps = 0;
-  DevBuf buf;                    // per step: state + per-world scratch copies
-  std::vector<long long> step_off;
-  long long state_bytes = 0;
-  long long scratch_bytes = 0;
+  int nsteps = 0;                                // steps recorded since dBatchTapeBegin
+  std::vector<std::unique_ptr<Snapshot>> ckpt;   // state at tape steps 0, interval, 2*interval, ...
+  float act_tol = 1e-6f;                         // active-set threshold on impulses
+  float lin_tol = 1e-6f;                         // relative residual of the adjoint solve
+  int lin_maxit = 500;
+  long long iters = 0, mismatches = 0;           // statistics of the last backward pass
 };
 
 // device arrays of a batch
@@ -81,6 +91,7 @@ int batch_set_profiling(BatchImpl *b, bool on);
 int batch_phase_times(BatchImpl *b, double *ms);
 int batch_autotune(BatchImpl *b, int trial_steps);
 
+int snapshot(BatchImpl *b, Snapshot &s, bool save);
 int tape_begin(BatchImpl *b, int max_steps);
 int tape_end(BatchImpl *b);
 int tape_step(BatchImpl *b, bool sync);

@@ -606,6 +606,33 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
 // the body data (every supported row has J_lin2 = -J_lin1).
 constexpr int SOR_MAX_ROWS = 6;   // rows of one constraint (contact 3, ball 3, hinge 6, hinge2 6)
 
+// largest complementarity residual |lambda - clamp(lambda - (A lambda - b)_i / A_ii)|
+// over the rows of one constraint of the coloured solver (A_ii = w / ad)
+__device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const SCons q, float sor_w) {
+  const SBody &B1 = sb[q.b1];
+  const bool two = q.b2 >= 0;
+  float r = 0.f;
+  for (int k = 0; k < q.m; k++) {
+    const SRow &w = sr[q.row0 + k];
+    float s = w.l[0] * B1.fc[0] + w.l[1] * B1.fc[1] + w.l[2] * B1.fc[2] + w.a1[0] * B1.fc[3] +
+              w.a1[1] * B1.fc[4] + w.a1[2] * B1.fc[5];
+    if (two) {
+      const SBody &B2 = sb[q.b2];
+      s += -w.l[0] * B2.fc[0] - w.l[1] * B2.fc[1] - w.l[2] * B2.fc[2] + w.a2[0] * B2.fc[3] + w.a2[1] * B2.fc[4] +
+           w.a2[2] * B2.fc[5];
+    }
+    float y = s + w.cfm * w.lambda - w.rhs;
+    float lo = w.lo, hi = w.hi;
+    if (w.findex >= 0) {
+      hi = fabsf(w.hi * sr[q.row0 + w.findex].lambda);
+      lo = -hi;
+    }
+    float p = fminf(fmaxf(w.lambda - y * (w.ad / sor_w), lo), hi);
+    r = fmaxf(r, fabsf(w.lambda - p));
+  }
+  return r;
+}
+
 __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
                             float *bt, const int *corder, const int *cstart, int nrows, int ngroups, Shared &sh) {
   const int nb = c.W.nb;
@@ -653,65 +680,83 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     rbase += tot;
   }
   __syncthreads();
-  for (int it = 0; it < P.iterations; it++) {
-    for (int g = 0; g < ngroups; g++) {
-      const int x1 = sstart[g + 1];
-      for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
-        const SCons q = sc[x];
-        const SBody B1 = sb[q.b1];
-        float f1[6], iI1[6];
-        for (int k = 0; k < 6; k++) { f1[k] = B1.fc[k]; iI1[k] = B1.iI[k]; }
-        const float im1 = B1.im;
-        const bool two = q.b2 >= 0;
-        float f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
-        if (two) {
-          const SBody B2 = sb[q.b2];
-          for (int k = 0; k < 6; k++) { f2[k] = B2.fc[k]; iI2[k] = B2.iI[k]; }
-          im2 = B2.im;
-        }
-        float lam0 = 0.f;
-        for (int k = 0; k < SOR_MAX_ROWS; k++) {
-          if (k >= q.m) break;
-          SRow &rr = sr[q.row0 + k];
-          const SRow r = rr;
-          float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] +
-                    r.a1[2] * f1[5];
-          if (two)
-            s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] +
-                 r.a2[2] * f2[5];
-          float old = r.lambda;
-          float delta = r.ad * (r.rhs - old * r.cfm - s);
-          float lo = r.lo, hi = r.hi;
-          if (r.findex >= 0) {
-            float ln = (r.findex == 0) ? lam0 : sr[q.row0 + r.findex].lambda;
-            hi = fabsf(r.hi * ln);
-            lo = -hi;
-          }
-          float nl = old + delta;
-          if (nl < lo) { delta = lo - old; nl = lo; }
-          else if (nl > hi) { delta = hi - old; nl = hi; }
-          rr.lambda = nl;
-          if (k == 0) lam0 = nl;
-          float d1 = delta * im1;
-          f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
-          f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
-          f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
-          f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
+  // fixed iteration count, or (tol > 0) iterate until the LCP residual of
+  // the world is at most tol, checked every check_every iterations, up to
+  // max_iterations
+  const bool use_tol = P.tol > 0.f;
+  const int maxit = use_tol ? P.max_iterations : P.iterations;
+  const int every = use_tol ? (P.check_every > 0 ? P.check_every : 1) : (maxit > 0 ? maxit : 1);
+  int it = 0;
+  while (it < maxit) {
+    const int n = every < maxit - it ? every : maxit - it;
+    for (int rep = 0; rep < n; rep++) {
+      for (int g = 0; g < ngroups; g++) {
+        const int x1 = sstart[g + 1];
+        for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
+          const SCons q = sc[x];
+          const SBody B1 = sb[q.b1];
+          float f1[6], iI1[6];
+          for (int k = 0; k < 6; k++) { f1[k] = B1.fc[k]; iI1[k] = B1.iI[k]; }
+          const float im1 = B1.im;
+          const bool two = q.b2 >= 0;
+          float f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
           if (two) {
-            float d2 = delta * im2;
-            f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
-            f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
-            f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
-            f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
+            const SBody B2 = sb[q.b2];
+            for (int k = 0; k < 6; k++) { f2[k] = B2.fc[k]; iI2[k] = B2.iI[k]; }
+            im2 = B2.im;
           }
+          float lam0 = 0.f;
+          for (int k = 0; k < SOR_MAX_ROWS; k++) {
+            if (k >= q.m) break;
+            SRow &rr = sr[q.row0 + k];
+            const SRow r = rr;
+            float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] +
+                      r.a1[2] * f1[5];
+            if (two)
+              s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] +
+                   r.a2[2] * f2[5];
+            float old = r.lambda;
+            float delta = r.ad * (r.rhs - old * r.cfm - s);
+            float lo = r.lo, hi = r.hi;
+            if (r.findex >= 0) {
+              float ln = (r.findex == 0) ? lam0 : sr[q.row0 + r.findex].lambda;
+              hi = fabsf(r.hi * ln);
+              lo = -hi;
+            }
+            float nl = old + delta;
+            if (nl < lo) { delta = lo - old; nl = lo; }
+            else if (nl > hi) { delta = hi - old; nl = hi; }
+            rr.lambda = nl;
+            if (k == 0) lam0 = nl;
+            float d1 = delta * im1;
+            f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
+            f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
+            f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
+            f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
+            if (two) {
+              float d2 = delta * im2;
+              f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
+              f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
+              f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
+              f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
+            }
+          }
+          for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
+          if (two)
+            for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
         }
-        for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
-        if (two)
-          for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
+        __syncthreads();
       }
-      __syncthreads();
+    }
+    it += n;
+    if (use_tol) {
+      float r = 0.f;
+      for (int x = threadIdx.x; x < nact; x += blockDim.x) r = fmaxf(r, sor_cons_residual(sb, sr, sc[x], P.sor_w));
+      r = block_max(r, sh);
+      if (r <= P.tol) break;
     }
   }
+  if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = it;
   // impulses back to the rows, fc back to the per-body scratch
   for (int x = threadIdx.x; x < nact; x += blockDim.x) {
     const Cons &q = cons[corder[x]];
@@ -1131,6 +1176,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
         for (int i = 0; i < m; i++) relax_row(rows, o[i], bt);
       }
     }
+    if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = iters;
   } else {
     sor_colored(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
   }

@@ -108,11 +108,49 @@ class Batch:
         return dict(pairs=c[0], contacts=c[1], rows=c[2], groups=c[3], overflow=c[4])
 
     def world_counters(self) -> np.ndarray:
-        """(nworlds, 5) int32: pairs, contacts, rows, islands/colours, overflow bits."""
-        out = np.zeros((self.nworlds, 5), np.int32)
+        """(nworlds, 6) int32: pairs, contacts, rows, islands/colours, overflow
+        bits, SOR iterations."""
+        out = np.zeros((self.nworlds, 6), np.int32)
         if not self.L.dBatchGetWorldCounters(self.h, out.ctypes.data_as(C.POINTER(C.c_int))):
             raise RuntimeError("dBatchGetWorldCounters failed")
         return out
+
+    def set_tolerance(self, tol: float, max_iterations: int = 400, check_every: int = 10):
+        """Coloured order: SOR runs until the LCP residual <= tol (0: fixed count)."""
+        if not self.L.dBatchSetSolverTolerance(self.h, tol, max_iterations, check_every):
+            raise ValueError("bad solver tolerance settings")
+
+    # ---- adjoint
+    def tape_options(self, checkpoint_interval: int = 10, act_tol: float = 1e-6, lin_tol: float = 1e-6,
+                     lin_maxit: int = 500):
+        if not self.L.dBatchTapeSetOptions(self.h, checkpoint_interval, act_tol, lin_tol, lin_maxit):
+            raise ValueError("bad tape options (or the tape is recording)")
+
+    def tape_begin(self, max_steps: int):
+        """Record the following steps for step_backward (checkpointed)."""
+        if not self.L.dBatchTapeBegin(self.h, max_steps):
+            raise RuntimeError("dBatchTapeBegin failed (see the error handler output)")
+
+    def tape_end(self):
+        self.L.dBatchTapeEnd(self.h)
+
+    def step_backward(self, nsteps: int, grad_out: np.ndarray):
+        """dL/d(state nsteps ago) and dL/d(mu), dL/d(bounce) per world, from
+        dL/d(current state) (nb x 13: pos, q, lvel, avel)."""
+        go = np.ascontiguousarray(grad_out, np.float32).reshape(self.nb, 13)
+        gi = np.zeros((self.nb, 13), np.float32)
+        gmu = np.zeros(self.nworlds, np.float32)
+        gb = np.zeros(self.nworlds, np.float32)
+        fp = C.POINTER(C.c_float)
+        if not self.L.dBatchStepBackward(self.h, nsteps, go.ctypes.data_as(fp), gi.ctypes.data_as(fp),
+                                         gmu.ctypes.data_as(fp), gb.ctypes.data_as(fp)):
+            raise RuntimeError("dBatchStepBackward failed (see the error handler output)")
+        return gi, gmu, gb
+
+    def tape_stats(self) -> dict:
+        out = (C.c_longlong * 2)()
+        self.L.dBatchTapeStats(self.h, out)
+        return dict(solve_iterations=out[0], collider_mismatches=out[1])
 
     def track_residual(self, on: bool = True):
         self.L.dBatchSetResidualTracking(self.h, 1 if on else 0)

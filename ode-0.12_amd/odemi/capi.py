@@ -61,6 +61,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchSetLaunchConfig": (C.c_int, [V, C.c_int, C.c_float]),
         "dBatchGetLds": (C.c_int, [V]),
         "dBatchGetWorldCounters": (C.c_int, [V, _ip]),
+        "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
         "dBatchGetSplit": (C.c_int, [V]),
         "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),
         "dBatchSetSplit": (C.c_int, [V, C.c_int]),
@@ -74,6 +75,8 @@ def load_library(path: str = LIB_PATH):
         "dBatchNumBodies": (C.c_int, [V]),
         "dBatchTapeBegin": (C.c_int, [V, C.c_int]),
         "dBatchTapeEnd": (C.c_int, [V]),
+        "dBatchTapeSetOptions": (C.c_int, [V, C.c_int, C.c_float, C.c_float, C.c_int]),
+        "dBatchTapeStats": (C.c_int, [V, C.POINTER(C.c_longlong)]),
         "dBatchStepBackward": (C.c_int, [V, C.c_int, _fp, _fp, _fp, _fp]),
         "odemiVersion": (C.c_char_p, []),
         "odemiDeviceCount": (C.c_int, []),

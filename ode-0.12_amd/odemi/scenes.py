@@ -230,15 +230,16 @@ def pile_world(rng: np.random.Generator, nb: int, n_chains: int, chain_len_lo: i
 
 
 def uneven_worlds(n_worlds: int = 4096, seed: int = 7, mean: float = 64.0, lo: int = 8, hi: int = 256,
-                  max_chains: int = 16) -> list[Scene]:
+                  max_chains: int = 16, dt: float = 0.005) -> list[Scene]:
     """Independent worlds with body count clamp(Poisson(64), 8, 256) plus
-    U{0..16} ball/hinge chains of U{4..12} links; h=0.01, mu=0.6."""
+    U{0..16} ball/hinge chains of U{4..12} links; the piles' step parameters
+    are those of the 4096-body pile: h=0.005, 20 iterations, mu=0.6."""
     rng = np.random.default_rng(seed)
     worlds = []
     for w in range(n_worlds):
         nb = int(np.clip(rng.poisson(mean), lo, hi))
         nch = int(rng.integers(0, max_chains + 1))
-        worlds.append(pile_world(rng, nb, nch, seed=int(rng.integers(0, 2 ** 31))))
+        worlds.append(pile_world(rng, nb, nch, dt=dt, seed=int(rng.integers(0, 2 ** 31))))
     return worlds
 
 

@@ -303,3 +303,66 @@ extern "C" void orc_plane_space(const double *n, double *p, double *q) {
   plane_space(nn, pp, qq);
   for (int k = 0; k < 3; k++) { p[k] = pp[k]; q[k] = qq[k]; }
 }
+
+// Step-by-step record of a run for the adjoint's checks: per step the body
+// state before the step (13 doubles per body: pos, q, lvel, avel) and every
+// contact of the step in creation order, 16 doubles each: pos 3, normal 3
+// (as the collider returned it), depth, impulses of its rows 3, g1, g2, b1,
+// b2 (after attach), flags, rows.  states holds (nsteps+1) records (the
+// last is the final state).  Returns the total contact count, or -1 when
+// max_total is exceeded.
+extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int *nk, double *contacts,
+                             long max_total) {
+  World<R> w;
+  build_world(s, w);
+  Surface<R> surf;
+  std::memset(&surf, 0, sizeof(surf));
+  surf.mode = s->surf_mode;
+  surf.mu = (R)s->mu;
+  surf.mu2 = (R)s->mu2;
+  surf.bounce = (R)s->bounce;
+  surf.bounce_vel = (R)s->bounce_vel;
+  surf.soft_erp = (R)s->soft_erp;
+  surf.soft_cfm = (R)s->soft_cfm;
+  w.rng.seed = s->seed;
+  std::vector<std::pair<int, int>> pairs;
+  std::vector<ContactGeom<R>> cbuf(s->max_contacts > 0 ? s->max_contacts : 1);
+  long total = 0;
+  for (int step = 0; step < nsteps; step++) {
+    record(w, states + (size_t)step * s->nb * 13);
+    w.hash_pairs(pairs);
+    std::vector<int> cj;
+    std::vector<ContactGeom<R>> cg;
+    for (auto &pr : pairs) {
+      int g1 = pr.first, g2 = pr.second;
+      int b1 = w.geoms[g1].body, b2 = w.geoms[g2].body;
+      if (s->skip_connected && b1 >= 0 && b2 >= 0 && w.connected_non_contact(b1, b2)) continue;
+      int n = collide_geoms(w.shape(g1), g1, w.shape(g2), g2, s->max_contacts, cbuf.data());
+      for (int i = 0; i < n; i++) {
+        cj.push_back(w.add_contact_joint(surf, cbuf[i], b1, b2));
+        cg.push_back(cbuf[i]);
+      }
+    }
+    w.quickstep((R)s->dt);
+    nk[step] = (int)cj.size();
+    if (total + (long)cj.size() > max_total) return -1;
+    for (size_t i = 0; i < cj.size(); i++) {
+      const Joint<R> &j = w.joints[cj[i]];
+      double *o = contacts + 16 * (size_t)(total + (long)i);
+      for (int k = 0; k < 3; k++) o[k] = cg[i].pos[k];
+      for (int k = 0; k < 3; k++) o[3 + k] = cg[i].normal[k];
+      o[6] = cg[i].depth;
+      for (int k = 0; k < 3; k++) o[7 + k] = k < j.nrows ? (double)j.lam[k] : 0.0;
+      o[10] = cg[i].g1;
+      o[11] = cg[i].g2;
+      o[12] = j.node0;
+      o[13] = j.node1;
+      o[14] = j.flags;
+      o[15] = j.nrows;
+    }
+    total += (long)cj.size();
+    w.empty_contacts();
+  }
+  record(w, states + (size_t)nsteps * s->nb * 13);
+  return total;
+}

@@ -115,6 +115,8 @@ template <class R> struct Joint {
   R susp_erp, susp_cfm;
   R v1[4] = {1, 0, 0, 0}, v2[4] = {0, 1, 0, 0};
   R c0 = 0, s0 = 0;
+  R lam[6] = {0, 0, 0, 0, 0, 0};   // impulses of the rows after the last step
+  int nrows = 0;
 };
 
 template <class R> struct WorldParams {
@@ -810,6 +812,8 @@ template <class R> struct World {
       int lo_ofs = 0;
       for (size_t a = 0; a < act.size(); a++) {
         Joint<R> &jt = joints[act[a]];
+        jt.nrows = jm[a];
+        for (int r = 0; r < jm[a] && r < 6; r++) jt.lam[r] = lambda[lo_ofs + r];
         if (jt.fb >= 0) {
           R d1[6] = {0, 0, 0, 0, 0, 0}, d2[6] = {0, 0, 0, 0, 0, 0};
           for (int r = 0; r < jm[a]; r++) {

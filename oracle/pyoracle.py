@@ -151,3 +151,21 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
                 contact_step_count=csc[:diag_steps].copy(), feedback=fb[:min(ncont, max_contacts)],
                 total_pairs=out.total_pairs, total_contacts=out.total_contacts,
                 sec_collide=out.sec_collide, sec_step=out.sec_step)
+
+
+def tape_run(lib, sc, nsteps: int, max_total: int = 200000) -> dict:
+    """Step-by-step record of a run (orc_tape_run): states before every step
+    (and the final one) and every step's contacts with their row impulses."""
+    fn = lib.orc_tape_run
+    fn.restype = C.c_long
+    fn.argtypes = [C.POINTER(SceneDesc), C.c_int, _dp, _ip, _dp, C.c_long]
+    desc, keep = make_desc(sc)
+    states = np.zeros((nsteps + 1, sc.nb, 13))
+    nk = np.zeros(nsteps, np.int32)
+    con = np.zeros((max_total, 16))
+    total = fn(C.byref(desc), nsteps, states.ctypes.data_as(_dp), nk.ctypes.data_as(_ip),
+               con.ctypes.data_as(_dp), max_total)
+    del keep
+    if total < 0:
+        raise RuntimeError("tape_run: contact buffer too small")
+    return dict(states=states, nk=nk, contacts=con[:total].copy())

@@ -21,3 +21,16 @@ def root():
 def golden(name):
     import numpy as np
     return dict(np.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz"), allow_pickle=False))
+
+
+def say(*a):
+    """Progress line naming the case (flushed before each GPU step)."""
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def odemi_mod():
+    import torch
+    assert torch.cuda.is_available(), "the GPU tests need an MI355X"
+    import odemi
+    return odemi

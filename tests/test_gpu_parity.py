@@ -16,22 +16,12 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import ROOT, golden
+from conftest import ROOT, golden, say
 from odemi import scenes
 
 pytestmark = pytest.mark.gpu
 
 
-def say(*a):
-    print(*a, flush=True)
-
-
-@pytest.fixture(scope="module")
-def odemi_mod():
-    import torch
-    assert torch.cuda.is_available()
-    import odemi
-    return odemi
 
 
 # ------------------------------------------------------------------ narrowphase

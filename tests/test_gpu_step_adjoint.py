@@ -1,0 +1,101 @@
+"""dBatchStepBackward on the GPU (FP32, through the C-ABI): gradients of the
+squared centre of mass after N steps with respect to initial velocities, mu
+and restitution, against FP64 central differences through the CPU
+restatement (converged solver) and against the host double adjoint; the
+checkpoint interval must not change the result; the torch.autograd.Function
+wrapper returns the same gradients."""
+import numpy as np
+import pytest
+
+from adjoint.harness import Rollout, com_loss, contact_scene, fd_grad, host_adjoint
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+N = 40
+TOL = 1e-3   # relative to the largest velocity gradient of the world
+
+
+def _scenes():
+    return [contact_scene(4, 4, (0,), bounce=0.5), contact_scene(5, 4, (1,)), contact_scene(6, 6, (0, 1))]
+
+
+def _gpu_grad(odemi_mod, scs, interval=10):
+    b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
+    b.set_tolerance(1e-6, 4000, 10)
+    b.tape_options(interval)
+    b.tape_begin(N)
+    b.step(N)
+    st = b.state().astype(np.float64)
+    finals, off = [], 0
+    for sc in scs:
+        finals.append(st[off:off + sc.nb])
+        off += sc.nb
+    L, gl = com_loss(scs, finals)
+    gi, gmu, gb = b.step_backward(N, np.concatenate(gl))
+    stats = b.tape_stats()
+    b.close()
+    return L, gi.astype(np.float64), gmu, gb, stats
+
+
+def test_gpu_adjoint_matches_central_differences_and_host(odemi_mod):
+    scs = _scenes()
+    say("adjoint: GPU backward over", N, "steps,", len(scs), "worlds")
+    L, gi, gmu, gb, stats = _gpu_grad(odemi_mod, scs)
+    say("adjoint stats", stats)
+    assert stats["collider_mismatches"] == 0
+    off = 0
+    for w, sc in enumerate(scs):
+        g = gi[off:off + sc.nb]
+        _, hg, hmu, hb, _ = host_adjoint(sc, N)
+        gmax = float(np.abs(hg[:, 7:13]).max())
+        err_host = float(np.abs(g[:, 7:13] - hg[:, 7:13]).max()) / gmax
+        worst = 0.0
+        for b in range(sc.nb):
+            for field, col in (("body_lvel", 7), ("body_avel", 10)):
+                k = (b + col) % 3
+                fd = fd_grad(sc, N, field, (b, k))
+                worst = max(worst, abs(fd - g[b, col + k]) / gmax)
+        fd_mu = fd_grad(sc, N, "mu")
+        say(f"world {w}: max rel err vs FD {worst:.2e}, vs host adjoint {err_host:.2e}, "
+            f"dL/dmu gpu {gmu[w]:.6g} fd {fd_mu:.6g}")
+        assert worst <= TOL and err_host <= TOL
+        assert abs(gmu[w] - fd_mu) <= TOL * max(gmax, abs(fd_mu))
+        if sc.bounce > 0:
+            fd_b = fd_grad(sc, N, "bounce")
+            assert abs(gb[w] - fd_b) <= TOL * max(gmax, abs(fd_b))
+        off += sc.nb
+
+
+def test_checkpoint_interval_does_not_change_gradient(odemi_mod):
+    scs = _scenes()
+    say("adjoint: checkpoint interval 3 vs 40")
+    a = _gpu_grad(odemi_mod, scs, interval=3)
+    b = _gpu_grad(odemi_mod, scs, interval=40)
+    scale = float(np.abs(a[1]).max())
+    assert np.abs(a[1] - b[1]).max() <= 1e-5 * scale
+    assert np.allclose(a[2], b[2], rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_torch_autograd_function(odemi_mod):
+    import torch
+    scs = _scenes()
+    say("adjoint: torch.autograd.Function")
+    Fn = Rollout.make(torch, odemi_mod, scs, N)
+    lv = torch.tensor(np.concatenate([s.body_lvel for s in scs]), dtype=torch.float64, requires_grad=True)
+    av = torch.tensor(np.concatenate([s.body_avel for s in scs]), dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor([s.mu for s in scs], dtype=torch.float64, requires_grad=True)
+    bo = torch.tensor([s.bounce for s in scs], dtype=torch.float64, requires_grad=True)
+    pos = Fn.apply(lv, av, mu, bo)
+    loss = 0
+    off = 0
+    for sc in scs:
+        m = torch.tensor(sc.body_mass, dtype=torch.float64)
+        com = (m[:, None] * pos[off:off + sc.nb]).sum(0) / m.sum()
+        loss = loss + (com ** 2).sum()
+        off += sc.nb
+    loss.backward()
+    _, gi, gmu, gb, _ = _gpu_grad(odemi_mod, scs)
+    assert np.allclose(lv.grad.numpy(), gi[:, 7:10], rtol=1e-4, atol=1e-6)
+    assert np.allclose(av.grad.numpy(), gi[:, 10:13], rtol=1e-4, atol=1e-6)
+    assert np.allclose(mu.grad.numpy(), gmu, rtol=1e-4, atol=1e-6)
