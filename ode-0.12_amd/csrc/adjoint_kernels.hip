@@ -57,6 +57,8 @@ __global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const l
   const Row *rows = (const Row *)(S + L.rows);
   int nk = B.counters[w * CNT_N + CNT_CONTACTS];
   if (nk > W.kcap) nk = W.kcap;
+  const int flags = B.counters[w * CNT_N + CNT_OVERFLOW];
+  if (flags & OVF_ROWS) nk = 0;   // the step left its rows unbuilt: nothing valid to record
   TapeContact<float> *dst = con + con_off[w];
   for (int i = threadIdx.x; i < nk; i += blockDim.x) {
     const ContactRec &c = cr[i];
@@ -66,12 +68,15 @@ __global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const l
     t.g1 = c.g1; t.g2 = c.g2; t.b1 = c.b1; t.b2 = c.b2; t.flags = c.flags;
     int m = cons[i].m;
     t.m = m;
-    for (int r = 0; r < 3; r++) t.lam[r] = (r < m) ? rows[cons[i].row0 + r].lambda : 0.f;
+    for (int r = 0; r < 3; r++) {
+      int ri = cons[i].row0 + r;
+      t.lam[r] = (r < m && ri < W.rcap) ? rows[ri].lambda : 0.f;
+    }
     dst[i] = t;
   }
   if (threadIdx.x == 0) {
     nk_out[w] = nk;
-    if (B.counters[w * CNT_N + CNT_OVERFLOW] & (OVF_PAIRS | OVF_CONTACTS | OVF_ROWS)) atomicOr(ovf, 1);
+    if (flags & (OVF_PAIRS | OVF_CONTACTS | OVF_ROWS)) atomicOr(ovf, 1);
   }
 }
 

@@ -371,9 +371,8 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
             int a, b;
             pkeys[o] = ref_pair_key(c, gi, gj, nref, a, b);
           } else {
-            pairs[o].g1 = gi < gj ? gi : gj;
-            pairs[o].g2 = gi < gj ? gj : gi;
-            pairs[o].key = 0;
+            pkeys[o] = gi < gj ? ((unsigned long long)gi << 32) | (unsigned)gj
+                               : ((unsigned long long)gj << 32) | (unsigned)gi;
           }
         }
       }
@@ -423,9 +422,8 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
               pkeys[o] = (3ULL << 62) | ((unsigned long long)r1 << 24) | (unsigned)r2;
             }
           } else {
-            pairs[o].g1 = g;
-            pairs[o].g2 = b;
-            pairs[o].key = 0;
+            pkeys[o] = g < b ? ((unsigned long long)g << 32) | (unsigned)b
+                             : ((unsigned long long)b << 32) | (unsigned)g;
           }
         }
         o++;
@@ -437,13 +435,24 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_PAIRS);
     npairs = c.W.pcap;
   }
-  if (ref) {
-    // sort the keys into the callback order and decode them
+  {
+    // sort the keys: in reference mode into the callback order, in coloured
+    // mode into (g1, g2) order with g1 < g2, so that the contact order (and
+    // with it the colouring) does not depend on the grid
     int np = pow2_at_least(npairs < 2 ? 2 : npairs);
     if (np > c.W.pcap) np = c.W.pcap;  // pcap is a power of two
     for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
     __syncthreads();
     bitonic_sort(pkeys, np, sh, B.lds_bytes);
+  }
+  if (!ref) {
+    for (int i = threadIdx.x; i < npairs; i += blockDim.x) {
+      unsigned long long k = pkeys[i];
+      pairs[i].g1 = (int)(k >> 32);
+      pairs[i].g2 = (int)(k & 0xffffffffu);
+      pairs[i].key = k;
+    }
+  } else {
     const int *gidx = c.at<int>(c.L.gidx);
     // inverse maps: hash index -> geom, big rank -> geom
     int *inv = c.at<int>(c.L.order);
@@ -1177,6 +1186,9 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
       }
     }
     if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = iters;
+    // islands are solved by lanes of every wavefront; the integration below
+    // reads the fc of all bodies
+    __syncthreads();
   } else {
     sor_colored(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
   }

@@ -7,6 +7,7 @@
 // scene_run() follows the same SceneDesc/SceneOut contract as the scene
 // driver (oracle/scene_driver.cpp), so the restatement, the reference build
 // and the MI355X library are interchangeable behind one harness.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -64,8 +65,21 @@ void record(const World<R> &w, double *dst) {
 
 }  // namespace
 
+// 1: the following runs solve in the coloured order of the MI355X coloured
+// mode instead of the reference order (not part of the reference).
+static int g_colored = 0;
+extern "C" void orc_set_colored(int on) { g_colored = on ? 1 : 0; }
+
+// the coloured mode's pair order: (g1, g2) with g1 < g2, sorted
+static void colored_pair_order(std::vector<std::pair<int, int>> &pairs) {
+  for (auto &p : pairs)
+    if (p.first > p.second) std::swap(p.first, p.second);
+  std::sort(pairs.begin(), pairs.end());
+}
+
 // Builds the world exactly as the scene driver does through the ODE API.
 static void build_world(const SceneDesc *s, World<R> &w) {
+  w.prm.colored = g_colored;
   for (int k = 0; k < 3; k++) w.prm.gravity[k] = (R)s->gravity[k];
   w.prm.erp = (R)s->erp;
   w.prm.cfm = (R)s->cfm;
@@ -188,6 +202,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     }
     auto t0 = clk::now();
     w.hash_pairs(pairs);
+    if (g_colored) colored_pair_order(pairs);
     int step_contacts = 0;
     for (auto &pr : pairs) {
       int g1 = pr.first, g2 = pr.second;
@@ -331,6 +346,7 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
   for (int step = 0; step < nsteps; step++) {
     record(w, states + (size_t)step * s->nb * 13);
     w.hash_pairs(pairs);
+    if (g_colored) colored_pair_order(pairs);
     std::vector<int> cj;
     std::vector<ContactGeom<R>> cg;
     for (auto &pr : pairs) {

@@ -126,6 +126,7 @@ template <class R> struct WorldParams {
   R w = R(1.3);
   R max_vel = INFINITY, min_depth = 0;
   int hash_min = -3, hash_max = 10;
+  int colored = 0;   // 1: graph-coloured Gauss-Seidel order (no shuffle), as the MI355X coloured mode
 };
 
 // row buffers of one island (quickstep.cpp:711-743)
@@ -802,7 +803,21 @@ template <class R> struct World {
       }
       for (int i = 0; i < m; i++) rhs[i] = c[i] * hinv - rhs[i];
       for (int i = 0; i < m; i++) cfm[i] *= hinv;
-      sor_lcp(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex);
+      if (prm.colored) {
+        // constraint ids of the coloured mode: contacts in creation order,
+        // then the permanent joints in creation order
+        const int nk = (int)joints.size() - n_perm;
+        std::vector<int> row0(act.size()), cid(act.size());
+        int o = 0;
+        for (size_t a = 0; a < act.size(); a++) {
+          row0[a] = o;
+          o += jm[a];
+          cid[a] = act[a] >= n_perm ? act[a] - n_perm : nk + act[a];
+        }
+        sor_colored(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex, row0, jm, cid);
+      } else {
+        sor_lcp(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex);
+      }
       for (int i = 0; i < nb; i++) {
         Body<R> &b = bodies[ib[i]];
         for (int k = 0; k < 3; k++) { b.lvel[k] += h * cforce[6 * i + k]; b.avel[k] += h * cforce[6 * i + 3 + k]; }
@@ -916,6 +931,92 @@ template <class R> struct World {
         if (f2) for (int k = 0; k < 6; k++) f2[k] += delta * M[6 + k];
       }
     }
+  }
+
+  // Graph-coloured Gauss-Seidel over the island's constraints (the order of
+  // the MI355X coloured mode, not a reference order): Jones-Plassmann
+  // colouring with priority hash(i) = (i * 2654435761) mod 2^31 of the
+  // constraint's id cid (contacts, then joints), colours swept in increasing order,
+  // each constraint's rows relaxed in row order with the per-row update of
+  // quickstep.cpp:484-582; no shuffle.
+  void sor_colored(int m, int nb, std::vector<R> &J, const std::vector<int> &jb, const std::vector<int> &ib,
+                   const std::vector<R> &invI, std::vector<R> &lambda, std::vector<R> &fc, std::vector<R> &b,
+                   const std::vector<R> &lo, const std::vector<R> &hi, const std::vector<R> &cfm,
+                   const std::vector<int> &findex, const std::vector<int> &row0, const std::vector<int> &jm,
+                   const std::vector<int> &cid) {
+    const int nc = (int)row0.size();
+    std::vector<R> iMJ(12 * (size_t)m, 0);
+    for (int i = 0; i < m; i++) {
+      int b1 = jb[2 * i], b2 = jb[2 * i + 1];
+      R k1 = bodies[ib[b1]].invMass;
+      for (int k = 0; k < 3; k++) iMJ[12 * i + k] = k1 * J[12 * i + k];
+      mv(&iMJ[12 * i + 3], &invI[12 * b1], &J[12 * i + 3]);
+      if (b2 != -1) {
+        R k2 = bodies[ib[b2]].invMass;
+        for (int k = 0; k < 3; k++) iMJ[12 * i + 6 + k] = k2 * J[12 * i + 6 + k];
+        mv(&iMJ[12 * i + 9], &invI[12 * b2], &J[12 * i + 9]);
+      }
+    }
+    std::fill(fc.begin(), fc.end(), R(0));
+    std::vector<R> Ad(m);
+    for (int i = 0; i < m; i++) {
+      R s = 0;
+      for (int k = 0; k < 6; k++) s += iMJ[12 * i + k] * J[12 * i + k];
+      if (jb[2 * i + 1] != -1) for (int k = 6; k < 12; k++) s += iMJ[12 * i + k] * J[12 * i + k];
+      Ad[i] = prm.w / (s + cfm[i]);
+    }
+    // colouring
+    std::vector<int> col(nc, -1);
+    std::vector<long long> bmax(nb);
+    int ncol = 0;
+    auto prio = [](int i) { return (long long)(((unsigned)i * 2654435761u) & 0x7fffffffu); };
+    while (true) {
+      std::fill(bmax.begin(), bmax.end(), -1LL);
+      int left = 0;
+      for (int i = 0; i < nc; i++) {
+        if (col[i] != -1 || jm[i] <= 0) continue;
+        left++;
+        int r = row0[i];
+        bmax[jb[2 * r]] = std::max(bmax[jb[2 * r]], prio(cid[i]));
+        if (jb[2 * r + 1] != -1) bmax[jb[2 * r + 1]] = std::max(bmax[jb[2 * r + 1]], prio(cid[i]));
+      }
+      if (!left) break;
+      for (int i = 0; i < nc; i++) {
+        if (col[i] != -1 || jm[i] <= 0) continue;
+        int r = row0[i];
+        if (bmax[jb[2 * r]] == prio(cid[i]) && (jb[2 * r + 1] == -1 || bmax[jb[2 * r + 1]] == prio(cid[i])))
+          col[i] = ncol;
+      }
+      ncol++;
+    }
+    for (int it = 0; it < prm.iterations; it++)
+      for (int g = 0; g < ncol; g++)
+        for (int i = 0; i < nc; i++) {
+          if (col[i] != g) continue;
+          for (int k = 0; k < jm[i]; k++) {
+            int r = row0[i] + k;
+            int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+            R *f1 = &fc[6 * (size_t)b1];
+            R *f2 = b2 != -1 ? &fc[6 * (size_t)b2] : nullptr;
+            const R *Jr = &J[12 * (size_t)r];
+            R s = f1[0] * Jr[0] + f1[1] * Jr[1] + f1[2] * Jr[2] + f1[3] * Jr[3] + f1[4] * Jr[4] + f1[5] * Jr[5];
+            if (f2) s += f2[0] * Jr[6] + f2[1] * Jr[7] + f2[2] * Jr[8] + f2[3] * Jr[9] + f2[4] * Jr[10] + f2[5] * Jr[11];
+            R old = lambda[r];
+            R delta = Ad[r] * (b[r] - old * cfm[r] - s);
+            R lo_a, hi_a;
+            if (findex[r] != -1) {
+              hi_a = std::fabs(hi[r] * lambda[findex[r]]);
+              lo_a = -hi_a;
+            } else { hi_a = hi[r]; lo_a = lo[r]; }
+            R nl = old + delta;
+            if (nl < lo_a) { delta = lo_a - old; nl = lo_a; }
+            else if (nl > hi_a) { delta = hi_a - old; nl = hi_a; }
+            lambda[r] = nl;
+            const R *M = &iMJ[12 * (size_t)r];
+            for (int q = 0; q < 6; q++) f1[q] += delta * M[q];
+            if (f2) for (int q = 0; q < 6; q++) f2[q] += delta * M[6 + q];
+          }
+        }
   }
 
   // dxStepBody, default flags: ode/src/util.cpp:255-360

@@ -264,3 +264,49 @@ def test_free_fall_closed_form():
     z = 50.0 + sum(-9.81 * h * (k + 1) * h for k in range(n))
     assert math.isclose(r["traj"][1][0, 9], vz, rel_tol=1e-12)
     assert math.isclose(r["traj"][1][0, 2], z, rel_tol=1e-12)
+
+
+def test_coloured_order_converges_to_the_reference_orders_solution():
+    """The coloured-order restatement (the order of the MI355X coloured
+    mode) and the reference order solve the same LCP: with the solver run to
+    convergence, one step from the same state gives the same contact
+    impulses (row order only changes the iterates)."""
+    from odemi import scenes
+    lib64 = po.oracle_lib("f64")
+    rng = np.random.default_rng(12)
+    for seed in (1, 2):
+        sc = scenes.pile_world(rng, 24, 2, seed=seed)
+        # advance to a state with resting contacts, then one converged step each way
+        st = po.run(lib64, sc, 60, 60)["traj"][-1]
+        sc.iterations = 100000
+        sc.body_pos, sc.body_q = st[:, :3].copy(), st[:, 3:7].copy()
+        sc.body_lvel, sc.body_avel = st[:, 7:10].copy(), st[:, 10:13].copy()
+        try:
+            lib64.orc_set_colored(1)
+            tc = po.tape_run(lib64, sc, 1)
+        finally:
+            lib64.orc_set_colored(0)
+        tr = po.tape_run(lib64, sc, 1)
+        assert tc["nk"][0] == tr["nk"][0] > 0
+        # contacts come in different orders: match them by (g1, g2, position)
+        def keyed(con):
+            out = {}
+            for c in con:
+                g = (min(int(c[10]), int(c[11])), max(int(c[10]), int(c[11])))
+                out.setdefault(g, []).append(c)
+            return out
+        kc, kr = keyed(tc["contacts"]), keyed(tr["contacts"])
+        assert kc.keys() == kr.keys()
+        scale = max(1.0, float(np.abs(tr["contacts"][:, 7]).max()))
+        for g in kr:
+            for c in kc[g]:
+                d = [np.abs(c[:3] - r[:3]).max() for r in kr[g]]
+                r = kr[g][int(np.argmin(d))]
+                assert min(d) < 1e-9
+                # the normal may be flipped when the pair order differs: compare the
+                # normal impulse (sign-free) and the friction impulse magnitude
+                assert abs(c[7] - r[7]) <= 1e-6 * scale, (g, c[7], r[7])
+                assert abs(np.hypot(c[8], c[9]) - np.hypot(r[8], r[9])) <= 1e-6 * scale
+        # and the bodies end in the same state
+        fc, fr = tc["states"][-1], tr["states"][-1]
+        assert np.abs(fc[:, 7:13] - fr[:, 7:13]).max() <= 1e-6 * max(1.0, np.abs(fr[:, 7:13]).max())
