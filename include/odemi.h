@@ -348,13 +348,19 @@ ODE_API int dBatchSetState(dBatchID b, const float *in);
 ODE_API int dBatchGetCounters(dBatchID b, long long *out5);
 /* Counters of the last step for every world, 6 ints per world: pairs,
  * contacts, rows, islands or colours, overflow bits (1 grid cells, 2 pairs,
- * 4 contacts, 8 rows), SOR iterations run. */
+ * 4 contacts, 8 rows, 16 non-finite state: the world is frozen), SOR
+ * iterations run. */
 ODE_API int dBatchGetWorldCounters(dBatchID b, int *out);
 /* Coloured order only: iterate each world's SOR until its LCP residual is
  * at most tol (checked every check_every iterations, at most max_iterations);
  * tol = 0 restores the fixed iteration count of the world.  The reference
  * order always runs the fixed count, as dWorldQuickStep does. */
 ODE_API int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, int check_every);
+/* As dBatchSetSolverTolerance; accel = 1 adds a nonsmooth nonlinear
+ * conjugate-gradient step after every sweep (same fixed point, fewer sweeps
+ * on stacked contacts).  A world whose state becomes non-finite is frozen
+ * and flagged (overflow bit 16) until dBatchSetState. */
+ODE_API int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterations, int check_every, int accel);
 /* Enables the LCP complementarity residual of the last step per world
  * (max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|). */
 ODE_API int dBatchSetResidualTracking(dBatchID b, int on);

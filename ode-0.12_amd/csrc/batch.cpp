@@ -349,10 +349,13 @@ static int persistent_grid(BatchImpl *b) {
 
 // one step of every world: one fused kernel, or collide and step kernels
 static hipError_t launch_step(BatchImpl *b, int first_world, int nworlds, int grid) {
-  if (!b->split) return launch_world_kernel(KM_FUSED, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
+  if (!b->split)
+    return launch_world_kernel(b->tol_on ? KM_FUSED_TOL : KM_FUSED, b->dev, first_world, nworlds, grid, b->wg, 0,
+                               b->stream, 0);
   hipError_t e = launch_world_kernel(KM_BATCH_COLLIDE, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
   if (e != hipSuccess) return e;
-  return launch_world_kernel(KM_BATCH_STEP, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 1);
+  return launch_world_kernel(b->tol_on ? KM_BATCH_STEP_TOL : KM_BATCH_STEP, b->dev, first_world, nworlds, grid,
+                             b->wg, 0, b->stream, 1);
 }
 
 static int build_graph(BatchImpl *b) {
@@ -439,6 +442,9 @@ int batch_get_state(BatchImpl *b, float *out) {
   return 1;
 }
 
+template <class F>
+static int patch_params(BatchImpl *b, F f);
+
 int batch_set_state(BatchImpl *b, const float *in) {
   OD_HIP(hipStreamSynchronize(b->stream));
   std::vector<float4> p(b->nb), q(b->nb), v(b->nb), w(b->nb);
@@ -458,7 +464,8 @@ int batch_set_state(BatchImpl *b, const float *in) {
   refresh_dev(b);
   if (launch_init_rot(b->dev, b->nb, b->stream) != hipSuccess) return 0;
   OD_HIP(hipStreamSynchronize(b->stream));
-  return 1;
+  // a new state revives frozen worlds
+  return patch_params(b, [](WorldParams &p) { p.dead = 0; });
 }
 
 int batch_counters(BatchImpl *b, long long *out5) {
@@ -572,13 +579,18 @@ static int patch_params(BatchImpl *b, F f) {
   return 1;
 }
 
-int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every) {
+int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every, int accel) {
   if (!(tol >= 0.f) || (tol > 0.f && (max_iterations <= 0 || check_every <= 0))) return 0;
-  return patch_params(b, [&](WorldParams &p) {
-    p.tol = tol;
-    p.max_iterations = max_iterations;
-    p.check_every = check_every;
-  });
+  if (!patch_params(b, [&](WorldParams &p) {
+        p.tol = tol;
+        p.max_iterations = max_iterations;
+        p.check_every = check_every;
+        p.accel = accel ? 1 : 0;
+      }))
+    return 0;
+  b->tol_on = tol > 0.f;
+  destroy_graph(b);
+  return 1;
 }
 
 static int set_cell_scale(BatchImpl *b, float s) {
@@ -751,7 +763,10 @@ int dBatchNumBodies(dBatchID b) { return b ? b->impl->nb : 0; }
 
 int dBatchGetLds(dBatchID b) { return b ? b->impl->lds : -1; }
 int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, int check_every) {
-  return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every) : 0;
+  return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every, 0) : 0;
+}
+int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterations, int check_every, int accel) {
+  return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every, accel) : 0;
 }
 int dBatchGetWorldCounters(dBatchID b, int *out) {
   if (!b || !out) return 0;

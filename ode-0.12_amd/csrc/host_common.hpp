@@ -266,7 +266,9 @@ enum {
   KM_BATCH_STEP = 2,      // quickstep on the contacts of KM_BATCH_COLLIDE
   KM_FUSED = 3,           // both in one kernel
   KM_API_COLLIDE = 5,     // one world, pairs and contacts copied out (dSpaceCollide)
-  KM_API_STEP = 10        // one world, contact joints from the host (dWorldQuickStep)
+  KM_API_STEP = 10,       // one world, contact joints from the host (dWorldQuickStep)
+  KM_BATCH_STEP_TOL = 18, // KM_BATCH_STEP honouring the solver tolerance
+  KM_FUSED_TOL = 19       // KM_FUSED honouring the solver tolerance
 };
 hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid, int wg,
                                int api_maxc, hipStream_t st, int queue = 0);

@@ -33,7 +33,7 @@ enum {
 };
 // per-world counters written by the step kernels
 enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_ITERS, CNT_N };
-enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8 };
+enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8, OVF_NONFINITE = 16 };
 
 static const int MAX_SLOTS = 8;   // most contacts a collider produces for one pair
 
@@ -66,6 +66,8 @@ struct WorldParams {
   float tol;               // coloured solver: stop once the LCP residual <= tol (0: fixed iterations)
   int max_iterations;      // iteration cap when tol > 0
   int check_every;         // iterations between residual checks when tol > 0
+  int accel;               // with tol: nonsmooth nonlinear conjugate-gradient steps between sweeps
+  int dead;                // set by the step when the world's state became non-finite; the world is frozen
 };
 
 struct Limot {             // ode/src/joints/joint.h:172-190
@@ -153,6 +155,13 @@ struct alignas(16) SBody {
   float pad[3];
 };
 
+// per-body state of the accelerated coloured solver: fc before the sweep and
+// the search direction's fc (M^-1 J^T p)
+struct alignas(16) SAccBody {
+  float fp[6];
+  float fdir[6];
+};
+
 // per-constraint record of the coloured solver, constraints sorted by colour
 // and their rows stored contiguously in that order
 struct alignas(16) SCons {
@@ -178,9 +187,11 @@ OD_HD inline long long od_pow2(long long n) {
 
 // bytes of the coloured solver's working set: bodies, rows, constraints,
 // colour order and colour starts
-OD_HD inline long long sor_bytes(int nb, int nrows, int ncons) {
-  return (long long)sizeof(SBody) * nb + (long long)sizeof(SRow) * nrows + (long long)sizeof(SCons) * ncons +
-         4LL * (2LL * ncons + 8);
+OD_HD inline long long sor_bytes(int nb, int nrows, int ncons, bool accel = false) {
+  long long b = (long long)sizeof(SBody) * nb + (long long)sizeof(SRow) * nrows + (long long)sizeof(SCons) * ncons +
+                4LL * (2LL * ncons + 8);
+  if (accel) b += (long long)sizeof(SAccBody) * nb + 8LL * nrows;   // conjugate-gradient state
+  return b;
 }
 
 // Per-world scratch layout.  nb/ng/nj: world sizes; pcap (a power of two),
@@ -211,7 +222,7 @@ OD_HD inline ScratchLayout scratch_layout(int nb, int ng, int nj, int pcap, int 
   L.island = o;   o = od_align(o + 4LL * (6LL * nb + 2 * ncons + 3LL * ng + 16));
   L.color = o;    o = od_align(o + 4LL * (ncons + nb));
   L.misc = o;     o = od_align(o + 4LL * 64);
-  L.sor = o;      o = od_align(o + sor_bytes(nb, rcap, (int)ncons));
+  L.sor = o;      o = od_align(o + sor_bytes(nb, rcap, (int)ncons, true));
   L.total = o;
   return L;
 }

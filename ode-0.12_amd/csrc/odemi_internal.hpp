@@ -58,6 +58,7 @@ struct BatchImpl {
   int lds_max = 65536;    // largest pool one workgroup may request
   bool prof_on = false;   // per-phase wall-clock counters
   bool split = false;     // collide and quickstep as two kernels per step
+  bool tol_on = false;    // some world iterates to a residual tolerance
   int wall_khz = 100000;  // wall clock rate
   Tape tape;              // forward tape for the adjoint
 };
@@ -87,6 +88,7 @@ int batch_counters(BatchImpl *b, long long *out5);
 int batch_set_controls(BatchImpl *b, const double *ctrl);
 int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid);
 int batch_set_lds(BatchImpl *b, int bytes);
+int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every, int accel);
 int batch_set_profiling(BatchImpl *b, bool on);
 int batch_phase_times(BatchImpl *b, double *ms);
 int batch_autotune(BatchImpl *b, int trial_steps);

@@ -95,6 +95,18 @@ __device__ inline float block_max(float v, Shared &sh) {
   return m;
 }
 
+__device__ inline float block_fsum(float v, Shared &sh) {
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) sh.wmax[wid] = v;
+  __syncthreads();
+  int nw = (blockDim.x + 63) >> 6;
+  float m = 0.f;
+  for (int k = 0; k < nw; k++) m += sh.wmax[k];
+  __syncthreads();
+  return m;
+}
+
 __device__ inline int block_sum(int v, Shared &sh) {
   int t;
   block_excl_scan(v, t, sh);
@@ -642,16 +654,24 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
   return r;
 }
 
+template <bool TOL>
 __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
                             float *bt, const int *corder, const int *cstart, int nrows, int ngroups, Shared &sh) {
   const int nb = c.W.nb;
   const int nact = cstart[ngroups];   // constraints with rows, in colour order
-  const long long need = sor_bytes(nb, nrows, nact);
+  const bool use_tol = TOL && P.tol > 0.f;
+  const bool accel = use_tol && P.accel != 0;
+  const long long need = sor_bytes(nb, nrows, nact, accel);
   char *base = (need <= B.lds_bytes) ? (char *)od_pool : c.at<char>(c.L.sor);
   SBody *sb = (SBody *)base;
   SRow *sr = (SRow *)(sb + nb);
   SCons *sc = (SCons *)(sr + nrows);
   int *sstart = (int *)(sc + nact);
+  // conjugate-gradient state (accelerated mode): per body fc before the
+  // sweep and the direction's fc; per row lambda before the sweep and the
+  // direction
+  SAccBody *sa = (SAccBody *)((char *)sstart + ((4LL * (2LL * nact + 8) + 15) & ~15LL));
+  float2 *ld = (float2 *)(sa + nb);
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     const float *t = bt + 32 * b;
     SBody q;
@@ -662,6 +682,11 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     sb[b] = q;
   }
   for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sstart[k] = cstart[k];
+  if (accel) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      for (int k = 0; k < 6; k++) sa[b].fdir[k] = 0.f;
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) ld[r] = make_float2(0.f, 0.f);
+  }
   // constraints and rows in colour order
   int rbase = 0;
   for (int x0 = 0; x0 < nact; x0 += blockDim.x) {
@@ -691,14 +716,22 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   __syncthreads();
   // fixed iteration count, or (tol > 0) iterate until the LCP residual of
   // the world is at most tol, checked every check_every iterations, up to
-  // max_iterations
-  const bool use_tol = P.tol > 0.f;
+  // max_iterations; accelerated: after each sweep a nonsmooth nonlinear
+  // conjugate-gradient step (Silcowitz, Niebe, Erleben 2010): with
+  // d = lambda_after - lambda_before, beta = |d|^2/|d_prev|^2 (0 if > 1),
+  // lambda += beta p, p = d + beta p, and fc follows through M^-1 J^T p
   const int maxit = use_tol ? P.max_iterations : P.iterations;
   const int every = use_tol ? (P.check_every > 0 ? P.check_every : 1) : (maxit > 0 ? maxit : 1);
+  float rr_prev = 0.f;
   int it = 0;
   while (it < maxit) {
     const int n = every < maxit - it ? every : maxit - it;
     for (int rep = 0; rep < n; rep++) {
+      if (accel) {
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+          for (int k = 0; k < 6; k++) sa[b].fp[k] = sb[b].fc[k];
+        __syncthreads();
+      }
       for (int g = 0; g < ngroups; g++) {
         const int x1 = sstart[g + 1];
         for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
@@ -725,6 +758,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
               s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] +
                    r.a2[2] * f2[5];
             float old = r.lambda;
+            if (accel) ld[q.row0 + k].x = old;
             float delta = r.ad * (r.rhs - old * r.cfm - s);
             float lo = r.lo, hi = r.hi;
             if (r.findex >= 0) {
@@ -756,6 +790,31 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         }
         __syncthreads();
       }
+      if (accel) {
+        float loc = 0.f;
+        for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+          float d = sr[r].lambda - ld[r].x;
+          loc += d * d;
+        }
+        const float rr = block_fsum(loc, sh);
+        float beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
+        if (!(beta <= 1.f)) beta = 0.f;
+        for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+          float d = sr[r].lambda - ld[r].x;
+          float dl = beta * ld[r].y;
+          ld[r].y = d + dl;
+          if (dl != 0.f) sr[r].lambda += dl;
+        }
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+          for (int k = 0; k < 6; k++) {
+            float dfc = sb[b].fc[k] - sa[b].fp[k];
+            float dfl = beta * sa[b].fdir[k];
+            sa[b].fdir[k] = dfc + dfl;
+            sb[b].fc[k] += dfl;
+          }
+        rr_prev = rr;
+        __syncthreads();
+      }
     }
     it += n;
     if (use_tol) {
@@ -777,6 +836,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   __syncthreads();
 }
 
+template <bool TOL>
 __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, const StepIn &in, Shared &sh,
                            bool ref) {
   const int nb = c.W.nb, nj = c.W.nj, nk = in.nk;
@@ -1190,7 +1250,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     // reads the fc of all bodies
     __syncthreads();
   } else {
-    sor_colored(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
+    sor_colored<TOL>(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
   }
   c.clk.lap(PH_SOR);
   // --- residual (optional), feedback
@@ -1217,6 +1277,8 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     }
   }
   // --- velocity update, integration (quickstep.cpp:905-1021, util.cpp:255-360)
+  if (threadIdx.x == 0) sh.ival[13] = 0;
+  __syncthreads();
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     int gb = c.W.b0 + b;
     float *t = bt + 32 * b;
@@ -1242,6 +1304,9 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     B.avel[gb] = f4(w);
     B.facc[gb] = make_float4(0.f, 0.f, 0.f, 0.f);
     B.tacc[gb] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.x) && isfinite(v.x) && isfinite(v.y) &&
+          isfinite(v.z) && isfinite(w.x) && isfinite(w.y) && isfinite(w.z)))
+      sh.ival[13] = 1;
   }
   // --- space list after dGeomMoved of every stepped body: moved geoms move
   //     to the front in reverse move order (collision_space.cpp:47-75, 209-213)
@@ -1275,6 +1340,11 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     B.counters[c.w * CNT_N + CNT_GROUPS] = ngroups;
   }
   __syncthreads();
+  // a world whose state became non-finite is frozen (and reported) from now on
+  if (threadIdx.x == 0 && sh.ival[13]) {
+    P.dead = 1;
+    atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_NONFINITE);
+  }
   c.clk.lap(PH_INTEGRATE);
 }
 
@@ -1312,13 +1382,17 @@ __device__ void phase_controls(const DevBatch &B, const Ctx &c) {
 // created (dWorldQuickStep).  Batch steps run COLLIDE|STEP in one kernel, or
 // COLLIDE then STEP as two kernels with the contact count passed through
 // the world's counters.
-enum { MF_COLLIDE = 1, MF_STEP = 2, MF_API_OUT = 4, MF_HOST_CONTACTS = 8 };
+// TOL: the coloured solver honours the world's residual tolerance (its own
+// instantiation, so the iterate-to-tolerance step is a separate kernel).
+enum { MF_COLLIDE = 1, MF_STEP = 2, MF_API_OUT = 4, MF_HOST_CONTACTS = 8, MF_TOL = 16 };
 enum {
   MODE_BATCH_COLLIDE = MF_COLLIDE,
   MODE_BATCH_STEP = MF_STEP,
   MODE_FUSED = MF_COLLIDE | MF_STEP,
   MODE_API_COLLIDE = MF_COLLIDE | MF_API_OUT,
-  MODE_API_STEP = MF_STEP | MF_HOST_CONTACTS
+  MODE_API_STEP = MF_STEP | MF_HOST_CONTACTS,
+  MODE_BATCH_STEP_TOL = MF_STEP | MF_TOL,
+  MODE_FUSED_TOL = MF_COLLIDE | MF_STEP | MF_TOL
 };
 
 template <int MODE>
@@ -1332,6 +1406,11 @@ __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
   WorldParams &P = B.wp[w];
   const bool ref = B.order_mode == ORDER_REFERENCE;
   constexpr bool api = (MODE & (MF_API_OUT | MF_HOST_CONTACTS)) != 0;
+  if (!api && P.dead) {
+    // frozen after its state became non-finite
+    if (threadIdx.x == 0) B.counters[w * CNT_N + CNT_OVERFLOW] = OVF_NONFINITE;
+    return;
+  }
   // the first kernel of a step clears the overflow flags
   if (((MODE & MF_COLLIDE) || api) && threadIdx.x == 0) B.counters[w * CNT_N + CNT_OVERFLOW] = 0;
   if ((MODE & MF_COLLIDE) && !api) phase_controls(B, c);
@@ -1371,7 +1450,7 @@ __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
       if (threadIdx.x == 0) atomicOr(&B.counters[w * CNT_N + CNT_OVERFLOW], OVF_CONTACTS);
       in.nk = c.W.kcap;
     }
-    phase_step(B, c, P, in, sh, ref);
+    phase_step<(MODE & MF_TOL) != 0>(B, c, P, in, sh, ref);
   }
   __syncthreads();
 }
@@ -1433,12 +1512,15 @@ static const void *kernel_of(int mode) {
     case MODE_FUSED: return world_kernel_fn<MODE_FUSED>();
     case MODE_API_COLLIDE: return world_kernel_fn<MODE_API_COLLIDE>();
     case MODE_API_STEP: return world_kernel_fn<MODE_API_STEP>();
+    case MODE_BATCH_STEP_TOL: return world_kernel_fn<MODE_BATCH_STEP_TOL>();
+    case MODE_FUSED_TOL: return world_kernel_fn<MODE_FUSED_TOL>();
     default: return nullptr;
   }
 }
 
 hipError_t allow_world_kernel_lds(int bytes) {
-  const int modes[] = {MODE_BATCH_COLLIDE, MODE_BATCH_STEP, MODE_FUSED, MODE_API_COLLIDE, MODE_API_STEP};
+  const int modes[] = {MODE_BATCH_COLLIDE, MODE_BATCH_STEP, MODE_FUSED, MODE_API_COLLIDE, MODE_API_STEP,
+                       MODE_BATCH_STEP_TOL, MODE_FUSED_TOL};
   for (int m : modes) {
     hipError_t e = hipFuncSetAttribute(kernel_of(m), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
@@ -1486,6 +1568,12 @@ hipError_t launch_world_kernel(int mode, const DevBatch &B0, int first_world, in
       break;
     case MODE_API_STEP:
       world_kernel<MODE_API_STEP><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_BATCH_STEP_TOL:
+      world_kernel<MODE_BATCH_STEP_TOL><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_FUSED_TOL:
+      world_kernel<MODE_FUSED_TOL><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
       break;
     default: return hipErrorInvalidValue;
   }

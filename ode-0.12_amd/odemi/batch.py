@@ -115,9 +115,10 @@ class Batch:
             raise RuntimeError("dBatchGetWorldCounters failed")
         return out
 
-    def set_tolerance(self, tol: float, max_iterations: int = 400, check_every: int = 10):
-        """Coloured order: SOR runs until the LCP residual <= tol (0: fixed count)."""
-        if not self.L.dBatchSetSolverTolerance(self.h, tol, max_iterations, check_every):
+    def set_tolerance(self, tol: float, max_iterations: int = 400, check_every: int = 10, accel: bool = False):
+        """Coloured order: SOR runs until the LCP residual <= tol (0: fixed
+        count); accel adds the conjugate-gradient step between sweeps."""
+        if not self.L.dBatchSetSolverToleranceAccel(self.h, tol, max_iterations, check_every, 1 if accel else 0):
             raise ValueError("bad solver tolerance settings")
 
     # ---- adjoint

@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchGetLds": (C.c_int, [V]),
         "dBatchGetWorldCounters": (C.c_int, [V, _ip]),
         "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
+        "dBatchSetSolverToleranceAccel": (C.c_int, [V, C.c_float, C.c_int, C.c_int, C.c_int]),
         "dBatchGetSplit": (C.c_int, [V]),
         "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),
         "dBatchSetSplit": (C.c_int, [V, C.c_int]),

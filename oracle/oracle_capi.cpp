@@ -69,6 +69,16 @@ void record(const World<R> &w, double *dst) {
 // mode instead of the reference order (not part of the reference).
 static int g_colored = 0;
 extern "C" void orc_set_colored(int on) { g_colored = on ? 1 : 0; }
+// coloured mode: iterate to a residual tolerance, optionally with the
+// conjugate-gradient acceleration; the sweep count of the runs since the
+// last call is returned by orc_colored_sweeps
+static double g_tol = 0;
+static int g_maxit = 0, g_check = 10, g_accel = 0;
+static long long g_sweeps = 0;
+extern "C" void orc_set_colored_solver(double tol, int max_iterations, int check_every, int accel) {
+  g_tol = tol; g_maxit = max_iterations; g_check = check_every > 0 ? check_every : 1; g_accel = accel;
+}
+extern "C" long long orc_colored_sweeps(void) { long long s = g_sweeps; g_sweeps = 0; return s; }
 
 // the coloured mode's pair order: (g1, g2) with g1 < g2, sorted
 static void colored_pair_order(std::vector<std::pair<int, int>> &pairs) {
@@ -80,6 +90,10 @@ static void colored_pair_order(std::vector<std::pair<int, int>> &pairs) {
 // Builds the world exactly as the scene driver does through the ODE API.
 static void build_world(const SceneDesc *s, World<R> &w) {
   w.prm.colored = g_colored;
+  w.prm.tol = (R)g_tol;
+  w.prm.max_iterations = g_maxit;
+  w.prm.check_every = g_check;
+  w.prm.accel = g_accel;
   for (int k = 0; k < 3; k++) w.prm.gravity[k] = (R)s->gravity[k];
   w.prm.erp = (R)s->erp;
   w.prm.cfm = (R)s->cfm;
@@ -248,6 +262,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
   }
   if (o->feedback)
     for (size_t k = 0; k < 12 * (size_t)o->max_contacts; k++) o->feedback[k] = (double)w.fbout[k];
+  g_sweeps += w.prm.sweeps;
   return o->status;
 }
 

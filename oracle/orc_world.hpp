@@ -127,6 +127,12 @@ template <class R> struct WorldParams {
   R max_vel = INFINITY, min_depth = 0;
   int hash_min = -3, hash_max = 10;
   int colored = 0;   // 1: graph-coloured Gauss-Seidel order (no shuffle), as the MI355X coloured mode
+  // coloured mode only: iterate until the LCP residual <= tol (checked every
+  // check_every sweeps, at most max_iterations); accel = 1 adds the
+  // nonsmooth nonlinear conjugate-gradient step between sweeps
+  R tol = 0;
+  int max_iterations = 0, check_every = 10, accel = 0;
+  long long sweeps = 0;   // sweeps run (statistics)
 };
 
 // row buffers of one island (quickstep.cpp:711-743)
@@ -989,7 +995,7 @@ template <class R> struct World {
       }
       ncol++;
     }
-    for (int it = 0; it < prm.iterations; it++)
+    auto sweep = [&]() {
       for (int g = 0; g < ncol; g++)
         for (int i = 0; i < nc; i++) {
           if (col[i] != g) continue;
@@ -1017,6 +1023,57 @@ template <class R> struct World {
             if (f2) for (int q = 0; q < 6; q++) f2[q] += delta * M[6 + q];
           }
         }
+    };
+    // LCP residual max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|
+    auto residual = [&]() {
+      R worst = 0;
+      for (int r = 0; r < m; r++) {
+        int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+        const R *Jr = &J[12 * (size_t)r];
+        const R *f1 = &fc[6 * (size_t)b1];
+        R s = 0;
+        for (int q = 0; q < 6; q++) s += Jr[q] * f1[q];
+        if (b2 != -1) for (int q = 0; q < 6; q++) s += Jr[6 + q] * fc[6 * (size_t)b2 + q];
+        R y = s + cfm[r] * lambda[r] - b[r];
+        R lo_a = lo[r], hi_a = hi[r];
+        if (findex[r] != -1) { hi_a = std::fabs(hi[r] * lambda[findex[r]]); lo_a = -hi_a; }
+        R p = lambda[r] - y * (Ad[r] / prm.w);
+        p = std::min(std::max(p, lo_a), hi_a);
+        worst = std::max(worst, std::fabs(lambda[r] - p));
+      }
+      return worst;
+    };
+    const bool use_tol = prm.tol > 0;
+    const int maxit = use_tol ? prm.max_iterations : prm.iterations;
+    std::vector<R> lprev(m, 0), dir(m, 0);
+    R rr_prev = 0;
+    for (int it = 0; it < maxit; it++) {
+      if (prm.accel) for (int r = 0; r < m; r++) lprev[r] = lambda[r];
+      sweep();
+      prm.sweeps++;
+      if (prm.accel) {
+        // nonsmooth nonlinear CG: r = PGS(l) - l, beta = |r|^2 / |r_prev|^2,
+        // restart when beta > 1; l += beta p, p = r + beta p
+        R rr = 0;
+        for (int r = 0; r < m; r++) { R d = lambda[r] - lprev[r]; rr += d * d; }
+        R beta = (it > 0 && rr_prev > 0) ? rr / rr_prev : 0;
+        if (beta > 1) beta = 0;
+        for (int r = 0; r < m; r++) {
+          R d = lambda[r] - lprev[r];
+          R dl = beta * dir[r];                 // momentum along the previous direction
+          dir[r] = d + dl;
+          if (dl != 0) {
+            lambda[r] += dl;
+            int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+            const R *M = &iMJ[12 * (size_t)r];
+            for (int q = 0; q < 6; q++) fc[6 * (size_t)b1 + q] += dl * M[q];
+            if (b2 != -1) for (int q = 0; q < 6; q++) fc[6 * (size_t)b2 + q] += dl * M[6 + q];
+          }
+        }
+        rr_prev = rr;
+      }
+      if (use_tol && (it + 1) % prm.check_every == 0 && residual() <= prm.tol) break;
+    }
   }
 
   // dxStepBody, default flags: ode/src/util.cpp:255-360
