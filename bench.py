@@ -155,6 +155,7 @@ def main():
                     help="run one device copy of this many bytes first (calibrates HBM byte counters)")
     ap.add_argument("--launch", default=None,
                     help="fixed launch config 'wg,cell_scale,lds,split,grid' instead of autotuning")
+    ap.add_argument("--launch-tol", default=None, help="the same for the iterate-to-tolerance phase")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -220,6 +221,13 @@ def main():
     # (2) the metric's operating point: every world iterates to residual <= tol
     b.set_state(s0)
     b.set_tolerance(args.tol, args.max_iters, args.check_every, accel=True)
+    if args.launch_tol:
+        wg, cs, lds, split, grid = args.launch_tol.split(",")
+        b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
+    elif args.autotune:
+        b.autotune(1, keep_cell=True, fused_only=True)   # launch shape for the iterate-to-tolerance solver
+    cfg_tol = b.launch_config()
+    log(f"launch config at tolerance {cfg_tol}")
     wall, dev_ms = _timed(b, args.steps)
     cnt = b.counters()
     wc = b.world_counters()
@@ -248,7 +256,7 @@ def main():
 
     plain = plain20 = None
     if args.plain_steps > 0:
-        t_tol = _plain_rate(odemi, worlds, cfg, s0, args.plain_steps, args.tol, args.max_iters, args.check_every)
+        t_tol = _plain_rate(odemi, worlds, cfg_tol, s0, args.plain_steps, args.tol, args.max_iters, args.check_every)
         t_20 = _plain_rate(odemi, worlds, cfg, s0, args.plain_steps, 0.0, 0, 0)
         plain, plain20 = nb / t_tol, nb / t_20
         log(f"plain: {t_tol * 1e3:.1f} ms/step at tolerance, {t_20 * 1e3:.1f} ms/step at 20 iterations")
@@ -285,7 +293,7 @@ def main():
                    "worlds": len(worlds), "bodies": nb, "geoms": ng, "solver_order": "colored",
                    "solver": f"coloured Gauss-Seidel with nonsmooth nonlinear CG steps until LCP residual "
                              f"<= {args.tol:g} (checked every {args.check_every}, cap {args.max_iters}), w=1.0",
-                   "launch": cfg},
+                   "launch": cfg_tol, "launch_fixed_iterations": cfg},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "lcp_residual": {"tol": args.tol, "converged_fraction": conv, "frozen_worlds": int(dead.sum()),

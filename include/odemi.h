@@ -368,6 +368,12 @@ ODE_API int dBatchGetResidual(dBatchID b, float *out_per_world);
 /* Chooses launch shape and grid cell scale by timing candidates on a copy
  * of the state; returns the chosen workgroup size. */
 ODE_API int dBatchAutotune(dBatchID b, int trial_steps);
+/* dBatchAutotune with a reduced search: ODEMI_TUNE_KEEP_CELL keeps the grid
+ * cell scale, ODEMI_TUNE_FUSED_ONLY tries only the fused kernel.  The trials
+ * run in the batch's current solver mode. */
+#define ODEMI_TUNE_KEEP_CELL 1
+#define ODEMI_TUNE_FUSED_ONLY 2
+ODE_API int dBatchAutotuneEx(dBatchID b, int trial_steps, int flags);
 ODE_API int dBatchGetLaunchConfig(dBatchID b, int *wg_size, float *cell_scale);
 ODE_API int dBatchSetLaunchConfig(dBatchID b, int wg_size, float cell_scale);
 /* LDS pool per workgroup in bytes (sort keys, colouring state and the

@@ -648,7 +648,7 @@ int batch_phase_times(BatchImpl *b, double *ms) {
 // the trials): broadphase cell scale first, then workgroup size x LDS pool x
 // grid (one workgroup per world, or a persistent grid filling the CUs at the
 // kernel's occupancy for that shape).
-int batch_autotune(BatchImpl *b, int trial_steps) {
+int batch_autotune(BatchImpl *b, int trial_steps, int flags) {
   if (trial_steps <= 0) trial_steps = 3;
   Snapshot snap;
   if (!snapshot(b, snap, true)) return 0;
@@ -663,16 +663,17 @@ int batch_autotune(BatchImpl *b, int trial_steps) {
   int bw = b->wg, bg = b->grid, bl = b->lds;
   float bs = b->cell_scale;
   const float scales[] = {1.0f, 1.5f, 2.0f};
-  for (float sc : scales) {
-    double t;
-    if (!batch_set_launch(b, bw, sc, 0) || !trial(t)) continue;
-    if (t < best) { best = t; bs = sc; bg = 0; }
-  }
+  if (!(flags & ODEMI_TUNE_KEEP_CELL))
+    for (float sc : scales) {
+      double t;
+      if (!batch_set_launch(b, bw, sc, 0) || !trial(t)) continue;
+      if (t < best) { best = t; bs = sc; bg = 0; }
+    }
   const int wgs[] = {64, 128, 256};
-  const int lds_div[] = {2, 3, 4, 8};   // pool = LDS per CU / k
+  const int lds_div[] = {1, 2, 3, 4, 8};   // pool = LDS per CU / k
   int st = world_kernel_static_lds();
   bool bsplit = b->split;
-  for (int split = 0; split < 2; split++)
+  for (int split = 0; split < ((flags & ODEMI_TUNE_FUSED_ONLY) ? 1 : 2); split++)
     for (int wg : wgs)
       for (int k : lds_div) {
         int lds = (b->lds_cu / k - st - 256) & ~255;
@@ -744,7 +745,10 @@ int dBatchGetResidual(dBatchID b, float *out) {
   return hipMemcpy(out, i->d.residual.p, sizeof(float) * i->nworlds, hipMemcpyDeviceToHost) == hipSuccess;
 }
 
-int dBatchAutotune(dBatchID b, int trial_steps) { return b ? batch_autotune(b->impl, trial_steps) : 0; }
+int dBatchAutotune(dBatchID b, int trial_steps) { return b ? batch_autotune(b->impl, trial_steps, 0) : 0; }
+int dBatchAutotuneEx(dBatchID b, int trial_steps, int flags) {
+  return b ? batch_autotune(b->impl, trial_steps, flags) : 0;
+}
 
 int dBatchGetLaunchConfig(dBatchID b, int *wg, float *cell_scale) {
   if (!b) return 0;

@@ -91,7 +91,7 @@ int batch_set_lds(BatchImpl *b, int bytes);
 int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every, int accel);
 int batch_set_profiling(BatchImpl *b, bool on);
 int batch_phase_times(BatchImpl *b, double *ms);
-int batch_autotune(BatchImpl *b, int trial_steps);
+int batch_autotune(BatchImpl *b, int trial_steps, int flags);
 
 int snapshot(BatchImpl *b, Snapshot &s, bool save);
 int tape_begin(BatchImpl *b, int max_steps);

@@ -716,18 +716,27 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   __syncthreads();
   // fixed iteration count, or (tol > 0) iterate until the LCP residual of
   // the world is at most tol, checked every check_every iterations, up to
-  // max_iterations; accelerated: after each sweep a nonsmooth nonlinear
+  // max_iterations; accelerated: between sweeps a nonsmooth nonlinear
   // conjugate-gradient step (Silcowitz, Niebe, Erleben 2010): with
   // d = lambda_after - lambda_before, beta = |d|^2/|d_prev|^2 (0 if > 1),
-  // lambda += beta p, p = d + beta p, and fc follows through M^-1 J^T p
+  // p = d + beta p and lambda += beta p before the next sweep; fc follows
+  // through M^-1 J^T p
   const int maxit = use_tol ? P.max_iterations : P.iterations;
   const int every = use_tol ? (P.check_every > 0 ? P.check_every : 1) : (maxit > 0 ? maxit : 1);
-  float rr_prev = 0.f;
+  float rr_prev = 0.f, beta = 0.f;
   int it = 0;
   while (it < maxit) {
     const int n = every < maxit - it ? every : maxit - it;
     for (int rep = 0; rep < n; rep++) {
       if (accel) {
+        // the pending momentum of the previous sweep (beta p, with p kept
+        // in ld.y / fdir), then fc before this sweep
+        if (beta > 0.f) {
+          for (int r = threadIdx.x; r < nrows; r += blockDim.x) sr[r].lambda += beta * ld[r].y;
+          for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            for (int k = 0; k < 6; k++) sb[b].fc[k] += beta * sa[b].fdir[k];
+          __syncthreads();
+        }
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
           for (int k = 0; k < 6; k++) sa[b].fp[k] = sb[b].fc[k];
         __syncthreads();
@@ -791,27 +800,23 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         __syncthreads();
       }
       if (accel) {
+        // d = this sweep's change; beta = |d|^2 / |d_prev|^2 (restart when
+        // > 1); the momentum beta p is applied before the next sweep, so the
+        // solver always ends on a projected sweep
         float loc = 0.f;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
           float d = sr[r].lambda - ld[r].x;
           loc += d * d;
         }
         const float rr = block_fsum(loc, sh);
-        float beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
+        beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
         if (!(beta <= 1.f)) beta = 0.f;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
           float d = sr[r].lambda - ld[r].x;
-          float dl = beta * ld[r].y;
-          ld[r].y = d + dl;
-          if (dl != 0.f) sr[r].lambda += dl;
+          ld[r].y = d + beta * ld[r].y;
         }
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-          for (int k = 0; k < 6; k++) {
-            float dfc = sb[b].fc[k] - sa[b].fp[k];
-            float dfl = beta * sa[b].fdir[k];
-            sa[b].fdir[k] = dfc + dfl;
-            sb[b].fc[k] += dfl;
-          }
+          for (int k = 0; k < 6; k++) sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + beta * sa[b].fdir[k];
         rr_prev = rr;
         __syncthreads();
       }

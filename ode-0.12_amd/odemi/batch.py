@@ -161,8 +161,11 @@ class Batch:
         self.L.dBatchGetResidual(self.h, out.ctypes.data_as(C.POINTER(C.c_float)))
         return out
 
-    def autotune(self, trial_steps: int = 3) -> int:
-        return self.L.dBatchAutotune(self.h, trial_steps)
+    def autotune(self, trial_steps: int = 3, keep_cell: bool = False, fused_only: bool = False) -> int:
+        """Times launch shapes on the live batch (state restored) in its
+        current solver mode and keeps the fastest; returns the workgroup size."""
+        flags = (1 if keep_cell else 0) | (2 if fused_only else 0)
+        return self.L.dBatchAutotuneEx(self.h, trial_steps, flags)
 
     def launch_config(self):
         wg = C.c_int()

@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
         "dBatchSetSolverToleranceAccel": (C.c_int, [V, C.c_float, C.c_int, C.c_int, C.c_int]),
         "dBatchGetSplit": (C.c_int, [V]),
+        "dBatchAutotuneEx": (C.c_int, [V, C.c_int, C.c_int]),
         "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),
         "dBatchSetSplit": (C.c_int, [V, C.c_int]),
         "dBatchSetLds": (C.c_int, [V, C.c_int]),

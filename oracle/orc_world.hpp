@@ -1046,30 +1046,32 @@ template <class R> struct World {
     const bool use_tol = prm.tol > 0;
     const int maxit = use_tol ? prm.max_iterations : prm.iterations;
     std::vector<R> lprev(m, 0), dir(m, 0);
-    R rr_prev = 0;
+    R rr_prev = 0, beta = 0;
     for (int it = 0; it < maxit; it++) {
-      if (prm.accel) for (int r = 0; r < m; r++) lprev[r] = lambda[r];
-      sweep();
-      prm.sweeps++;
       if (prm.accel) {
-        // nonsmooth nonlinear CG: r = PGS(l) - l, beta = |r|^2 / |r_prev|^2,
-        // restart when beta > 1; l += beta p, p = r + beta p
-        R rr = 0;
-        for (int r = 0; r < m; r++) { R d = lambda[r] - lprev[r]; rr += d * d; }
-        R beta = (it > 0 && rr_prev > 0) ? rr / rr_prev : 0;
-        if (beta > 1) beta = 0;
-        for (int r = 0; r < m; r++) {
-          R d = lambda[r] - lprev[r];
-          R dl = beta * dir[r];                 // momentum along the previous direction
-          dir[r] = d + dl;
-          if (dl != 0) {
+        // the pending momentum beta p of the previous sweep
+        if (beta > 0)
+          for (int r = 0; r < m; r++) {
+            R dl = beta * dir[r];
+            if (dl == 0) continue;
             lambda[r] += dl;
             int b1 = jb[2 * r], b2 = jb[2 * r + 1];
             const R *M = &iMJ[12 * (size_t)r];
             for (int q = 0; q < 6; q++) fc[6 * (size_t)b1 + q] += dl * M[q];
             if (b2 != -1) for (int q = 0; q < 6; q++) fc[6 * (size_t)b2 + q] += dl * M[6 + q];
           }
-        }
+        for (int r = 0; r < m; r++) lprev[r] = lambda[r];
+      }
+      sweep();
+      prm.sweeps++;
+      if (prm.accel) {
+        // nonsmooth nonlinear CG: d = PGS(l) - l, beta = |d|^2 / |d_prev|^2,
+        // restart when beta > 1; p = d + beta p, applied before the next sweep
+        R rr = 0;
+        for (int r = 0; r < m; r++) { R d = lambda[r] - lprev[r]; rr += d * d; }
+        beta = rr_prev > 0 ? rr / rr_prev : 0;
+        if (!(beta <= 1)) beta = 0;
+        for (int r = 0; r < m; r++) dir[r] = (lambda[r] - lprev[r]) + beta * dir[r];
         rr_prev = rr;
       }
       if (use_tol && (it + 1) % prm.check_every == 0 && residual() <= prm.tol) break;
