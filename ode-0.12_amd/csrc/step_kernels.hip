@@ -684,7 +684,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sstart[k] = cstart[k];
   if (accel) {
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      for (int k = 0; k < 6; k++) sa[b].fdir[k] = 0.f;
+      for (int k = 0; k < 6; k++) { sa[b].fdir[k] = 0.f; sa[b].fp[k] = 0.f; }
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) ld[r] = make_float2(0.f, 0.f);
   }
   // constraints and rows in colour order
@@ -719,26 +719,26 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   // max_iterations; accelerated: between sweeps a nonsmooth nonlinear
   // conjugate-gradient step (Silcowitz, Niebe, Erleben 2010): with
   // d = lambda_after - lambda_before, beta = |d|^2/|d_prev|^2 (0 if > 1),
-  // p = d + beta p and lambda += beta p before the next sweep; fc follows
+  // lambda += beta p_prev (applied before the next sweep, so the solver
+  // always ends on a projected sweep), p = d + beta p_prev; fc follows
   // through M^-1 J^T p
   const int maxit = use_tol ? P.max_iterations : P.iterations;
   const int every = use_tol ? (P.check_every > 0 ? P.check_every : 1) : (maxit > 0 ? maxit : 1);
-  float rr_prev = 0.f, beta = 0.f;
+  float rr_prev = 0.f;
   int it = 0;
   while (it < maxit) {
     const int n = every < maxit - it ? every : maxit - it;
     for (int rep = 0; rep < n; rep++) {
       if (accel) {
-        // the pending momentum of the previous sweep (beta p, with p kept
-        // in ld.y / fdir), then fc before this sweep
-        if (beta > 0.f) {
-          for (int r = threadIdx.x; r < nrows; r += blockDim.x) sr[r].lambda += beta * ld[r].y;
-          for (int b = threadIdx.x; b < nb; b += blockDim.x)
-            for (int k = 0; k < 6; k++) sb[b].fc[k] += beta * sa[b].fdir[k];
-          __syncthreads();
-        }
+        // the pending momentum beta p_prev of the previous sweep (kept in
+        // ld.x for the rows and fp for the bodies), then fc before this sweep
+        for (int r = threadIdx.x; r < nrows; r += blockDim.x)
+          if (ld[r].x != 0.f) sr[r].lambda += ld[r].x;
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-          for (int k = 0; k < 6; k++) sa[b].fp[k] = sb[b].fc[k];
+          for (int k = 0; k < 6; k++) {
+            sb[b].fc[k] += sa[b].fp[k];
+            sa[b].fp[k] = sb[b].fc[k];
+          }
         __syncthreads();
       }
       for (int g = 0; g < ngroups; g++) {
@@ -801,22 +801,27 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       }
       if (accel) {
         // d = this sweep's change; beta = |d|^2 / |d_prev|^2 (restart when
-        // > 1); the momentum beta p is applied before the next sweep, so the
-        // solver always ends on a projected sweep
+        // > 1); the momentum beta p_prev is applied before the next sweep
         float loc = 0.f;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
           float d = sr[r].lambda - ld[r].x;
           loc += d * d;
         }
         const float rr = block_fsum(loc, sh);
-        beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
+        float beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
         if (!(beta <= 1.f)) beta = 0.f;
         for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
           float d = sr[r].lambda - ld[r].x;
-          ld[r].y = d + beta * ld[r].y;
+          float pend = beta * ld[r].y;   // momentum along the previous direction
+          ld[r].y = d + pend;            // new direction
+          ld[r].x = pend;
         }
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-          for (int k = 0; k < 6; k++) sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + beta * sa[b].fdir[k];
+          for (int k = 0; k < 6; k++) {
+            float pend = beta * sa[b].fdir[k];
+            sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + pend;
+            sa[b].fp[k] = pend;
+          }
         rr_prev = rr;
         __syncthreads();
       }
@@ -1083,8 +1088,10 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
       round++;
     }
     ngroups = round;
-    // bucket constraints by colour (order inside a colour is immaterial:
-    // its constraints touch disjoint bodies)
+    // bucket constraints by colour, stably (index order inside a colour):
+    // the sweep result does not depend on the order inside a colour, but
+    // the block sums of the stopping test and the CG step do, so the
+    // layout is kept reproducible from run to run
     int *cstart = c.at<int>(c.L.island);   // ngroups+1 starts
     if (ngroups <= MAX_COLORS_LDS) {
       for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sh.ccount[k] = 0;
@@ -1099,8 +1106,18 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
       __syncthreads();
       for (int k = threadIdx.x; k < ngroups; k += blockDim.x) sh.ccur[k] = sh.ccount[k];
       __syncthreads();
-      for (int i = threadIdx.x; i < ncons; i += blockDim.x)
-        if (col[i] >= 0) corder[atomicAdd(&sh.ccur[col[i]], 1)] = i;
+      for (int start = 0; start < ncons; start += blockDim.x) {
+        const int i = start + threadIdx.x;
+        const int ci = (i < ncons) ? col[i] : -1;
+        if (ci >= 0) {
+          int rank = 0;
+          for (int j = start; j < i; j++) rank += (col[j] == ci);
+          corder[sh.ccur[ci] + rank] = i;
+        }
+        __syncthreads();
+        if (ci >= 0) atomicAdd(&sh.ccur[ci], 1);
+        __syncthreads();
+      }
     } else if (threadIdx.x == 0) {
       for (int k = 0; k <= ngroups; k++) cstart[k] = 0;
       for (int i = 0; i < ncons; i++)

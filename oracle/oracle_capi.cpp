@@ -74,17 +74,31 @@ extern "C" void orc_set_colored(int on) { g_colored = on ? 1 : 0; }
 // last call is returned by orc_colored_sweeps
 static double g_tol = 0;
 static int g_maxit = 0, g_check = 10, g_accel = 0;
-static long long g_sweeps = 0;
+static double g_warm = 0;
+static long long g_sweeps = 0, g_unconv = 0;
 extern "C" void orc_set_colored_solver(double tol, int max_iterations, int check_every, int accel) {
   g_tol = tol; g_maxit = max_iterations; g_check = check_every > 0 ? check_every : 1; g_accel = accel;
 }
+// coloured mode: start each solve from warm x the previous step's impulses
+extern "C" void orc_set_warm_start(double warm) { g_warm = warm; }
+extern "C" void orc_set_exact_delta(int on) { orc::g_exact_delta = on; }
 extern "C" long long orc_colored_sweeps(void) { long long s = g_sweeps; g_sweeps = 0; return s; }
+// tolerance-mode solves that ended at the iteration cap since the last call
+extern "C" long long orc_colored_unconverged(void) { long long s = g_unconv; g_unconv = 0; return s; }
 
 // the coloured mode's pair order: (g1, g2) with g1 < g2, sorted
 static void colored_pair_order(std::vector<std::pair<int, int>> &pairs) {
   for (auto &p : pairs)
     if (p.first > p.second) std::swap(p.first, p.second);
   std::sort(pairs.begin(), pairs.end());
+}
+
+// Optional body state applied after the world is built (joints keep the
+// anchors of the scene's own poses): 13 doubles per body, pos q lvel avel.
+static std::vector<double> g_state_override;
+extern "C" void orc_set_state_override(const double *st, int nb) {
+  if (st && nb > 0) g_state_override.assign(st, st + 13 * (size_t)nb);
+  else g_state_override.clear();
 }
 
 // Builds the world exactly as the scene driver does through the ODE API.
@@ -94,6 +108,7 @@ static void build_world(const SceneDesc *s, World<R> &w) {
   w.prm.max_iterations = g_maxit;
   w.prm.check_every = g_check;
   w.prm.accel = g_accel;
+  w.prm.warm = (R)g_warm;
   for (int k = 0; k < 3; k++) w.prm.gravity[k] = (R)s->gravity[k];
   w.prm.erp = (R)s->erp;
   w.prm.cfm = (R)s->cfm;
@@ -172,6 +187,15 @@ static void build_world(const SceneDesc *s, World<R> &w) {
     }
   }
   w.n_perm = (int)w.joints.size();
+  if (g_state_override.size() == 13 * (size_t)s->nb) {
+    for (int i = 0; i < s->nb; i++) {
+      const double *r = &g_state_override[13 * (size_t)i];
+      for (int k = 0; k < 3; k++) w.bodies[i].pos[k] = (R)r[k];
+      R q[4] = {(R)r[3], (R)r[4], (R)r[5], (R)r[6]};
+      w.set_quat(i, q);
+      for (int k = 0; k < 3; k++) { w.bodies[i].lvel[k] = (R)r[7 + k]; w.bodies[i].avel[k] = (R)r[10 + k]; }
+    }
+  }
 }
 
 extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
@@ -263,6 +287,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
   if (o->feedback)
     for (size_t k = 0; k < 12 * (size_t)o->max_contacts; k++) o->feedback[k] = (double)w.fbout[k];
   g_sweeps += w.prm.sweeps;
+  g_unconv += w.prm.unconverged;
   return o->status;
 }
 

@@ -10,6 +10,7 @@
 // explicitly because pair, island and row order follow from them.
 #pragma once
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -19,6 +20,7 @@
 #include "orc_math.hpp"
 
 namespace orc {
+inline int g_exact_delta = 0;
 
 // ---------------------------------------------------------------- RNG
 // Linear congruential generator and the xor-folded bounded draw:
@@ -117,6 +119,7 @@ template <class R> struct Joint {
   R c0 = 0, s0 = 0;
   R lam[6] = {0, 0, 0, 0, 0, 0};   // impulses of the rows after the last step
   int nrows = 0;
+  long long wkey = -1;          // contact: (g1, g2, index within the pair), the warm-start key
 };
 
 template <class R> struct WorldParams {
@@ -132,7 +135,12 @@ template <class R> struct WorldParams {
   // nonsmooth nonlinear conjugate-gradient step between sweeps
   R tol = 0;
   int max_iterations = 0, check_every = 10, accel = 0;
+  // coloured mode only: warm start; the solve starts from warm times the
+  // impulses of the previous step (contacts matched by geom pair and index
+  // within the pair, permanent joints by joint); 0 = start from zero
+  R warm = 0;
   long long sweeps = 0;   // sweeps run (statistics)
+  long long unconverged = 0;   // tolerance-mode solves that ended at the cap
 };
 
 // row buffers of one island (quickstep.cpp:711-743)
@@ -454,6 +462,10 @@ template <class R> struct World {
     j.surf = s;
     for (int k = 0; k < 3; k++) { j.cpos[k] = cg.pos[k]; j.cnormal[k] = cg.normal[k]; j.fdir1[k] = 0; }
     j.cdepth = cg.depth;
+    int slot = 0;
+    const long long pk = ((long long)cg.g1 << 32) | ((long long)cg.g2 << 8);
+    if ((int)joints.size() > n_perm && (joints.back().wkey & ~255LL) == pk) slot = (int)(joints.back().wkey & 255) + 1;
+    j.wkey = pk | (slot & 255);
     joints.push_back(j);
     int ji = (int)joints.size() - 1;
     attach(ji, b1, b2);
@@ -820,6 +832,22 @@ template <class R> struct World {
           o += jm[a];
           cid[a] = act[a] >= n_perm ? act[a] - n_perm : nk + act[a];
         }
+        if (prm.warm != 0) {
+          for (size_t a = 0; a < act.size(); a++) {
+            const Joint<R> &jt = joints[act[a]];
+            const R *prev = nullptr;
+            int np = 0;
+            if (act[a] >= n_perm) {
+              auto f = std::lower_bound(warm_prev.begin(), warm_prev.end(), std::make_pair(jt.wkey, std::array<R, 3>{}),
+                                        [](const auto &x, const auto &y) { return x.first < y.first; });
+              if (f != warm_prev.end() && f->first == jt.wkey) { prev = f->second.data(); np = 3; }
+            } else if (jt.nrows == jm[a]) {
+              prev = jt.lam;
+              np = 6;
+            }
+            for (int r = 0; r < jm[a] && r < np; r++) lambda[row0[a] + r] = prm.warm * prev[r];
+          }
+        }
         sor_colored(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex, row0, jm, cid);
       } else {
         sor_lcp(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex);
@@ -963,7 +991,14 @@ template <class R> struct World {
         mv(&iMJ[12 * i + 9], &invI[12 * b2], &J[12 * i + 9]);
       }
     }
+    // fc = inv(M) J^T lambda of the starting impulses
     std::fill(fc.begin(), fc.end(), R(0));
+    for (int i = 0; i < m; i++) {
+      if (lambda[i] == 0) continue;
+      int b1 = jb[2 * i], b2 = jb[2 * i + 1];
+      for (int q = 0; q < 6; q++) fc[6 * (size_t)b1 + q] += lambda[i] * iMJ[12 * i + q];
+      if (b2 != -1) for (int q = 0; q < 6; q++) fc[6 * (size_t)b2 + q] += lambda[i] * iMJ[12 * i + 6 + q];
+    }
     std::vector<R> Ad(m);
     for (int i = 0; i < m; i++) {
       R s = 0;
@@ -1017,6 +1052,7 @@ template <class R> struct World {
             R nl = old + delta;
             if (nl < lo_a) { delta = lo_a - old; nl = lo_a; }
             else if (nl > hi_a) { delta = hi_a - old; nl = hi_a; }
+            else if (g_exact_delta && prm.tol > 0) delta = nl - old;
             lambda[r] = nl;
             const R *M = &iMJ[12 * (size_t)r];
             for (int q = 0; q < 6; q++) f1[q] += delta * M[q];
@@ -1045,14 +1081,13 @@ template <class R> struct World {
     };
     const bool use_tol = prm.tol > 0;
     const int maxit = use_tol ? prm.max_iterations : prm.iterations;
-    std::vector<R> lprev(m, 0), dir(m, 0);
-    R rr_prev = 0, beta = 0;
+    std::vector<R> lprev(m, 0), dir(m, 0), pend(m, 0);
+    R rr_prev = 0;
     for (int it = 0; it < maxit; it++) {
       if (prm.accel) {
-        // the pending momentum beta p of the previous sweep
-        if (beta > 0)
-          for (int r = 0; r < m; r++) {
-            R dl = beta * dir[r];
+        // the pending momentum beta p_prev of the previous sweep
+        for (int r = 0; r < m; r++) {
+            R dl = pend[r];
             if (dl == 0) continue;
             lambda[r] += dl;
             int b1 = jb[2 * r], b2 = jb[2 * r + 1];
@@ -1066,16 +1101,21 @@ template <class R> struct World {
       prm.sweeps++;
       if (prm.accel) {
         // nonsmooth nonlinear CG: d = PGS(l) - l, beta = |d|^2 / |d_prev|^2,
-        // restart when beta > 1; p = d + beta p, applied before the next sweep
+        // restart when beta > 1; l += beta p_prev (before the next sweep),
+        // p = d + beta p_prev
         R rr = 0;
         for (int r = 0; r < m; r++) { R d = lambda[r] - lprev[r]; rr += d * d; }
-        beta = rr_prev > 0 ? rr / rr_prev : 0;
+        R beta = rr_prev > 0 ? rr / rr_prev : 0;
         if (!(beta <= 1)) beta = 0;
-        for (int r = 0; r < m; r++) dir[r] = (lambda[r] - lprev[r]) + beta * dir[r];
+        for (int r = 0; r < m; r++) {
+          pend[r] = beta * dir[r];
+          dir[r] = (lambda[r] - lprev[r]) + pend[r];
+        }
         rr_prev = rr;
       }
-      if (use_tol && (it + 1) % prm.check_every == 0 && residual() <= prm.tol) break;
+      if (use_tol && (it + 1) % prm.check_every == 0 && residual() <= prm.tol) return;
     }
+    if (use_tol) prm.unconverged++;
   }
 
   // dxStepBody, default flags: ode/src/util.cpp:255-360
@@ -1132,7 +1172,28 @@ template <class R> struct World {
       islands_b.push_back(ib);
       islands_j.push_back(ij);
     }
-    for (size_t k = 0; k < islands_b.size(); k++) quickstep_island(islands_b[k], islands_j[k], h);
+    if (prm.colored) {
+      // the coloured mode solves the whole world at once (one colouring, one
+      // sweep over all islands, world-wide stopping test and CG step), as
+      // the MI355X coloured mode does; islands are independent, so a plain
+      // sweep is the same either way
+      std::vector<int> ab, aj;
+      for (size_t k = 0; k < islands_b.size(); k++) {
+        ab.insert(ab.end(), islands_b[k].begin(), islands_b[k].end());
+        aj.insert(aj.end(), islands_j[k].begin(), islands_j[k].end());
+      }
+      quickstep_island(ab, aj, h);
+      // this step's contact impulses, the next step's starting point
+      warm_prev.clear();
+      if (prm.warm != 0)
+        for (size_t k = n_perm; k < joints.size(); k++) {
+          const Joint<R> &jt = joints[k];
+          warm_prev.push_back({jt.wkey, std::array<R, 3>{jt.lam[0], jt.lam[1], jt.lam[2]}});
+        }
+      std::sort(warm_prev.begin(), warm_prev.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    } else {
+      for (size_t k = 0; k < islands_b.size(); k++) quickstep_island(islands_b[k], islands_j[k], h);
+    }
     islands_b.clear();
     islands_j.clear();
     // new space list: moved geoms in reverse order of their move, then the
@@ -1146,6 +1207,7 @@ template <class R> struct World {
     space.swap(ns);
   }
   std::vector<std::vector<int>> islands_b, islands_j;
+  std::vector<std::pair<long long, std::array<R, 3>>> warm_prev;   // sorted by key
 };
 
 }  // namespace orc
