@@ -8,8 +8,9 @@ world.  Steps 0..warmup settle the piles; the timed steps follow.
 
 metric: body-steps/s of the batched persistent path (all worlds in one launch
 per step, HIP-graph replay) with every world's SOR iterated until its LCP
-residual is <= 1e-4.  Also reported: the plain one-world-per-launch path on
-the same worlds and state, the same two paths at the quickstep's fixed 20
+residual is <= 1e-4 (warm-started from the previous step's impulses; the
+first --tol-warmup steps at tolerance are untimed).  Also reported: the plain
+one-world-per-launch path continuing the same batch and state, the same two paths at the quickstep's fixed 20
 iterations, device counters, per-phase clocks, the HBM roofline of the step
 and the reference CPU build on the host.
 
@@ -147,6 +148,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4, help="LCP residual the solver iterates down to")
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--warm", type=float, default=1.0,
+                    help="warm start of the iterate-to-tolerance solver (factor on the previous impulses; 0: off)")
+    ap.add_argument("--tol-warmup", type=int, default=10,
+                    help="untimed steps at tolerance before the timed ones (the solver's warm start settles)")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="steps of the adjoint run (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=1024, help="worlds in the CPU baseline sample")
     ap.add_argument("--cpu-procs", type=int, default=16, help="host processes of the CPU baseline")
@@ -221,6 +226,7 @@ def main():
     # (2) the metric's operating point: every world iterates to residual <= tol
     b.set_state(s0)
     b.set_tolerance(args.tol, args.max_iters, args.check_every, accel=True)
+    b.set_warm_start(args.warm)
     if args.launch_tol:
         wg, cs, lds, split, grid = args.launch_tol.split(",")
         b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
@@ -228,6 +234,8 @@ def main():
         b.autotune(1, keep_cell=True, fused_only=True)   # launch shape for the iterate-to-tolerance solver
     cfg_tol = b.launch_config()
     log(f"launch config at tolerance {cfg_tol}")
+    if args.tol_warmup:
+        b.step(args.tol_warmup)
     wall, dev_ms = _timed(b, args.steps)
     cnt = b.counters()
     wc = b.world_counters()
@@ -246,6 +254,15 @@ def main():
     b.step(5)
     ph = {k: v / 5 for k, v in b.phase_times().items()}
     b.profile(False)
+    # the plain path at the same tolerance, continuing the same state (and
+    # warm-start records): one world per launch
+    t_tol = None
+    if args.plain_steps > 0:
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        b.step_plain(args.plain_steps)
+        torch.cuda.synchronize()
+        t_tol = (time.perf_counter() - tp) / args.plain_steps
     b.close()
 
     ms_step = wall / args.steps * 1e3
@@ -256,7 +273,6 @@ def main():
 
     plain = plain20 = None
     if args.plain_steps > 0:
-        t_tol = _plain_rate(odemi, worlds, cfg_tol, s0, args.plain_steps, args.tol, args.max_iters, args.check_every)
         t_20 = _plain_rate(odemi, worlds, cfg, s0, args.plain_steps, 0.0, 0, 0)
         plain, plain20 = nb / t_tol, nb / t_20
         log(f"plain: {t_tol * 1e3:.1f} ms/step at tolerance, {t_20 * 1e3:.1f} ms/step at 20 iterations")
@@ -292,7 +308,8 @@ def main():
                                f"steps {args.warmup}-{args.warmup + args.steps}, seed 7",
                    "worlds": len(worlds), "bodies": nb, "geoms": ng, "solver_order": "colored",
                    "solver": f"coloured Gauss-Seidel with nonsmooth nonlinear CG steps until LCP residual "
-                             f"<= {args.tol:g} (checked every {args.check_every}, cap {args.max_iters}), w=1.0",
+                             f"<= {args.tol:g} (checked every {args.check_every}, cap {args.max_iters}), w=1.0, "
+                             f"warm start {args.warm:g} x the previous step's impulses",
                    "launch": cfg_tol, "launch_fixed_iterations": cfg},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,

@@ -361,6 +361,15 @@ ODE_API int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, 
  * on stacked contacts).  A world whose state becomes non-finite is frozen
  * and flagged (overflow bit 16) until dBatchSetState. */
 ODE_API int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterations, int check_every, int accel);
+/* Coloured order only: warm start.  Each solve starts from warm (0..1) times
+ * the impulses of the previous step -- contacts matched by geom pair and
+ * index within the pair, permanent joints by joint when their row count is
+ * unchanged -- instead of from zero; the solution at a tolerance is the same
+ * LCP's.  The reference keeps this behind WARM_STARTING for joints only
+ * (ode/src/quickstep.cpp:43-47, 348-372, 864-895), as contacts are
+ * recreated every step.  dBatchSetState and this call clear the stored
+ * impulses.  warm = 0 (the default) restores the reference's zero start. */
+ODE_API int dBatchSetWarmStart(dBatchID b, float warm);
 /* Enables the LCP complementarity residual of the last step per world
  * (max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|). */
 ODE_API int dBatchSetResidualTracking(dBatchID b, int on);

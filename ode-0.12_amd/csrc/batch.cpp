@@ -48,6 +48,8 @@ void refresh_dev(BatchImpl *b) {
   D.work_counter = b->d.work_counter.as<int>();
   D.world_order = b->d.world_order.as<int>();
   D.scratch = b->d.scratch.as<char>();
+  D.warm = b->d.warm.as<char>();
+  D.warm_n = b->d.warm_n.as<int>();
   D.tape = nullptr;
   D.tape_off = nullptr;
   D.prof = b->prof_on ? b->d.prof.as<unsigned long long>() : nullptr;
@@ -284,10 +286,12 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   b->nj = (int)A.joints.size();
   if (!A.wp.empty()) b->cell_scale = A.wp[0].cell_scale;
   derive_lists(A);
-  long long scratch = 0;
+  long long scratch = 0, warm = 0;
   for (auto &wi : A.wi) {
     wi.scratch = scratch;
     scratch += scratch_layout(wi.nb, wi.ng, wi.nj, wi.pcap, wi.kcap, wi.rcap).total;
+    wi.warm = warm;
+    warm += warm_bytes(wi.kcap, wi.nj);
   }
   b->wi = A.wi;
   b->wp = A.wp;
@@ -309,7 +313,8 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
             upload(d.world_order, order) && d.R.alloc(sizeof(float4) * 3 * (size_t)(nb ? nb : 1)) &&
             d.counters.alloc(sizeof(int) * CNT_N * (size_t)b->nworlds) &&
             d.residual.alloc(sizeof(float) * (size_t)b->nworlds) && d.work_counter.alloc(64 * WORK_QUEUES) &&
-            d.scratch.alloc((size_t)scratch) && d.ctrl.alloc(sizeof(float4) * (size_t)(nj ? nj : 1)) &&
+            d.scratch.alloc((size_t)scratch) && d.warm.alloc((size_t)(warm > 0 ? warm : 256)) &&
+            d.warm_n.alloc(sizeof(int) * (size_t)b->nworlds) && d.ctrl.alloc(sizeof(float4) * (size_t)(nj ? nj : 1)) &&
             d.prof.alloc(sizeof(unsigned long long) * PH_N * (size_t)b->nworlds);
   if (!ok) {
     report_error("batch setup failed (%lld scratch bytes)", scratch);
@@ -319,6 +324,7 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   (void)hipMemset(d.counters.p, 0, d.counters.n);
   (void)hipMemset(d.residual.p, 0, d.residual.n);
   (void)hipMemset(d.prof.p, 0, d.prof.n);
+  (void)hipMemset(d.warm_n.p, 0xff, d.warm_n.n);   // no warm-start records yet
   refresh_dev(b);
   if (launch_init_rot(b->dev, nb, b->stream) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
     report_error("batch setup: rotation init failed");
@@ -461,6 +467,8 @@ int batch_set_state(BatchImpl *b, const float *in) {
     OD_HIP(hipMemcpy(b->d.lvel.p, v.data(), sizeof(float4) * b->nb, hipMemcpyHostToDevice));
     OD_HIP(hipMemcpy(b->d.avel.p, w.data(), sizeof(float4) * b->nb, hipMemcpyHostToDevice));
   }
+  // the previous impulses belong to the old state: the next solve starts from zero
+  OD_HIP(hipMemset(b->d.warm_n.p, 0xff, b->d.warm_n.n));
   refresh_dev(b);
   if (launch_init_rot(b->dev, b->nb, b->stream) != hipSuccess) return 0;
   OD_HIP(hipStreamSynchronize(b->stream));
@@ -553,7 +561,8 @@ int snapshot(BatchImpl *b, Snapshot &s, bool save) {
   struct P { DevBuf *src, *dst; };
   P items[] = {{&b->d.pos, &s.pos}, {&b->d.q, &s.q}, {&b->d.lvel, &s.lvel}, {&b->d.avel, &s.avel},
                {&b->d.facc, &s.facc}, {&b->d.tacc, &s.tacc}, {&b->d.R, &s.R}, {&b->d.grank, &s.grank},
-               {&b->d.joints, &s.joints}, {&b->d.wp, &s.wp}};
+               {&b->d.joints, &s.joints}, {&b->d.wp, &s.wp}, {&b->d.warm, &s.warm},
+               {&b->d.warm_n, &s.warm_n}};
   for (auto &it : items) {
     if (save) {
       if (!it.dst->alloc(it.src->n)) return 0;
@@ -590,6 +599,13 @@ int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_e
     return 0;
   b->tol_on = tol > 0.f;
   destroy_graph(b);
+  return 1;
+}
+
+int batch_set_warm_start(BatchImpl *b, float warm) {
+  if (!(warm >= 0.f && warm <= 1.f)) return 0;
+  if (!patch_params(b, [&](WorldParams &p) { p.warm = warm; })) return 0;
+  OD_HIP(hipMemset(b->d.warm_n.p, 0xff, b->d.warm_n.n));
   return 1;
 }
 
@@ -772,6 +788,7 @@ int dBatchSetSolverTolerance(dBatchID b, float tol, int max_iterations, int chec
 int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterations, int check_every, int accel) {
   return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every, accel) : 0;
 }
+int dBatchSetWarmStart(dBatchID b, float warm) { return b ? batch_set_warm_start(b->impl, warm) : 0; }
 int dBatchGetWorldCounters(dBatchID b, int *out) {
   if (!b || !out) return 0;
   BatchImpl *i = b->impl;

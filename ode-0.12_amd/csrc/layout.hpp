@@ -46,6 +46,7 @@ struct WorldInfo {
   int rcap;          // row capacity
   int adj0;          // offset into the permanent-joint adjacency arrays
   long long scratch; // byte offset of the world's scratch region
+  long long warm;    // byte offset of the world's warm-start records
 };
 
 struct WorldParams {
@@ -68,6 +69,7 @@ struct WorldParams {
   int check_every;         // iterations between residual checks when tol > 0
   int accel;               // with tol: nonsmooth nonlinear conjugate-gradient steps between sweeps
   int dead;                // set by the step when the world's state became non-finite; the world is frozen
+  float warm;              // coloured solver: start from warm x the previous step's impulses (0: from zero)
 };
 
 struct Limot {             // ode/src/joints/joint.h:172-190
@@ -106,8 +108,22 @@ struct ContactRec {
   float normal[3];
   int g1, g2;
   int b1, b2;              // node order after attach (b1 >= 0 unless both static)
-  int flags;               // JF_REVERSE
+  int flags;               // JF_REVERSE; bits 8+: index of the contact within its pair
 };
+
+// Warm-start record of one contact: the contact's identity (geom pair and
+// index within the pair) and the impulses of its rows after the step.
+struct WarmRec {
+  unsigned long long key;
+  float lam[3];
+  int pad;
+};
+
+// identity of a contact across steps: geom ids (ascending) and index within the pair
+OD_HD inline unsigned long long contact_key(int g1, int g2, int slot) {
+  unsigned long long lo = (unsigned)(g1 < g2 ? g1 : g2), hi = (unsigned)(g1 < g2 ? g2 : g1);
+  return (lo << 36) | (hi << 12) | (unsigned long long)(slot & 0xfff);
+}
 
 // one constraint row, 144 bytes.  J and iMJ are unscaled; the solver
 // applies ad = w / (J inv(M) J^T + cfm) (ode/src/quickstep.cpp:370-402
@@ -184,6 +200,10 @@ OD_HD inline long long od_pow2(long long n) {
   while (p < n) p <<= 1;
   return p;
 }
+
+// warm-start records of one world: kcap contact records, then 8 floats per
+// permanent joint (impulses of up to 6 rows, then the row count as int bits)
+OD_HD inline long long warm_bytes(int kcap, int nj) { return od_align(24LL * kcap) + 32LL * nj; }
 
 // bytes of the coloured solver's working set: bodies, rows, constraints,
 // colour order and colour starts
@@ -267,6 +287,10 @@ struct DevBatch {
   int lds_bytes;           // dynamic LDS pool per workgroup
   const int *world_order;  // processing order of worlds (largest first)
   char *scratch;
+  // warm start: per-world records (WorldInfo::warm) and the number of valid
+  // contact records per world (-1: none, the next solve starts from zero)
+  char *warm;
+  int *warm_n;
   // tape for the adjoint (null when not recording)
   char *tape;
   const long long *tape_off;   // per world byte offset into the tape

@@ -11,7 +11,7 @@ namespace odemi {
 // device state that one step mutates; saved and restored around autotuning
 // trials and as the adjoint's checkpoints
 struct Snapshot {
-  DevBuf pos, q, lvel, avel, facc, tacc, R, grank, joints, wp;
+  DevBuf pos, q, lvel, avel, facc, tacc, R, grank, joints, wp, warm, warm_n;
 };
 
 // Forward tape for the adjoint: checkpoints of the state every `interval`
@@ -34,7 +34,7 @@ struct BatchBufs {
   DevBuf wi, wp, pos, q, lvel, avel, facc, tacc, R, I, invI, mass, invMass;
   DevBuf gtype, gbody, gparam, gpos, gR, grank, gcat, gcol;
   DevBuf joints, adj_start, adj_other, bgeom_start, bgeom;
-  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback, prof;
+  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback, prof, warm, warm_n;
 };
 
 struct BatchImpl {
@@ -89,6 +89,7 @@ int batch_set_controls(BatchImpl *b, const double *ctrl);
 int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid);
 int batch_set_lds(BatchImpl *b, int bytes);
 int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every, int accel);
+int batch_set_warm_start(BatchImpl *b, float warm);
 int batch_set_profiling(BatchImpl *b, bool on);
 int batch_phase_times(BatchImpl *b, double *ms);
 int batch_autotune(BatchImpl *b, int trial_steps, int flags);

@@ -498,7 +498,7 @@ __device__ inline void store_contacts(const DevBatch &B, const Ctx &c, int g1, i
     r.pos[0] = pts[i].pos.x; r.pos[1] = pts[i].pos.y; r.pos[2] = pts[i].pos.z;
     r.depth = pts[i].depth;
     r.normal[0] = pts[i].n.x; r.normal[1] = pts[i].n.y; r.normal[2] = pts[i].n.z;
-    r.g1 = g1; r.g2 = g2; r.b1 = b1; r.b2 = b2; r.flags = flags;
+    r.g1 = g1; r.g2 = g2; r.b1 = b1; r.b2 = b2; r.flags = flags | (i << 8);
   }
 }
 
@@ -654,13 +654,61 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
   return r;
 }
 
+// fc += delta * inv(M) J^T of one row for both bodies of its constraint
+__device__ inline void sor_push(const SRow &r, float delta, float im1, const float *iI1, float *f1, bool two,
+                                float im2, const float *iI2, float *f2) {
+  float d1 = delta * im1;
+  f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
+  f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
+  f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
+  f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
+  if (two) {
+    float d2 = delta * im2;
+    f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
+    f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
+    f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
+    f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
+  }
+}
+
+// Starting impulses of one constraint's rows from the warm-start records of
+// the previous step: a contact by its key (records sorted by key, binary
+// search), a permanent joint by index when its row count is unchanged.
+__device__ inline void warm_lookup(const WorldParams &P, const Cons &q, const ContactRec *con, const WarmRec *wrec,
+                                   int nprev, const float *wjnt, float *wl) {
+  if (q.kind == JT_CONTACT) {
+    const ContactRec &cr = con[q.src];
+    const unsigned long long key = contact_key(cr.g1, cr.g2, cr.flags >> 8);
+    int lo = 0, hi = nprev;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (wrec[mid].key < key) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < nprev && wrec[lo].key == key)
+      for (int k = 0; k < q.m && k < 3; k++) wl[k] = P.warm * wrec[lo].lam[k];
+  } else {
+    const float *jw = wjnt + 8 * q.src;
+    if (__float_as_int(jw[6]) == q.m)
+      for (int k = 0; k < q.m && k < 6; k++) wl[k] = P.warm * jw[k];
+  }
+}
+
 template <bool TOL>
 __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
-                            float *bt, const int *corder, const int *cstart, int nrows, int ngroups, Shared &sh) {
+                            float *bt, const int *corder, const int *cstart, int nrows, int ngroups, int nk,
+                            int ncons, bool host_contacts, Shared &sh) {
   const int nb = c.W.nb;
   const int nact = cstart[ngroups];   // constraints with rows, in colour order
   const bool use_tol = TOL && P.tol > 0.f;
   const bool accel = use_tol && P.accel != 0;
+  // warm start (contacts found on the device): records of the previous step
+  const bool warm_on = P.warm != 0.f && !host_contacts && B.warm_n != nullptr;
+  const int nprev = warm_on ? B.warm_n[c.w] : -1;
+  const bool warm = warm_on && nprev >= 0;
+  WarmRec *wrec = warm_on ? (WarmRec *)(B.warm + c.W.warm) : nullptr;
+  float *wjnt = warm_on ? (float *)(B.warm + c.W.warm + od_align(24LL * c.W.kcap)) : nullptr;
+  const ContactRec *con = c.at<ContactRec>(c.L.contacts);
   const long long need = sor_bytes(nb, nrows, nact, accel);
   char *base = (need <= B.lds_bytes) ? (char *)od_pool : c.at<char>(c.L.sor);
   SBody *sb = (SBody *)base;
@@ -699,6 +747,8 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       SCons o;
       o.row0 = rbase + off; o.m = m; o.b1 = q->b1; o.b2 = q->b2;
       sc[x] = o;
+      float wl[SOR_MAX_ROWS] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (warm) warm_lookup(P, *q, con, wrec, nprev, wjnt, wl);
       for (int k = 0; k < m; k++) {
         const Row &R = rows[q->row0 + k];
         SRow w;
@@ -706,7 +756,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         w.a1[0] = R.J[3]; w.a1[1] = R.J[4]; w.a1[2] = R.J[5];
         w.a2[0] = R.J[9]; w.a2[1] = R.J[10]; w.a2[2] = R.J[11];
         w.rhs = R.rhs; w.cfm = R.cfm; w.ad = R.ad; w.lo = R.lo; w.hi = R.hi;
-        w.lambda = 0.f;
+        w.lambda = k < SOR_MAX_ROWS ? wl[k] : 0.f;
         w.findex = R.findex >= 0 ? R.findex - q->row0 : -1;
         sr[o.row0 + k] = w;
       }
@@ -714,6 +764,33 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     rbase += tot;
   }
   __syncthreads();
+  if (warm) {
+    // fc = inv(M) J^T lambda of the starting impulses, colour by colour
+    // (a colour's constraints share no body)
+    for (int g = 0; g < ngroups; g++) {
+      const int x1 = sstart[g + 1];
+      for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
+        const SCons q = sc[x];
+        const bool two = q.b2 >= 0;
+        float f1[6], f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
+        for (int k = 0; k < 6; k++) f1[k] = sb[q.b1].fc[k];
+        if (two)
+          for (int k = 0; k < 6; k++) { f2[k] = sb[q.b2].fc[k]; iI2[k] = sb[q.b2].iI[k]; }
+        if (two) im2 = sb[q.b2].im;
+        float iI1[6];
+        for (int k = 0; k < 6; k++) iI1[k] = sb[q.b1].iI[k];
+        const float im1 = sb[q.b1].im;
+        for (int k = 0; k < q.m && k < SOR_MAX_ROWS; k++) {
+          const SRow r = sr[q.row0 + k];
+          if (r.lambda != 0.f) sor_push(r, r.lambda, im1, iI1, f1, two, im2, iI2, f2);
+        }
+        for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
+        if (two)
+          for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
+      }
+      __syncthreads();
+    }
+  }
   // fixed iteration count, or (tol > 0) iterate until the LCP residual of
   // the world is at most tol, checked every check_every iterations, up to
   // max_iterations; accelerated: between sweeps a nonsmooth nonlinear
@@ -780,18 +857,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
             else if (nl > hi) { delta = hi - old; nl = hi; }
             rr.lambda = nl;
             if (k == 0) lam0 = nl;
-            float d1 = delta * im1;
-            f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
-            f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
-            f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
-            f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
-            if (two) {
-              float d2 = delta * im2;
-              f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
-              f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
-              f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
-              f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
-            }
+            sor_push(r, delta, im1, iI1, f1, two, im2, iI2, f2);
           }
           for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
           if (two)
@@ -844,6 +910,28 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = sb[b].fc[k];
   __syncthreads();
+  if (warm_on) {
+    // this step's impulses, the next step's starting point; contacts are in
+    // pair order, so the records come out sorted by key
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const Cons &q = cons[i];
+      const ContactRec &cr = con[q.src];
+      WarmRec r;
+      r.key = contact_key(cr.g1, cr.g2, cr.flags >> 8);
+      for (int k = 0; k < 3; k++) r.lam[k] = k < q.m ? rows[q.row0 + k].lambda : 0.f;
+      r.pad = 0;
+      wrec[i] = r;
+    }
+    for (int i = nk + threadIdx.x; i < ncons; i += blockDim.x) {
+      const Cons &q = cons[i];
+      float *jw = wjnt + 8 * q.src;
+      for (int k = 0; k < 6; k++) jw[k] = k < q.m ? rows[q.row0 + k].lambda : 0.f;
+      jw[6] = __int_as_float(q.m);
+      jw[7] = 0.f;
+    }
+    if (threadIdx.x == 0) B.warm_n[c.w] = nk;
+    __syncthreads();
+  }
 }
 
 template <bool TOL>
@@ -1132,8 +1220,12 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
   }
   c.clk.lap(PH_ORDER);
   if (nrows > c.W.rcap) {
-    // not enough row storage: leave the world unchanged for this step
-    if (threadIdx.x == 0) atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_ROWS);
+    // not enough row storage: leave the world unchanged for this step (and
+    // start the next solve from zero)
+    if (threadIdx.x == 0) {
+      atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_ROWS);
+      if (B.warm_n) B.warm_n[c.w] = -1;
+    }
     __syncthreads();
     return;
   }
@@ -1272,7 +1364,8 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     // reads the fc of all bodies
     __syncthreads();
   } else {
-    sor_colored<TOL>(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, sh);
+    sor_colored<TOL>(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, nk, ncons,
+                     in.from_host, sh);
   }
   c.clk.lap(PH_SOR);
   // --- residual (optional), feedback

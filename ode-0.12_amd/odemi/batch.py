@@ -121,6 +121,13 @@ class Batch:
         if not self.L.dBatchSetSolverToleranceAccel(self.h, tol, max_iterations, check_every, 1 if accel else 0):
             raise ValueError("bad solver tolerance settings")
 
+    def set_warm_start(self, warm: float = 1.0):
+        """Coloured order: start each solve from warm x the previous step's
+        impulses (contacts matched by geom pair and index within the pair);
+        0 restores the zero start."""
+        if not self.L.dBatchSetWarmStart(self.h, warm):
+            raise ValueError("bad warm-start factor")
+
     # ---- adjoint
     def tape_options(self, checkpoint_interval: int = 10, act_tol: float = 1e-6, lin_tol: float = 1e-6,
                      lin_maxit: int = 500):

@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchGetWorldCounters": (C.c_int, [V, _ip]),
         "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
         "dBatchSetSolverToleranceAccel": (C.c_int, [V, C.c_float, C.c_int, C.c_int, C.c_int]),
+        "dBatchSetWarmStart": (C.c_int, [V, C.c_float]),
         "dBatchGetSplit": (C.c_int, [V]),
         "dBatchAutotuneEx": (C.c_int, [V, C.c_int, C.c_int]),
         "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),

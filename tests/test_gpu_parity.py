@@ -381,36 +381,45 @@ def test_coloured_order_matches_coloured_restatement(odemi_mod):
     say("coloured trajectories: worst err/tol", worst)
 
 
-def test_accelerated_tolerance_solver_matches_restatement(odemi_mod):
+@pytest.mark.parametrize("warm", [0.0, 1.0], ids=["cold", "warm"])
+def test_accelerated_tolerance_solver_matches_restatement(odemi_mod, warm):
     """Coloured order iterated to a residual tolerance with the conjugate-
-    gradient steps, against the same solver in the CPU restatement: 20 steps
-    within 4x the restatement's FP32-FP64 distance + 1e-3 m; every world
-    reaches the tolerance."""
+    gradient steps (cold, or warm-started from the previous step's impulses),
+    against the same solver in the CPU restatement: 20 steps within 4x the
+    restatement's FP32-FP64 distance + 1e-3 m; every world reaches the
+    tolerance."""
     ws = scenes.uneven_worlds(4, seed=13)
     lib32, lib64 = po.oracle_lib("f32"), po.oracle_lib("f64")
     n, tol = 20, 1e-3
-    lib32.orc_set_colored_solver.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
-    lib64.orc_set_colored_solver.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
     refs = []
     try:
         for lib in (lib32, lib64):
+            lib.orc_set_colored_solver.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
+            lib.orc_set_warm_start.argtypes = [C.c_double]
             lib.orc_set_colored(1)
             lib.orc_set_colored_solver(tol, 3000, 10, 1)
+            lib.orc_set_warm_start(warm)
         for sc in ws:
             refs.append((po.run(lib32, sc, n, n)["traj"][-1], po.run(lib64, sc, n, n)["traj"][-1]))
     finally:
         for lib in (lib32, lib64):
             lib.orc_set_colored(0)
             lib.orc_set_colored_solver(0.0, 0, 10, 0)
+            lib.orc_set_warm_start(0.0)
     b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
     b.set_tolerance(tol, 3000, 10, accel=True)
+    b.set_warm_start(warm)
     b.track_residual(True)
-    b.step(n)
+    its = np.zeros(len(ws), np.int64)
+    for _ in range(n):
+        b.step(1)
+        its += b.world_counters()[:, 5]
     res = b.residual()
     st = b.state()
     wc = b.world_counters()
     b.close()
-    say("accelerated solver: residual", res.tolist(), "iterations", wc[:, 5].tolist())
+    say(f"accelerated solver (warm {warm}): residual", res.tolist(), "iterations", wc[:, 5].tolist(),
+        "total sweeps", its.tolist())
     assert (res <= tol).all() and (wc[:, 5] < 3000).all()
     offs = np.concatenate([[0], np.cumsum([w.nb for w in ws])])
     for w, (t32, t64) in enumerate(refs):

@@ -310,3 +310,48 @@ def test_coloured_order_converges_to_the_reference_orders_solution():
         # and the bodies end in the same state
         fc, fr = tc["states"][-1], tr["states"][-1]
         assert np.abs(fc[:, 7:13] - fr[:, 7:13]).max() <= 1e-6 * max(1.0, np.abs(fr[:, 7:13]).max())
+
+
+def test_warm_start_changes_only_the_starting_point():
+    """Warm start (coloured mode) starts each solve from the previous step's
+    impulses instead of zero.  The frictionless LCP (regularised by cfm, so
+    its solution is unique) is unchanged: solved to 1e-10, warm and cold runs
+    agree to round-off after the warm-started steps, and at 1e-6 the warm run
+    needs fewer sweeps on a resting pile.  A start whose fc does not match
+    its impulses converges to a different solution and fails the first
+    check."""
+    import ctypes as C
+    from odemi import scenes
+    lib64 = po.oracle_lib("f64")
+    lib64.orc_set_colored_solver.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
+    lib64.orc_set_warm_start.argtypes = [C.c_double]
+    lib64.orc_colored_sweeps.restype = C.c_longlong
+    lib64.orc_colored_unconverged.restype = C.c_longlong
+    rng = np.random.default_rng(21)
+    sc = scenes.pile_world(rng, 24, 2, seed=3)
+    sc.mu = 0.0
+    st = po.run(lib64, sc, 80, 80)["traj"][-1]
+    sc.body_pos, sc.body_q = st[:, :3].copy(), st[:, 3:7].copy()
+    sc.body_lvel, sc.body_avel = st[:, 7:10].copy(), st[:, 10:13].copy()
+    out = {}
+    try:
+        lib64.orc_set_colored(1)
+        for tol, steps, cap in ((1e-10, 3, 1000000), (1e-6, 15, 400000)):
+            lib64.orc_set_colored_solver(tol, cap, 10, 1)
+            for warm in (0.0, 1.0):
+                lib64.orc_set_warm_start(warm)
+                lib64.orc_colored_sweeps()
+                lib64.orc_colored_unconverged()
+                t = po.run(lib64, sc, steps, steps)["traj"][-1]
+                out[tol, warm] = (t, lib64.orc_colored_sweeps(), lib64.orc_colored_unconverged())
+    finally:
+        lib64.orc_set_colored(0)
+        lib64.orc_set_colored_solver(0.0, 0, 10, 0)
+        lib64.orc_set_warm_start(0.0)
+    (tc, _, uc), (tw, _, uw) = out[1e-10, 0.0], out[1e-10, 1.0]
+    assert uc == uw == 0
+    assert np.abs(tw[:, :3] - tc[:, :3]).max() < 1e-9
+    assert np.abs(tw[:, 7:13] - tc[:, 7:13]).max() < 1e-7
+    sw_c, sw_w = out[1e-6, 0.0][1], out[1e-6, 1.0][1]
+    assert out[1e-6, 0.0][2] == out[1e-6, 1.0][2] == 0
+    assert sw_w < 0.85 * sw_c, (sw_w, sw_c)
