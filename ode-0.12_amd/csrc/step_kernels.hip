@@ -598,6 +598,16 @@ __device__ inline void relax_row(Row *rows, int r, float *fc) {
     for (int k = 0; k < 6; k++) f2[k] += delta * R.iMJ[6 + k];
 }
 
+// |lambda - clamp(lambda - d, lo, hi)| evaluated without the cancellation of
+// lambda - (lambda - d): |d| when the step stays inside the bounds, else the
+// distance from lambda to the bound it crosses
+__device__ inline float proj_residual(float lambda, float d, float lo, float hi) {
+  const float p = lambda - d;
+  if (p < lo) return fabsf(lambda - lo);
+  if (p > hi) return fabsf(hi - lambda);
+  return fabsf(d);
+}
+
 // complementarity residual of one row: |lambda - clamp(lambda - (A lambda - b)/A_ii)|
 __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
   const Row &R = rows[r];
@@ -612,9 +622,7 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
   float y = s + R.cfm * R.lambda - R.rhs;
   float lo = R.lo, hi = R.hi;
   if (R.findex >= 0) { hi = fabsf(R.hi * rows[R.findex].lambda); lo = -hi; }
-  float p = R.lambda - y / Aii;
-  p = fminf(fmaxf(p, lo), hi);
-  return fabsf(R.lambda - p);
+  return proj_residual(R.lambda, y / Aii, lo, hi);
 }
 
 // Coloured SOR-LCP.  The working set -- per-body fc, world inverse inertia
@@ -648,8 +656,7 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
       hi = fabsf(w.hi * sr[q.row0 + w.findex].lambda);
       lo = -hi;
     }
-    float p = fminf(fmaxf(w.lambda - y * (w.ad / sor_w), lo), hi);
-    r = fmaxf(r, fabsf(w.lambda - p));
+    r = fmaxf(r, proj_residual(w.lambda, y * (w.ad / sor_w), lo, hi));
   }
   return r;
 }
@@ -801,12 +808,14 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   // through M^-1 J^T p
   const int maxit = use_tol ? P.max_iterations : P.iterations;
   const int every = use_tol ? (P.check_every > 0 ? P.check_every : 1) : (maxit > 0 ? maxit : 1);
-  float rr_prev = 0.f;
+  float rr_prev = 0.f, res_prev = INFINITY;
+  bool acc = accel;        // CG steps still on for this solve
+  float res_last = 0.f;    // residual at the last check
   int it = 0;
   while (it < maxit) {
     const int n = every < maxit - it ? every : maxit - it;
     for (int rep = 0; rep < n; rep++) {
-      if (accel) {
+      if (acc) {
         // the pending momentum beta p_prev of the previous sweep (kept in
         // ld.x for the rows and fp for the bodies), then fc before this sweep
         for (int r = threadIdx.x; r < nrows; r += blockDim.x)
@@ -855,6 +864,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
             float nl = old + delta;
             if (nl < lo) { delta = lo - old; nl = lo; }
             else if (nl > hi) { delta = hi - old; nl = hi; }
+            else if (use_tol) delta = nl - old;   // fc follows the stored impulse
             rr.lambda = nl;
             if (k == 0) lam0 = nl;
             sor_push(r, delta, im1, iI1, f1, two, im2, iI2, f2);
@@ -865,7 +875,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         }
         __syncthreads();
       }
-      if (accel) {
+      if (acc) {
         // d = this sweep's change; beta = |d|^2 / |d_prev|^2 (restart when
         // > 1); the momentum beta p_prev is applied before the next sweep
         float loc = 0.f;
@@ -897,7 +907,20 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       float r = 0.f;
       for (int x = threadIdx.x; x < nact; x += blockDim.x) r = fmaxf(r, sor_cons_residual(sb, sr, sc[x], P.sor_w));
       r = block_max(r, sh);
+      res_last = r;
       if (r <= P.tol) break;
+      if (acc && !(r <= res_prev)) {
+        // the residual rose since the last check: restart the CG steps, and
+        // stop them for the rest of the solve when the residual is still far
+        // (> 1000 tol) from the tolerance
+        for (int q = threadIdx.x; q < nrows; q += blockDim.x) ld[q] = make_float2(0.f, 0.f);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+          for (int k = 0; k < 6; k++) { sa[b].fdir[k] = 0.f; sa[b].fp[k] = 0.f; }
+        rr_prev = 0.f;
+        if (r > 1000.f * P.tol) acc = false;
+        __syncthreads();
+      }
+      res_prev = r;
     }
   }
   if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = it;
@@ -910,7 +933,12 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = sb[b].fc[k];
   __syncthreads();
-  if (warm_on) {
+  if (warm_on && !(res_last <= 100.f * P.tol)) {
+    // a solve that ended at the cap far from the tolerance (residual > 100
+    // tol) is no starting point: the next solve starts from zero
+    if (threadIdx.x == 0) B.warm_n[c.w] = -1;
+    __syncthreads();
+  } else if (warm_on) {
     // this step's impulses, the next step's starting point; contacts are in
     // pair order, so the records come out sorted by key
     for (int i = threadIdx.x; i < nk; i += blockDim.x) {

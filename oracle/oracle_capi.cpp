@@ -81,7 +81,6 @@ extern "C" void orc_set_colored_solver(double tol, int max_iterations, int check
 }
 // coloured mode: start each solve from warm x the previous step's impulses
 extern "C" void orc_set_warm_start(double warm) { g_warm = warm; }
-extern "C" void orc_set_exact_delta(int on) { orc::g_exact_delta = on; }
 extern "C" long long orc_colored_sweeps(void) { long long s = g_sweeps; g_sweeps = 0; return s; }
 // tolerance-mode solves that ended at the iteration cap since the last call
 extern "C" long long orc_colored_unconverged(void) { long long s = g_unconv; g_unconv = 0; return s; }

@@ -20,7 +20,6 @@
 #include "orc_math.hpp"
 
 namespace orc {
-inline int g_exact_delta = 0;
 
 // ---------------------------------------------------------------- RNG
 // Linear congruential generator and the xor-folded bounded draw:
@@ -832,7 +831,7 @@ template <class R> struct World {
           o += jm[a];
           cid[a] = act[a] >= n_perm ? act[a] - n_perm : nk + act[a];
         }
-        if (prm.warm != 0) {
+        if (prm.warm != 0 && warm_ok) {
           for (size_t a = 0; a < act.size(); a++) {
             const Joint<R> &jt = joints[act[a]];
             const R *prev = nullptr;
@@ -1052,7 +1051,7 @@ template <class R> struct World {
             R nl = old + delta;
             if (nl < lo_a) { delta = lo_a - old; nl = lo_a; }
             else if (nl > hi_a) { delta = hi_a - old; nl = hi_a; }
-            else if (g_exact_delta && prm.tol > 0) delta = nl - old;
+            else if (prm.tol > 0) delta = nl - old;   // fc follows the stored impulse
             lambda[r] = nl;
             const R *M = &iMJ[12 * (size_t)r];
             for (int q = 0; q < 6; q++) f1[q] += delta * M[q];
@@ -1073,18 +1072,25 @@ template <class R> struct World {
         R y = s + cfm[r] * lambda[r] - b[r];
         R lo_a = lo[r], hi_a = hi[r];
         if (findex[r] != -1) { hi_a = std::fabs(hi[r] * lambda[findex[r]]); lo_a = -hi_a; }
-        R p = lambda[r] - y * (Ad[r] / prm.w);
-        p = std::min(std::max(p, lo_a), hi_a);
-        worst = std::max(worst, std::fabs(lambda[r] - p));
+        // |lambda - clamp(lambda - d)| without the cancellation of
+        // lambda - (lambda - d): |d| when the step stays inside the bounds
+        R d = y * (Ad[r] / prm.w);
+        R p = lambda[r] - d, e;
+        if (p < lo_a) e = lambda[r] - lo_a;
+        else if (p > hi_a) e = hi_a - lambda[r];
+        else e = d;
+        worst = std::max(worst, std::fabs(e));
       }
       return worst;
     };
     const bool use_tol = prm.tol > 0;
+    last_res = 0;
     const int maxit = use_tol ? prm.max_iterations : prm.iterations;
     std::vector<R> lprev(m, 0), dir(m, 0), pend(m, 0);
-    R rr_prev = 0;
+    R rr_prev = 0, res_prev = INFINITY;
+    bool accel_off = false;
     for (int it = 0; it < maxit; it++) {
-      if (prm.accel) {
+      if (prm.accel && !accel_off) {
         // the pending momentum beta p_prev of the previous sweep
         for (int r = 0; r < m; r++) {
             R dl = pend[r];
@@ -1099,7 +1105,7 @@ template <class R> struct World {
       }
       sweep();
       prm.sweeps++;
-      if (prm.accel) {
+      if (prm.accel && !accel_off) {
         // nonsmooth nonlinear CG: d = PGS(l) - l, beta = |d|^2 / |d_prev|^2,
         // restart when beta > 1; l += beta p_prev (before the next sweep),
         // p = d + beta p_prev
@@ -1113,7 +1119,21 @@ template <class R> struct World {
         }
         rr_prev = rr;
       }
-      if (use_tol && (it + 1) % prm.check_every == 0 && residual() <= prm.tol) return;
+      if (use_tol && (it + 1) % prm.check_every == 0) {
+        const R res = residual();
+        last_res = res;
+        if (res <= prm.tol) return;
+        if (prm.accel && !accel_off && !(res <= res_prev)) {
+          // the residual rose since the last check: restart the CG steps,
+          // and stop them for the rest of the solve when the residual is
+          // still far (> 1000 tol) from the tolerance
+          std::fill(pend.begin(), pend.end(), R(0));
+          std::fill(dir.begin(), dir.end(), R(0));
+          rr_prev = 0;
+          if (res > 1000 * prm.tol) accel_off = true;
+        }
+        res_prev = res;
+      }
     }
     if (use_tol) prm.unconverged++;
   }
@@ -1182,10 +1202,14 @@ template <class R> struct World {
         ab.insert(ab.end(), islands_b[k].begin(), islands_b[k].end());
         aj.insert(aj.end(), islands_j[k].begin(), islands_j[k].end());
       }
+      last_res = 0;
       quickstep_island(ab, aj, h);
       // this step's contact impulses, the next step's starting point
       warm_prev.clear();
-      if (prm.warm != 0)
+      // a solve that ended at the cap far from the tolerance (residual
+      // > 100 tol) is no starting point: the next one starts from zero
+      warm_ok = prm.warm != 0 && last_res <= 100 * prm.tol;
+      if (warm_ok)
         for (size_t k = n_perm; k < joints.size(); k++) {
           const Joint<R> &jt = joints[k];
           warm_prev.push_back({jt.wkey, std::array<R, 3>{jt.lam[0], jt.lam[1], jt.lam[2]}});
@@ -1208,6 +1232,8 @@ template <class R> struct World {
   }
   std::vector<std::vector<int>> islands_b, islands_j;
   std::vector<std::pair<long long, std::array<R, 3>>> warm_prev;   // sorted by key
+  R last_res = 0;   // residual at the last check of the last coloured solve (0: none)
+  bool warm_ok = false;   // warm_prev and the joints' impulses hold a usable start
 };
 
 }  // namespace orc
