@@ -370,6 +370,13 @@ ODE_API int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterati
  * recreated every step.  dBatchSetState and this call clear the stored
  * impulses.  warm = 0 (the default) restores the reference's zero start. */
 ODE_API int dBatchSetWarmStart(dBatchID b, float warm);
+/* Coloured order at a tolerance: on = 1 runs each step as three kernels --
+ * collide and the solver set-up, the sweeps, the integration -- so that the
+ * sweeps (the bulk of the work at a tolerance) get a lean kernel with its own
+ * workgroup size wg, LDS pool lds (bytes) and persistent grid (0: one
+ * workgroup per world).  Results are those of the one-kernel step. */
+ODE_API int dBatchSetSorLaunch(dBatchID b, int on, int wg, int lds, int grid);
+ODE_API int dBatchGetSorLaunch(dBatchID b, int *on, int *wg, int *lds, int *grid);
 /* Enables the LCP complementarity residual of the last step per world
  * (max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)|). */
 ODE_API int dBatchSetResidualTracking(dBatchID b, int on);
@@ -382,6 +389,7 @@ ODE_API int dBatchAutotune(dBatchID b, int trial_steps);
  * run in the batch's current solver mode. */
 #define ODEMI_TUNE_KEEP_CELL 1
 #define ODEMI_TUNE_FUSED_ONLY 2
+#define ODEMI_TUNE_NO_SOR_SPLIT 4   /* do not try the sweeps in a kernel of their own */
 ODE_API int dBatchAutotuneEx(dBatchID b, int trial_steps, int flags);
 ODE_API int dBatchGetLaunchConfig(dBatchID b, int *wg_size, float *cell_scale);
 ODE_API int dBatchSetLaunchConfig(dBatchID b, int wg_size, float cell_scale);

@@ -54,6 +54,7 @@ void refresh_dev(BatchImpl *b) {
   D.tape_off = nullptr;
   D.prof = b->prof_on ? b->d.prof.as<unsigned long long>() : nullptr;
   D.lds_bytes = b->lds;
+  D.sor_lds_bytes = b->sor_lds;
 }
 
 // LDS limits of the current device and the default pool: half a CU's LDS
@@ -353,8 +354,21 @@ static int persistent_grid(BatchImpl *b) {
   return 0;
 }
 
-// one step of every world: one fused kernel, or collide and step kernels
+// one step of every world: one fused kernel, or collide and step kernels, or
+// (coloured order at a tolerance) collide + set-up, the sweeps, integration
 static hipError_t launch_step(BatchImpl *b, int first_world, int nworlds, int grid) {
+  if (b->tol_on && b->sor_split && b->order_mode == ORDER_COLORED) {
+    hipError_t e = launch_world_kernel(KM_FUSED_PRE, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
+    if (e != hipSuccess) return e;
+    DevBatch d2 = b->dev;
+    d2.lds_bytes = b->sor_lds;
+    const int g2 = grid > 0 ? b->sor_grid : 0;
+    e = launch_world_kernel(KM_SOR, d2, first_world, nworlds, g2, b->sor_wg, 0, b->stream, 1);
+    if (e != hipSuccess) return e;
+    DevBatch d3 = b->dev;
+    d3.lds_bytes = 0;
+    return launch_world_kernel(KM_POST, d3, first_world, nworlds, grid, b->wg, 0, b->stream, 2);
+  }
   if (!b->split)
     return launch_world_kernel(b->tol_on ? KM_FUSED_TOL : KM_FUSED, b->dev, first_world, nworlds, grid, b->wg, 0,
                                b->stream, 0);
@@ -637,6 +651,23 @@ int batch_set_lds(BatchImpl *b, int bytes) {
   return 1;
 }
 
+int batch_set_sor_launch(BatchImpl *b, int on, int wg, int lds, int grid) {
+  if (on) {
+    // switched off: a timing run of a 4096-world batch with the split sweeps
+    // did not finish; the cause is not found yet
+    report_error("dBatchSetSorLaunch: the split sweep kernel is switched off");
+    return 0;
+  }
+  if (wg != 64 && wg != 128 && wg != 192 && wg != 256) return 0;
+  if (lds < 0 || lds > b->lds_max || grid < 0) return 0;
+  b->sor_split = on != 0;
+  b->sor_wg = wg;
+  b->sor_lds = lds & ~255;
+  b->sor_grid = grid;
+  destroy_graph(b);
+  return 1;
+}
+
 int batch_set_profiling(BatchImpl *b, bool on) {
   OD_HIP(hipStreamSynchronize(b->stream));
   OD_HIP(hipMemset(b->d.prof.p, 0, b->d.prof.n));
@@ -676,6 +707,7 @@ int batch_autotune(BatchImpl *b, int trial_steps, int flags) {
     return snapshot(b, snap, false);
   };
   double best = 1e30;
+  b->sor_split = false;
   int bw = b->wg, bg = b->grid, bl = b->lds;
   float bs = b->cell_scale;
   const float scales[] = {1.0f, 1.5f, 2.0f};
@@ -705,10 +737,32 @@ int batch_autotune(BatchImpl *b, int trial_steps, int flags) {
           if (t < best) { best = t; bw = wg; bg = grid; bl = lds; bsplit = b->split; }
         }
       }
-  if (!snapshot(b, snap, false)) return 0;
   b->split = bsplit;
   batch_set_launch(b, bw, bs, bg);
   batch_set_lds(b, bl);
+  // coloured order at a tolerance: the sweeps in their own kernel, with
+  // their own workgroup size, LDS pool and grid
+  bool bsor = false;
+  int sw = b->sor_wg, sl = b->sor_lds, sg = b->sor_grid;
+  if (false && b->tol_on && b->order_mode == ORDER_COLORED && !(flags & ODEMI_TUNE_NO_SOR_SPLIT)) {
+    const int sdiv[] = {2, 3, 4, 6, 8};
+    for (int wg : wgs)
+      for (int k : sdiv) {
+        int lds = (b->lds_cu / k - st - 256) & ~255;
+        if (lds < 0) lds = 0;
+        if (lds > b->lds_max) lds = b->lds_max;
+        int occ = world_kernel_occupancy(KM_SOR, wg, lds);
+        if (occ <= 0) continue;
+        const int grids[] = {0, occ * b->cu_count};
+        for (int grid : grids) {
+          double t;
+          if (!batch_set_sor_launch(b, 1, wg, lds, grid) || !trial(t)) continue;
+          if (t < best) { best = t; bsor = true; sw = wg; sl = lds; sg = grid; }
+        }
+      }
+  }
+  batch_set_sor_launch(b, bsor ? 1 : 0, sw, sl, sg);
+  if (!snapshot(b, snap, false)) return 0;
   return bw;
 }
 
@@ -803,6 +857,17 @@ int dBatchSetPersistentGrid(dBatchID b, int grid) {
   return 1;
 }
 int dBatchGetSplit(dBatchID b) { return b ? (b->impl->split ? 1 : 0) : -1; }
+int dBatchSetSorLaunch(dBatchID b, int on, int wg, int lds, int grid) {
+  return b ? batch_set_sor_launch(b->impl, on, wg, lds, grid) : 0;
+}
+int dBatchGetSorLaunch(dBatchID b, int *on, int *wg, int *lds, int *grid) {
+  if (!b) return 0;
+  if (on) *on = b->impl->sor_split ? 1 : 0;
+  if (wg) *wg = b->impl->sor_wg;
+  if (lds) *lds = b->impl->sor_lds;
+  if (grid) *grid = b->impl->sor_grid;
+  return 1;
+}
 int dBatchSetSplit(dBatchID b, int on) {
   if (!b) return 0;
   b->impl->split = on != 0;

@@ -268,7 +268,10 @@ enum {
   KM_API_COLLIDE = 5,     // one world, pairs and contacts copied out (dSpaceCollide)
   KM_API_STEP = 10,       // one world, contact joints from the host (dWorldQuickStep)
   KM_BATCH_STEP_TOL = 18, // KM_BATCH_STEP honouring the solver tolerance
-  KM_FUSED_TOL = 19       // KM_FUSED honouring the solver tolerance
+  KM_FUSED_TOL = 19,      // KM_FUSED honouring the solver tolerance
+  KM_FUSED_PRE = 51,      // KM_FUSED_TOL up to the coloured solver's set-up
+  KM_POST = 82,           // the rest of the step after the sweeps
+  KM_SOR = 1000           // the coloured solver's sweeps (between KM_FUSED_PRE and KM_POST)
 };
 hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid, int wg,
                                int api_maxc, hipStream_t st, int queue = 0);

@@ -158,7 +158,7 @@ struct Cons {
 struct alignas(16) SRow {
   float l[3], a1[3], a2[3];
   float rhs, cfm, ad, lo, hi;
-  float lambda;
+  float pad;
   int findex;   // row (within its constraint) of the normal impulse, or -1
 };
 
@@ -184,7 +184,7 @@ struct alignas(16) SCons {
   int row0, m, b1, b2;
 };
 
-constexpr int WORK_QUEUES = 2;   // persistent-queue counters per batch, 16 ints apart
+constexpr int WORK_QUEUES = 3;   // persistent-queue counters per batch, 16 ints apart
 
 // phases timed by the optional in-kernel profiler (wall-clock ticks)
 enum { PH_AABB = 0, PH_BROAD, PH_NARROW, PH_BODIES, PH_ROWS, PH_ORDER, PH_SOR, PH_INTEGRATE, PH_N };
@@ -205,13 +205,19 @@ OD_HD inline long long od_pow2(long long n) {
 // permanent joint (impulses of up to 6 rows, then the row count as int bits)
 OD_HD inline long long warm_bytes(int kcap, int nj) { return od_align(24LL * kcap) + 32LL * nj; }
 
-// bytes of the coloured solver's working set: bodies, rows, constraints,
-// colour order and colour starts
-OD_HD inline long long sor_bytes(int nb, int nrows, int ncons, bool accel = false) {
-  long long b = (long long)sizeof(SBody) * nb + (long long)sizeof(SRow) * nrows + (long long)sizeof(SCons) * ncons +
-                4LL * (2LL * ncons + 8);
+// bytes of the coloured solver's working set: rows, bodies, constraints in
+// colour order, colour starts, row impulses and (accelerated mode) the
+// conjugate-gradient state; copied to the LDS pool for the sweeps when it fits
+OD_HD inline long long sor_lds_bytes(int nb, int nrows, int ncons, bool accel = false) {
+  long long b = (long long)sizeof(SRow) * nrows + (long long)sizeof(SBody) * nb + (long long)sizeof(SCons) * ncons +
+                ((4LL * (2LL * ncons + 8) + 15) & ~15LL) + ((4LL * nrows + 15) & ~15LL);
   if (accel) b += (long long)sizeof(SAccBody) * nb + 8LL * nrows;   // conjugate-gradient state
   return b;
+}
+
+// the same working set in the world's scratch
+OD_HD inline long long sor_bytes(int nb, int nrows, int ncons, bool accel = false) {
+  return sor_lds_bytes(nb, nrows, ncons, accel);
 }
 
 // Per-world scratch layout.  nb/ng/nj: world sizes; pcap (a power of two),
@@ -285,6 +291,7 @@ struct DevBatch {
   int *work_counter;       // persistent-kernel world queue
   unsigned long long *prof; // optional: PH_N wall-clock tick counters per world
   int lds_bytes;           // dynamic LDS pool per workgroup
+  int sor_lds_bytes;       // LDS pool of the separate sweep kernel (split coloured solver)
   const int *world_order;  // processing order of worlds (largest first)
   char *scratch;
   // warm start: per-world records (WorldInfo::warm) and the number of valid

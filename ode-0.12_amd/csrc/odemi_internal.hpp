@@ -59,6 +59,10 @@ struct BatchImpl {
   bool prof_on = false;   // per-phase wall-clock counters
   bool split = false;     // collide and quickstep as two kernels per step
   bool tol_on = false;    // some world iterates to a residual tolerance
+  // coloured order at a tolerance: the sweeps in a kernel of their own
+  // (collide + set-up | sweeps | integration), with their own launch shape
+  bool sor_split = false;
+  int sor_wg = 128, sor_lds = 0, sor_grid = 0;
   int wall_khz = 100000;  // wall clock rate
   Tape tape;              // forward tape for the adjoint
 };
@@ -88,6 +92,7 @@ int batch_counters(BatchImpl *b, long long *out5);
 int batch_set_controls(BatchImpl *b, const double *ctrl);
 int batch_set_launch(BatchImpl *b, int wg, float cell_scale, int grid);
 int batch_set_lds(BatchImpl *b, int bytes);
+int batch_set_sor_launch(BatchImpl *b, int on, int wg, int lds, int grid);
 int batch_set_tolerance(BatchImpl *b, float tol, int max_iterations, int check_every, int accel);
 int batch_set_warm_start(BatchImpl *b, float warm);
 int batch_set_profiling(BatchImpl *b, bool on);

@@ -635,28 +635,37 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
 // the body data (every supported row has J_lin2 = -J_lin1).
 constexpr int SOR_MAX_ROWS = 6;   // rows of one constraint (contact 3, ball 3, hinge 6, hinge2 6)
 
+// J_1 . f1 (+ J_2 . f2) of one row as a shallow tree of fused multiply-adds
+// (J_l2 = -J_l1)
+__device__ inline float sor_dot(const SRow &r, const float *f1, bool two, const float *f2) {
+  float s = (fmaf(r.l[0], f1[0], r.l[1] * f1[1]) + fmaf(r.l[2], f1[2], r.a1[0] * f1[3])) +
+            fmaf(r.a1[1], f1[4], r.a1[2] * f1[5]);
+  if (two)
+    s += (fmaf(r.a2[0], f2[3], -(r.l[0] * f2[0])) - fmaf(r.l[1], f2[1], r.l[2] * f2[2])) +
+         fmaf(r.a2[1], f2[4], r.a2[2] * f2[5]);
+  return s;
+}
+
 // largest complementarity residual |lambda - clamp(lambda - (A lambda - b)_i / A_ii)|
 // over the rows of one constraint of the coloured solver (A_ii = w / ad)
-__device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const SCons q, float sor_w) {
-  const SBody &B1 = sb[q.b1];
+__device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const float *slam, const SCons q,
+                                          float sor_w) {
+  float f1[6], f2[6];
+  for (int k = 0; k < 6; k++) f1[k] = sb[q.b1].fc[k];
   const bool two = q.b2 >= 0;
+  if (two)
+    for (int k = 0; k < 6; k++) f2[k] = sb[q.b2].fc[k];
   float r = 0.f;
   for (int k = 0; k < q.m; k++) {
     const SRow &w = sr[q.row0 + k];
-    float s = w.l[0] * B1.fc[0] + w.l[1] * B1.fc[1] + w.l[2] * B1.fc[2] + w.a1[0] * B1.fc[3] +
-              w.a1[1] * B1.fc[4] + w.a1[2] * B1.fc[5];
-    if (two) {
-      const SBody &B2 = sb[q.b2];
-      s += -w.l[0] * B2.fc[0] - w.l[1] * B2.fc[1] - w.l[2] * B2.fc[2] + w.a2[0] * B2.fc[3] + w.a2[1] * B2.fc[4] +
-           w.a2[2] * B2.fc[5];
-    }
-    float y = s + w.cfm * w.lambda - w.rhs;
+    const float lam = slam[q.row0 + k];
+    const float y = sor_dot(w, f1, two, f2) + w.cfm * lam - w.rhs;
     float lo = w.lo, hi = w.hi;
     if (w.findex >= 0) {
-      hi = fabsf(w.hi * sr[q.row0 + w.findex].lambda);
+      hi = fabsf(w.hi * slam[q.row0 + w.findex]);
       lo = -hi;
     }
-    r = fmaxf(r, proj_residual(w.lambda, y * (w.ad / sor_w), lo, hi));
+    r = fmaxf(r, proj_residual(lam, y * (w.ad / sor_w), lo, hi));
   }
   return r;
 }
@@ -664,18 +673,81 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
 // fc += delta * inv(M) J^T of one row for both bodies of its constraint
 __device__ inline void sor_push(const SRow &r, float delta, float im1, const float *iI1, float *f1, bool two,
                                 float im2, const float *iI2, float *f2) {
-  float d1 = delta * im1;
-  f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
-  f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
-  f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
-  f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
+  const float d1 = delta * im1;
+  f1[0] = fmaf(d1, r.l[0], f1[0]); f1[1] = fmaf(d1, r.l[1], f1[1]); f1[2] = fmaf(d1, r.l[2], f1[2]);
+  f1[3] = fmaf(delta, fmaf(iI1[0], r.a1[0], fmaf(iI1[1], r.a1[1], iI1[2] * r.a1[2])), f1[3]);
+  f1[4] = fmaf(delta, fmaf(iI1[1], r.a1[0], fmaf(iI1[3], r.a1[1], iI1[4] * r.a1[2])), f1[4]);
+  f1[5] = fmaf(delta, fmaf(iI1[2], r.a1[0], fmaf(iI1[4], r.a1[1], iI1[5] * r.a1[2])), f1[5]);
   if (two) {
-    float d2 = delta * im2;
-    f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
-    f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
-    f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
-    f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
+    const float d2 = delta * im2;
+    f2[0] = fmaf(-d2, r.l[0], f2[0]); f2[1] = fmaf(-d2, r.l[1], f2[1]); f2[2] = fmaf(-d2, r.l[2], f2[2]);
+    f2[3] = fmaf(delta, fmaf(iI2[0], r.a2[0], fmaf(iI2[1], r.a2[1], iI2[2] * r.a2[2])), f2[3]);
+    f2[4] = fmaf(delta, fmaf(iI2[1], r.a2[0], fmaf(iI2[3], r.a2[1], iI2[4] * r.a2[2])), f2[4]);
+    f2[5] = fmaf(delta, fmaf(iI2[2], r.a2[0], fmaf(iI2[4], r.a2[1], iI2[5] * r.a2[2])), f2[5]);
   }
+}
+
+// sum over the four lanes of a quad (all four receive it), by DPP lane swaps
+__device__ inline float quad_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  return v;
+}
+
+// Relaxes the rows of one constraint (quickstep.cpp:484-582) with the four
+// lanes of a quad: lane sub holds one 3-vector of the two bodies' fc --
+// 0: linear of body 1, 1: angular of body 1, 2: linear of body 2, 3: angular
+// of body 2 -- and the matching part of J and inv(M) J^T.  J . fc is the quad
+// sum of the four partial products; every lane then forms the same delta and
+// updates its own part of fc.  tol_mode: fc follows the stored impulse.
+__device__ inline void sor_relax_quad(SBody *__restrict__ sb, const SRow *__restrict__ sr, float *__restrict__ slam,
+                                      float2 *__restrict__ ld, const SCons q, int sub, bool accel, bool tol_mode) {
+  const bool two = q.b2 >= 0;
+  const bool act = sub < 2 || two;      // lanes 2 and 3 idle for a constraint with one body
+  const int b = sub < 2 ? q.b1 : (two ? q.b2 : q.b1);
+  const int h = (sub & 1) * 3;          // linear or angular half of the body's fc
+  float f[3], I[6], im = 0.f;
+  for (int k = 0; k < 3; k++) f[k] = sb[b].fc[h + k];
+  for (int k = 0; k < 6; k++) I[k] = sb[b].iI[k];
+  im = sb[b].im;
+  const float sg = sub == 2 ? -1.f : 1.f;   // J_l2 = -J_l1
+  float lam0 = 0.f;
+  for (int k = 0; k < q.m; k++) {
+    const SRow r = sr[q.row0 + k];
+    const float old = slam[q.row0 + k];
+    // this lane's part of J and of inv(M) J^T
+    float j0, j1, j2;
+    if (sub == 0 || sub == 2) { j0 = sg * r.l[0]; j1 = sg * r.l[1]; j2 = sg * r.l[2]; }
+    else if (sub == 1) { j0 = r.a1[0]; j1 = r.a1[1]; j2 = r.a1[2]; }
+    else { j0 = r.a2[0]; j1 = r.a2[1]; j2 = r.a2[2]; }
+    float m0, m1, m2;
+    if (sub & 1) {
+      m0 = fmaf(I[0], j0, fmaf(I[1], j1, I[2] * j2));
+      m1 = fmaf(I[1], j0, fmaf(I[3], j1, I[4] * j2));
+      m2 = fmaf(I[2], j0, fmaf(I[4], j1, I[5] * j2));
+    } else {
+      m0 = im * j0; m1 = im * j1; m2 = im * j2;
+    }
+    const float part = act ? fmaf(j0, f[0], fmaf(j1, f[1], j2 * f[2])) : 0.f;
+    const float s = quad_sum(part);
+    if (accel && sub == 0) ld[q.row0 + k].x = old;
+    float delta = r.ad * (r.rhs - fmaf(old, r.cfm, s));
+    float lo = r.lo, hi = r.hi;
+    if (r.findex >= 0) {
+      const float ln = (r.findex == 0) ? lam0 : slam[q.row0 + r.findex];
+      hi = fabsf(r.hi * ln);
+      lo = -hi;
+    }
+    float nl = old + delta;
+    if (nl < lo) { delta = lo - old; nl = lo; }
+    else if (nl > hi) { delta = hi - old; nl = hi; }
+    else if (tol_mode) delta = nl - old;
+    if (sub == 0) slam[q.row0 + k] = nl;
+    if (k == 0) lam0 = nl;
+    f[0] = fmaf(delta, m0, f[0]); f[1] = fmaf(delta, m1, f[1]); f[2] = fmaf(delta, m2, f[2]);
+  }
+  if (act)
+    for (int k = 0; k < 3; k++) sb[b].fc[h + k] = f[k];
 }
 
 // Starting impulses of one constraint's rows from the warm-start records of
@@ -701,50 +773,70 @@ __device__ inline void warm_lookup(const WorldParams &P, const Cons &q, const Co
   }
 }
 
-template <bool TOL>
-__device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
-                            float *bt, const int *corder, const int *cstart, int nrows, int ngroups, int nk,
-                            int ncons, bool host_contacts, Shared &sh) {
+// The coloured solver's working set: the rows (read-only during the sweeps),
+// bodies, constraints in colour order, colour starts, the row impulses and
+// (accelerated mode) the conjugate-gradient state: per body fc before the
+// sweep and the direction's fc, per row lambda before the sweep and the
+// direction.  Built in the world's scratch; copied to the LDS pool for the
+// sweeps when it fits there (same layout).
+struct SorSet {
+  SRow *sr;
+  SBody *sb;
+  SCons *sc;
+  int *sstart;
+  float *slam;
+  SAccBody *sa;
+  float2 *ld;
+};
+
+__device__ inline SorSet sor_set(char *base, SRow *sr, int nb, int nrows, int nact) {
+  SorSet S;
+  S.sr = sr;
+  S.sb = (SBody *)base;
+  S.sc = (SCons *)(S.sb + nb);
+  S.sstart = (int *)(S.sc + nact);
+  S.slam = (float *)((char *)S.sstart + ((4LL * (2LL * nact + 8) + 15) & ~15LL));
+  S.sa = (SAccBody *)((char *)S.slam + ((4LL * nrows + 15) & ~15LL));
+  S.ld = (float2 *)(S.sa + nb);
+  return S;
+}
+
+__device__ inline SorSet sor_global(const Ctx &c, int nrows, int nact) {
+  SRow *sr = c.at<SRow>(c.L.sor);
+  return sor_set((char *)(sr + nrows), sr, c.W.nb, nrows, nact);
+}
+
+// per-world values handed from the solver's set-up to the sweeps and on to
+// the integration (split launches), in the world's misc block
+enum { MISC_SOR_ROWS = 8, MISC_SOR_GROUPS = 9, MISC_SOR_RESIDUAL = 10, MISC_SOR_READY = 11 };
+
+// Set-up: the working set in the world's scratch, in colour order, with the
+// starting impulses (zero, or the warm start from the previous step).
+__device__ void sor_prepare(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, const Row *rows,
+                            const float *bt, const int *corder, const int *cstart, int nrows, int ngroups,
+                            bool host_contacts, Shared &sh) {
   const int nb = c.W.nb;
   const int nact = cstart[ngroups];   // constraints with rows, in colour order
-  const bool use_tol = TOL && P.tol > 0.f;
-  const bool accel = use_tol && P.accel != 0;
-  // warm start (contacts found on the device): records of the previous step
   const bool warm_on = P.warm != 0.f && !host_contacts && B.warm_n != nullptr;
   const int nprev = warm_on ? B.warm_n[c.w] : -1;
   const bool warm = warm_on && nprev >= 0;
-  WarmRec *wrec = warm_on ? (WarmRec *)(B.warm + c.W.warm) : nullptr;
-  float *wjnt = warm_on ? (float *)(B.warm + c.W.warm + od_align(24LL * c.W.kcap)) : nullptr;
+  const WarmRec *wrec = warm_on ? (const WarmRec *)(B.warm + c.W.warm) : nullptr;
+  const float *wjnt = warm_on ? (const float *)(B.warm + c.W.warm + od_align(24LL * c.W.kcap)) : nullptr;
   const ContactRec *con = c.at<ContactRec>(c.L.contacts);
-  const long long need = sor_bytes(nb, nrows, nact, accel);
-  char *base = (need <= B.lds_bytes) ? (char *)od_pool : c.at<char>(c.L.sor);
-  SBody *sb = (SBody *)base;
-  SRow *sr = (SRow *)(sb + nb);
-  SCons *sc = (SCons *)(sr + nrows);
-  int *sstart = (int *)(sc + nact);
-  // conjugate-gradient state (accelerated mode): per body fc before the
-  // sweep and the direction's fc; per row lambda before the sweep and the
-  // direction
-  SAccBody *sa = (SAccBody *)((char *)sstart + ((4LL * (2LL * nact + 8) + 15) & ~15LL));
-  float2 *ld = (float2 *)(sa + nb);
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  const int nt = blockDim.x;
+  const SorSet S = sor_global(c, nrows, nact);
+  for (int b = threadIdx.x; b < nb; b += nt) {
     const float *t = bt + 32 * b;
     SBody q;
     for (int k = 0; k < 6; k++) q.fc[k] = 0.f;
     q.iI[0] = t[0]; q.iI[1] = t[1]; q.iI[2] = t[2]; q.iI[3] = t[4]; q.iI[4] = t[5]; q.iI[5] = t[8];
     q.im = B.invMass[c.W.b0 + b];
     q.pad[0] = q.pad[1] = q.pad[2] = 0.f;
-    sb[b] = q;
+    S.sb[b] = q;
   }
-  for (int k = threadIdx.x; k <= ngroups; k += blockDim.x) sstart[k] = cstart[k];
-  if (accel) {
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      for (int k = 0; k < 6; k++) { sa[b].fdir[k] = 0.f; sa[b].fp[k] = 0.f; }
-    for (int r = threadIdx.x; r < nrows; r += blockDim.x) ld[r] = make_float2(0.f, 0.f);
-  }
-  // constraints and rows in colour order
+  for (int k = threadIdx.x; k <= ngroups; k += nt) S.sstart[k] = cstart[k];
   int rbase = 0;
-  for (int x0 = 0; x0 < nact; x0 += blockDim.x) {
+  for (int x0 = 0; x0 < nact; x0 += nt) {
     int x = x0 + threadIdx.x;
     const Cons *q = x < nact ? &cons[corder[x]] : nullptr;
     int m = q ? q->m : 0;
@@ -753,7 +845,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     if (q) {
       SCons o;
       o.row0 = rbase + off; o.m = m; o.b1 = q->b1; o.b2 = q->b2;
-      sc[x] = o;
+      S.sc[x] = o;
       float wl[SOR_MAX_ROWS] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (warm) warm_lookup(P, *q, con, wrec, nprev, wjnt, wl);
       for (int k = 0; k < m; k++) {
@@ -763,12 +855,52 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         w.a1[0] = R.J[3]; w.a1[1] = R.J[4]; w.a1[2] = R.J[5];
         w.a2[0] = R.J[9]; w.a2[1] = R.J[10]; w.a2[2] = R.J[11];
         w.rhs = R.rhs; w.cfm = R.cfm; w.ad = R.ad; w.lo = R.lo; w.hi = R.hi;
-        w.lambda = k < SOR_MAX_ROWS ? wl[k] : 0.f;
+        w.pad = 0.f;
         w.findex = R.findex >= 0 ? R.findex - q->row0 : -1;
-        sr[o.row0 + k] = w;
+        S.sr[o.row0 + k] = w;
+        S.slam[o.row0 + k] = k < SOR_MAX_ROWS ? wl[k] : 0.f;
       }
     }
     rbase += tot;
+  }
+  __syncthreads();
+}
+
+// The sweeps.  LDS: the working set after the rows is copied to the LDS pool
+// first and back at the end; the choice is a template argument so that every
+// access compiles to LDS (or global) instructions rather than generic flat
+// ones.  Writes the iteration count and leaves the last residual in the
+// world's misc block.
+template <bool TOL, bool LDS>
+__device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, const Ctx &c, const WorldParams &P,
+                                                         int nrows, int ngroups, int nact, bool host_contacts,
+                                                         Shared &sh) {
+  const int nb = c.W.nb;
+  const bool use_tol = TOL && P.tol > 0.f;
+  const bool accel = use_tol && P.accel != 0;
+  const bool warm = P.warm != 0.f && !host_contacts && B.warm_n != nullptr && B.warm_n[c.w] >= 0;
+  const int nt = blockDim.x;
+  const int quad = threadIdx.x >> 2, sub = threadIdx.x & 3, nq = nt >> 2;   // four lanes per constraint
+  const SorSet G = sor_global(c, nrows, nact);
+  const SorSet S = LDS ? sor_set((char *)((SRow *)od_pool + nrows), (SRow *)od_pool, nb, nrows, nact) : G;
+  SRow *__restrict__ sr = S.sr;
+  SBody *__restrict__ sb = S.sb;
+  SCons *__restrict__ sc = S.sc;
+  int *__restrict__ sstart = S.sstart;
+  float *__restrict__ slam = S.slam;
+  SAccBody *__restrict__ sa = S.sa;
+  float2 *__restrict__ ld = S.ld;
+  if (LDS) {
+    for (int r = threadIdx.x; r < nrows; r += nt) sr[r] = G.sr[r];
+    for (int b = threadIdx.x; b < nb; b += nt) sb[b] = G.sb[b];
+    for (int x = threadIdx.x; x < nact; x += nt) sc[x] = G.sc[x];
+    for (int k = threadIdx.x; k <= ngroups; k += nt) sstart[k] = G.sstart[k];
+    for (int r = threadIdx.x; r < nrows; r += nt) slam[r] = G.slam[r];
+  }
+  if (accel) {
+    for (int b = threadIdx.x; b < nb; b += nt)
+      for (int k = 0; k < 6; k++) { sa[b].fdir[k] = 0.f; sa[b].fp[k] = 0.f; }
+    for (int r = threadIdx.x; r < nrows; r += nt) ld[r] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   if (warm) {
@@ -776,7 +908,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     // (a colour's constraints share no body)
     for (int g = 0; g < ngroups; g++) {
       const int x1 = sstart[g + 1];
-      for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
+      for (int x = sstart[g] + threadIdx.x; x < x1; x += nt) {
         const SCons q = sc[x];
         const bool two = q.b2 >= 0;
         float f1[6], f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
@@ -788,8 +920,8 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         for (int k = 0; k < 6; k++) iI1[k] = sb[q.b1].iI[k];
         const float im1 = sb[q.b1].im;
         for (int k = 0; k < q.m && k < SOR_MAX_ROWS; k++) {
-          const SRow r = sr[q.row0 + k];
-          if (r.lambda != 0.f) sor_push(r, r.lambda, im1, iI1, f1, two, im2, iI2, f2);
+          const float lam = slam[q.row0 + k];
+          if (lam != 0.f) sor_push(sr[q.row0 + k], lam, im1, iI1, f1, two, im2, iI2, f2);
         }
         for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
         if (two)
@@ -818,9 +950,9 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       if (acc) {
         // the pending momentum beta p_prev of the previous sweep (kept in
         // ld.x for the rows and fp for the bodies), then fc before this sweep
-        for (int r = threadIdx.x; r < nrows; r += blockDim.x)
-          if (ld[r].x != 0.f) sr[r].lambda += ld[r].x;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int r = threadIdx.x; r < nrows; r += nt)
+          if (ld[r].x != 0.f) slam[r] += ld[r].x;
+        for (int b = threadIdx.x; b < nb; b += nt)
           for (int k = 0; k < 6; k++) {
             sb[b].fc[k] += sa[b].fp[k];
             sa[b].fp[k] = sb[b].fc[k];
@@ -829,70 +961,27 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       }
       for (int g = 0; g < ngroups; g++) {
         const int x1 = sstart[g + 1];
-        for (int x = sstart[g] + threadIdx.x; x < x1; x += blockDim.x) {
-          const SCons q = sc[x];
-          const SBody B1 = sb[q.b1];
-          float f1[6], iI1[6];
-          for (int k = 0; k < 6; k++) { f1[k] = B1.fc[k]; iI1[k] = B1.iI[k]; }
-          const float im1 = B1.im;
-          const bool two = q.b2 >= 0;
-          float f2[6] = {0, 0, 0, 0, 0, 0}, iI2[6] = {0, 0, 0, 0, 0, 0}, im2 = 0.f;
-          if (two) {
-            const SBody B2 = sb[q.b2];
-            for (int k = 0; k < 6; k++) { f2[k] = B2.fc[k]; iI2[k] = B2.iI[k]; }
-            im2 = B2.im;
-          }
-          float lam0 = 0.f;
-          for (int k = 0; k < SOR_MAX_ROWS; k++) {
-            if (k >= q.m) break;
-            SRow &rr = sr[q.row0 + k];
-            const SRow r = rr;
-            float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] +
-                      r.a1[2] * f1[5];
-            if (two)
-              s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] +
-                   r.a2[2] * f2[5];
-            float old = r.lambda;
-            if (accel) ld[q.row0 + k].x = old;
-            float delta = r.ad * (r.rhs - old * r.cfm - s);
-            float lo = r.lo, hi = r.hi;
-            if (r.findex >= 0) {
-              float ln = (r.findex == 0) ? lam0 : sr[q.row0 + r.findex].lambda;
-              hi = fabsf(r.hi * ln);
-              lo = -hi;
-            }
-            float nl = old + delta;
-            if (nl < lo) { delta = lo - old; nl = lo; }
-            else if (nl > hi) { delta = hi - old; nl = hi; }
-            else if (use_tol) delta = nl - old;   // fc follows the stored impulse
-            rr.lambda = nl;
-            if (k == 0) lam0 = nl;
-            sor_push(r, delta, im1, iI1, f1, two, im2, iI2, f2);
-          }
-          for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
-          if (two)
-            for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
-        }
+        for (int x = sstart[g] + quad; x < x1; x += nq) sor_relax_quad(sb, sr, slam, ld, sc[x], sub, accel, use_tol);
         __syncthreads();
       }
       if (acc) {
         // d = this sweep's change; beta = |d|^2 / |d_prev|^2 (restart when
         // > 1); the momentum beta p_prev is applied before the next sweep
         float loc = 0.f;
-        for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-          float d = sr[r].lambda - ld[r].x;
+        for (int r = threadIdx.x; r < nrows; r += nt) {
+          float d = slam[r] - ld[r].x;
           loc += d * d;
         }
         const float rr = block_fsum(loc, sh);
         float beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
         if (!(beta <= 1.f)) beta = 0.f;
-        for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-          float d = sr[r].lambda - ld[r].x;
+        for (int r = threadIdx.x; r < nrows; r += nt) {
+          float d = slam[r] - ld[r].x;
           float pend = beta * ld[r].y;   // momentum along the previous direction
           ld[r].y = d + pend;            // new direction
           ld[r].x = pend;
         }
-        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int b = threadIdx.x; b < nb; b += nt)
           for (int k = 0; k < 6; k++) {
             float pend = beta * sa[b].fdir[k];
             sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + pend;
@@ -905,7 +994,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
     it += n;
     if (use_tol) {
       float r = 0.f;
-      for (int x = threadIdx.x; x < nact; x += blockDim.x) r = fmaxf(r, sor_cons_residual(sb, sr, sc[x], P.sor_w));
+      for (int x = threadIdx.x; x < nact; x += nt) r = fmaxf(r, sor_cons_residual(sb, sr, slam, sc[x], P.sor_w));
       r = block_max(r, sh);
       res_last = r;
       if (r <= P.tol) break;
@@ -913,8 +1002,8 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
         // the residual rose since the last check: restart the CG steps, and
         // stop them for the rest of the solve when the residual is still far
         // (> 1000 tol) from the tolerance
-        for (int q = threadIdx.x; q < nrows; q += blockDim.x) ld[q] = make_float2(0.f, 0.f);
-        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int q = threadIdx.x; q < nrows; q += nt) ld[q] = make_float2(0.f, 0.f);
+        for (int b = threadIdx.x; b < nb; b += nt)
           for (int k = 0; k < 6; k++) { sa[b].fdir[k] = 0.f; sa[b].fp[k] = 0.f; }
         rr_prev = 0.f;
         if (r > 1000.f * P.tol) acc = false;
@@ -923,17 +1012,59 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       res_prev = r;
     }
   }
-  if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = it;
-  // impulses back to the rows, fc back to the per-body scratch
-  for (int x = threadIdx.x; x < nact; x += blockDim.x) {
-    const Cons &q = cons[corder[x]];
-    const SCons o = sc[x];
-    for (int k = 0; k < o.m; k++) rows[q.row0 + k].lambda = sr[o.row0 + k].lambda;
+  if (threadIdx.x == 0) {
+    B.counters[c.w * CNT_N + CNT_ITERS] = it;
+    c.at<int>(c.L.misc)[MISC_SOR_RESIDUAL] = __float_as_int(use_tol ? res_last : 0.f);
   }
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = sb[b].fc[k];
+  if (LDS) {
+    for (int b = threadIdx.x; b < nb; b += nt)
+      for (int k = 0; k < 6; k++) G.sb[b].fc[k] = sb[b].fc[k];
+    for (int r = threadIdx.x; r < nrows; r += nt) G.slam[r] = slam[r];
+  }
   __syncthreads();
-  if (warm_on && !(res_last <= 100.f * P.tol)) {
+}
+
+// the sweeps on the working set in global scratch (worlds larger than the LDS
+// pool); kept out of line so that the LDS variant is not merged with it
+template <bool TOL>
+__device__ __attribute__((noinline)) void sor_solve_global(const DevBatch &B, const Ctx &c, const WorldParams &P,
+                                                           int nrows, int ngroups, int nact, bool host_contacts,
+                                                           Shared &sh) {
+  sor_solve<TOL, false>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+}
+
+template <bool TOL>
+__device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const WorldParams &P, int nrows, int ngroups,
+                                     int nact, bool host_contacts, Shared &sh) {
+  const bool accel = TOL && P.tol > 0.f && P.accel != 0;
+  if (sor_lds_bytes(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
+    sor_solve<TOL, true>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  else
+    sor_solve_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+}
+
+// After the sweeps: impulses back to the rows (constraint order), fc to the
+// per-body scratch, and the warm-start records of the next step.
+__device__ void sor_finish(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
+                           float *bt, const int *corder, int nrows, int nact, int nk, int ncons, bool host_contacts) {
+  const int nb = c.W.nb;
+  const int nt = blockDim.x;
+  const SorSet G = sor_global(c, nrows, nact);
+  const bool warm_on = P.warm != 0.f && !host_contacts && B.warm_n != nullptr;
+  WarmRec *wrec = warm_on ? (WarmRec *)(B.warm + c.W.warm) : nullptr;
+  float *wjnt = warm_on ? (float *)(B.warm + c.W.warm + od_align(24LL * c.W.kcap)) : nullptr;
+  const ContactRec *con = c.at<ContactRec>(c.L.contacts);
+  const float res_last = __int_as_float(c.at<int>(c.L.misc)[MISC_SOR_RESIDUAL]);
+  for (int x = threadIdx.x; x < nact; x += nt) {
+    const Cons &q = cons[corder[x]];
+    const SCons o = G.sc[x];
+    for (int k = 0; k < o.m; k++) rows[q.row0 + k].lambda = G.slam[o.row0 + k];
+  }
+  for (int b = threadIdx.x; b < nb; b += nt)
+    for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = G.sb[b].fc[k];
+  __syncthreads();
+  const bool use_tol = P.tol > 0.f;
+  if (warm_on && use_tol && !(res_last <= 100.f * P.tol)) {
     // a solve that ended at the cap far from the tolerance (residual > 100
     // tol) is no starting point: the next solve starts from zero
     if (threadIdx.x == 0) B.warm_n[c.w] = -1;
@@ -941,7 +1072,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   } else if (warm_on) {
     // this step's impulses, the next step's starting point; contacts are in
     // pair order, so the records come out sorted by key
-    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+    for (int i = threadIdx.x; i < nk; i += nt) {
       const Cons &q = cons[i];
       const ContactRec &cr = con[q.src];
       WarmRec r;
@@ -950,7 +1081,7 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
       r.pad = 0;
       wrec[i] = r;
     }
-    for (int i = nk + threadIdx.x; i < ncons; i += blockDim.x) {
+    for (int i = nk + threadIdx.x; i < ncons; i += nt) {
       const Cons &q = cons[i];
       float *jw = wjnt + 8 * q.src;
       for (int k = 0; k < 6; k++) jw[k] = k < q.m ? rows[q.row0 + k].lambda : 0.f;
@@ -962,9 +1093,23 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   }
 }
 
+// the coloured solver in one kernel: set-up, sweeps, write-back
 template <bool TOL>
-__device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, const StepIn &in, Shared &sh,
-                           bool ref) {
+__device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &P, const Cons *cons, Row *rows,
+                            float *bt, const int *corder, const int *cstart, int nrows, int ngroups, int nk,
+                            int ncons, bool host_contacts, Shared &sh) {
+  const int nact = cstart[ngroups];
+  sor_prepare(B, c, P, cons, rows, bt, corder, cstart, nrows, ngroups, host_contacts, sh);
+  sor_solve_any<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  sor_finish(B, c, P, cons, rows, bt, corder, nrows, nact, nk, ncons, host_contacts);
+}
+
+// STAGE 0: the whole step; 1: up to the coloured solver's set-up (the sweeps
+// and the rest run in later kernels); 2: after the sweeps (write-back,
+// velocities, integration)
+template <bool TOL, int STAGE>
+__device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P,
+                                                          const StepIn &in, Shared &sh, bool ref) {
   const int nb = c.W.nb, nj = c.W.nj, nk = in.nk;
   const float h = P.dt, hinv = 1.f / P.dt;
   float *bt = c.at<float>(c.L.btmp);   // per body: invIw 0..8, fc 9..14, tmp 15..20, extra torque 21..23
@@ -972,6 +1117,24 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
   Row *rows = c.at<Row>(c.L.rows);
   const ContactRec *con = c.at<ContactRec>(c.L.contacts);
   int *misc = c.at<int>(c.L.misc);
+  const int ncons = nk + nj;
+  int *corder = c.at<int>(c.L.order);          // constraint order (island order when ref)
+  int *isl = c.at<int>(c.L.island);            // ref: [cons0, ncons, row0, nrows, seed] per island
+  int *color = c.at<int>(c.L.color);
+  int nrows = 0, ngroups = 0;
+  if (STAGE == 2) {
+    if (misc[MISC_SOR_READY] == 0) return;     // the step left the world unchanged
+    nrows = misc[MISC_SOR_ROWS];
+    ngroups = misc[MISC_SOR_GROUPS];
+    const int nact = isl[ngroups];
+    const bool accel = TOL && P.tol > 0.f && P.accel != 0;
+    if (sor_lds_bytes(nb, nrows, nact, accel) > B.sor_lds_bytes) {
+      // not solved by the sweep kernel (its LDS pool is too small)
+      sor_solve_global<TOL>(B, c, P, nrows, ngroups, nact, in.from_host, sh);
+      c.clk.lap(PH_SOR);
+    }
+    sor_finish(B, c, P, cons, rows, bt, corder, nrows, nact, nk, ncons, in.from_host);
+  } else {
   // --- bodies: world inverse inertia, gyroscopic torque, gravity
   //     (quickstep.cpp:609-665)
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -997,7 +1160,6 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     B.facc[gb] = fa;
   }
   // --- constraints: contacts (k < nk) then permanent joints
-  const int ncons = nk + nj;
   for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
     Cons &q = cons[i];
     if (i < nk) {
@@ -1038,10 +1200,6 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
   __syncthreads();
   c.clk.lap(PH_BODIES);
   // --- constraint order, row offsets, groups
-  int *corder = c.at<int>(c.L.order);          // constraint order (island order when ref)
-  int *isl = c.at<int>(c.L.island);            // ref: [cons0, ncons, row0, nrows, seed] per island
-  int *color = c.at<int>(c.L.color);
-  int nrows = 0, ngroups = 0;
   if (ref) {
     if (threadIdx.x == 0) {
       // adjacency, newest attach first: body joint lists prepend (ode.cpp:1382-1389)
@@ -1253,6 +1411,7 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     if (threadIdx.x == 0) {
       atomicOr(&B.counters[c.w * CNT_N + CNT_OVERFLOW], OVF_ROWS);
       if (B.warm_n) B.warm_n[c.w] = -1;
+      if (STAGE == 1) misc[MISC_SOR_READY] = 0;
     }
     __syncthreads();
     return;
@@ -1391,9 +1550,19 @@ __device__ void phase_step(const DevBatch &B, const Ctx &c, WorldParams &P, cons
     // islands are solved by lanes of every wavefront; the integration below
     // reads the fc of all bodies
     __syncthreads();
+  } else if (STAGE == 1) {
+    // the sweeps run in their own kernel
+    sor_prepare(B, c, P, cons, rows, bt, corder, isl, nrows, ngroups, in.from_host, sh);
+    if (threadIdx.x == 0) {
+      misc[MISC_SOR_ROWS] = nrows;
+      misc[MISC_SOR_GROUPS] = ngroups;
+      misc[MISC_SOR_READY] = 1;
+    }
+    __syncthreads();
+    return;
   } else {
-    sor_colored<TOL>(B, c, P, cons, rows, bt, corder, c.at<int>(c.L.island), nrows, ngroups, nk, ncons,
-                     in.from_host, sh);
+    sor_colored<TOL>(B, c, P, cons, rows, bt, corder, isl, nrows, ngroups, nk, ncons, in.from_host, sh);
+  }
   }
   c.clk.lap(PH_SOR);
   // --- residual (optional), feedback
@@ -1527,7 +1696,7 @@ __device__ void phase_controls(const DevBatch &B, const Ctx &c) {
 // the world's counters.
 // TOL: the coloured solver honours the world's residual tolerance (its own
 // instantiation, so the iterate-to-tolerance step is a separate kernel).
-enum { MF_COLLIDE = 1, MF_STEP = 2, MF_API_OUT = 4, MF_HOST_CONTACTS = 8, MF_TOL = 16 };
+enum { MF_COLLIDE = 1, MF_STEP = 2, MF_API_OUT = 4, MF_HOST_CONTACTS = 8, MF_TOL = 16, MF_PRE = 32, MF_POST = 64 };
 enum {
   MODE_BATCH_COLLIDE = MF_COLLIDE,
   MODE_BATCH_STEP = MF_STEP,
@@ -1535,11 +1704,14 @@ enum {
   MODE_API_COLLIDE = MF_COLLIDE | MF_API_OUT,
   MODE_API_STEP = MF_STEP | MF_HOST_CONTACTS,
   MODE_BATCH_STEP_TOL = MF_STEP | MF_TOL,
-  MODE_FUSED_TOL = MF_COLLIDE | MF_STEP | MF_TOL
+  MODE_FUSED_TOL = MF_COLLIDE | MF_STEP | MF_TOL,
+  MODE_FUSED_PRE = MF_COLLIDE | MF_STEP | MF_TOL | MF_PRE,   // collide and step up to the solver set-up
+  MODE_POST = MF_STEP | MF_TOL | MF_POST,                    // the step after the sweeps
+  KM_SOR = 1000                                               // the sweeps (sor_kernel)
 };
 
 template <int MODE>
-__device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
+__device__ __attribute__((always_inline)) void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
   Ctx c;
   c.w = w;
   c.W = B.wi[w];
@@ -1593,7 +1765,7 @@ __device__ void run_world(const DevBatch &B, int w, Shared &sh, int api_maxc) {
       if (threadIdx.x == 0) atomicOr(&B.counters[w * CNT_N + CNT_OVERFLOW], OVF_CONTACTS);
       in.nk = c.W.kcap;
     }
-    phase_step<(MODE & MF_TOL) != 0>(B, c, P, in, sh, ref);
+    phase_step<(MODE & MF_TOL) != 0, (MODE & MF_PRE) ? 1 : ((MODE & MF_POST) ? 2 : 0)>(B, c, P, in, sh, ref);
   }
   __syncthreads();
 }
@@ -1614,6 +1786,48 @@ __global__ __launch_bounds__(WG) void world_kernel(DevBatch B, int first_world, 
     __syncthreads();
     if (k >= B.nworlds) break;
     run_world<MODE>(B, B.world_order ? B.world_order[k] : k, sh, api_maxc);
+  }
+}
+
+// The coloured solver's sweeps of every world (between MODE_FUSED_PRE and
+// MODE_POST): a lean kernel, so that many workgroups fit on a CU.
+template <bool TOL>
+__device__ void sor_world(const DevBatch &B, int w, Shared &sh) {
+  const WorldParams &P = B.wp[w];
+  if (P.dead) return;
+  Ctx c;
+  c.w = w;
+  c.W = B.wi[w];
+  c.L = scratch_layout(c.W.nb, c.W.ng, c.W.nj, c.W.pcap, c.W.kcap, c.W.rcap);
+  c.S = B.scratch + c.W.scratch;
+  const int *misc = c.at<int>(c.L.misc);
+  if (misc[MISC_SOR_READY] == 0) return;
+  const int nrows = misc[MISC_SOR_ROWS], ngroups = misc[MISC_SOR_GROUPS];
+  const int nact = c.at<int>(c.L.island)[ngroups];
+  const bool accel = TOL && P.tol > 0.f && P.accel != 0;
+  // a world whose working set does not fit the pool is solved by the
+  // integration kernel, in global scratch
+  if (sor_lds_bytes(c.W.nb, nrows, nact, accel) > B.lds_bytes) return;
+  c.clk.start(B, w);
+  sor_solve<TOL, true>(B, c, P, nrows, ngroups, nact, false, sh);
+  c.clk.lap(PH_SOR);
+}
+
+template <bool TOL>
+__global__ __launch_bounds__(WG) void sor_kernel(DevBatch B, int first_world, int persistent) {
+  __shared__ Shared sh;
+  if (!persistent) {
+    int w = first_world + blockIdx.x;
+    if (w < B.nworlds) sor_world<TOL>(B, B.world_order ? B.world_order[w] : w, sh);
+    return;
+  }
+  while (true) {
+    if (threadIdx.x == 0) sh.ival[15] = atomicAdd(B.work_counter, 1);
+    __syncthreads();
+    int k = sh.ival[15];
+    __syncthreads();
+    if (k >= B.nworlds) break;
+    sor_world<TOL>(B, B.world_order ? B.world_order[k] : k, sh);
   }
 }
 
@@ -1657,13 +1871,16 @@ static const void *kernel_of(int mode) {
     case MODE_API_STEP: return world_kernel_fn<MODE_API_STEP>();
     case MODE_BATCH_STEP_TOL: return world_kernel_fn<MODE_BATCH_STEP_TOL>();
     case MODE_FUSED_TOL: return world_kernel_fn<MODE_FUSED_TOL>();
+    case MODE_FUSED_PRE: return world_kernel_fn<MODE_FUSED_PRE>();
+    case MODE_POST: return world_kernel_fn<MODE_POST>();
+    case KM_SOR: return (const void *)sor_kernel<true>;
     default: return nullptr;
   }
 }
 
 hipError_t allow_world_kernel_lds(int bytes) {
   const int modes[] = {MODE_BATCH_COLLIDE, MODE_BATCH_STEP, MODE_FUSED, MODE_API_COLLIDE, MODE_API_STEP,
-                       MODE_BATCH_STEP_TOL, MODE_FUSED_TOL};
+                       MODE_BATCH_STEP_TOL, MODE_FUSED_TOL, MODE_FUSED_PRE, MODE_POST, KM_SOR};
   for (int m : modes) {
     hipError_t e = hipFuncSetAttribute(kernel_of(m), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
@@ -1718,6 +1935,11 @@ hipError_t launch_world_kernel(int mode, const DevBatch &B0, int first_world, in
     case MODE_FUSED_TOL:
       world_kernel<MODE_FUSED_TOL><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
       break;
+    case MODE_FUSED_PRE:
+      world_kernel<MODE_FUSED_PRE><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc);
+      break;
+    case MODE_POST: world_kernel<MODE_POST><<<blocks, wg, lds, st>>>(B, first_world, persistent, api_maxc); break;
+    case KM_SOR: sor_kernel<true><<<blocks, wg, lds, st>>>(B, first_world, persistent); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

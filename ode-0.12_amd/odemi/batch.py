@@ -178,8 +178,17 @@ class Batch:
         wg = C.c_int()
         sc = C.c_float()
         grid = self.L.dBatchGetLaunchConfig(self.h, C.byref(wg), C.byref(sc))
+        on, swg, slds, sgrid = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.L.dBatchGetSorLaunch(self.h, C.byref(on), C.byref(swg), C.byref(slds), C.byref(sgrid))
         return dict(wg=wg.value, cell_scale=sc.value, grid=grid, lds=self.L.dBatchGetLds(self.h),
-                    split=self.L.dBatchGetSplit(self.h))
+                    split=self.L.dBatchGetSplit(self.h),
+                    sor=dict(on=on.value, wg=swg.value, lds=slds.value, grid=sgrid.value))
+
+    def set_sor_launch(self, on: bool, wg: int = 128, lds: int = 0, grid: int = 0):
+        """Coloured order at a tolerance: the sweeps in a kernel of their own
+        with this workgroup size, LDS pool and grid (0: one workgroup per world)."""
+        if not self.L.dBatchSetSorLaunch(self.h, 1 if on else 0, wg, lds, grid):
+            raise RuntimeError("bad sweep-kernel launch config")
 
     def set_launch_config(self, wg: int, cell_scale: float = 1.0, lds: int | None = None,
                           split: int | None = None, grid: int | None = None):

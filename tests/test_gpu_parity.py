@@ -427,3 +427,39 @@ def test_accelerated_tolerance_solver_matches_restatement(odemi_mod, warm):
         err = float(np.abs(st[offs[w]:offs[w + 1], :3] - t32[:, :3]).max())
         say(f"world {w}: err {err:.3g} envelope {env:.3g}")
         assert err <= 4 * env + 1e-3
+
+
+@pytest.mark.skip(reason="split sweep kernel switched off: a 4096-world timing run with it did not finish; "
+                         "cause not found")
+def test_split_sweep_kernel_matches_fused(odemi_mod):
+    """The coloured solver at a tolerance with its sweeps in a kernel of their
+    own (collide + set-up | sweeps | integration) runs the same arithmetic as
+    the one-kernel step: states agree bit for bit, with an LDS pool that holds
+    every world and with one so small that the integration kernel solves the
+    worlds it cannot hold.  (Same workgroup size as the one-kernel step: the
+    block sums of the conjugate-gradient step follow the workgroup size.)"""
+    ws = scenes.uneven_worlds(24, seed=17)
+    n = 12
+    ref = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    ref.set_launch_config(128, 1.0, 65536, 0, 0)
+    ref.set_tolerance(1e-4, 400, 10, accel=True)
+    ref.set_warm_start(1.0)
+    ref.step(n)
+    want = ref.state()
+    want_it = ref.world_counters()[:, 5]
+    ref.close()
+    for wg, lds, grid in ((128, 65536, 0), (128, 8192, 0), (128, 32768, 64)):
+        say("split sweep kernel", wg, lds, grid)
+        b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+        b.set_launch_config(128, 1.0, 65536, 0, 0)
+        b.set_tolerance(1e-4, 400, 10, accel=True)
+        b.set_warm_start(1.0)
+        b.set_sor_launch(True, wg, lds, grid)
+        b.step(n)
+        got = b.state()
+        got_it = b.world_counters()[:, 5]
+        cfg = b.launch_config()
+        b.close()
+        assert cfg["sor"]["on"] == 1
+        assert np.array_equal(got_it, want_it)
+        assert np.array_equal(got, want), float(np.abs(got - want).max())
