@@ -635,14 +635,11 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
 // the body data (every supported row has J_lin2 = -J_lin1).
 constexpr int SOR_MAX_ROWS = 6;   // rows of one constraint (contact 3, ball 3, hinge 6, hinge2 6)
 
-// J_1 . f1 (+ J_2 . f2) of one row as a shallow tree of fused multiply-adds
-// (J_l2 = -J_l1)
+// J_1 . f1 (+ J_2 . f2) of one row (J_l2 = -J_l1), summed in the order written
 __device__ inline float sor_dot(const SRow &r, const float *f1, bool two, const float *f2) {
-  float s = (fmaf(r.l[0], f1[0], r.l[1] * f1[1]) + fmaf(r.l[2], f1[2], r.a1[0] * f1[3])) +
-            fmaf(r.a1[1], f1[4], r.a1[2] * f1[5]);
+  float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] + r.a1[2] * f1[5];
   if (two)
-    s += (fmaf(r.a2[0], f2[3], -(r.l[0] * f2[0])) - fmaf(r.l[1], f2[1], r.l[2] * f2[2])) +
-         fmaf(r.a2[1], f2[4], r.a2[2] * f2[5]);
+    s += -r.l[0] * f2[0] - r.l[1] * f2[1] - r.l[2] * f2[2] + r.a2[0] * f2[3] + r.a2[1] * f2[4] + r.a2[2] * f2[5];
   return s;
 }
 
@@ -673,81 +670,71 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
 // fc += delta * inv(M) J^T of one row for both bodies of its constraint
 __device__ inline void sor_push(const SRow &r, float delta, float im1, const float *iI1, float *f1, bool two,
                                 float im2, const float *iI2, float *f2) {
-  const float d1 = delta * im1;
-  f1[0] = fmaf(d1, r.l[0], f1[0]); f1[1] = fmaf(d1, r.l[1], f1[1]); f1[2] = fmaf(d1, r.l[2], f1[2]);
-  f1[3] = fmaf(delta, fmaf(iI1[0], r.a1[0], fmaf(iI1[1], r.a1[1], iI1[2] * r.a1[2])), f1[3]);
-  f1[4] = fmaf(delta, fmaf(iI1[1], r.a1[0], fmaf(iI1[3], r.a1[1], iI1[4] * r.a1[2])), f1[4]);
-  f1[5] = fmaf(delta, fmaf(iI1[2], r.a1[0], fmaf(iI1[4], r.a1[1], iI1[5] * r.a1[2])), f1[5]);
+  float d1 = delta * im1;
+  f1[0] += d1 * r.l[0]; f1[1] += d1 * r.l[1]; f1[2] += d1 * r.l[2];
+  f1[3] += delta * (iI1[0] * r.a1[0] + iI1[1] * r.a1[1] + iI1[2] * r.a1[2]);
+  f1[4] += delta * (iI1[1] * r.a1[0] + iI1[3] * r.a1[1] + iI1[4] * r.a1[2]);
+  f1[5] += delta * (iI1[2] * r.a1[0] + iI1[4] * r.a1[1] + iI1[5] * r.a1[2]);
   if (two) {
-    const float d2 = delta * im2;
-    f2[0] = fmaf(-d2, r.l[0], f2[0]); f2[1] = fmaf(-d2, r.l[1], f2[1]); f2[2] = fmaf(-d2, r.l[2], f2[2]);
-    f2[3] = fmaf(delta, fmaf(iI2[0], r.a2[0], fmaf(iI2[1], r.a2[1], iI2[2] * r.a2[2])), f2[3]);
-    f2[4] = fmaf(delta, fmaf(iI2[1], r.a2[0], fmaf(iI2[3], r.a2[1], iI2[4] * r.a2[2])), f2[4]);
-    f2[5] = fmaf(delta, fmaf(iI2[2], r.a2[0], fmaf(iI2[4], r.a2[1], iI2[5] * r.a2[2])), f2[5]);
+    float d2 = delta * im2;
+    f2[0] -= d2 * r.l[0]; f2[1] -= d2 * r.l[1]; f2[2] -= d2 * r.l[2];
+    f2[3] += delta * (iI2[0] * r.a2[0] + iI2[1] * r.a2[1] + iI2[2] * r.a2[2]);
+    f2[4] += delta * (iI2[1] * r.a2[0] + iI2[3] * r.a2[1] + iI2[4] * r.a2[2]);
+    f2[5] += delta * (iI2[2] * r.a2[0] + iI2[4] * r.a2[1] + iI2[5] * r.a2[2]);
   }
 }
 
-// sum over the four lanes of a quad (all four receive it), by DPP lane swaps
-__device__ inline float quad_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-  return v;
-}
-
-// Relaxes the rows of one constraint (quickstep.cpp:484-582) with the four
-// lanes of a quad: lane sub holds one 3-vector of the two bodies' fc --
-// 0: linear of body 1, 1: angular of body 1, 2: linear of body 2, 3: angular
-// of body 2 -- and the matching part of J and inv(M) J^T.  J . fc is the quad
-// sum of the four partial products; every lane then forms the same delta and
-// updates its own part of fc.  tol_mode: fc follows the stored impulse.
-__device__ inline void sor_relax_quad(SBody *__restrict__ sb, const SRow *__restrict__ sr, float *__restrict__ slam,
-                                      float2 *__restrict__ ld, const SCons q, int sub, bool accel, bool tol_mode) {
+// Relaxes the rows of one constraint with one lane (quickstep.cpp:484-582):
+// both bodies' fc in registers, the rows read three at a time so that the
+// LDS latency is paid once per group of rows.  tol_mode: fc follows the
+// stored impulse.
+__device__ inline void sor_relax(SBody *__restrict__ sb, const SRow *__restrict__ sr, float *__restrict__ slam,
+                                 float2 *__restrict__ ld, const SCons q, bool accel, bool tol_mode) {
   const bool two = q.b2 >= 0;
-  const bool act = sub < 2 || two;      // lanes 2 and 3 idle for a constraint with one body
-  const int b = sub < 2 ? q.b1 : (two ? q.b2 : q.b1);
-  const int h = (sub & 1) * 3;          // linear or angular half of the body's fc
-  float f[3], I[6], im = 0.f;
-  for (int k = 0; k < 3; k++) f[k] = sb[b].fc[h + k];
-  for (int k = 0; k < 6; k++) I[k] = sb[b].iI[k];
-  im = sb[b].im;
-  const float sg = sub == 2 ? -1.f : 1.f;   // J_l2 = -J_l1
-  float lam0 = 0.f;
-  for (int k = 0; k < q.m; k++) {
-    const SRow r = sr[q.row0 + k];
-    const float old = slam[q.row0 + k];
-    // this lane's part of J and of inv(M) J^T
-    float j0, j1, j2;
-    if (sub == 0 || sub == 2) { j0 = sg * r.l[0]; j1 = sg * r.l[1]; j2 = sg * r.l[2]; }
-    else if (sub == 1) { j0 = r.a1[0]; j1 = r.a1[1]; j2 = r.a1[2]; }
-    else { j0 = r.a2[0]; j1 = r.a2[1]; j2 = r.a2[2]; }
-    float m0, m1, m2;
-    if (sub & 1) {
-      m0 = fmaf(I[0], j0, fmaf(I[1], j1, I[2] * j2));
-      m1 = fmaf(I[1], j0, fmaf(I[3], j1, I[4] * j2));
-      m2 = fmaf(I[2], j0, fmaf(I[4], j1, I[5] * j2));
-    } else {
-      m0 = im * j0; m1 = im * j1; m2 = im * j2;
-    }
-    const float part = act ? fmaf(j0, f[0], fmaf(j1, f[1], j2 * f[2])) : 0.f;
-    const float s = quad_sum(part);
-    if (accel && sub == 0) ld[q.row0 + k].x = old;
-    float delta = r.ad * (r.rhs - fmaf(old, r.cfm, s));
-    float lo = r.lo, hi = r.hi;
-    if (r.findex >= 0) {
-      const float ln = (r.findex == 0) ? lam0 : slam[q.row0 + r.findex];
-      hi = fabsf(r.hi * ln);
-      lo = -hi;
-    }
-    float nl = old + delta;
-    if (nl < lo) { delta = lo - old; nl = lo; }
-    else if (nl > hi) { delta = hi - old; nl = hi; }
-    else if (tol_mode) delta = nl - old;
-    if (sub == 0) slam[q.row0 + k] = nl;
-    if (k == 0) lam0 = nl;
-    f[0] = fmaf(delta, m0, f[0]); f[1] = fmaf(delta, m1, f[1]); f[2] = fmaf(delta, m2, f[2]);
+  const SBody B1 = sb[q.b1];
+  SBody B2;
+  if (two) B2 = sb[q.b2];
+  float f1[6], iI1[6], f2[6], iI2[6], im2 = 0.f;
+  for (int k = 0; k < 6; k++) { f1[k] = B1.fc[k]; iI1[k] = B1.iI[k]; f2[k] = 0.f; iI2[k] = 0.f; }
+  const float im1 = B1.im;
+  if (two) {
+    for (int k = 0; k < 6; k++) { f2[k] = B2.fc[k]; iI2[k] = B2.iI[k]; }
+    im2 = B2.im;
   }
-  if (act)
-    for (int k = 0; k < 3; k++) sb[b].fc[h + k] = f[k];
+  float lam0 = 0.f;
+  for (int k0 = 0; k0 < q.m; k0 += 3) {
+    SRow R3[3];
+    float L3[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (k0 + j < q.m) { R3[j] = sr[q.row0 + k0 + j]; L3[j] = slam[q.row0 + k0 + j]; }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int k = k0 + j;
+      if (k >= q.m) break;
+      const SRow &r = R3[j];
+      const float s = sor_dot(r, f1, two, f2);
+      const float old = L3[j];
+      if (accel) ld[q.row0 + k].x = old;
+      float delta = r.ad * (r.rhs - old * r.cfm - s);
+      float lo = r.lo, hi = r.hi;
+      if (r.findex >= 0) {
+        float ln = (r.findex == 0) ? lam0 : slam[q.row0 + r.findex];
+        hi = fabsf(r.hi * ln);
+        lo = -hi;
+      }
+      float nl = old + delta;
+      if (nl < lo) { delta = lo - old; nl = lo; }
+      else if (nl > hi) { delta = hi - old; nl = hi; }
+      else if (tol_mode) delta = nl - old;
+      slam[q.row0 + k] = nl;
+      if (k == 0) lam0 = nl;
+      sor_push(r, delta, im1, iI1, f1, two, im2, iI2, f2);
+    }
+  }
+  for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
+  if (two)
+    for (int k = 0; k < 6; k++) sb[q.b2].fc[k] = f2[k];
 }
 
 // Starting impulses of one constraint's rows from the warm-start records of
@@ -880,7 +867,6 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   const bool accel = use_tol && P.accel != 0;
   const bool warm = P.warm != 0.f && !host_contacts && B.warm_n != nullptr && B.warm_n[c.w] >= 0;
   const int nt = blockDim.x;
-  const int quad = threadIdx.x >> 2, sub = threadIdx.x & 3, nq = nt >> 2;   // four lanes per constraint
   const SorSet G = sor_global(c, nrows, nact);
   const SorSet S = LDS ? sor_set((char *)((SRow *)od_pool + nrows), (SRow *)od_pool, nb, nrows, nact) : G;
   SRow *__restrict__ sr = S.sr;
@@ -961,7 +947,7 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
       }
       for (int g = 0; g < ngroups; g++) {
         const int x1 = sstart[g + 1];
-        for (int x = sstart[g] + quad; x < x1; x += nq) sor_relax_quad(sb, sr, slam, ld, sc[x], sub, accel, use_tol);
+        for (int x = sstart[g] + threadIdx.x; x < x1; x += nt) sor_relax(sb, sr, slam, ld, sc[x], accel, use_tol);
         __syncthreads();
       }
       if (acc) {
