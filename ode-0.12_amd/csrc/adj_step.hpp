@@ -73,7 +73,7 @@ struct StepAdj {
 
 // scratch sizes
 constexpr int ADJ_BODY_FLOATS = 64;
-constexpr int ADJ_ROW_FLOATS = 32;
+constexpr int ADJ_ROW_FLOATS = 33;
 template <class T> OD_HD inline long long adj_scratch_elems(int nb, int nk) {
   return (long long)ADJ_BODY_FLOATS * nb + (long long)ADJ_ROW_FLOATS * 3 * nk + 16;
 }
@@ -98,7 +98,8 @@ enum {
   RJB = 12,      // J_bar 12
   RLB = 24,      // lambda_bar
   RY = 25,       // BiCGSTAB x (the adjoint y)
-  RR = 26, RRH = 27, RP = 28, RV = 29, RS = 30, RT = 31   // BiCGSTAB vectors
+  RR = 26, RRH = 27, RP = 28, RV = 29, RS = 30, RT = 31,  // BiCGSTAB vectors
+  RYB = 32       // the iterate with the smallest residual so far
 };
 
 template <class T> OD_HD inline V3<T> ld3(const T *p) { return v3(p[0], p[1], p[2]); }
@@ -408,6 +409,13 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
   if (gnorm2 > T(0)) {
     T rho0 = T(1), alpha = T(1), omega = T(1);
     const T stop2 = P.lin_tol * P.lin_tol * gnorm2;
+    // safeguard: the iterate with the smallest residual is kept (RYB, from
+    // y = 0 with residual |g|^2); a non-finite or exploding residual ends the
+    // iteration and the kept iterate is used
+    T best2 = gnorm2;
+    for (int k = tid; k < nk; k += nt)
+      for (int r = 0; r < 3; r++) rsc[ADJ_ROW_FLOATS * (3 * k + r) + RYB] = T(0);
+    bool use_best = false;
     // fields: RY x, RR r, RRH r_hat, RP p, RV v, RS s, RT t; a hat vector is
     // kept in the J_bar-free slots RJB..: no -- use the dedicated scratch
     // below (wsc is scalars only), so the hat vectors reuse RS / RT in turn.
@@ -487,7 +495,25 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
         }
       T rr = par.sum(loc);
       rho0 = rho1;
+      if (!(rr <= T(1e12) * gnorm2)) { use_best = true; iters++; break; }   // diverged or not finite
+      if (rr < best2) {
+        best2 = rr;
+        for (int k = tid; k < nk; k += nt)
+          for (int r = 0; r < 3; r++) {
+            T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
+            R[RYB] = R[RY];
+          }
+      }
       if (rr <= stop2 || omega == T(0)) { iters++; break; }
+      if (iters + 1 == P.lin_maxit && rr > best2) use_best = true;   // ended worse than the best iterate
+    }
+    if (use_best) {
+      par.sync();
+      for (int k = tid; k < nk; k += nt)
+        for (int r = 0; r < 3; r++) {
+          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
+          R[RY] = R[RYB];
+        }
     }
   }
   if (tid == 0 && G.stats) G.stats[0] += iters;
