@@ -20,9 +20,10 @@ def _scenes():
     return [contact_scene(4, 4, (0,), bounce=0.5), contact_scene(5, 4, (1,)), contact_scene(6, 6, (0, 1))]
 
 
-def _gpu_grad(odemi_mod, scs, interval=10):
+def _gpu_grad(odemi_mod, scs, interval=10, warm=0.0):
     b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
     b.set_tolerance(1e-6, 4000, 10)
+    b.set_warm_start(warm)
     b.tape_options(interval)
     b.tape_begin(N)
     b.step(N)
@@ -67,11 +68,16 @@ def test_gpu_adjoint_matches_central_differences_and_host(odemi_mod):
         off += sc.nb
 
 
-def test_checkpoint_interval_does_not_change_gradient(odemi_mod):
+@pytest.mark.parametrize("warm", [0.0, 1.0], ids=["cold", "warm"])
+def test_checkpoint_interval_does_not_change_gradient(odemi_mod, warm):
+    """Segments are replayed from checkpoints; a checkpoint holds the
+    warm-start records too, so replays of warm-started steps reproduce the
+    forward and the gradient does not depend on the interval."""
     scs = _scenes()
-    say("adjoint: checkpoint interval 3 vs 40")
-    a = _gpu_grad(odemi_mod, scs, interval=3)
-    b = _gpu_grad(odemi_mod, scs, interval=40)
+    say("adjoint: checkpoint interval 3 vs 40, warm start", warm)
+    a = _gpu_grad(odemi_mod, scs, interval=3, warm=warm)
+    b = _gpu_grad(odemi_mod, scs, interval=40, warm=warm)
+    assert a[4]["collider_mismatches"] == 0 and b[4]["collider_mismatches"] == 0
     scale = float(np.abs(a[1]).max())
     assert np.abs(a[1] - b[1]).max() <= 1e-5 * scale
     assert np.allclose(a[2], b[2], rtol=1e-5, atol=1e-6 * scale)
