@@ -141,6 +141,83 @@ __device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh, in
   }
 }
 
+// LDS bytes the block radix sort needs for a workgroup of nt lanes: digit
+// bases (256) and per-wavefront digit counts of one tile
+__host__ __device__ inline int radix_lds_bytes(int nt) { return 4 * (256 + 256 * ((nt + 63) / 64)); }
+
+// Ascending LSD radix sort of n distinct u64 keys in global memory by one
+// workgroup, 8-bit digits over the bits that are set in some key; tmp holds n
+// keys.  Each pass counts digits (LDS atomics), scans the 256 counts and
+// scatters tile by tile, stably: the rank of a key among the equal digits of
+// its wavefront comes from eight ballots, the wavefronts of a tile are
+// ordered through per-wavefront counts in LDS.  The result is in keys.
+__device__ inline void block_radix_sort(unsigned long long *keys, unsigned long long *tmp, int n, Shared &sh) {
+  const int nt = blockDim.x, nw = (nt + 63) >> 6;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int *base = (int *)od_pool;      // 256 digit bases
+  int *wc = base + 256;            // per wavefront: 256 digit counts of the current tile
+  unsigned long long any = 0;
+  for (int i = threadIdx.x; i < n; i += nt) any |= keys[i];
+  // bits present in some key (OR over the workgroup, 32 bits at a time)
+  unsigned lo = (unsigned)any, hi = (unsigned)(any >> 32);
+  for (int o = 32; o > 0; o >>= 1) { lo |= __shfl_xor(lo, o, 64); hi |= __shfl_xor(hi, o, 64); }
+  if (lane == 0) { sh.ival[0] = 0; sh.ival[1] = 0; }
+  __syncthreads();
+  if (lane == 0) { atomicOr(&sh.ival[0], (int)lo); atomicOr(&sh.ival[1], (int)hi); }
+  __syncthreads();
+  const unsigned long long all = ((unsigned long long)(unsigned)sh.ival[1] << 32) | (unsigned)sh.ival[0];
+  __syncthreads();
+  int passes = 0;
+  while (passes < 8 && (all >> (8 * passes)) != 0) passes++;
+  unsigned long long *src = keys, *dst = tmp;
+  for (int p = 0; p < passes; p++) {
+    const int shift = 8 * p;
+    for (int d = threadIdx.x; d < 256; d += nt) base[d] = 0;
+    for (int d = threadIdx.x; d < 256 * nw; d += nt) wc[d] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += nt) atomicAdd(&base[(int)((src[i] >> shift) & 255)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int d = 0; d < 256; d++) { int c = base[d]; base[d] = run; run += c; }
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < n; t0 += nt) {
+      const int i = t0 + threadIdx.x;
+      const bool valid = i < n;
+      const unsigned long long k = valid ? src[i] : 0ULL;
+      const int d = (int)((k >> shift) & 255);
+      unsigned long long peers = __ballot(valid);
+      for (int b = 0; b < 8; b++) {
+        const unsigned long long m = __ballot(valid && ((d >> b) & 1));
+        peers &= ((d >> b) & 1) ? m : ~m;
+      }
+      const unsigned long long lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+      const int rank = __popcll(peers & lt);
+      const int cnt = __popcll(peers);
+      if (valid && rank == cnt - 1) wc[wid * 256 + d] = cnt;
+      __syncthreads();
+      if (valid) {
+        int off = base[d] + rank;
+        for (int w = 0; w < wid; w++) off += wc[w * 256 + d];
+        dst[off] = k;
+      }
+      __syncthreads();
+      for (int dd = threadIdx.x; dd < 256; dd += nt) {
+        int tot = 0;
+        for (int w = 0; w < nw; w++) { tot += wc[w * 256 + dd]; wc[w * 256 + dd] = 0; }
+        base[dd] += tot;
+      }
+      __syncthreads();
+    }
+    unsigned long long *t = src; src = dst; dst = t;
+  }
+  if (src != keys) {
+    for (int i = threadIdx.x; i < n; i += nt) keys[i] = src[i];
+    __syncthreads();
+  }
+}
+
 __device__ inline int pow2_at_least(int n) {
   int p = 1;
   while (p < n) p <<= 1;
@@ -348,9 +425,17 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
   }
   const int ne = base;
   const int np2 = pow2_at_least(ne < 2 ? 2 : ne);
-  for (int i = ne + threadIdx.x; i < np2; i += blockDim.x) cells[i] = ~0ULL;
-  __syncthreads();
-  bitonic_sort(cells, np2, sh, B.lds_bytes);
+  if ((long long)np2 * 8 > B.lds_bytes && B.lds_bytes >= radix_lds_bytes(blockDim.x) && ne <= c.W.pcap) {
+    // too many keys for the LDS pool: radix sort in global memory (pkeys,
+    // pcap entries written only by the pair scan below, holds the second
+    // buffer)
+    __syncthreads();
+    block_radix_sort(cells, pkeys, ne, sh);
+  } else {
+    for (int i = ne + threadIdx.x; i < np2; i += blockDim.x) cells[i] = ~0ULL;
+    __syncthreads();
+    bitonic_sort(cells, np2, sh, B.lds_bytes);
+  }
   // pair scan, counting pass then writing pass
   auto visit = [&](int i, bool write, int &cnt, int out0) {
     unsigned long long ei = cells[i];
@@ -404,13 +489,26 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     }
     npairs += tot;
   }
-  // big geoms against the rest, then big against big
+  // big geoms against the rest, then big against big; the big geoms first
+  // listed in ascending index order (ecount is free scratch here)
+  int *blist = ecount;
+  int nbig = 0;
+  for (int start = 0; start < ng; start += blockDim.x) {
+    int g = start + threadIdx.x;
+    int isb = (g < ng && gbig[g]) ? 1 : 0;
+    int tot;
+    int off = block_excl_scan(isb, tot, sh);
+    if (isb) blist[nbig + off] = g;
+    nbig += tot;
+  }
+  __syncthreads();
   for (int start = 0; start < ng; start += blockDim.x) {
     int g = start + threadIdx.x;
     int cnt = 0;
     if (g < ng)
-      for (int b = 0; b < ng; b++) {
-        if (!gbig[b] || b == g) continue;
+      for (int t = 0; t < nbig; t++) {
+        const int b = blist[t];
+        if (b == g) continue;
         if (gbig[g] && !(b > g)) continue;  // big-big once
         if (pair_passes(B, c, g, b, aabb)) cnt++;
       }
@@ -418,8 +516,9 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     int off = block_excl_scan(cnt, tot, sh);
     if (cnt) {
       int o = npairs + off;
-      for (int b = 0; b < ng; b++) {
-        if (!gbig[b] || b == g) continue;
+      for (int t = 0; t < nbig; t++) {
+        const int b = blist[t];
+        if (b == g) continue;
         if (gbig[g] && !(b > g)) continue;
         if (!pair_passes(B, c, g, b, aabb)) continue;
         if (o < c.W.pcap) {
@@ -453,9 +552,16 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     // with it the colouring) does not depend on the grid
     int np = pow2_at_least(npairs < 2 ? 2 : npairs);
     if (np > c.W.pcap) np = c.W.pcap;  // pcap is a power of two
-    for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
-    __syncthreads();
-    bitonic_sort(pkeys, np, sh, B.lds_bytes);
+    if ((long long)np * 8 > B.lds_bytes && B.lds_bytes >= radix_lds_bytes(blockDim.x)) {
+      // radix sort in global memory; the pair records (written after the
+      // sort) hold the second buffer
+      __syncthreads();
+      block_radix_sort(pkeys, (unsigned long long *)pairs, npairs, sh);
+    } else {
+      for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
+      __syncthreads();
+      bitonic_sort(pkeys, np, sh, B.lds_bytes);
+    }
   }
   if (!ref) {
     for (int i = threadIdx.x; i < npairs; i += blockDim.x) {
