@@ -1714,29 +1714,51 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
   }
   // --- space list after dGeomMoved of every stepped body: moved geoms move
   //     to the front in reverse move order (collision_space.cpp:47-75, 209-213)
-  if (ref && threadIdx.x == 0) {
-    int nib = misc[3];
+  if (ref) {
+    // in parallel: old rank -> geom; the moved geoms in move order (bodies
+    // in island visit order, each body's geoms newest first) take ranks
+    // M-1..0; the others follow in their old relative order
+    const int nib = misc[3];
     const int *border = corder + ncons;
     int *old = isl + 5 * (ngroups + 1);
     const int ng = c.W.ng;
     int *inv = old;                     // old rank -> geom
-    for (int g = 0; g < ng; g++) inv[B.grank[c.W.g0 + g]] = g;
     int *moved = old + ng;
-    int M = 0;
-    for (int x = 0; x < nib; x++) {
-      int gb = c.W.b0 + border[x];
-      for (int k = B.bgeom_start[gb]; k < B.bgeom_start[gb + 1]; k++) moved[M++] = B.bgeom[k];
+    int *flag = moved + ng;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+      inv[B.grank[c.W.g0 + g]] = g;
+      flag[g] = 0;
     }
-    int *flag = moved + M;
-    for (int g = 0; g < ng; g++) flag[g] = 0;
-    for (int x = 0; x < M; x++) {
+    __syncthreads();
+    int M = 0;
+    for (int x0 = 0; x0 < nib; x0 += blockDim.x) {
+      const int x = x0 + threadIdx.x;
+      int k0 = 0, cnt = 0;
+      if (x < nib) {
+        const int gb = c.W.b0 + border[x];
+        k0 = B.bgeom_start[gb];
+        cnt = B.bgeom_start[gb + 1] - k0;
+      }
+      int tot;
+      const int off = block_excl_scan(cnt, tot, sh);
+      for (int j = 0; j < cnt; j++) moved[M + off + j] = B.bgeom[k0 + j];
+      M += tot;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < M; x += blockDim.x) {
       B.grank[c.W.g0 + moved[x]] = M - 1 - x;
       flag[moved[x]] = 1;
     }
+    __syncthreads();
     int r = M;
-    for (int k = 0; k < ng; k++) {
-      int g = inv[k];
-      if (!flag[g]) B.grank[c.W.g0 + g] = r++;
+    for (int k0 = 0; k0 < ng; k0 += blockDim.x) {
+      const int k = k0 + threadIdx.x;
+      const int g = k < ng ? inv[k] : -1;
+      const int keep = (g >= 0 && !flag[g]) ? 1 : 0;
+      int tot;
+      const int off = block_excl_scan(keep, tot, sh);
+      if (keep) B.grank[c.W.g0 + g] = r + off;
+      r += tot;
     }
   }
   if (threadIdx.x == 0) {
