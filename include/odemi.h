@@ -385,7 +385,8 @@ ODE_API int dBatchGetResidual(dBatchID b, float *out_per_world);
  * of the state; returns the chosen workgroup size. */
 ODE_API int dBatchAutotune(dBatchID b, int trial_steps);
 /* dBatchAutotune with a reduced search: ODEMI_TUNE_KEEP_CELL keeps the grid
- * cell scale, ODEMI_TUNE_FUSED_ONLY tries only the fused kernel.  The trials
+ * cell scale, ODEMI_TUNE_FUSED_ONLY tries only the fused kernel (while the
+ * two-kernel step is switched off, every search is fused only).  The trials
  * run in the batch's current solver mode. */
 #define ODEMI_TUNE_KEEP_CELL 1
 #define ODEMI_TUNE_FUSED_ONLY 2
@@ -399,7 +400,10 @@ ODE_API int dBatchSetLaunchConfig(dBatchID b, int wg_size, float cell_scale);
  * per-workgroup limit. */
 ODE_API int dBatchGetLds(dBatchID b);
 /* 1: a step runs as two kernels (collision, then quickstep), each with its
- * own register budget; 0: one fused kernel.  The autotuner picks one. */
+ * own register budget; 0: one fused kernel.  The two-kernel step is switched
+ * off for now (a 4096-world timing run with it did not finish, cause not
+ * found): dBatchSetSplit(b, 1) returns 0 and the autotuner keeps the fused
+ * kernel. */
 ODE_API int dBatchGetSplit(dBatchID b);
 /* Persistent grid size (workgroups pulling worlds from a queue); 0 launches
  * one workgroup per world.  dBatchGetLaunchConfig returns the current one. */

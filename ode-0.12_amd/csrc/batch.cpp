@@ -720,24 +720,22 @@ int batch_autotune(BatchImpl *b, int trial_steps, int flags) {
   const int wgs[] = {64, 128, 256};
   const int lds_div[] = {1, 2, 3, 4, 8};   // pool = LDS per CU / k
   int st = world_kernel_static_lds();
-  bool bsplit = b->split;
-  for (int split = 0; split < ((flags & ODEMI_TUNE_FUSED_ONLY) ? 1 : 2); split++)
-    for (int wg : wgs)
-      for (int k : lds_div) {
-        int lds = (b->lds_cu / k - st - 256) & ~255;
-        if (lds < 0) lds = 0;
-        if (lds > b->lds_max) lds = b->lds_max;
-        int occ = world_kernel_occupancy(split ? KM_BATCH_STEP : KM_FUSED, wg, lds);
-        if (occ <= 0) continue;
-        const int grids[] = {0, occ * b->cu_count};
-        for (int grid : grids) {
-          double t;
-          b->split = split != 0;
-          if (!batch_set_launch(b, wg, bs, grid) || !batch_set_lds(b, lds) || !trial(t)) continue;
-          if (t < best) { best = t; bw = wg; bg = grid; bl = lds; bsplit = b->split; }
-        }
+  // the fused kernel only: the two-kernel step (split) is switched off
+  b->split = false;
+  for (int wg : wgs)
+    for (int k : lds_div) {
+      int lds = (b->lds_cu / k - st - 256) & ~255;
+      if (lds < 0) lds = 0;
+      if (lds > b->lds_max) lds = b->lds_max;
+      int occ = world_kernel_occupancy(KM_FUSED, wg, lds);
+      if (occ <= 0) continue;
+      const int grids[] = {0, occ * b->cu_count};
+      for (int grid : grids) {
+        double t;
+        if (!batch_set_launch(b, wg, bs, grid) || !batch_set_lds(b, lds) || !trial(t)) continue;
+        if (t < best) { best = t; bw = wg; bg = grid; bl = lds; }
       }
-  b->split = bsplit;
+    }
   batch_set_launch(b, bw, bs, bg);
   batch_set_lds(b, bl);
   // coloured order at a tolerance: the sweeps in their own kernel, with
@@ -870,7 +868,13 @@ int dBatchGetSorLaunch(dBatchID b, int *on, int *wg, int *lds, int *grid) {
 }
 int dBatchSetSplit(dBatchID b, int on) {
   if (!b) return 0;
-  b->impl->split = on != 0;
+  if (on) {
+    // switched off: a 4096-world timing run with collision and quickstep in
+    // two persistent kernels did not finish; the cause is not found yet
+    report_error("dBatchSetSplit: the two-kernel step is switched off");
+    return 0;
+  }
+  b->impl->split = false;
   destroy_graph(b->impl);
   return 1;
 }

@@ -196,8 +196,8 @@ class Batch:
             raise RuntimeError("bad launch config")
         if lds is not None and not self.L.dBatchSetLds(self.h, lds):
             raise RuntimeError(f"bad LDS pool size {lds}")
-        if split is not None:
-            self.L.dBatchSetSplit(self.h, int(split))
+        if split is not None and not self.L.dBatchSetSplit(self.h, int(split)):
+            raise RuntimeError("the two-kernel step (split=1) is switched off")
         if grid is not None and not self.L.dBatchSetPersistentGrid(self.h, int(grid)):
             raise RuntimeError(f"bad grid {grid}")
 

@@ -332,6 +332,11 @@ def test_autotune_keeps_state(odemi_mod):
     a.step(5)
     b.step(5)
     assert np.array_equal(a.state(), b.state())
+    # the two-kernel step is switched off: refused, never chosen
+    assert a.launch_config()["split"] == 0
+    with pytest.raises(RuntimeError):
+        a.set_launch_config(wg, 1.0, split=1)
+    assert a.launch_config()["split"] == 0
 
 
 def test_coloured_order_matches_coloured_restatement(odemi_mod):
