@@ -675,15 +675,17 @@ __device__ inline Surf host_surface(const ContactIn &ci) {
   return s;
 }
 
-// relax one row: ode/src/quickstep.cpp:484-582 (J and b scaled by ad there)
+// relax one row: ode/src/quickstep.cpp:484-582 (J and b scaled by ad there).
+// fc holds a body's 6 constraint-force floats at fc + FS * body + FO.
+template <int FS, int FO>
 __device__ inline void relax_row(Row *rows, int r, float *fc) {
   Row &R = rows[r];
   float old = R.lambda;
-  float *f1 = fc + 32 * R.b1 + 9;
+  float *f1 = fc + FS * R.b1 + FO;
   float s = R.J[0] * f1[0] + R.J[1] * f1[1] + R.J[2] * f1[2] + R.J[3] * f1[3] + R.J[4] * f1[4] + R.J[5] * f1[5];
   float *f2 = nullptr;
   if (R.b2 >= 0) {
-    f2 = fc + 32 * R.b2 + 9;
+    f2 = fc + FS * R.b2 + FO;
     s += R.J[6] * f2[0] + R.J[7] * f2[1] + R.J[8] * f2[2] + R.J[9] * f2[3] + R.J[10] * f2[4] + R.J[11] * f2[5];
   }
   float delta = R.ad * (R.rhs - old * R.cfm - s);
@@ -702,6 +704,63 @@ __device__ inline void relax_row(Row *rows, int r, float *fc) {
   for (int k = 0; k < 6; k++) f1[k] += delta * R.iMJ[k];
   if (f2)
     for (int k = 0; k < 6; k++) f2[k] += delta * R.iMJ[6 + k];
+}
+
+// LDS bytes of the reference-order solve of a world: its rows, the row
+// order and 6 fc floats per body
+__device__ inline long long ref_sor_lds_bytes(int nrows, int nb) {
+  return (long long)sizeof(Row) * nrows + 4LL * nrows + 24LL * nb;
+}
+
+// SOR_LCP in the reference order (quickstep.cpp:406-582): one island per
+// lane; rows ordered with the findex = -1 rows first and the others from the
+// tail, reshuffled every 8 iterations with the island's own RNG state.  LDS:
+// rows, order and fc are copied to the LDS pool first and back after the
+// iterations (same arithmetic, same order); otherwise they stay in the
+// world's scratch (rows, ord, and fc at bt[32 b + 9]).
+template <bool LDS>
+__device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, int nrows, int ngroups, int nb,
+                              int iters) {
+  constexpr int FS = LDS ? 6 : 32, FO = LDS ? 0 : 9;
+  Row *R = LDS ? (Row *)od_pool : rows;
+  int *O = LDS ? (int *)(R + nrows) : ord;
+  float *F = LDS ? (float *)(O + nrows) : bt;
+  const int nt = blockDim.x;
+  if (LDS) {
+    for (int i = threadIdx.x; i < nrows; i += nt) R[i] = rows[i];
+    for (int b = threadIdx.x; b < nb; b += nt)
+      for (int k = 0; k < 6; k++) F[6 * b + k] = bt[32 * b + 9 + k];
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < ngroups; k += nt) {
+    int r0 = isl[5 * k + 2], m = isl[5 * k + 3];
+    if (m <= 0) continue;
+    unsigned seed = (unsigned)isl[5 * k + 4];
+    int *o = O + r0;
+    int head = 0, tail = m - 1;
+    for (int i = 0; i < m; i++) {
+      if (R[r0 + i].findex == -1) o[head++] = r0 + i;
+      else o[tail--] = r0 + i;
+    }
+    for (int it = 0; it < iters; it++) {
+      if ((it & 7) == 0) {
+        for (int i = 1; i < m; i++) {
+          int sw = lcg_below(seed, i + 1);
+          int t = o[i]; o[i] = o[sw]; o[sw] = t;
+        }
+      }
+      for (int i = 0; i < m; i++) relax_row<FS, FO>(R, o[i], F);
+    }
+  }
+  // islands are solved by lanes of every wavefront; the integration reads
+  // the fc of all bodies
+  __syncthreads();
+  if (LDS) {
+    for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = R[i].lambda;
+    for (int b = threadIdx.x; b < nb; b += nt)
+      for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = F[6 * b + k];
+    __syncthreads();
+  }
 }
 
 // |lambda - clamp(lambda - d, lo, hi)| evaluated without the cancellation of
@@ -1616,32 +1675,12 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
   // --- SOR-LCP
   const int iters = P.iterations;
   if (ref) {
-    // one island per lane, rows in the reference order with its shuffle
     int *ord = corder + ncons + nb;
-    for (int k = threadIdx.x; k < ngroups; k += blockDim.x) {
-      int r0 = isl[5 * k + 2], m = isl[5 * k + 3];
-      if (m <= 0) continue;
-      unsigned seed = (unsigned)isl[5 * k + 4];
-      int *o = ord + r0;
-      int head = 0, tail = m - 1;
-      for (int i = 0; i < m; i++) {
-        if (rows[r0 + i].findex == -1) o[head++] = r0 + i;
-        else o[tail--] = r0 + i;
-      }
-      for (int it = 0; it < iters; it++) {
-        if ((it & 7) == 0) {
-          for (int i = 1; i < m; i++) {
-            int sw = lcg_below(seed, i + 1);
-            int t = o[i]; o[i] = o[sw]; o[sw] = t;
-          }
-        }
-        for (int i = 0; i < m; i++) relax_row(rows, o[i], bt);
-      }
-    }
+    if (ref_sor_lds_bytes(nrows, nb) <= B.lds_bytes)
+      sor_reference<true>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
+    else
+      sor_reference<false>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
     if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = iters;
-    // islands are solved by lanes of every wavefront; the integration below
-    // reads the fc of all bodies
-    __syncthreads();
   } else if (STAGE == 1) {
     // the sweeps run in their own kernel
     sor_prepare(B, c, P, cons, rows, bt, corder, isl, nrows, ngroups, in.from_host, sh);
