@@ -319,21 +319,38 @@ __device__ void phase_ref_index(const DevBatch &B, const Ctx &c, Shared &sh) {
   int *gbig = c.at<int>(c.L.gbig);
   int *gidx = c.at<int>(c.L.gidx);
   int *inv = c.at<int>(c.L.order);   // rank -> geom, then big list
-  for (int g = threadIdx.x; g < c.W.ng; g += blockDim.x) inv[B.grank[c.W.g0 + g]] = g;
+  const int ng = c.W.ng;
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) inv[B.grank[c.W.g0 + g]] = g;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int n = 0, nbig = 0;
-    for (int r = 0; r < c.W.ng; r++) {
-      int g = inv[r];
-      if (gbig[g]) { gidx[g] = -1; nbig++; }
-      else gidx[g] = n++;
+  // in space-list order (block scans): non-big geoms get consecutive hash
+  // indices; big geoms note their position among the big ones (-2 - pos)
+  int n = 0, nbig = 0;
+  for (int r0 = 0; r0 < ng; r0 += blockDim.x) {
+    const int r = r0 + threadIdx.x;
+    const int g = r < ng ? inv[r] : -1;
+    const int isb = (g >= 0 && gbig[g]) ? 1 : 0;
+    const int nonb = (g >= 0 && !isb) ? 1 : 0;
+    int t1, t2;
+    const int o1 = block_excl_scan(nonb, t1, sh);
+    const int o2 = block_excl_scan(isb, t2, sh);
+    if (nonb) gidx[g] = n + o1;
+    if (isb) gidx[g] = -2 - (nbig + o2);
+    n += t1;
+    nbig += t2;
+  }
+  __syncthreads();
+  // big geoms in reverse space order: big rank k = nbig - 1 - pos
+  int *big = inv + ng;
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+    const int v = gidx[g];
+    if (v <= -2) {
+      const int k = nbig - 1 - (-2 - v);
+      big[k] = g;
+      gbig[g] = 1 + k;   // 1 + big rank
+      gidx[g] = -1;
     }
-    // big geoms: reverse space order
-    int *big = inv + c.W.ng;
-    int k = 0;
-    for (int r = c.W.ng - 1; r >= 0; r--)
-      if (gbig[inv[r]]) big[k++] = inv[r];
-    for (int r = 0; r < nbig; r++) gbig[big[r]] = 1 + r;   // 1 + big rank
+  }
+  if (threadIdx.x == 0) {
     sh.ival[0] = n;
     sh.ival[1] = nbig;
   }
@@ -1255,6 +1272,142 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
   sor_finish(B, c, P, cons, rows, bt, corder, nrows, nact, nk, ncons, host_contacts);
 }
 
+// LDS bytes of the reference-order constraint ordering: 12 ints per
+// constraint, 10 per body, one per joint
+__device__ inline long long ref_order_lds_bytes(int nb, int ncons, int nj) {
+  return 4LL * (12LL * ncons + 10LL * nb + nj + 16);
+}
+
+// The reference order of constraints and rows in the LDS pool: the same
+// steps as the global-memory version in phase_step (adjacency newest attach
+// first, island walk, rows in island order, per-island RNG state and Info2
+// erp), with the constraint fields gathered into LDS by all lanes first, one
+// lane walking the islands there, and all lanes writing the results back
+// (constraint order, body visit order, island table, row offsets, erp, and
+// m = 0 for the constraints no island reaches).
+__device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, Cons *cons, int nk, int ncons,
+                              int *corder, int *isl, float *erp_out, int *misc, Shared &sh) {
+  const int nb = c.W.nb, nj = ncons - nk, nt = blockDim.x;
+  int *cb1 = (int *)od_pool, *cb2 = cb1 + ncons, *cm = cb2 + ncons, *ckind = cm + ncons;
+  int *cseq = ckind + ncons;               // attach sequence, then first row
+  int *cen = cseq + ncons, *ctag = cen + ncons, *cord = ctag + ncons;
+  float *cerp = (float *)(cord + ncons);   // Info2 erp seen by the constraint
+  float *cbe = cerp + ncons;               // a ball joint's own erp
+  int *adj = (int *)(cbe + ncons);         // 2 per constraint
+  int *adjs = adj + 2 * ncons;             // nb + 1
+  int *fill = adjs + nb + 1, *tag = fill + nb, *bord = tag + nb, *stack = bord + nb;
+  int *jord = stack + nb;                  // nj
+  int *itab = jord + nj;                   // island table, 5 per island
+  for (int i = threadIdx.x; i < ncons; i += nt) {
+    const Cons q = cons[i];
+    cb1[i] = q.b1; cb2[i] = q.b2; cm[i] = q.m; ckind[i] = q.kind; cseq[i] = q.seq;
+    cen[i] = q.b1 >= 0 && (B.invMass[c.W.b0 + q.b1] > 0.f || (q.b2 >= 0 && B.invMass[c.W.b0 + q.b2] > 0.f));
+    cbe[i] = q.kind == JT_BALL ? B.joints[c.W.j0 + q.src].erp : 0.f;
+    ctag[i] = 0;
+  }
+  for (int b = threadIdx.x; b <= nb; b += nt) adjs[b] = 0;
+  for (int b = threadIdx.x; b < nb; b += nt) tag[b] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < ncons; i++) {
+      if (cm[i] <= 0 && ckind[i] != JT_CONTACT) continue;
+      if (cb1[i] < 0) continue;
+      adjs[cb1[i] + 1]++;
+      if (cb2[i] >= 0) adjs[cb2[i] + 1]++;
+    }
+    for (int b = 0; b < nb; b++) adjs[b + 1] += adjs[b];
+    int nj2 = 0;
+    for (int i = nk; i < ncons; i++) {
+      if (cb1[i] < 0) continue;
+      int s = cseq[i], p = nj2++;
+      while (p > 0 && cseq[jord[p - 1]] > s) { jord[p] = jord[p - 1]; p--; }
+      jord[p] = i;
+    }
+    for (int b = 0; b < nb; b++) fill[b] = adjs[b];
+    int ci = nk - 1, jj = nj2 - 1;
+    while (ci >= 0 || jj >= 0) {
+      int pick;
+      if (jj < 0) pick = ci--;
+      else if (ci < 0) pick = jord[jj--];
+      else if (cseq[ci] > cseq[jord[jj]]) pick = ci--;
+      else pick = jord[jj--];
+      if (cb1[pick] < 0) continue;
+      adj[fill[cb1[pick]]++] = pick;
+      if (cb2[pick] >= 0) adj[fill[cb2[pick]]++] = pick;
+    }
+    int nic = 0, nib = 0, nisl = 0;
+    for (int bb = nb - 1; bb >= 0; bb--) {
+      if (tag[bb]) continue;
+      tag[bb] = 1;
+      int c0 = nic;
+      bord[nib++] = bb;
+      int sp = 0, b = bb;
+      while (true) {
+        for (int k = adjs[b]; k < adjs[b + 1]; k++) {
+          int qi = adj[k];
+          if (ctag[qi]) continue;
+          if (!cen[qi]) { ctag[qi] = 2; continue; }
+          ctag[qi] = 1;
+          cord[nic++] = qi;
+          int other = (cb1[qi] == b) ? cb2[qi] : cb1[qi];
+          if (other >= 0 && tag[other] <= 0) {
+            tag[other] = 1;
+            stack[sp++] = other;
+          }
+        }
+        if (sp == 0) break;
+        b = stack[--sp];
+        bord[nib++] = b;
+      }
+      itab[5 * nisl + 0] = c0;
+      itab[5 * nisl + 1] = nic - c0;
+      nisl++;
+    }
+    unsigned seed = P.seed;
+    unsigned long long shuffles = (unsigned long long)((P.iterations + 7) / 8);
+    int r = 0;
+    for (int k = 0; k < nisl; k++) {
+      int c0 = itab[5 * k], cn = itab[5 * k + 1];
+      int r0 = r;
+      float erp = P.erp;
+      for (int x = 0; x < cn; x++) {
+        const int i = cord[c0 + x];
+        cseq[i] = r;
+        r += cm[i];
+        if (ckind[i] == JT_BALL) erp = cbe[i];
+        cerp[i] = erp;
+      }
+      int m = r - r0;
+      itab[5 * k + 2] = r0;
+      itab[5 * k + 3] = m;
+      itab[5 * k + 4] = (int)seed;
+      if (m > 0 && m - 1 > 0) seed = lcg_skip(seed, shuffles * (unsigned long long)(m - 1));
+    }
+    P.seed = seed;
+    misc[0] = r;
+    misc[1] = nisl;
+    misc[2] = nic;
+    misc[3] = nib;
+    sh.ival[4] = nisl;
+    sh.ival[5] = nic;
+    sh.ival[6] = nib;
+  }
+  __syncthreads();
+  const int nisl = sh.ival[4], nic = sh.ival[5], nib = sh.ival[6];
+  for (int i = threadIdx.x; i < ncons; i += nt) {
+    if (ctag[i] != 1) {
+      cons[i].m = 0;
+    } else {
+      cons[i].row0 = cseq[i];
+      erp_out[i] = cerp[i];
+    }
+  }
+  for (int x = threadIdx.x; x < nic; x += nt) corder[x] = cord[x];
+  for (int x = threadIdx.x; x < nib; x += nt) corder[ncons + x] = bord[x];
+  for (int x = threadIdx.x; x < 5 * nisl; x += nt) isl[x] = itab[x];
+  __syncthreads();
+}
+
 // STAGE 0: the whole step; 1: up to the coloured solver's set-up (the sweeps
 // and the rest run in later kernels); 2: after the sweeps (write-back,
 // velocities, integration)
@@ -1351,7 +1504,11 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
   __syncthreads();
   c.clk.lap(PH_BODIES);
   // --- constraint order, row offsets, groups
-  if (ref) {
+  if (ref && ref_order_lds_bytes(nb, ncons, nj) <= B.lds_bytes) {
+    ref_order_lds(B, c, P, cons, nk, ncons, corder, isl, (float *)(color + nb), misc, sh);
+    nrows = misc[0];
+    ngroups = misc[1];
+  } else if (ref) {
     if (threadIdx.x == 0) {
       // adjacency, newest attach first: body joint lists prepend (ode.cpp:1382-1389)
       int *adjs = c.at<int>(c.L.adjs);
