@@ -15,6 +15,10 @@ iterations, device counters, per-phase clocks, the HBM roofline of the step
 and the reference CPU build on the host.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds N]
+       python bench.py --workload pile4096|vehicles   (the single 4096-body
+       pile and the 8192 vehicle worlds at the quickstep's 20 iterations, in
+       the reference and the coloured order, with the reference CPU build on a
+       bounded sample)
 """
 from __future__ import annotations
 
@@ -39,30 +43,38 @@ def algorithmic_bytes(nb, ng, pairs, contacts, rows):
 
 def _cpu_worker(job):
     """One host process: the reference single-precision build over a slice of
-    worlds; returns (body-steps, seconds)."""
-    scenes_, steps = job
+    worlds; returns (body-steps, seconds) of steps skip..skip+steps, timed as
+    a run of skip+steps steps minus a run of skip steps of the same world."""
+    scenes_, skip, steps = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     lib = po.ref_lib("single")
-    t0 = time.perf_counter()
+    secs = 0.0
     bodies = 0
     for sc in scenes_:
-        po.run(lib, sc, steps, 0)
+        t0 = time.perf_counter()
+        po.run(lib, sc, skip + steps, 0)
+        t1 = time.perf_counter()
+        if skip > 0:
+            po.run(lib, sc, skip, 0)
+        t2 = time.perf_counter()
+        secs += (t1 - t0) - (t2 - t1)
         bodies += sc.nb
-    return bodies * steps, time.perf_counter() - t0
+    return bodies * steps, secs
 
 
-def cpu_baseline(worlds, steps, procs):
+def cpu_baseline(worlds, skip, steps, procs):
     """Reference ODE 0.12 single-precision build (oracle/_ref) on `procs` host
     processes (ODE keeps its RNG and collider table in process globals, so
     worlds are split over processes, not threads), each stepping a slice of
-    the sample.  Runs before the GPU is touched."""
+    the sample; the rate of steps skip..skip+steps (the GPU's timed window).
+    Runs before the GPU is touched."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import multiprocessing as mp
     import pyoracle as po
     if not po.have_ref("single"):
         return None
-    jobs = [(worlds[k::procs], steps) for k in range(procs)]
+    jobs = [(worlds[k::procs], skip, steps) for k in range(procs)]
     with mp.get_context("spawn").Pool(procs) as pool:
         res = pool.map(_cpu_worker, jobs)
     work = sum(r[0] for r in res)
@@ -137,6 +149,95 @@ def adjoint_run(odemi, scenes, np, steps, interval):
             "worlds_with_finite_gradients": int(fin_w.sum()), "frozen_worlds": int(((wc[:, 4] & 16) != 0).sum())}
 
 
+def config_run(args):
+    """The single 4096-body pile (500-step config: seed 12345, h=0.005) or the
+    8192 vehicle worlds (per-step steer stops and wheel torques applied before
+    every step, as the reference frame loop does) at the quickstep's fixed 20
+    iterations, in the reference order and the coloured order; one JSON line."""
+    import numpy as np
+    import odemi
+    from odemi import scenes
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    steps, warmup = args.steps, args.warmup
+    if args.workload == "pile4096":
+        worlds = [scenes.random_pile(4096, seed=12345)]
+        desc = "single-world random pile: 4096 bodies (boxes/spheres), h=0.005, mu=0.6, 20 iterations, seed 12345"
+        sample, procs = worlds, 1
+    else:
+        worlds = scenes.vehicle_worlds(8192, warmup + steps + 1, seed=11)
+        desc = ("batched vehicle worlds: 8192 worlds, chassis + 4 sphere wheels on hinge2 joints, 32 static ramps, "
+                "per-step steer/torque controls, h=0.01, mu=1.0, 20 iterations, seed 11")
+        sample, procs = worlds[:2048], max(1, min(args.cpu_procs, os.cpu_count() or 1))
+    nb = sum(w.nb for w in worlds)
+    log(f"workload: {desc}; {len(worlds)} worlds, {nb} bodies")
+    cpu = None
+    if not args.no_cpu:
+        r = cpu_baseline(sample, warmup, steps, procs)
+        if r:
+            cpu = dict(value=r[0], unit="body-steps/s", cores=procs, kind="reference",
+                       sample=f"reference ODE 0.12 dSINGLE build (-O2), dSpaceCollide + dWorldQuickStep (20 "
+                              f"iterations) + dJointGroupEmpty, {len(sample)} world(s) ({sum(w.nb for w in sample)} "
+                              f"bodies) x steps {warmup}-{warmup + steps} (the GPU's timed window; run of "
+                              f"{warmup + steps} steps minus run of {warmup}) on {procs} host process(es), "
+                              f"{r[1]:.1f}s")
+            log(f"cpu baseline: {r[0]:.3g} body-steps/s on {procs} process(es) ({r[1]:.1f}s)")
+    import torch
+    torch.cuda.set_device(0)
+    has_ctrl = worlds[0].step_ctrl is not None
+    ctrl = None
+    if has_ctrl:
+        ctrl = np.concatenate([w.step_ctrl[:warmup + steps] for w in worlds], axis=1)   # (steps, nj, 4)
+    out = {}
+    orders = [(odemi.ORDER_REFERENCE, "reference_order"), (odemi.ORDER_COLORED, "coloured_order")]
+    if args.order != "both":
+        orders = [o for o in orders if o[1].startswith(args.order)]
+    for order, name in orders:
+        b = odemi.Batch(worlds, order)
+        if args.launch:
+            wg, cs, lds, split, grid = args.launch.split(",")
+            b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
+        else:
+            b.autotune(2)
+        cfg = b.launch_config()
+
+        def run(k0, k1):
+            if has_ctrl:
+                for k in range(k0, k1):
+                    b.set_controls(ctrl[k])
+                    b.step(1)
+            else:
+                b.step(k1 - k0)
+            torch.cuda.synchronize()
+
+        run(0, warmup)
+        t0 = time.perf_counter()
+        run(warmup, warmup + steps)
+        wall = time.perf_counter() - t0
+        b.track_residual(True)
+        b.step(1)
+        res = b.residual()
+        b.track_residual(False)
+        b.profile(True)
+        b.step(3)
+        ph = {k: v / 3 for k, v in b.phase_times().items()}
+        b.profile(False)
+        cnt = b.counters()
+        b.close()
+        out[name] = {"value": nb * steps / wall, "ms_per_step": wall / steps * 1e3, "launch": cfg,
+                     "lcp_residual_median": float(np.median(res)), "counters_per_step": cnt,
+                     "phase_wg_ms_per_step": ph}
+        log(f"{name}: {wall / steps * 1e3:.3f} ms/step, {nb * steps / wall:.3g} body-steps/s, launch {cfg}")
+    head = out.get("coloured_order") or out["reference_order"]
+    line = {"metric": "body-steps/s at the quickstep's 20 iterations", "value": head["value"],
+            "unit": "body-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scenes)",
+            "config": {"workload": desc, "worlds": len(worlds), "bodies": nb},
+            "reference_order": out.get("reference_order"), "coloured_order": out.get("coloured_order"),
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,10 +262,17 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="fixed launch config 'wg,cell_scale,lds,split,grid' instead of autotuning")
     ap.add_argument("--launch-tol", default=None, help="the same for the iterate-to-tolerance phase")
+    ap.add_argument("--workload", default="uneven", choices=("uneven", "pile4096", "vehicles"),
+                    help="uneven: the metric's batched uneven worlds; pile4096 / vehicles: the other configurations")
+    ap.add_argument("--order", default="both", choices=("both", "reference", "coloured"),
+                    help="pile4096 / vehicles: the solver order(s) to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "uneven":
+        config_run(args)
+        return
 
     import numpy as np
     import odemi
@@ -182,13 +290,14 @@ def main():
     if not args.no_cpu:
         procs = max(1, min(args.cpu_procs, os.cpu_count() or 1))
         sample = worlds[: args.cpu_sample]
-        csteps = args.warmup + args.steps
-        r = cpu_baseline(sample, csteps, procs)
+        r = cpu_baseline(sample, args.warmup, args.steps, procs)
         if r:
             cpu = dict(value=r[0], unit="body-steps/s", cores=procs, kind="reference",
                        sample=f"reference ODE 0.12 dSINGLE build (-O2), dSpaceCollide + dWorldQuickStep "
                               f"(20 iterations) + dJointGroupEmpty, first {len(sample)} worlds "
-                              f"({sum(w.nb for w in sample)} bodies) x steps 0-{csteps}, split over {procs} "
+                              f"({sum(w.nb for w in sample)} bodies) x steps {args.warmup}-"
+                              f"{args.warmup + args.steps} (the GPU's timed window at 20 iterations; run of "
+                              f"{args.warmup + args.steps} steps minus run of {args.warmup}), split over {procs} "
                               f"host processes, {r[1]:.1f}s")
             log(f"cpu baseline: {r[0]:.3g} body-steps/s on {procs} processes ({r[1]:.1f}s)")
 

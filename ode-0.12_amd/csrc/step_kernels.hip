@@ -145,13 +145,17 @@ __device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh, in
 // bases (256) and per-wavefront digit counts of one tile
 __host__ __device__ inline int radix_lds_bytes(int nt) { return 4 * (256 + 256 * ((nt + 63) / 64)); }
 
-// Ascending LSD radix sort of n distinct u64 keys in global memory by one
-// workgroup, 8-bit digits over the bits that are set in some key; tmp holds n
-// keys.  Each pass counts digits (LDS atomics), scans the 256 counts and
-// scatters tile by tile, stably: the rank of a key among the equal digits of
-// its wavefront comes from eight ballots, the wavefronts of a tile are
-// ordered through per-wavefront counts in LDS.  The result is in keys.
-__device__ inline void block_radix_sort(unsigned long long *keys, unsigned long long *tmp, int n, Shared &sh) {
+// Ascending LSD radix sort of n u64 keys in global memory by one workgroup,
+// by bits [lo_bit, 64) in 8-bit digits (lo_bit a multiple of 8); tmp holds n
+// keys.  A digit that is zero in every key is skipped (a stable pass over
+// one digit value is the identity).  Each pass counts digits (LDS atomics),
+// scans the 256 counts (block scan) and scatters tile by tile, stably: the
+// rank of a key among the equal digits of its wavefront comes from eight
+// ballots, the wavefronts of a tile are ordered through per-wavefront counts
+// in LDS.  Keys equal in the sorted bits keep their input order.  The result
+// is in keys.
+__device__ inline void block_radix_sort(unsigned long long *keys, unsigned long long *tmp, int n, int lo_bit,
+                                        Shared &sh) {
   const int nt = blockDim.x, nw = (nt + 63) >> 6;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int *base = (int *)od_pool;      // 256 digit bases
@@ -167,19 +171,23 @@ __device__ inline void block_radix_sort(unsigned long long *keys, unsigned long 
   __syncthreads();
   const unsigned long long all = ((unsigned long long)(unsigned)sh.ival[1] << 32) | (unsigned)sh.ival[0];
   __syncthreads();
-  int passes = 0;
-  while (passes < 8 && (all >> (8 * passes)) != 0) passes++;
   unsigned long long *src = keys, *dst = tmp;
-  for (int p = 0; p < passes; p++) {
-    const int shift = 8 * p;
+  for (int shift = lo_bit; shift < 64; shift += 8) {
+    if (((all >> shift) & 255) == 0) continue;
     for (int d = threadIdx.x; d < 256; d += nt) base[d] = 0;
     for (int d = threadIdx.x; d < 256 * nw; d += nt) wc[d] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += nt) atomicAdd(&base[(int)((src[i] >> shift) & 255)], 1);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int run = 0;
-      for (int d = 0; d < 256; d++) { int c = base[d]; base[d] = run; run += c; }
+    {
+      // exclusive scan of the digit counts, a run of digits per lane
+      const int per = (256 + nt - 1) / nt, d0 = threadIdx.x * per;
+      const int d1 = d0 + per < 256 ? d0 + per : 256;
+      int loc = 0;
+      for (int d = d0; d < d1; d++) loc += base[d];
+      int tot;
+      int off = block_excl_scan(loc, tot, sh);
+      for (int d = d0; d < d1; d++) { const int cnt = base[d]; base[d] = off; off += cnt; }
     }
     __syncthreads();
     for (int t0 = 0; t0 < n; t0 += nt) {
@@ -445,9 +453,11 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
   if ((long long)np2 * 8 > B.lds_bytes && B.lds_bytes >= radix_lds_bytes(blockDim.x) && ne <= c.W.pcap) {
     // too many keys for the LDS pool: radix sort in global memory (pkeys,
     // pcap entries written only by the pair scan below, holds the second
-    // buffer)
+    // buffer), by the cell hash only: the pair scan needs the entries of a
+    // cell together, and the pairs it finds are sorted by their own keys
+    // below, so the order inside a cell does not reach the result
     __syncthreads();
-    block_radix_sort(cells, pkeys, ne, sh);
+    block_radix_sort(cells, pkeys, ne, 32, sh);
   } else {
     for (int i = ne + threadIdx.x; i < np2; i += blockDim.x) cells[i] = ~0ULL;
     __syncthreads();
@@ -573,7 +583,7 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
       // radix sort in global memory; the pair records (written after the
       // sort) hold the second buffer
       __syncthreads();
-      block_radix_sort(pkeys, (unsigned long long *)pairs, npairs, sh);
+      block_radix_sort(pkeys, (unsigned long long *)pairs, npairs, 0, sh);
     } else {
       for (int i = npairs + threadIdx.x; i < np; i += blockDim.x) pkeys[i] = ~0ULL;
       __syncthreads();

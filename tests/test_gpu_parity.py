@@ -155,6 +155,24 @@ def test_first_step_pairs_and_contacts_match_reference(odemi_mod, name):
     assert b.counters()["overflow"] == 0
 
 
+@pytest.mark.parametrize("wg,lds", [(256, 8192), (64, 2048), (128, 0)])
+def test_broadphase_sort_paths_keep_reference_pair_order(odemi_mod, wg, lds):
+    """The 4096-body pile's first-step pairs with small LDS pools: cell keys
+    and pair keys sorted by the block radix sort in global memory (pools of
+    8 KB and 2 KB), or by the bitonic sort in global memory (no pool); the
+    pair order must stay the reference's, bit for bit."""
+    say("broadphase sort path", wg, lds)
+    from test_golden import CASES
+    sc = CASES["pile4096"][0]()
+    g = golden("pile4096")
+    b = _batch(odemi_mod, [sc])
+    b.set_launch_config(wg, 1.0, lds)
+    b.step(1)
+    n0 = int(g["pair_step_count"][0])
+    assert np.array_equal(b.pairs(0), g["pairs"][:n0])
+    assert b.counters()["contacts"] == int(g["contact_step_count"][0])
+
+
 def _match_within_pairs(con, ref):
     """Contacts of one pair whose depths tie (a box resting flat on a plane:
     its four lowest vertices) come out in an order decided by the last bit of
