@@ -335,8 +335,93 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   return b;
 }
 
+// The descriptor checks the ODE setters make by assertion (sphere radius and
+// box sides >= 0, sphere.cpp:52, box.cpp:51; only placeable geoms take a
+// body, collision_kernel.cpp:510; dJointAttach refuses body1 == body2,
+// ode.cpp:1352; dMassCheck wants mass > 0, mass.cpp:48), plus index ranges:
+// a descriptor that fails one is refused before any device work, with the
+// reason reported.
+static bool validate_desc(const dBatchDesc *D) {
+  if (!D || D->nworlds <= 0 || !D->worlds) {
+    report_error("dBatchCreate: no worlds");
+    return false;
+  }
+  long long nb = 0, ng = 0, nj = 0;
+  for (int w = 0; w < D->nworlds; w++) {
+    const dBatchWorldDesc &W = D->worlds[w];
+    if (W.nb < 0 || W.ng < 0 || W.nj < 0) {
+      report_error("dBatchCreate: world %d has a negative body, geom or joint count", w);
+      return false;
+    }
+    if (!(W.dt > 0 && std::isfinite(W.dt)) || W.iterations < 0 || W.hash_minlevel > W.hash_maxlevel) {
+      report_error("dBatchCreate: world %d: step size, iteration count or hash levels out of range", w);
+      return false;
+    }
+    nb += W.nb;
+    ng += W.ng;
+    nj += W.nj;
+  }
+  if (nb > INT32_MAX / 4 || ng > INT32_MAX / 8 || nj > INT32_MAX / 8) {
+    report_error("dBatchCreate: batch too large");
+    return false;
+  }
+  if ((nb && (!D->body_pos || !D->body_q || !D->body_lvel || !D->body_avel || !D->body_mass || !D->body_I)) ||
+      (ng && (!D->geom_type || !D->geom_body || !D->geom_param || !D->geom_pos || !D->geom_q)) ||
+      (nj && (!D->joint_type || !D->joint_b1 || !D->joint_b2 || !D->joint_anchor || !D->joint_axis1 ||
+              !D->joint_axis2 || !D->joint_param))) {
+    report_error("dBatchCreate: a body, geom or joint array is missing");
+    return false;
+  }
+  long long b0 = 0, g0 = 0, j0 = 0;
+  for (int w = 0; w < D->nworlds; w++) {
+    const dBatchWorldDesc &W = D->worlds[w];
+    for (int i = 0; i < W.nb; i++) {
+      const double m = D->body_mass[b0 + i];
+      if (!(m > 0 && std::isfinite(m))) {
+        report_error("dBatchCreate: world %d body %d: mass must be positive and finite", w, i);
+        return false;
+      }
+    }
+    for (int g = 0; g < W.ng; g++) {
+      const long long gg = g0 + g;
+      const int t = D->geom_type[gg], body = D->geom_body[gg];
+      const double *p = D->geom_param + 4 * gg;
+      if (t != GEOM_SPHERE && t != GEOM_BOX && t != GEOM_PLANE) {
+        report_error("dBatchCreate: world %d geom %d: unknown type %d", w, g, t);
+        return false;
+      }
+      if (body < -1 || body >= W.nb || (t == GEOM_PLANE && body != -1)) {
+        report_error("dBatchCreate: world %d geom %d: body %d out of range (planes take none)", w, g, body);
+        return false;
+      }
+      const int nsz = t == GEOM_SPHERE ? 1 : (t == GEOM_BOX ? 3 : 0);
+      for (int k = 0; k < nsz; k++)
+        if (!(p[k] >= 0 && std::isfinite(p[k]))) {
+          report_error("dBatchCreate: world %d geom %d: size must be >= 0 and finite", w, g);
+          return false;
+        }
+    }
+    for (int j = 0; j < W.nj; j++) {
+      const long long jg = j0 + j;
+      const int t = D->joint_type[jg], a = D->joint_b1[jg], b = D->joint_b2[jg];
+      if (t != JT_BALL && t != JT_HINGE && t != JT_HINGE2) {
+        report_error("dBatchCreate: world %d joint %d: unknown type %d", w, j, t);
+        return false;
+      }
+      if (a < -1 || a >= W.nb || b < -1 || b >= W.nb || (a >= 0 && a == b)) {
+        report_error("dBatchCreate: world %d joint %d: bodies %d, %d out of range or equal", w, j, a, b);
+        return false;
+      }
+    }
+    b0 += W.nb;
+    g0 += W.ng;
+    j0 += W.nj;
+  }
+  return true;
+}
+
 BatchImpl *batch_create(const dBatchDesc *D, int order_mode, double cap_scale) {
-  if (!D || D->nworlds <= 0) return nullptr;
+  if (!validate_desc(D)) return nullptr;
   BatchArrays A = arrays_from_desc(D, cap_scale, 1.0f);
   return batch_create_arrays(A, order_mode);
 }

@@ -55,3 +55,40 @@ def test_scene_driver_links_against_the_library(root):
     assert os.path.exists(p)
     out = subprocess.run(["nm", "-D", "--undefined-only", p], capture_output=True, text=True, check=True).stdout
     assert "dSpaceCollide" in out and "dWorldQuickStep" in out
+
+
+def _bad_scenes():
+    """Descriptors the ODE setters would refuse by assertion, one fault each."""
+    import numpy as np
+    from odemi import scenes
+    out = []
+    base = lambda: scenes.box_stack(3)
+    sc = base(); sc.geom_body = sc.geom_body.copy(); sc.geom_body[-1] = sc.nb
+    out.append(("geom body out of range", sc, "out of range"))
+    sc = base(); sc.geom_body = sc.geom_body.copy(); sc.geom_body[0] = 0   # geom 0 is the plane
+    out.append(("plane on a body", sc, "planes take none"))
+    sc = base(); sc.geom_type = sc.geom_type.copy(); sc.geom_type[-1] = 7
+    out.append(("unknown geom type", sc, "unknown type"))
+    sc = base(); sc.geom_param = sc.geom_param.copy(); sc.geom_param[-1, 1] = -1.0
+    out.append(("negative box side", sc, "size must be"))
+    sc = base(); sc.body_mass = sc.body_mass.copy(); sc.body_mass[1] = 0.0
+    out.append(("zero mass", sc, "mass must be"))
+    sc = base(); sc.dt = 0.0
+    out.append(("zero step size", sc, "step size"))
+    ch = scenes.pile_world(np.random.default_rng(1), 8, 1, seed=3)
+    assert ch.nj > 0
+    sc = ch; sc.joint_b2 = sc.joint_b2.copy(); sc.joint_b1 = sc.joint_b1.copy(); sc.joint_b2[0] = sc.joint_b1[0]
+    out.append(("joint on one body twice", sc, "out of range or equal"))
+    return out
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_batch_descriptor_faults_are_refused(root, capfd, case):
+    """dBatchCreate refuses a descriptor the reference's setters would assert
+    on, before any device work (no GPU needed), and says why."""
+    import odemi
+    name, sc, msg = _bad_scenes()[case]
+    with pytest.raises(RuntimeError):
+        odemi.Batch([sc], odemi.ORDER_REFERENCE)
+    err = capfd.readouterr().err
+    assert "dBatchCreate" in err and msg in err, (name, err)
