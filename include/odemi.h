@@ -254,6 +254,9 @@ ODE_API int dSpaceGetNumGeoms(dSpaceID space);                                /*
 
 /* collision: include/ode/collision.h:68-1048 */
 ODE_API void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback); /* collision.h:825 */
+/* flags bit: any contacts will do (collision.h:746); the colliders here
+ * refine nothing, so it changes no result */
+#define CONTACTS_UNIMPORTANT 0x80000000
 ODE_API int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip); /* collision.h:795 */
 ODE_API dGeomID dCreateSphere(dSpaceID space, dReal radius);                  /* collision.h:924 */
 ODE_API dGeomID dCreateBox(dSpaceID space, dReal lx, dReal ly, dReal lz);     /* collision.h:1002 */

@@ -145,6 +145,16 @@ __device__ inline void bitonic_sort(unsigned long long *a, int n, Shared &sh, in
 // bases (256) and per-wavefront digit counts of one tile
 __host__ __device__ inline int radix_lds_bytes(int nt) { return 4 * (256 + 256 * ((nt + 63) / 64)); }
 
+// Workgroup barrier that orders LDS accesses only: global loads and stores
+// in flight stay in flight (a full __syncthreads waits for them).  For
+// phases whose cross-lane exchange is in LDS and whose global writes are
+// read only after a later full barrier.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Ascending LSD radix sort of n u64 keys in global memory by one workgroup,
 // by bits [lo_bit, 64) in 8-bit digits (lo_bit a multiple of 8); tmp holds n
 // keys.  A digit that is zero in every key is skipped (a stable pass over
@@ -177,7 +187,13 @@ __device__ inline void block_radix_sort(unsigned long long *keys, unsigned long 
     for (int d = threadIdx.x; d < 256; d += nt) base[d] = 0;
     for (int d = threadIdx.x; d < 256 * nw; d += nt) wc[d] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += nt) atomicAdd(&base[(int)((src[i] >> shift) & 255)], 1);
+    // digit counts, four keys per lane in flight at a time
+    for (int i0 = threadIdx.x; i0 < n; i0 += 4 * nt) {
+      unsigned long long k4[4];
+      for (int u = 0; u < 4; u++) k4[u] = (i0 + u * nt < n) ? src[i0 + u * nt] : 0ULL;
+      for (int u = 0; u < 4; u++)
+        if (i0 + u * nt < n) atomicAdd(&base[(int)((k4[u] >> shift) & 255)], 1);
+    }
     __syncthreads();
     {
       // exclusive scan of the digit counts, a run of digits per lane
@@ -190,10 +206,14 @@ __device__ inline void block_radix_sort(unsigned long long *keys, unsigned long 
       for (int d = d0; d < d1; d++) { const int cnt = base[d]; base[d] = off; off += cnt; }
     }
     __syncthreads();
+    // tiles of nt keys; the next tile's key is loaded before this tile's
+    // LDS-only barriers, the scattered stores are waited for once per pass
+    unsigned long long knext = threadIdx.x < n ? src[threadIdx.x] : 0ULL;
     for (int t0 = 0; t0 < n; t0 += nt) {
       const int i = t0 + threadIdx.x;
       const bool valid = i < n;
-      const unsigned long long k = valid ? src[i] : 0ULL;
+      const unsigned long long k = knext;
+      knext = (i + nt < n) ? src[i + nt] : 0ULL;
       const int d = (int)((k >> shift) & 255);
       unsigned long long peers = __ballot(valid);
       for (int b = 0; b < 8; b++) {
@@ -204,20 +224,21 @@ __device__ inline void block_radix_sort(unsigned long long *keys, unsigned long 
       const int rank = __popcll(peers & lt);
       const int cnt = __popcll(peers);
       if (valid && rank == cnt - 1) wc[wid * 256 + d] = cnt;
-      __syncthreads();
+      lds_barrier();
       if (valid) {
         int off = base[d] + rank;
         for (int w = 0; w < wid; w++) off += wc[w * 256 + d];
         dst[off] = k;
       }
-      __syncthreads();
+      lds_barrier();
       for (int dd = threadIdx.x; dd < 256; dd += nt) {
         int tot = 0;
         for (int w = 0; w < nw; w++) { tot += wc[w * 256 + dd]; wc[w * 256 + dd] = 0; }
         base[dd] += tot;
       }
-      __syncthreads();
+      lds_barrier();
     }
+    __syncthreads();   // this pass's stores before the next pass reads them
     unsigned long long *t = src; src = dst; dst = t;
   }
   if (src != keys) {
@@ -463,20 +484,36 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
     __syncthreads();
     bitonic_sort(cells, np2, sh, B.lds_bytes);
   }
-  // pair scan, counting pass then writing pass
-  auto visit = [&](int i, bool write, int &cnt, int out0) {
-    unsigned long long ei = cells[i];
-    unsigned hi = (unsigned)(ei >> 32);
-    int gi = (int)((ei & 0xffffffffu) >> 3), si = (int)(ei & 7);
-    const float *ai = aabb + 6 * gi;
+  // pair scan: one lane per cell entry walks the rest of its cell's run.
+  // A pair is found once, from the entry of the pair's home cell (the low
+  // corner of the two cell ranges' overlap) that comes first in the run.
+  auto pair_key = [&](int gi, int gj) -> unsigned long long {
+    if (ref) {
+      int a, b;
+      return ref_pair_key(c, gi, gj, nref, a, b);
+    }
+    return gi < gj ? ((unsigned long long)gi << 32) | (unsigned)gj : ((unsigned long long)gj << 32) | (unsigned)gi;
+  };
+  auto visit = [&](int i, auto &&emit) {
+    const unsigned long long ei = cells[i];
+    const unsigned hi = (unsigned)(ei >> 32);
+    const int gi = (int)((ei & 0xffffffffu) >> 3), si = (int)(ei & 7);
+    const int G = c.W.g0;
+    float ai[6];
+    for (int k = 0; k < 6; k++) ai[k] = aabb[6 * gi + k];
+    const int bi = B.gbody[G + gi];
+    const unsigned ci = B.gcat[G + gi], ki = B.gcol[G + gi];
     int li[3];
     for (int k = 0; k < 3; k++) li[k] = (int)floorf(ai[2 * k] * inv) + ((si >> k) & 1);
     for (int j = i + 1; j < ne; j++) {
-      unsigned long long ej = cells[j];
+      const unsigned long long ej = cells[j];
       if ((unsigned)(ej >> 32) != hi) break;
-      int gj = (int)((ej & 0xffffffffu) >> 3), sj = (int)(ej & 7);
+      const int gj = (int)((ej & 0xffffffffu) >> 3), sj = (int)(ej & 7);
       if (gj == gi) continue;
-      const float *aj = aabb + 6 * gj;
+      float aj[6];
+      for (int k = 0; k < 6; k++) aj[k] = aabb[6 * gj + k];
+      const int bj = B.gbody[G + gj];
+      const unsigned cj = B.gcat[G + gj], kj = B.gcol[G + gj];
       bool same = true, home = true;
       for (int k = 0; k < 3; k++) {
         int ljk0 = (int)floorf(aj[2 * k] * inv);
@@ -487,34 +524,44 @@ __device__ int phase_broadphase(const DevBatch &B, const Ctx &c, const WorldPara
         home &= hc == li[k];
       }
       if (!same || !home) continue;
-      if (!pair_passes(B, c, gi, gj, aabb)) continue;
-      if (write) {
-        int o = out0 + cnt;
-        if (o < c.W.pcap) {
-          if (ref) {
-            int a, b;
-            pkeys[o] = ref_pair_key(c, gi, gj, nref, a, b);
-          } else {
-            pkeys[o] = gi < gj ? ((unsigned long long)gi << 32) | (unsigned)gj
-                               : ((unsigned long long)gj << 32) | (unsigned)gi;
-          }
-        }
-      }
-      cnt++;
+      // collideAABBs filters (collision_space_internal.h:54-71)
+      if (bi == bj && bi >= 0) continue;
+      if (((ci & kj) || (cj & ki)) == 0) continue;
+      if (!aabb_overlap(ai, aj)) continue;
+      emit(gi, gj);
     }
   };
-  int npairs = 0;
-  for (int start = 0; start < ne; start += blockDim.x) {
-    int i = start + threadIdx.x;
-    int cnt = 0;
-    if (i < ne) visit(i, false, cnt, 0);
-    int tot;
-    int off = block_excl_scan(cnt, tot, sh);
-    if (i < ne && cnt) {
-      int c2 = 0;
-      visit(i, true, c2, npairs + off);
+  // every pair takes the next slot of a workgroup counter: the keys are
+  // sorted below, so the slot order does not reach the result
+  if (threadIdx.x == 0) sh.ival[2] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < ne; i += blockDim.x)
+    visit(i, [&](int gi, int gj) {
+      const int o = atomicAdd(&sh.ival[2], 1);
+      if (o < c.W.pcap) pkeys[o] = pair_key(gi, gj);
+    });
+  __syncthreads();
+  int npairs = sh.ival[2];
+  __syncthreads();
+  if (npairs > c.W.pcap) {
+    // more pairs than slots: scan again in entry order (counting pass, then
+    // writing pass) so that the pairs kept are the same on every run
+    npairs = 0;
+    for (int start = 0; start < ne; start += blockDim.x) {
+      const int i = start + threadIdx.x;
+      int cnt = 0;
+      if (i < ne) visit(i, [&](int, int) { cnt++; });
+      int tot;
+      const int off = block_excl_scan(cnt, tot, sh);
+      if (i < ne && cnt) {
+        int o = npairs + off;
+        visit(i, [&](int gi, int gj) {
+          if (o < c.W.pcap) pkeys[o] = pair_key(gi, gj);
+          o++;
+        });
+      }
+      npairs += tot;
     }
-    npairs += tot;
   }
   // big geoms against the rest, then big against big; the big geoms first
   // listed in ascending index order (ecount is free scratch here)

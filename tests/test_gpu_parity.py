@@ -122,6 +122,37 @@ def test_narrowphase_pairs_vs_oracle(odemi_mod):
     assert n_cmp > 150 and n_disc < 10
 
 
+def test_dcollide_refusals_match_the_reference(odemi_mod):
+    """dCollide returns 0 for a request of no contacts, a geom against itself
+    and two geoms on one body (collision_kernel.cpp:306-311); the flag bits
+    above the contact count (CONTACTS_UNIMPORTANT) do not change the count."""
+    say("dCollide refusals")
+    api = Api()
+    L = api.L
+    V, F = C.c_void_p, C.c_float
+    for name, res, args in [("dWorldCreate", V, []), ("dWorldDestroy", None, [V]), ("dBodyCreate", V, [V]),
+                            ("dBodySetPosition", None, [V, F, F, F]), ("dGeomSetBody", None, [V, V])]:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    R0 = _rot(np.array([1.0, 0, 0, 0]))
+    s1 = api.geom(0, [0.5, 0, 0, 0], np.zeros(3), R0)
+    s2 = api.geom(0, [0.5, 0, 0, 0], np.array([0.6, 0, 0]), R0)
+    assert len(api.collide(s1, s2, 1)) == 1
+    buf = (C.c_byte * 256)()
+    assert L.dCollide(s1, s2, 0, buf, 64) == 0
+    assert L.dCollide(s1, s2, 0x80000000 | 4, buf, 64) == 1
+    assert L.dCollide(s1, s1, 4, buf, 64) == 0
+    w = L.dWorldCreate()
+    body = L.dBodyCreate(w)
+    L.dGeomSetBody(s1, body)
+    L.dGeomSetBody(s2, body)
+    assert L.dCollide(s1, s2, 4, buf, 64) == 0
+    L.dGeomDestroy(s1)
+    L.dGeomDestroy(s2)
+    L.dWorldDestroy(w)
+
+
 # ------------------------------------------------------------------ broadphase + contacts
 
 
