@@ -574,3 +574,50 @@ def test_ragged_batch_worlds_step_independently(odemi_mod):
     x = np.array([0.5, 0.25, 3.0]) + n * h * np.array([1.0, -0.5, 2.0]) + h * h * g * n * (n + 1) / 2
     assert np.abs(st[0, :3] - x).max() <= 1e-4, (st[0, :3], x)
     b.close()
+
+
+def _packed_spheres():
+    """252 spheres (r 0.3 m) on a 0.5 m lattice over a plane: every sphere
+    overlaps its lattice neighbours, ~700 candidate pairs on the first step."""
+    from odemi.scenes import _Builder, PLANE
+    b = _Builder()
+    b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
+    for x in range(6):
+        for y in range(6):
+            for z in range(7):
+                b.sphere_body((0.5 * x, 0.5 * y, 0.35 + 0.5 * z), 0.3, 1.0)
+    return b.scene(dt=0.01, mu=0.5, seed=3)
+
+
+def test_capacity_overflow_is_flagged_and_deterministic(odemi_mod, monkeypatch):
+    """Packed spheres with pair, contact and row capacities cut to 5 %
+    (ODEMI_CAPACITY_SCALE): the first step flags the pair overflow, keeps
+    exactly as many pairs as there are slots, all of them pairs of the
+    full-capacity step, and two runs agree bit for bit (the pairs kept when
+    they overflow are chosen in entry order, not by the order lanes finish)."""
+    sc = _packed_spheres()
+    say("capacity overflow: full capacity")
+    full = _batch(odemi_mod, [sc])
+    full.step(1)
+    full_pairs = {tuple(p) for p in full.pairs(0)}
+    full.close()
+    monkeypatch.setenv("ODEMI_CAPACITY_SCALE", "0.05")
+    runs = []
+    for rep in range(2):
+        say("capacity overflow: run", rep)
+        b = _batch(odemi_mod, [sc])
+        b.step(1)
+        p1 = b.pairs(0)
+        f1 = int(b.world_counters()[0, 4])
+        b.step(2)
+        runs.append((p1, f1, b.state()))
+        b.close()
+    p1, f1, _ = runs[0]
+    assert f1 & 2, f1                                   # OVF_PAIRS
+    pcap = 1
+    while pcap < int(0.05 * (16 * sc.ng + 64)):   # the library's capacity rule
+        pcap <<= 1
+    assert len(p1) == pcap and len(full_pairs) > pcap
+    assert {tuple(p) for p in p1} <= full_pairs
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2], equal_nan=True)
