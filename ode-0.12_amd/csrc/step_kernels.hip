@@ -785,23 +785,79 @@ __device__ inline void relax_row(Row *rows, int r, float *fc) {
 __device__ inline long long ref_sor_lds_bytes(int nrows, int nb) {
   return (long long)sizeof(Row) * nrows + 4LL * nrows + 24LL * nb;
 }
+// the same with the rows left in global memory: impulses, order and fc only
+__device__ inline long long ref_sor_lds_bytes_streamed(int nrows, int nb) {
+  return 8LL * nrows + 24LL * nb;
+}
+
+// the read-only part of a row (everything but lambda), held in registers
+struct RowC {
+  float J[12], iMJ[12];
+  float rhs, cfm, lo, hi, ad;
+  int findex, b1, b2;
+};
+
+__device__ inline RowC load_rowc(const Row &R) {
+  RowC q;
+  for (int k = 0; k < 12; k++) { q.J[k] = R.J[k]; q.iMJ[k] = R.iMJ[k]; }
+  q.rhs = R.rhs; q.cfm = R.cfm; q.lo = R.lo; q.hi = R.hi; q.ad = R.ad;
+  q.findex = R.findex; q.b1 = R.b1; q.b2 = R.b2;
+  return q;
+}
+
+// relax_row on a row held in registers, its impulse in lam[r] and fc in
+// F[6 b]: the same arithmetic in the same order
+__device__ inline void relax_rowc(const RowC &R, int r, float *lam, float *F) {
+  float old = lam[r];
+  float *f1 = F + 6 * R.b1;
+  float s = R.J[0] * f1[0] + R.J[1] * f1[1] + R.J[2] * f1[2] + R.J[3] * f1[3] + R.J[4] * f1[4] + R.J[5] * f1[5];
+  float *f2 = nullptr;
+  if (R.b2 >= 0) {
+    f2 = F + 6 * R.b2;
+    s += R.J[6] * f2[0] + R.J[7] * f2[1] + R.J[8] * f2[2] + R.J[9] * f2[3] + R.J[10] * f2[4] + R.J[11] * f2[5];
+  }
+  float delta = R.ad * (R.rhs - old * R.cfm - s);
+  float lo, hi;
+  if (R.findex >= 0) {
+    hi = fabsf(R.hi * lam[R.findex]);
+    lo = -hi;
+  } else {
+    lo = R.lo;
+    hi = R.hi;
+  }
+  float nl = old + delta;
+  if (nl < lo) { delta = lo - old; lam[r] = lo; }
+  else if (nl > hi) { delta = hi - old; lam[r] = hi; }
+  else lam[r] = nl;
+  for (int k = 0; k < 6; k++) f1[k] += delta * R.iMJ[k];
+  if (f2)
+    for (int k = 0; k < 6; k++) f2[k] += delta * R.iMJ[6 + k];
+}
 
 // SOR_LCP in the reference order (quickstep.cpp:406-582): one island per
 // lane; rows ordered with the findex = -1 rows first and the others from the
-// tail, reshuffled every 8 iterations with the island's own RNG state.  LDS:
-// rows, order and fc are copied to the LDS pool first and back after the
-// iterations (same arithmetic, same order); otherwise they stay in the
-// world's scratch (rows, ord, and fc at bt[32 b + 9]).
-template <bool LDS>
+// tail, reshuffled every 8 iterations with the island's own RNG state.
+// MODE 1: rows, order and fc are copied to the LDS pool first and back after
+// the iterations; MODE 2 (worlds whose rows exceed the pool): impulses,
+// order and fc in LDS, the read-only rows streamed from global memory with
+// the next row's load in flight while the current one is relaxed; MODE 0:
+// everything in the world's scratch (rows, ord, and fc at bt[32 b + 9]).
+// The arithmetic and its order are the same in all three.
+template <int MODE>
 __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, int nrows, int ngroups, int nb,
                               int iters) {
+  constexpr bool LDS = MODE == 1;
   constexpr int FS = LDS ? 6 : 32, FO = LDS ? 0 : 9;
   Row *R = LDS ? (Row *)od_pool : rows;
-  int *O = LDS ? (int *)(R + nrows) : ord;
-  float *F = LDS ? (float *)(O + nrows) : bt;
+  int *O = LDS ? (int *)(R + nrows) : (MODE == 2 ? (int *)od_pool : ord);
+  float *F = LDS ? (float *)(O + nrows) : (MODE == 2 ? (float *)(O + nrows) : bt);
+  float *LAM = MODE == 2 ? F + 6 * nb : nullptr;
   const int nt = blockDim.x;
-  if (LDS) {
-    for (int i = threadIdx.x; i < nrows; i += nt) R[i] = rows[i];
+  if (MODE != 0) {
+    if (LDS)
+      for (int i = threadIdx.x; i < nrows; i += nt) R[i] = rows[i];
+    else
+      for (int i = threadIdx.x; i < nrows; i += nt) LAM[i] = rows[i].lambda;
     for (int b = threadIdx.x; b < nb; b += nt)
       for (int k = 0; k < 6; k++) F[6 * b + k] = bt[32 * b + 9 + k];
     __syncthreads();
@@ -823,14 +879,31 @@ __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, in
           int t = o[i]; o[i] = o[sw]; o[sw] = t;
         }
       }
-      for (int i = 0; i < m; i++) relax_row<FS, FO>(R, o[i], F);
+      if (MODE == 2) {
+        int rn = o[0];
+        RowC nx = load_rowc(rows[rn]);
+        for (int i = 0; i < m; i++) {
+          const RowC cu = nx;
+          const int r = rn;
+          if (i + 1 < m) {
+            rn = o[i + 1];
+            nx = load_rowc(rows[rn]);
+          }
+          relax_rowc(cu, r, LAM, F);
+        }
+      } else {
+        for (int i = 0; i < m; i++) relax_row<FS, FO>(R, o[i], F);
+      }
     }
   }
   // islands are solved by lanes of every wavefront; the integration reads
   // the fc of all bodies
   __syncthreads();
-  if (LDS) {
-    for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = R[i].lambda;
+  if (MODE != 0) {
+    if (LDS)
+      for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = R[i].lambda;
+    else
+      for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = LAM[i];
     for (int b = threadIdx.x; b < nb; b += nt)
       for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = F[6 * b + k];
     __syncthreads();
@@ -1330,18 +1403,19 @@ __device__ void sor_colored(const DevBatch &B, const Ctx &c, const WorldParams &
 }
 
 // LDS bytes of the reference-order constraint ordering: 12 ints per
-// constraint, 10 per body, one per joint
+// constraint, 5 per body, one per joint
 __device__ inline long long ref_order_lds_bytes(int nb, int ncons, int nj) {
-  return 4LL * (12LL * ncons + 10LL * nb + nj + 16);
+  return 4LL * (12LL * ncons + 5LL * nb + nj + 16);
 }
 
 // The reference order of constraints and rows in the LDS pool: the same
 // steps as the global-memory version in phase_step (adjacency newest attach
 // first, island walk, rows in island order, per-island RNG state and Info2
-// erp), with the constraint fields gathered into LDS by all lanes first, one
-// lane walking the islands there, and all lanes writing the results back
-// (constraint order, body visit order, island table, row offsets, erp, and
-// m = 0 for the constraints no island reaches).
+// erp), with the constraint fields gathered into LDS by all lanes first and
+// one lane walking the islands there.  The lane writes the body visit order
+// and the island table straight to global memory (nothing reads them back
+// during the walk); all lanes then write the constraint order, row offsets,
+// erp, and m = 0 for the constraints no island reaches.
 __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, Cons *cons, int nk, int ncons,
                               int *corder, int *isl, float *erp_out, int *misc, Shared &sh) {
   const int nb = c.W.nb, nj = ncons - nk, nt = blockDim.x;
@@ -1352,9 +1426,12 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
   float *cbe = cerp + ncons;               // a ball joint's own erp
   int *adj = (int *)(cbe + ncons);         // 2 per constraint
   int *adjs = adj + 2 * ncons;             // nb + 1
-  int *fill = adjs + nb + 1, *tag = fill + nb, *bord = tag + nb, *stack = bord + nb;
-  int *jord = stack + nb;                  // nj
-  int *itab = jord + nj;                   // island table, 5 per island
+  int *fill = adjs + nb + 1;               // fill cursors, then body tags
+  int *tag = fill;
+  int *stack = fill + nb;
+  int *ispan = stack + nb;                 // first constraint and count per island
+  int *jord = ispan + 2 * nb;              // nj
+  int *border = corder + ncons;            // body visit order (global)
   for (int i = threadIdx.x; i < ncons; i += nt) {
     const Cons q = cons[i];
     cb1[i] = q.b1; cb2[i] = q.b2; cm[i] = q.m; ckind[i] = q.kind; cseq[i] = q.seq;
@@ -1363,7 +1440,6 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
     ctag[i] = 0;
   }
   for (int b = threadIdx.x; b <= nb; b += nt) adjs[b] = 0;
-  for (int b = threadIdx.x; b < nb; b += nt) tag[b] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < ncons; i++) {
@@ -1392,12 +1468,13 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
       adj[fill[cb1[pick]]++] = pick;
       if (cb2[pick] >= 0) adj[fill[cb2[pick]]++] = pick;
     }
+    for (int b = 0; b < nb; b++) tag[b] = 0;
     int nic = 0, nib = 0, nisl = 0;
     for (int bb = nb - 1; bb >= 0; bb--) {
       if (tag[bb]) continue;
       tag[bb] = 1;
       int c0 = nic;
-      bord[nib++] = bb;
+      border[nib++] = bb;
       int sp = 0, b = bb;
       while (true) {
         for (int k = adjs[b]; k < adjs[b + 1]; k++) {
@@ -1414,17 +1491,17 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
         }
         if (sp == 0) break;
         b = stack[--sp];
-        bord[nib++] = b;
+        border[nib++] = b;
       }
-      itab[5 * nisl + 0] = c0;
-      itab[5 * nisl + 1] = nic - c0;
+      ispan[2 * nisl + 0] = c0;
+      ispan[2 * nisl + 1] = nic - c0;
       nisl++;
     }
     unsigned seed = P.seed;
     unsigned long long shuffles = (unsigned long long)((P.iterations + 7) / 8);
     int r = 0;
     for (int k = 0; k < nisl; k++) {
-      int c0 = itab[5 * k], cn = itab[5 * k + 1];
+      int c0 = ispan[2 * k], cn = ispan[2 * k + 1];
       int r0 = r;
       float erp = P.erp;
       for (int x = 0; x < cn; x++) {
@@ -1435,9 +1512,11 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
         cerp[i] = erp;
       }
       int m = r - r0;
-      itab[5 * k + 2] = r0;
-      itab[5 * k + 3] = m;
-      itab[5 * k + 4] = (int)seed;
+      isl[5 * k + 0] = c0;
+      isl[5 * k + 1] = cn;
+      isl[5 * k + 2] = r0;
+      isl[5 * k + 3] = m;
+      isl[5 * k + 4] = (int)seed;
       if (m > 0 && m - 1 > 0) seed = lcg_skip(seed, shuffles * (unsigned long long)(m - 1));
     }
     P.seed = seed;
@@ -1445,12 +1524,10 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
     misc[1] = nisl;
     misc[2] = nic;
     misc[3] = nib;
-    sh.ival[4] = nisl;
     sh.ival[5] = nic;
-    sh.ival[6] = nib;
   }
   __syncthreads();
-  const int nisl = sh.ival[4], nic = sh.ival[5], nib = sh.ival[6];
+  const int nic = sh.ival[5];
   for (int i = threadIdx.x; i < ncons; i += nt) {
     if (ctag[i] != 1) {
       cons[i].m = 0;
@@ -1460,8 +1537,6 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
     }
   }
   for (int x = threadIdx.x; x < nic; x += nt) corder[x] = cord[x];
-  for (int x = threadIdx.x; x < nib; x += nt) corder[ncons + x] = bord[x];
-  for (int x = threadIdx.x; x < 5 * nisl; x += nt) isl[x] = itab[x];
   __syncthreads();
 }
 
@@ -1891,9 +1966,11 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
   if (ref) {
     int *ord = corder + ncons + nb;
     if (ref_sor_lds_bytes(nrows, nb) <= B.lds_bytes)
-      sor_reference<true>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
+      sor_reference<1>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
+    else if (ref_sor_lds_bytes_streamed(nrows, nb) <= B.lds_bytes)
+      sor_reference<2>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
     else
-      sor_reference<false>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
+      sor_reference<0>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
     if (threadIdx.x == 0) B.counters[c.w * CNT_N + CNT_ITERS] = iters;
   } else if (STAGE == 1) {
     // the sweeps run in their own kernel

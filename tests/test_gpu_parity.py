@@ -344,9 +344,11 @@ def test_reference_order_independent_of_launch_shape(odemi_mod):
     say("launch shapes")
     ws = scenes.uneven_worlds(8, seed=5)
     outs = []
-    # the last shape has no LDS pool: sorts and the solve run in global
-    # scratch (the others solve the worlds whose rows fit in LDS)
-    for wg, cs, lds in ((256, 1.0, None), (64, 2.0, None), (128, 1.5, None), (128, 1.0, 0)):
+    # the last shapes have a 16 KB pool (rows streamed from global memory,
+    # impulses and forces in LDS) and no LDS pool (sorts, ordering and the
+    # solve in global scratch); the others solve the worlds whose rows fit
+    # in LDS there
+    for wg, cs, lds in ((256, 1.0, None), (64, 2.0, None), (128, 1.5, None), (128, 1.0, 16384), (128, 1.0, 0)):
         b = _batch(odemi_mod, ws)
         b.set_launch_config(wg, cs, lds)
         b.step(12)
