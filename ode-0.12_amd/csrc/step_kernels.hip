@@ -783,7 +783,7 @@ __device__ inline void relax_row(Row *rows, int r, float *fc) {
 // LDS bytes of the reference-order solve of a world: its rows, the row
 // order and 6 fc floats per body
 __device__ inline long long ref_sor_lds_bytes(int nrows, int nb) {
-  return (long long)sizeof(Row) * nrows + 4LL * nrows + 24LL * nb;
+  return (long long)sizeof(Row) * nrows + 8LL * nrows + 24LL * nb;
 }
 // the same with the rows left in global memory: impulses, order and fc only
 __device__ inline long long ref_sor_lds_bytes_streamed(int nrows, int nb) {
@@ -837,27 +837,26 @@ __device__ inline void relax_rowc(const RowC &R, int r, float *lam, float *F) {
 // SOR_LCP in the reference order (quickstep.cpp:406-582): one island per
 // lane; rows ordered with the findex = -1 rows first and the others from the
 // tail, reshuffled every 8 iterations with the island's own RNG state.
-// MODE 1: rows, order and fc are copied to the LDS pool first and back after
-// the iterations; MODE 2 (worlds whose rows exceed the pool): impulses,
-// order and fc in LDS, the read-only rows streamed from global memory with
-// the next row's load in flight while the current one is relaxed; MODE 0:
-// everything in the world's scratch (rows, ord, and fc at bt[32 b + 9]).
-// The arithmetic and its order are the same in all three.
+// MODE 1: rows, impulses, order and fc in the LDS pool (copied in first and
+// back after the iterations); MODE 2 (worlds whose rows exceed the pool):
+// impulses, order and fc in LDS, the read-only rows streamed from global
+// memory; in both the next row's read-only part is loaded while the current
+// one is relaxed.  MODE 0: everything in the world's scratch (rows, ord, and
+// fc at bt[32 b + 9]).  The arithmetic and its order are the same in all
+// three.
 template <int MODE>
 __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, int nrows, int ngroups, int nb,
                               int iters) {
   constexpr bool LDS = MODE == 1;
-  constexpr int FS = LDS ? 6 : 32, FO = LDS ? 0 : 9;
   Row *R = LDS ? (Row *)od_pool : rows;
   int *O = LDS ? (int *)(R + nrows) : (MODE == 2 ? (int *)od_pool : ord);
-  float *F = LDS ? (float *)(O + nrows) : (MODE == 2 ? (float *)(O + nrows) : bt);
-  float *LAM = MODE == 2 ? F + 6 * nb : nullptr;
+  float *F = MODE == 0 ? bt : (float *)(O + nrows);
+  float *LAM = MODE == 0 ? nullptr : F + 6 * nb;
   const int nt = blockDim.x;
   if (MODE != 0) {
     if (LDS)
       for (int i = threadIdx.x; i < nrows; i += nt) R[i] = rows[i];
-    else
-      for (int i = threadIdx.x; i < nrows; i += nt) LAM[i] = rows[i].lambda;
+    for (int i = threadIdx.x; i < nrows; i += nt) LAM[i] = rows[i].lambda;
     for (int b = threadIdx.x; b < nb; b += nt)
       for (int k = 0; k < 6; k++) F[6 * b + k] = bt[32 * b + 9 + k];
     __syncthreads();
@@ -879,20 +878,20 @@ __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, in
           int t = o[i]; o[i] = o[sw]; o[sw] = t;
         }
       }
-      if (MODE == 2) {
+      if (MODE != 0) {
         int rn = o[0];
-        RowC nx = load_rowc(rows[rn]);
+        RowC nx = load_rowc(R[rn]);
         for (int i = 0; i < m; i++) {
           const RowC cu = nx;
           const int r = rn;
           if (i + 1 < m) {
             rn = o[i + 1];
-            nx = load_rowc(rows[rn]);
+            nx = load_rowc(R[rn]);
           }
           relax_rowc(cu, r, LAM, F);
         }
       } else {
-        for (int i = 0; i < m; i++) relax_row<FS, FO>(R, o[i], F);
+        for (int i = 0; i < m; i++) relax_row<32, 9>(R, o[i], F);
       }
     }
   }
@@ -900,10 +899,7 @@ __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, in
   // the fc of all bodies
   __syncthreads();
   if (MODE != 0) {
-    if (LDS)
-      for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = R[i].lambda;
-    else
-      for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = LAM[i];
+    for (int i = threadIdx.x; i < nrows; i += nt) rows[i].lambda = LAM[i];
     for (int b = threadIdx.x; b < nb; b += nt)
       for (int k = 0; k < 6; k++) bt[32 * b + 9 + k] = F[6 * b + k];
     __syncthreads();
@@ -1441,14 +1437,28 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
   }
   for (int b = threadIdx.x; b <= nb; b += nt) adjs[b] = 0;
   __syncthreads();
+  // adjacency list sizes (LDS atomics), then their prefix (block scan) and
+  // the fill cursors, all lanes
+  for (int i = threadIdx.x; i < ncons; i += nt) {
+    if (cm[i] <= 0 && ckind[i] != JT_CONTACT) continue;
+    if (cb1[i] < 0) continue;
+    atomicAdd(&adjs[cb1[i] + 1], 1);
+    if (cb2[i] >= 0) atomicAdd(&adjs[cb2[i] + 1], 1);
+  }
+  __syncthreads();
+  {
+    const int per = (nb + 1 + nt - 1) / nt, d0 = threadIdx.x * per;
+    const int d1 = d0 + per < nb + 1 ? d0 + per : nb + 1;
+    int loc = 0;
+    for (int d = d0; d < d1; d++) loc += adjs[d];
+    int tot;
+    int run = block_excl_scan(loc, tot, sh);
+    for (int d = d0; d < d1; d++) { run += adjs[d]; adjs[d] = run; }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += nt) fill[b] = adjs[b];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 0; i < ncons; i++) {
-      if (cm[i] <= 0 && ckind[i] != JT_CONTACT) continue;
-      if (cb1[i] < 0) continue;
-      adjs[cb1[i] + 1]++;
-      if (cb2[i] >= 0) adjs[cb2[i] + 1]++;
-    }
-    for (int b = 0; b < nb; b++) adjs[b + 1] += adjs[b];
     int nj2 = 0;
     for (int i = nk; i < ncons; i++) {
       if (cb1[i] < 0) continue;
@@ -1456,7 +1466,6 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
       while (p > 0 && cseq[jord[p - 1]] > s) { jord[p] = jord[p - 1]; p--; }
       jord[p] = i;
     }
-    for (int b = 0; b < nb; b++) fill[b] = adjs[b];
     int ci = nk - 1, jj = nj2 - 1;
     while (ci >= 0 || jj >= 0) {
       int pick;
