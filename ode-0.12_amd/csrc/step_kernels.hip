@@ -806,20 +806,23 @@ __device__ inline RowC load_rowc(const Row &R) {
 }
 
 // relax_row on a row held in registers, its impulse in lam[r] and fc in
-// F[6 b]: the same arithmetic in the same order
+// F[6 b]: the same arithmetic in the same order.  All reads (the row's
+// impulse, its normal row's impulse, both bodies' fc) are issued before any
+// write, so they share one LDS round trip.
 __device__ inline void relax_rowc(const RowC &R, int r, float *lam, float *F) {
-  float old = lam[r];
+  const bool two = R.b2 >= 0;
+  const float old = lam[r];
+  const float lf = lam[R.findex >= 0 ? R.findex : r];
   float *f1 = F + 6 * R.b1;
-  float s = R.J[0] * f1[0] + R.J[1] * f1[1] + R.J[2] * f1[2] + R.J[3] * f1[3] + R.J[4] * f1[4] + R.J[5] * f1[5];
-  float *f2 = nullptr;
-  if (R.b2 >= 0) {
-    f2 = F + 6 * R.b2;
-    s += R.J[6] * f2[0] + R.J[7] * f2[1] + R.J[8] * f2[2] + R.J[9] * f2[3] + R.J[10] * f2[4] + R.J[11] * f2[5];
-  }
+  float *f2 = F + 6 * (two ? R.b2 : R.b1);
+  float fa[6], fb[6];
+  for (int k = 0; k < 6; k++) { fa[k] = f1[k]; fb[k] = f2[k]; }
+  float s = R.J[0] * fa[0] + R.J[1] * fa[1] + R.J[2] * fa[2] + R.J[3] * fa[3] + R.J[4] * fa[4] + R.J[5] * fa[5];
+  if (two) s += R.J[6] * fb[0] + R.J[7] * fb[1] + R.J[8] * fb[2] + R.J[9] * fb[3] + R.J[10] * fb[4] + R.J[11] * fb[5];
   float delta = R.ad * (R.rhs - old * R.cfm - s);
   float lo, hi;
   if (R.findex >= 0) {
-    hi = fabsf(R.hi * lam[R.findex]);
+    hi = fabsf(R.hi * lf);
     lo = -hi;
   } else {
     lo = R.lo;
@@ -829,9 +832,9 @@ __device__ inline void relax_rowc(const RowC &R, int r, float *lam, float *F) {
   if (nl < lo) { delta = lo - old; lam[r] = lo; }
   else if (nl > hi) { delta = hi - old; lam[r] = hi; }
   else lam[r] = nl;
-  for (int k = 0; k < 6; k++) f1[k] += delta * R.iMJ[k];
-  if (f2)
-    for (int k = 0; k < 6; k++) f2[k] += delta * R.iMJ[6 + k];
+  for (int k = 0; k < 6; k++) f1[k] = fa[k] + delta * R.iMJ[k];
+  if (two)
+    for (int k = 0; k < 6; k++) f2[k] = fb[k] + delta * R.iMJ[6 + k];
 }
 
 // SOR_LCP in the reference order (quickstep.cpp:406-582): one island per
@@ -879,16 +882,15 @@ __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, in
         }
       }
       if (MODE != 0) {
-        int rn = o[0];
-        RowC nx = load_rowc(R[rn]);
-        for (int i = 0; i < m; i++) {
-          const RowC cu = nx;
-          const int r = rn;
-          if (i + 1 < m) {
-            rn = o[i + 1];
-            nx = load_rowc(R[rn]);
-          }
-          relax_rowc(cu, r, LAM, F);
+        // two row buffers in turn: the next row loads while this one relaxes
+        int ra = o[0], rb = ra;
+        RowC qa = load_rowc(R[ra]), qb;
+        for (int i = 0; i < m; i += 2) {
+          if (i + 1 < m) { rb = o[i + 1]; qb = load_rowc(R[rb]); }
+          relax_rowc(qa, ra, LAM, F);
+          if (i + 1 >= m) break;
+          if (i + 2 < m) { ra = o[i + 2]; qa = load_rowc(R[ra]); }
+          relax_rowc(qb, rb, LAM, F);
         }
       } else {
         for (int i = 0; i < m; i++) relax_row<32, 9>(R, o[i], F);
