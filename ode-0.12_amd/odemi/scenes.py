@@ -160,6 +160,28 @@ class _Builder:
             **kw)
 
 
+def joint_zoo(seed: int = 5) -> Scene:
+    """Joint limits, stops and motors over a plane: a powered hinge to the
+    world with stops (motor 2 rad/s, fmax 3, stops +-0.4 rad), a hinge with
+    bouncy soft stops below it, a ball joint, and a wheel on a hinge2 with a
+    locked steering axis (stops 0.1..0.1), a spin motor (vel2 8 rad/s, fmax2
+    15) and a soft suspension; h=0.01, mu=0.8."""
+    b = _Builder()
+    b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
+    a = b.box_body((0.0, 0.0, 1.6), (0.6, 0.2, 0.2), 1.0)
+    c = b.box_body((0.0, 0.0, 1.1), (0.2, 0.2, 0.6), 1.0)
+    d = b.sphere_body((0.0, 0.0, 0.65), 0.15, 1.0)
+    b.joint(HINGE, a, -1, (0.0, 0.0, 1.6), axis1=(0.0, 0.0, 1.0), lo=-0.4, hi=0.4, vel=2.0, fmax=3.0)
+    b.joint(HINGE, c, a, (0.0, 0.0, 1.4), axis1=(1.0, 0.0, 0.0), lo=-0.3, hi=0.25, bounce=0.5, stop_erp=0.6,
+            stop_cfm=1e-4)
+    b.joint(BALL, d, c, (0.0, 0.0, 0.8))
+    ch = b.box_body((2.0, 0.0, 0.6), (1.0, 0.6, 0.2), 2.0)
+    wh = b.sphere_body((2.0, 0.45, 0.35), 0.3, 1.0)
+    b.joint(HINGE2, ch, wh, (2.0, 0.45, 0.35), axis1=(0.0, 0.0, 1.0), axis2=(0.0, 1.0, 0.0), lo=0.1, hi=0.1,
+            vel2=8.0, fmax2=15.0, susp_erp=0.4, susp_cfm=0.02)
+    return b.scene(dt=0.01, mu=0.8, seed=seed)
+
+
 def box_stack(n: int = 10, gap: float = 1e-3, seed: int = 0) -> Scene:
     """10 boxes 1x1x1 m, density 1, on a z=0 plane with 1 mm gaps; h=0.01,
     20 iterations, ERP 0.2, CFM 1e-5, mu=0.5 (Coulomb friction through findex)."""

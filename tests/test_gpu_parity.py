@@ -280,7 +280,7 @@ def _envelope(sc, n, every):
     return np.abs(a[:, :, :3] - b[:, :, :3]).max(axis=(1, 2))
 
 
-@pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle", "chains200", "vehicle200"])
+@pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle", "chains200", "vehicle200", "joints"])
 def test_trajectory_within_fp32_envelope(odemi_mod, name):
     say("trajectory", name)
     from test_golden import CASES
