@@ -8,6 +8,10 @@ Each generator returns ``Scene`` objects (one per world) built from a seed:
   ball/hinge chains.
 * ``vehicle_worlds`` -- four-wheeled cars on hinge2 suspensions over ramps.
 * ``diff_piles``     -- small piles for the adjoint.
+* ``joint_zoo``      -- hinge stops and motors, ball, hinge2 steering stops,
+  spin motor and suspension.
+* ``surface_pile``   -- a pile with every contact surface option (mu2,
+  bounce, soft ERP/CFM), or frictionless.
 
 Bodies, geoms and joints are listed in creation order; the order matters for
 the reference's list-order effects (ode/src/ode.cpp:62-68 prepends bodies and
@@ -180,6 +184,20 @@ def joint_zoo(seed: int = 5) -> Scene:
     b.joint(HINGE2, ch, wh, (2.0, 0.45, 0.35), axis1=(0.0, 0.0, 1.0), axis2=(0.0, 1.0, 0.0), lo=0.1, hi=0.1,
             vel2=8.0, fmax2=15.0, susp_erp=0.4, susp_cfm=0.02)
     return b.scene(dt=0.01, mu=0.8, seed=seed)
+
+
+def surface_pile(seed: int = 13, mu: float = 0.7, mu2: float = 0.3) -> Scene:
+    """A 40-body pile with every contact surface option of the batch policy:
+    dContactMu2 (mu along the first friction direction, mu2 along the
+    second), dContactBounce (0.3 above 0.05 m/s), dContactSoftERP (0.5) and
+    dContactSoftCFM (1e-4), dContactApprox1; mu = 0 gives frictionless
+    contacts (one row each)."""
+    sc = pile_world(np.random.default_rng(seed), 40, 0, seed=seed)
+    sc.surf_mode = APPROX1 | MU2 | BOUNCE | SOFT_ERP | SOFT_CFM
+    sc.mu, sc.mu2 = mu, mu2
+    sc.bounce, sc.bounce_vel = 0.3, 0.05
+    sc.soft_erp, sc.soft_cfm = 0.5, 1e-4
+    return sc
 
 
 def box_stack(n: int = 10, gap: float = 1e-3, seed: int = 0) -> Scene:

@@ -160,7 +160,7 @@ def _batch(odemi_mod, scs, order=None):
     return odemi_mod.Batch(scs, odemi_mod.ORDER_REFERENCE if order is None else order)
 
 
-@pytest.mark.parametrize("name", ["pile4096", "pile256_dense", "chains", "vehicle"])
+@pytest.mark.parametrize("name", ["pile4096", "pile256_dense", "chains", "vehicle", "surfaces"])
 def test_first_step_pairs_and_contacts_match_reference(odemi_mod, name):
     say("first-step parity", name)
     from test_golden import CASES
@@ -280,7 +280,8 @@ def _envelope(sc, n, every):
     return np.abs(a[:, :, :3] - b[:, :, :3]).max(axis=(1, 2))
 
 
-@pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle", "chains200", "vehicle200", "joints"])
+@pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle", "chains200", "vehicle200", "joints", "surfaces",
+                                  "frictionless"])
 def test_trajectory_within_fp32_envelope(odemi_mod, name):
     say("trajectory", name)
     from test_golden import CASES
