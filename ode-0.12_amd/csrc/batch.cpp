@@ -338,9 +338,10 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
 // The descriptor checks the ODE setters make by assertion (sphere radius and
 // box sides >= 0, sphere.cpp:52, box.cpp:51; only placeable geoms take a
 // body, collision_kernel.cpp:510; dJointAttach refuses body1 == body2,
-// ode.cpp:1352; dMassCheck wants mass > 0, mass.cpp:48), plus index ranges:
-// a descriptor that fails one is refused before any device work, with the
-// reason reported.
+// ode.cpp:1352; dMassCheck wants mass > 0 and a positive definite inertia,
+// mass.cpp:46-53; dBodySetQuaternion normalises, so a zero quaternion has no
+// meaning), plus index ranges and finite poses: a descriptor that fails one
+// is refused before any device work, with the reason reported.
 static bool validate_desc(const dBatchDesc *D) {
   if (!D || D->nworlds <= 0 || !D->worlds) {
     report_error("dBatchCreate: no worlds");
@@ -376,9 +377,39 @@ static bool validate_desc(const dBatchDesc *D) {
   for (int w = 0; w < D->nworlds; w++) {
     const dBatchWorldDesc &W = D->worlds[w];
     for (int i = 0; i < W.nb; i++) {
-      const double m = D->body_mass[b0 + i];
+      const long long bi = b0 + i;
+      const double m = D->body_mass[bi];
       if (!(m > 0 && std::isfinite(m))) {
         report_error("dBatchCreate: world %d body %d: mass must be positive and finite", w, i);
+        return false;
+      }
+      // inertia positive definite (Cholesky of the symmetric part)
+      const double *I = D->body_I + 9 * bi;
+      const double a = I[0], b = 0.5 * (I[1] + I[3]), c = 0.5 * (I[2] + I[6]);
+      const double e = I[4], f = 0.5 * (I[5] + I[7]), k = I[8];
+      bool pd = a > 0;
+      if (pd) {
+        const double l11 = std::sqrt(a), l21 = b / l11, l31 = c / l11;
+        const double d2 = e - l21 * l21;
+        pd = d2 > 0;
+        if (pd) {
+          const double l22 = std::sqrt(d2), l32 = (f - l31 * l21) / l22;
+          pd = k - l31 * l31 - l32 * l32 > 0;
+        }
+      }
+      if (!pd) {
+        report_error("dBatchCreate: world %d body %d: inertia must be positive definite", w, i);
+        return false;
+      }
+      const double *q = D->body_q + 4 * bi;
+      double qq = 0;
+      bool fin = true;
+      for (int t = 0; t < 4; t++) { qq += q[t] * q[t]; fin &= std::isfinite(q[t]); }
+      for (int t = 0; t < 3; t++)
+        fin &= std::isfinite(D->body_pos[3 * bi + t]) && std::isfinite(D->body_lvel[3 * bi + t]) &&
+               std::isfinite(D->body_avel[3 * bi + t]);
+      if (!fin || !(qq > 0)) {
+        report_error("dBatchCreate: world %d body %d: pose or velocity not finite, or zero quaternion", w, i);
         return false;
       }
     }

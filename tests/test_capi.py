@@ -79,10 +79,16 @@ def _bad_scenes():
     assert ch.nj > 0
     sc = ch; sc.joint_b2 = sc.joint_b2.copy(); sc.joint_b1 = sc.joint_b1.copy(); sc.joint_b2[0] = sc.joint_b1[0]
     out.append(("joint on one body twice", sc, "out of range or equal"))
+    sc = base(); sc.body_I = sc.body_I.copy(); sc.body_I[2] = np.diag([1.0, -1.0, 1.0]).reshape(9)
+    out.append(("inertia not positive definite", sc, "positive definite"))
+    sc = base(); sc.body_q = sc.body_q.copy(); sc.body_q[0] = 0.0
+    out.append(("zero quaternion", sc, "zero quaternion"))
+    sc = base(); sc.body_pos = sc.body_pos.copy(); sc.body_pos[1, 2] = float("nan")
+    out.append(("NaN position", sc, "not finite"))
     return out
 
 
-@pytest.mark.parametrize("case", range(7))
+@pytest.mark.parametrize("case", range(10))
 def test_batch_descriptor_faults_are_refused(root, capfd, case):
     """dBatchCreate refuses a descriptor the reference's setters would assert
     on, before any device work (no GPU needed), and says why."""

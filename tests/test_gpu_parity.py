@@ -533,10 +533,12 @@ def _free_fall_world(v0=(1.0, -0.5, 2.0)):
 
 
 def _nan_world():
+    """Two spheres over a plane; the test makes the first one's velocity NaN
+    through dBatchSetState (dBatchCreate refuses non-finite input)."""
     from odemi.scenes import _Builder, PLANE
     b = _Builder()
     b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
-    b.sphere_body((0.0, 0.0, 0.5), 0.3, 1.0, lvel=(float("nan"), 0.0, 0.0))
+    b.sphere_body((0.0, 0.0, 0.5), 0.3, 1.0)
     b.sphere_body((2.0, 0.0, 0.25), 0.3, 1.0)
     return b.scene(dt=0.01, mu=0.5, seed=2)
 
@@ -544,9 +546,9 @@ def _nan_world():
 def test_ragged_batch_worlds_step_independently(odemi_mod):
     """Worlds of very different kinds and sizes in one batch (a lone body in
     free fall with no geoms to meet, the box stack, uneven piles with chains, a
-    vehicle, a world whose state is non-finite from the start), reference
-    order: every finite world ends bit-identical to the same world stepped
-    alone; the non-finite world is frozen and flagged without touching the
+    vehicle, a world whose state is made non-finite before the first step),
+    reference order: every finite world ends bit-identical to the same world
+    stepped alone; the non-finite world is frozen and flagged without touching the
     others; the free-falling body follows the integrator's closed form
     (v += h g, x += h v: x_n = x_0 + n h v_0 + h^2 g n (n + 1) / 2) within
     FP32 rounding (1e-4 m)."""
@@ -556,6 +558,9 @@ def test_ragged_batch_worlds_step_independently(odemi_mod):
     nsteps = 25
     say("ragged batch", [w.nb for w in worlds])
     b = _batch(odemi_mod, worlds)
+    st0 = b.state()
+    st0[sum(w.nb for w in worlds[:-1]), 7] = np.nan      # lvel.x of the last world's first body
+    b.set_state(st0)
     b.step(nsteps)
     st = b.state()
     wc = b.world_counters()
