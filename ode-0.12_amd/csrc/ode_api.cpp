@@ -679,6 +679,10 @@ void dJointDestroy(dJointID j) {
 }
 // dJointAttach (ode.cpp:1348-1399)
 void dJointAttach(dJointID j, dBodyID b1, dBodyID b2) {
+  if (b1 && b1 == b2) {   // dUASSERT in the reference (ode.cpp:1352)
+    report_debug("can't have body1==body2");
+    return;
+  }
   if (j->node[0] || j->node[1]) detach(j);
   if (!b1) { b1 = b2; b2 = nullptr; j->flags |= JF_REVERSE; }
   else j->flags &= ~JF_REVERSE;
@@ -876,6 +880,10 @@ void dGeomDestroy(dGeomID g) {
 void dGeomSetData(dGeomID g, void *d) { g->data = d; }
 void *dGeomGetData(dGeomID g) { return g->data; }
 void dGeomSetBody(dGeomID g, dBodyID b) {  // collision_kernel.cpp:507-556
+  if (b && g->cls == dPlaneClass) {   // planes are not placeable (collision_kernel.cpp:510)
+    report_debug("geom must be placeable");
+    return;
+  }
   if (b) {
     if (g->body != b) {
       if (g->body) {

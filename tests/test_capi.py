@@ -98,3 +98,35 @@ def test_batch_descriptor_faults_are_refused(root, capfd, case):
         odemi.Batch([sc], odemi.ORDER_REFERENCE)
     err = capfd.readouterr().err
     assert "dBatchCreate" in err and msg in err, (name, err)
+
+
+def test_api_assertions_follow_the_reference(root, capfd):
+    """The reference's argument assertions on the API path go to the debug
+    handler and leave the object unchanged (no GPU work): dGeomSetBody on a
+    plane ("geom must be placeable", collision_kernel.cpp:510) and
+    dJointAttach of one body to itself ("can't have body1==body2",
+    ode.cpp:1352)."""
+    import ctypes as C
+    import odemi
+    L = odemi.lib()
+    V = C.c_void_p
+    for name, res, args in [("dWorldCreate", V, []), ("dWorldDestroy", None, [V]), ("dBodyCreate", V, [V]),
+                            ("dCreatePlane", V, [V, C.c_float, C.c_float, C.c_float, C.c_float]),
+                            ("dGeomSetBody", None, [V, V]), ("dGeomGetBody", V, [V]), ("dGeomDestroy", None, [V]),
+                            ("dJointCreateBall", V, [V, V]), ("dJointAttach", None, [V, V, V]),
+                            ("dJointGetBody", V, [V, C.c_int])]:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    w = L.dWorldCreate()
+    b = L.dBodyCreate(w)
+    p = L.dCreatePlane(None, 0.0, 0.0, 1.0, 0.0)
+    L.dGeomSetBody(p, b)
+    assert not L.dGeomGetBody(p)
+    j = L.dJointCreateBall(w, None)
+    L.dJointAttach(j, b, b)
+    assert not L.dJointGetBody(j, 0) and not L.dJointGetBody(j, 1)
+    err = capfd.readouterr().err
+    assert "geom must be placeable" in err and "body1==body2" in err, err
+    L.dGeomDestroy(p)
+    L.dWorldDestroy(w)
