@@ -750,16 +750,15 @@ __device__ inline Surf host_surface(const ContactIn &ci) {
 }
 
 // relax one row: ode/src/quickstep.cpp:484-582 (J and b scaled by ad there).
-// fc holds a body's 6 constraint-force floats at fc + FS * body + FO.
-template <int FS, int FO>
-__device__ inline void relax_row(Row *rows, int r, float *fc) {
+// A body's 6 constraint-force floats are at bt[32 b + 9] (per-body scratch).
+__device__ inline void relax_row(Row *rows, int r, float *bt) {
   Row &R = rows[r];
   float old = R.lambda;
-  float *f1 = fc + FS * R.b1 + FO;
+  float *f1 = bt + 32 * R.b1 + 9;
   float s = R.J[0] * f1[0] + R.J[1] * f1[1] + R.J[2] * f1[2] + R.J[3] * f1[3] + R.J[4] * f1[4] + R.J[5] * f1[5];
   float *f2 = nullptr;
   if (R.b2 >= 0) {
-    f2 = fc + FS * R.b2 + FO;
+    f2 = bt + 32 * R.b2 + 9;
     s += R.J[6] * f2[0] + R.J[7] * f2[1] + R.J[8] * f2[2] + R.J[9] * f2[3] + R.J[10] * f2[4] + R.J[11] * f2[5];
   }
   float delta = R.ad * (R.rhs - old * R.cfm - s);
@@ -893,7 +892,7 @@ __device__ void sor_reference(Row *rows, float *bt, const int *isl, int *ord, in
           relax_rowc(qb, rb, LAM, F);
         }
       } else {
-        for (int i = 0; i < m; i++) relax_row<32, 9>(R, o[i], F);
+        for (int i = 0; i < m; i++) relax_row(R, o[i], F);
       }
     }
   }
