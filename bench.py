@@ -339,12 +339,14 @@ def main():
     if args.launch_tol:
         wg, cs, lds, split, grid = args.launch_tol.split(",")
         b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
-    elif args.autotune:
-        b.autotune(1, keep_cell=True, fused_only=True)   # launch shape for the iterate-to-tolerance solver
-    cfg_tol = b.launch_config()
-    log(f"launch config at tolerance {cfg_tol}")
     if args.tol_warmup:
         b.step(args.tol_warmup)
+    if not args.launch_tol and args.autotune:
+        # launch shape for the iterate-to-tolerance solver, timed on
+        # warm-started steps like the ones measured (state restored after)
+        b.autotune(2, keep_cell=True, fused_only=True)
+    cfg_tol = b.launch_config()
+    log(f"launch config at tolerance {cfg_tol}")
     wall, dev_ms = _timed(b, args.steps)
     cnt = b.counters()
     wc = b.world_counters()

@@ -1242,9 +1242,11 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   while (it < maxit) {
     const int n = every < maxit - it ? every : maxit - it;
     for (int rep = 0; rep < n; rep++) {
-      if (acc) {
+      if (acc && rep == 0) {
         // the pending momentum beta p_prev of the previous sweep (kept in
-        // ld.x for the rows and fp for the bodies), then fc before this sweep
+        // ld.x for the rows and fp for the bodies), then fc before this
+        // sweep; inside a block of sweeps between two checks this is folded
+        // into the previous sweep's CG update below
         for (int r = threadIdx.x; r < nrows; r += nt)
           if (ld[r].x != 0.f) slam[r] += ld[r].x;
         for (int b = threadIdx.x; b < nb; b += nt)
@@ -1270,17 +1272,28 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
         const float rr = block_fsum(loc, sh);
         float beta = (rr_prev > 0.f) ? rr / rr_prev : 0.f;
         if (!(beta <= 1.f)) beta = 0.f;
+        // the next sweep of this block takes the momentum now (the same
+        // operations as the pending-momentum step above); the last sweep
+        // before a check leaves it pending, so the check sees the sweep's
+        // own result
+        const bool now = rep + 1 < n;
         for (int r = threadIdx.x; r < nrows; r += nt) {
           float d = slam[r] - ld[r].x;
           float pend = beta * ld[r].y;   // momentum along the previous direction
           ld[r].y = d + pend;            // new direction
-          ld[r].x = pend;
+          if (!now) ld[r].x = pend;
+          else if (pend != 0.f) slam[r] += pend;
         }
         for (int b = threadIdx.x; b < nb; b += nt)
           for (int k = 0; k < 6; k++) {
             float pend = beta * sa[b].fdir[k];
             sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + pend;
-            sa[b].fp[k] = pend;
+            if (!now) {
+              sa[b].fp[k] = pend;
+            } else {
+              sb[b].fc[k] += pend;
+              sa[b].fp[k] = sb[b].fc[k];
+            }
           }
         rr_prev = rr;
         __syncthreads();
