@@ -629,3 +629,23 @@ def test_capacity_overflow_is_flagged_and_deterministic(odemi_mod, monkeypatch):
     assert {tuple(p) for p in p1} <= full_pairs
     assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
     assert np.array_equal(runs[0][2], runs[1][2], equal_nan=True)
+
+
+def test_coloured_order_is_reproducible(odemi_mod):
+    """The coloured order at a tolerance (CG steps, warm start) gives the same
+    bits on every run at one launch shape: pair slots are taken in any order
+    but sorted by key, colours are bucketed stably, and the block sums of the
+    stopping test and the CG step run in a fixed order."""
+    ws = scenes.uneven_worlds(12, seed=11)
+    outs = []
+    for rep in range(2):
+        say("coloured reproducibility run", rep)
+        b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+        b.set_launch_config(128, 1.5, None, 0, 256)
+        b.set_tolerance(1e-4, 400, 10, accel=True)
+        b.set_warm_start(1.0)
+        b.step(15)
+        outs.append((b.state(), b.world_counters()))
+        b.close()
+    assert np.array_equal(outs[0][0], outs[1][0], equal_nan=True)
+    assert np.array_equal(outs[0][1], outs[1][1])
