@@ -945,6 +945,27 @@ __device__ inline float row_residual(const Row *rows, int r, const float *fc) {
 constexpr int SOR_MAX_ROWS = 6;   // rows of one constraint (contact 3, ball 3, hinge 6, hinge2 6)
 
 // J_1 . f1 (+ J_2 . f2) of one row (J_l2 = -J_l1), summed in the order written
+// Rows of the coloured solver's working set are stored with their four
+// 16-byte chunks rotated by ((r >> 1) & 3) for row r: neighbouring lanes read
+// rows 3 (contacts) or 1 apart, whose 64-byte starts fall on two LDS banks
+// only; the rotation spreads one chunk read of eight such lanes over all 32
+// banks.  The layout is the same in global scratch and in LDS.
+__device__ inline int srow_chunk(int r, int j) { return j ^ ((r >> 1) & 3); }
+
+__device__ inline void store_srow(SRow *base, int r, const SRow &v) {
+  char *dst = (char *)(base + r);
+  const char *src = (const char *)&v;
+  for (int j = 0; j < 4; j++) __builtin_memcpy(dst + 16 * srow_chunk(r, j), src + 16 * j, 16);
+}
+
+__device__ inline SRow load_srow(const SRow *base, int r) {
+  SRow v;
+  const char *src = (const char *)(base + r);
+  char *dst = (char *)&v;
+  for (int j = 0; j < 4; j++) __builtin_memcpy(dst + 16 * j, src + 16 * srow_chunk(r, j), 16);
+  return v;
+}
+
 __device__ inline float sor_dot(const SRow &r, const float *f1, bool two, const float *f2) {
   float s = r.l[0] * f1[0] + r.l[1] * f1[1] + r.l[2] * f1[2] + r.a1[0] * f1[3] + r.a1[1] * f1[4] + r.a1[2] * f1[5];
   if (two)
@@ -963,7 +984,7 @@ __device__ inline float sor_cons_residual(const SBody *sb, const SRow *sr, const
     for (int k = 0; k < 6; k++) f2[k] = sb[q.b2].fc[k];
   float r = 0.f;
   for (int k = 0; k < q.m; k++) {
-    const SRow &w = sr[q.row0 + k];
+    const SRow w = load_srow(sr, q.row0 + k);
     const float lam = slam[q.row0 + k];
     const float y = sor_dot(w, f1, two, f2) + w.cfm * lam - w.rhs;
     float lo = w.lo, hi = w.hi;
@@ -1016,7 +1037,7 @@ __device__ inline void sor_relax(SBody *__restrict__ sb, const SRow *__restrict_
     float L3[3];
 #pragma unroll
     for (int j = 0; j < 3; j++)
-      if (k0 + j < q.m) { R3[j] = sr[q.row0 + k0 + j]; L3[j] = slam[q.row0 + k0 + j]; }
+      if (k0 + j < q.m) { R3[j] = load_srow(sr, q.row0 + k0 + j); L3[j] = slam[q.row0 + k0 + j]; }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       const int k = k0 + j;
@@ -1153,7 +1174,7 @@ __device__ void sor_prepare(const DevBatch &B, const Ctx &c, const WorldParams &
         w.rhs = R.rhs; w.cfm = R.cfm; w.ad = R.ad; w.lo = R.lo; w.hi = R.hi;
         w.pad = 0.f;
         w.findex = R.findex >= 0 ? R.findex - q->row0 : -1;
-        S.sr[o.row0 + k] = w;
+        store_srow(S.sr, o.row0 + k, w);
         S.slam[o.row0 + k] = k < SOR_MAX_ROWS ? wl[k] : 0.f;
       }
     }
@@ -1216,7 +1237,7 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
         const float im1 = sb[q.b1].im;
         for (int k = 0; k < q.m && k < SOR_MAX_ROWS; k++) {
           const float lam = slam[q.row0 + k];
-          if (lam != 0.f) sor_push(sr[q.row0 + k], lam, im1, iI1, f1, two, im2, iI2, f2);
+          if (lam != 0.f) sor_push(load_srow(sr, q.row0 + k), lam, im1, iI1, f1, two, im2, iI2, f2);
         }
         for (int k = 0; k < 6; k++) sb[q.b1].fc[k] = f1[k];
         if (two)
