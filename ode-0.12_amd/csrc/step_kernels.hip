@@ -1270,11 +1270,11 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
         // into the previous sweep's CG update below
         for (int r = threadIdx.x; r < nrows; r += nt)
           if (ld[r].x != 0.f) slam[r] += ld[r].x;
-        for (int b = threadIdx.x; b < nb; b += nt)
-          for (int k = 0; k < 6; k++) {
-            sb[b].fc[k] += sa[b].fp[k];
-            sa[b].fp[k] = sb[b].fc[k];
-          }
+        for (int e = threadIdx.x; e < 6 * nb; e += nt) {   // one fc component per lane
+          const int b = e / 6, k = e - 6 * b;
+          sb[b].fc[k] += sa[b].fp[k];
+          sa[b].fp[k] = sb[b].fc[k];
+        }
         __syncthreads();
       }
       for (int g = 0; g < ngroups; g++) {
@@ -1305,17 +1305,17 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
           if (!now) ld[r].x = pend;
           else if (pend != 0.f) slam[r] += pend;
         }
-        for (int b = threadIdx.x; b < nb; b += nt)
-          for (int k = 0; k < 6; k++) {
-            float pend = beta * sa[b].fdir[k];
-            sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + pend;
-            if (!now) {
-              sa[b].fp[k] = pend;
-            } else {
-              sb[b].fc[k] += pend;
-              sa[b].fp[k] = sb[b].fc[k];
-            }
+        for (int e = threadIdx.x; e < 6 * nb; e += nt) {   // one fc component per lane
+          const int b = e / 6, k = e - 6 * b;
+          float pend = beta * sa[b].fdir[k];
+          sa[b].fdir[k] = (sb[b].fc[k] - sa[b].fp[k]) + pend;
+          if (!now) {
+            sa[b].fp[k] = pend;
+          } else {
+            sb[b].fc[k] += pend;
+            sa[b].fp[k] = sb[b].fc[k];
           }
+        }
         rr_prev = rr;
         __syncthreads();
       }
