@@ -15,6 +15,7 @@ iterations, device counters, per-phase clocks, the HBM roofline of the step
 and the reference CPU build on the host.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds N]
+       python bench.py --workload adjoint   (the differentiable piles only)
        python bench.py --workload pile4096|vehicles   (the single 4096-body
        pile and the 8192 vehicle worlds at the quickstep's 20 iterations, in
        the reference and the coloured order, with the reference CPU build on a
@@ -143,7 +144,7 @@ def adjoint_run(odemi, scenes, np, steps, interval):
     fin_w = np.array([np.isfinite(gi[offs[k]:offs[k + 1]]).all() and np.isfinite(gmu[k]) for k in range(len(worlds))])
     return {"workload": f"differentiable piles: {len(worlds)} worlds, U{{32..128}} boxes/spheres ({nb} bodies), "
                         f"bounce 0.2, h=0.01, {steps} steps, checkpoint every {interval}",
-            "forward_ms_per_step": fwd / steps * 1e3, "backward_ms_per_step": bwd / steps * 1e3,
+            "bodies": nb, "forward_ms_per_step": fwd / steps * 1e3, "backward_ms_per_step": bwd / steps * 1e3,
             "backward_body_steps_per_s": nb * steps / bwd, "adjoint_solve_iterations": stats["solve_iterations"],
             "collider_mismatches": stats["collider_mismatches"],
             "worlds_with_finite_gradients": int(fin_w.sum()), "frozen_worlds": int(((wc[:, 4] & 16) != 0).sum())}
@@ -262,14 +263,32 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="fixed launch config 'wg,cell_scale,lds,split,grid' instead of autotuning")
     ap.add_argument("--launch-tol", default=None, help="the same for the iterate-to-tolerance phase")
-    ap.add_argument("--workload", default="uneven", choices=("uneven", "pile4096", "vehicles"),
-                    help="uneven: the metric's batched uneven worlds; pile4096 / vehicles: the other configurations")
+    ap.add_argument("--workload", default="uneven", choices=("uneven", "pile4096", "vehicles", "adjoint"),
+                    help="uneven: the metric's batched uneven worlds; pile4096 / vehicles: the other configurations; "
+                         "adjoint: the differentiable piles (forward with the tape, then dBatchStepBackward)")
     ap.add_argument("--order", default="both", choices=("both", "reference", "coloured"),
                     help="pile4096 / vehicles: the solver order(s) to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "adjoint":
+        import numpy as np
+        import torch
+        import odemi
+        from odemi import scenes
+        torch.cuda.set_device(0)
+        adj = adjoint_run(odemi, scenes, np, args.adjoint_steps, 10)
+        print("#", adj, file=sys.stderr, flush=True)
+        steps = args.adjoint_steps
+        nb = adj["bodies"]
+        print(json.dumps({"metric": "body-steps/s of the step adjoint (dBatchStepBackward, replay + reverse)",
+                          "value": adj["backward_body_steps_per_s"], "unit": "body-steps/s", "n_gpus": 1,
+                          "steps": steps, "warmup": 0, "ms_per_step": adj["backward_ms_per_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic (seeded scenes)", "config": {"workload": adj["workload"], "bodies": nb},
+                          "adjoint": adj}), flush=True)
+        return
     if args.workload != "uneven":
         config_run(args)
         return
