@@ -108,7 +108,7 @@ def _plain_rate(odemi, worlds, cfg, state, steps, tol, max_iters, check_every):
     return dt / steps
 
 
-def adjoint_run(odemi, scenes, np, steps, interval):
+def adjoint_run(odemi, scenes, np, steps, interval, keep=False):
     """Differentiable batched piles: 1024 worlds of U{32..128} boxes/spheres
     with restitution, h=0.01; forward with the tape (checkpoint every
     `interval` steps, SOR iterated to residual 1e-5) and dBatchStepBackward
@@ -120,6 +120,7 @@ def adjoint_run(odemi, scenes, np, steps, interval):
     b = odemi.Batch(worlds, odemi.ORDER_COLORED)
     b.set_tolerance(1e-5, 1000, 10, accel=True)
     b.tape_options(interval)
+    b.tape_keep_steps(keep)
     b.tape_begin(steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -280,6 +281,11 @@ def main():
         torch.cuda.set_device(0)
         adj = adjoint_run(odemi, scenes, np, args.adjoint_steps, 10)
         print("#", adj, file=sys.stderr, flush=True)
+        kept = adjoint_run(odemi, scenes, np, args.adjoint_steps, 10, keep=True)
+        print("# every step kept (no replay):", kept, file=sys.stderr, flush=True)
+        adj["kept_steps"] = {k: kept[k] for k in ("forward_ms_per_step", "backward_ms_per_step",
+                                                  "backward_body_steps_per_s", "worlds_with_finite_gradients",
+                                                  "collider_mismatches")}
         steps = args.adjoint_steps
         nb = adj["bodies"]
         print(json.dumps({"metric": "body-steps/s of the step adjoint (dBatchStepBackward, replay + reverse)",

@@ -447,6 +447,13 @@ ODE_API int dBatchTapeEnd(dBatchID b);
 /* Checkpoint interval, active-set threshold on impulses, relative residual
  * and iteration cap of the adjoint linear solve (not while recording). */
 ODE_API int dBatchTapeSetOptions(dBatchID b, int checkpoint_interval, float act_tol, float lin_tol, int lin_maxit);
+/* on = 1 (before dBatchTapeBegin): the forward pass also keeps every
+ * recorded step's pre-state and contacts on the device (64 bytes per body
+ * plus the contact capacity, per step), and dBatchStepBackward runs the
+ * reverse sweep without replaying the checkpoint segments; the gradients
+ * are the same bits.  No counterpart in the reference (which has no
+ * adjoint). */
+ODE_API int dBatchTapeKeepSteps(dBatchID b, int on);
 /* Totals of the last backward pass: adjoint-solve iterations, and contact
  * pairs whose dual-number collider disagreed with the tape (0 expected). */
 ODE_API int dBatchTapeStats(dBatchID b, long long *out2);

@@ -56,10 +56,44 @@ int tape_begin(BatchImpl *b, int max_steps) {
   t.ckpt.clear();
   t.nsteps = 0;
   t.max_steps = max_steps;
+  if (t.keep) {
+    const int nb = b->nb, nw = b->nworlds;
+    std::vector<long long> con_off(nw);
+    long long ncon = 0;
+    for (int w = 0; w < nw; w++) {
+      con_off[w] = ncon;
+      ncon += b->wi[w].kcap;
+    }
+    t.kncon = ncon;
+    const size_t stbytes = sizeof(float4) * (size_t)nb;
+    if (!(t.kst.alloc(4 * stbytes * (size_t)max_steps) &&
+          t.kcon.alloc(sizeof(adj::TapeContact<float>) * (size_t)ncon * max_steps) &&
+          t.knk.alloc(sizeof(int) * (size_t)nw * max_steps) && t.kovf.alloc(sizeof(int)) &&
+          t.kcon_off.alloc(sizeof(long long) * nw))) {
+      report_error("dBatchTapeBegin: device allocation for %d kept steps failed", max_steps);
+      return 0;
+    }
+    OD_HIP(hipMemcpy(t.kcon_off.p, con_off.data(), sizeof(long long) * nw, hipMemcpyHostToDevice));
+    OD_HIP(hipMemset(t.kovf.p, 0, sizeof(int)));
+  }
   t.ckpt.emplace_back(new Snapshot());
   if (!snapshot(b, *t.ckpt.back(), true)) return 0;
   t.recording = true;
   destroy_graph(b);
+  return 1;
+}
+
+int tape_keep_steps(BatchImpl *b, bool on) {
+  if (b->tape.recording) {
+    report_error("dBatchTapeKeepSteps: set before dBatchTapeBegin");
+    return 0;
+  }
+  b->tape.keep = on;
+  if (!on) {
+    b->tape.kst.release();
+    b->tape.kcon.release();
+    b->tape.knk.release();
+  }
   return 1;
 }
 
@@ -76,7 +110,22 @@ int tape_step(BatchImpl *b, bool sync) {
     report_error("dBatchStep: the tape holds at most %d steps", t.max_steps);
     return 0;
   }
-  if (!batch_launch_once(b)) return 0;
+  if (t.keep) {
+    // this step's pre-state, then (after the step) its contacts
+    const int nb = b->nb, nw = b->nworlds;
+    const size_t stbytes = sizeof(float4) * (size_t)nb;
+    float4 *slot = t.kst.as<float4>() + (size_t)4 * t.nsteps * nb;
+    OD_HIP(hipMemcpyAsync(slot, b->d.pos.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
+    OD_HIP(hipMemcpyAsync(slot + nb, b->d.q.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
+    OD_HIP(hipMemcpyAsync(slot + 2 * (size_t)nb, b->d.lvel.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
+    OD_HIP(hipMemcpyAsync(slot + 3 * (size_t)nb, b->d.avel.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
+    if (!batch_launch_once(b)) return 0;
+    OD_HIP(launch_tape_capture(b->dev, t.kcon.as<adj::TapeContact<float>>() + (size_t)t.nsteps * t.kncon,
+                               t.kcon_off.as<long long>(), t.knk.as<int>() + (size_t)t.nsteps * nw,
+                               t.kovf.as<int>(), b->stream));
+  } else if (!batch_launch_once(b)) {
+    return 0;
+  }
   t.nsteps++;
   if (t.nsteps % t.interval == 0 && t.nsteps < t.max_steps) {
     t.ckpt.emplace_back(new Snapshot());
@@ -146,10 +195,30 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
   Snapshot endsnap;
   if (!snapshot(b, endsnap, true)) return 0;
   const int T = t.nsteps, first = T - nsteps;
+  bool fail = false;
+  if (t.keep) {
+    // the forward pass kept every step: reverse sweep only
+    int ovf = 0;
+    OD_HIP(hipMemcpy(&ovf, t.kovf.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (ovf) {
+      report_error("dBatchStepBackward: a recorded step overflowed its pair/contact/row capacities");
+      fail = true;
+    }
+    for (int s = T - 1; s >= first && !fail; s--) {
+      const float4 *slot = t.kst.as<float4>() + (size_t)4 * s * nb;
+      const float4 *st4[4] = {slot, slot + nb, slot + 2 * (size_t)nb, slot + 3 * (size_t)nb};
+      const float4 *gin[4] = {G[cur][0], G[cur][1], G[cur][2], G[cur][3]};
+      float4 *gout[4] = {G[1 - cur][0], G[1 - cur][1], G[1 - cur][2], G[1 - cur][3]};
+      OD_HIP(launch_adjoint_step(b->dev, st4, t.kcon.as<adj::TapeContact<float>>() + (size_t)s * t.kncon,
+                                 t.kcon_off.as<long long>(), t.knk.as<int>() + (size_t)s * nw, gin, gout,
+                                 gmu.as<float>(), gbn.as<float>(), stats.as<int>(), scr.as<float>(),
+                                 d_scr_off.as<long long>(), t.act_tol, t.lin_tol, t.lin_maxit, 256, b->stream));
+      cur = 1 - cur;
+    }
+  }
   int seg0 = ((T - 1) / K) * K;
   const int seg_first = (first / K) * K;
-  bool fail = false;
-  for (; seg0 >= seg_first && !fail; seg0 -= K) {
+  for (; !t.keep && seg0 >= seg_first && !fail; seg0 -= K) {
     const int seg1 = std::min(seg0 + K, T);
     if (!snapshot(b, *t.ckpt[seg0 / K], false)) return 0;
     // replay the segment, recording each step's pre-state and contacts

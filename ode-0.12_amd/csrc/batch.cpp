@@ -1016,6 +1016,7 @@ int dBatchTapeSetOptions(dBatchID b, int checkpoint_interval, float act_tol, flo
   t.lin_maxit = lin_maxit;
   return 1;
 }
+int dBatchTapeKeepSteps(dBatchID b, int on) { return b ? tape_keep_steps(b->impl, on != 0) : 0; }
 int dBatchTapeStats(dBatchID b, long long *out2) {
   if (!b || !out2) return 0;
   out2[0] = b->impl->tape.iters;

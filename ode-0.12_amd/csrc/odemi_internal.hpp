@@ -27,6 +27,12 @@ struct Tape {
   float lin_tol = 1e-6f;                         // relative residual of the adjoint solve
   int lin_maxit = 500;
   long long iters = 0, mismatches = 0;           // statistics of the last backward pass
+  // keep mode: every recorded step's pre-state (x, q, v, w) and contacts,
+  // written during the forward pass, so that the backward pass needs no
+  // replay (memory per step: 64 B per body plus the contact capacity)
+  bool keep = false;
+  DevBuf kst, kcon, knk, kovf, kcon_off;
+  long long kncon = 0;
 };
 
 // device arrays of a batch
@@ -101,6 +107,7 @@ int batch_autotune(BatchImpl *b, int trial_steps, int flags);
 
 int snapshot(BatchImpl *b, Snapshot &s, bool save);
 int tape_begin(BatchImpl *b, int max_steps);
+int tape_keep_steps(BatchImpl *b, bool on);
 int tape_end(BatchImpl *b);
 int tape_step(BatchImpl *b, bool sync);
 int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_in, float *grad_mu,

@@ -134,6 +134,12 @@ class Batch:
         if not self.L.dBatchTapeSetOptions(self.h, checkpoint_interval, act_tol, lin_tol, lin_maxit):
             raise ValueError("bad tape options (or the tape is recording)")
 
+    def tape_keep_steps(self, on: bool = True):
+        """Keep every recorded step on the device so that step_backward
+        needs no replay (set before tape_begin)."""
+        if not self.L.dBatchTapeKeepSteps(self.h, 1 if on else 0):
+            raise RuntimeError("dBatchTapeKeepSteps failed (the tape is recording)")
+
     def tape_begin(self, max_steps: int):
         """Record the following steps for step_backward (checkpointed)."""
         if not self.L.dBatchTapeBegin(self.h, max_steps):

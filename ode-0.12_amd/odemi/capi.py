@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchTapeEnd": (C.c_int, [V]),
         "dBatchTapeSetOptions": (C.c_int, [V, C.c_int, C.c_float, C.c_float, C.c_int]),
         "dBatchTapeStats": (C.c_int, [V, C.POINTER(C.c_longlong)]),
+        "dBatchTapeKeepSteps": (C.c_int, [V, C.c_int]),
         "dBatchStepBackward": (C.c_int, [V, C.c_int, _fp, _fp, _fp, _fp]),
         "odemiVersion": (C.c_char_p, []),
         "odemiDeviceCount": (C.c_int, []),

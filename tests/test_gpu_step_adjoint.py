@@ -20,11 +20,12 @@ def _scenes():
     return [contact_scene(4, 4, (0,), bounce=0.5), contact_scene(5, 4, (1,)), contact_scene(6, 6, (0, 1))]
 
 
-def _gpu_grad(odemi_mod, scs, interval=10, warm=0.0):
+def _gpu_grad(odemi_mod, scs, interval=10, warm=0.0, keep=False):
     b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
     b.set_tolerance(1e-6, 4000, 10)
     b.set_warm_start(warm)
     b.tape_options(interval)
+    b.tape_keep_steps(keep)
     b.tape_begin(N)
     b.step(N)
     st = b.state().astype(np.float64)
@@ -81,6 +82,21 @@ def test_checkpoint_interval_does_not_change_gradient(odemi_mod, warm):
     scale = float(np.abs(a[1]).max())
     assert np.abs(a[1] - b[1]).max() <= 1e-5 * scale
     assert np.allclose(a[2], b[2], rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_kept_steps_give_the_replayed_gradient(odemi_mod):
+    """With every step kept by the forward pass the backward pass runs no
+    replay; the records are those a replay writes, so the gradient is the
+    checkpointed one."""
+    scs = _scenes()
+    say("adjoint: kept steps vs checkpoint interval 10")
+    a = _gpu_grad(odemi_mod, scs, interval=10)
+    b = _gpu_grad(odemi_mod, scs, interval=10, keep=True)
+    assert b[4]["collider_mismatches"] == 0
+    scale = float(np.abs(a[1]).max())
+    assert np.abs(a[1] - b[1]).max() <= 1e-5 * scale
+    assert np.allclose(a[2], b[2], rtol=1e-5, atol=1e-6 * scale)
+    say("kept vs replayed: max |d grad| / max |grad|", float(np.abs(a[1] - b[1]).max()) / scale)
 
 
 def test_torch_autograd_function(odemi_mod):
