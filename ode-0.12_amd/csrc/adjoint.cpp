@@ -55,7 +55,8 @@ int tape_begin(BatchImpl *b, int max_steps) {
   Tape &t = b->tape;
   t.ckpt.clear();
   t.nsteps = 0;
-  t.max_steps = max_steps;
+  t.max_steps = 0;
+  t.kept = false;
   if (t.keep) {
     const int nb = b->nb, nw = b->nworlds;
     std::vector<long long> con_off(nw);
@@ -78,6 +79,10 @@ int tape_begin(BatchImpl *b, int max_steps) {
   }
   t.ckpt.emplace_back(new Snapshot());
   if (!snapshot(b, *t.ckpt.back(), true)) return 0;
+  // the mode is latched only once every buffer it needs exists; the backward
+  // pass follows the latched mode, whatever dBatchTapeKeepSteps says later
+  t.kept = t.keep;
+  t.max_steps = max_steps;
   t.recording = true;
   destroy_graph(b);
   return 1;
@@ -88,11 +93,22 @@ int tape_keep_steps(BatchImpl *b, bool on) {
     report_error("dBatchTapeKeepSteps: set before dBatchTapeBegin");
     return 0;
   }
-  b->tape.keep = on;
+  Tape &t = b->tape;
+  t.keep = on;
   if (!on) {
-    b->tape.kst.release();
-    b->tape.kcon.release();
-    b->tape.knk.release();
+    // the buffers go with the tape recorded in keep mode: that tape can no
+    // longer be differentiated
+    t.kst.release();
+    t.kcon.release();
+    t.knk.release();
+    t.kovf.release();
+    t.kcon_off.release();
+    if (t.kept) {
+      t.kept = false;
+      t.nsteps = 0;
+      t.max_steps = 0;
+      t.ckpt.clear();
+    }
   }
   return 1;
 }
@@ -110,7 +126,7 @@ int tape_step(BatchImpl *b, bool sync) {
     report_error("dBatchStep: the tape holds at most %d steps", t.max_steps);
     return 0;
   }
-  if (t.keep) {
+  if (t.kept) {
     // this step's pre-state, then (after the step) its contacts
     const int nb = b->nb, nw = b->nworlds;
     const size_t stbytes = sizeof(float4) * (size_t)nb;
@@ -196,7 +212,7 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
   if (!snapshot(b, endsnap, true)) return 0;
   const int T = t.nsteps, first = T - nsteps;
   bool fail = false;
-  if (t.keep) {
+  if (t.kept) {
     // the forward pass kept every step: reverse sweep only
     int ovf = 0;
     OD_HIP(hipMemcpy(&ovf, t.kovf.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -218,7 +234,7 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
   }
   int seg0 = ((T - 1) / K) * K;
   const int seg_first = (first / K) * K;
-  for (; !t.keep && seg0 >= seg_first && !fail; seg0 -= K) {
+  for (; !t.kept && seg0 >= seg_first && !fail; seg0 -= K) {
     const int seg1 = std::min(seg0 + K, T);
     if (!snapshot(b, *t.ckpt[seg0 / K], false)) return 0;
     // replay the segment, recording each step's pre-state and contacts

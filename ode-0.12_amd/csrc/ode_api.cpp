@@ -114,6 +114,10 @@ struct CollideCtx {
 
 int g_init = 0;
 
+// every live space, so that a destroyed world can be unlinked from the spaces
+// that cache it
+std::vector<dxSpace *> g_spaces;
+
 void body_rot_from_q(dxBody *b) { h_quat_to_rot(b->q, b->R); }
 
 void mark_state(dxWorld *w) { if (w) w->state_dirty = true; }
@@ -133,6 +137,24 @@ void geom_moved(dxGeom *g) {
 void body_moved(dxBody *b) {
   for (dxGeom *g : b->geoms) geom_moved(g);
   mark_state(b->world);
+}
+
+int pull_state(dxWorld *w);
+
+// dGeomSetBody(g, 0) (collision_kernel.cpp:530-547): the geom keeps the
+// body's last pose as its own and leaves the body's geom list; it is not
+// marked as moved, since its effective pose does not change
+void geom_detach_body(dxGeom *g) {
+  dxBody *b = g->body;
+  if (!b) return;
+  pull_state(b->world);
+  std::memcpy(g->pos, b->pos, sizeof(g->pos));
+  std::memcpy(g->R, b->R, sizeof(g->R));
+  auto &L = b->geoms;
+  auto it = std::find(L.begin(), L.end(), g);
+  if (it != L.end()) L.erase(it);
+  mark_struct(b->world);
+  g->body = nullptr;
 }
 
 // device state -> host copies
@@ -369,15 +391,27 @@ void dCloseODE(void) { if (g_init > 0) g_init--; }
 
 // ---------------------------------------------------------------- world
 dWorldID dWorldCreate(void) { return new dxWorld(); }
+// dWorldDestroy (ode.cpp:1585-1612): every body is destroyed as dBodyDestroy
+// does (its geoms keep the body's last pose), grouped joints are only
+// deactivated (their group still owns them), and spaces that cached this
+// world forget it.
 void dWorldDestroy(dWorldID w) {
   if (!w) return;
+  pull_state(w);
+  for (dxBody *b : w->bodies) {
+    std::vector<dxGeom *> gs = b->geoms;
+    for (dxGeom *g : gs) geom_detach_body(g);
+  }
   if (w->mir) batch_destroy(w->mir);
+  w->mir = nullptr;
   for (dxJoint *j : w->joints) {
     j->world = nullptr;
     detach(j);
     if (!j->group) delete j;
   }
   for (dxBody *b : w->bodies) delete b;
+  for (dxSpace *s : g_spaces)
+    if (s->world == w) s->world = nullptr;
   delete w;
 }
 void dWorldSetGravity(dWorldID w, dReal x, dReal y, dReal z) { w->gravity[0] = x; w->gravity[1] = y; w->gravity[2] = z; }
@@ -515,7 +549,9 @@ dBodyID dBodyCreate(dWorldID w) {
 void dBodyDestroy(dBodyID b) {
   dxWorld *w = b->world;
   pull_state(w);
-  for (dxGeom *g : b->geoms) g->body = nullptr;
+  // dGeomSetBody(geom, 0) for every geom of the body (ode.cpp:296-309)
+  std::vector<dxGeom *> gs = b->geoms;
+  for (dxGeom *g : gs) geom_detach_body(g);
   std::vector<dxJoint *> js = b->jlist;
   for (dxJoint *j : js) detach(j);
   w->bodies.erase(std::find(w->bodies.begin(), w->bodies.end(), b));
@@ -807,9 +843,14 @@ int dAreConnectedExcluding(dBodyID a, dBodyID b, int joint_type) {
 dSpaceID dSimpleSpaceCreate(dSpaceID) {
   dxSpace *s = new dxSpace();
   s->cls = dSimpleSpaceClass;
+  g_spaces.push_back(s);
   return s;
 }
-dSpaceID dHashSpaceCreate(dSpaceID) { return new dxSpace(); }
+dSpaceID dHashSpaceCreate(dSpaceID) {
+  dxSpace *s = new dxSpace();
+  g_spaces.push_back(s);
+  return s;
+}
 void dHashSpaceSetLevels(dSpaceID s, int lo, int hi) {
   s->minlevel = lo;
   s->maxlevel = hi;
@@ -820,6 +861,8 @@ void dSpaceDestroy(dSpaceID s) {
   for (dxGeom *g : gl) dGeomDestroy(g);
   for (dxWorld *w : std::vector<dxWorld *>{s->world})
     if (w && w->mir_space == s) { w->mir_space = nullptr; mark_struct(w); }
+  auto it = std::find(g_spaces.begin(), g_spaces.end(), s);
+  if (it != g_spaces.end()) g_spaces.erase(it);
   delete s;
 }
 void dSpaceAdd(dSpaceID s, dGeomID g) {
@@ -897,13 +940,7 @@ void dGeomSetBody(dGeomID g, dBodyID b) {  // collision_kernel.cpp:507-556
     if (g->space && !g->space->world) g->space->world = b->world;
     mark_struct(b->world);
   } else if (g->body) {
-    pull_state(g->body->world);
-    std::memcpy(g->pos, g->body->pos, sizeof(g->pos));
-    std::memcpy(g->R, g->body->R, sizeof(g->R));
-    auto &L = g->body->geoms;
-    L.erase(std::find(L.begin(), L.end(), g));
-    mark_struct(g->body->world);
-    g->body = nullptr;
+    geom_detach_body(g);
   }
 }
 dBodyID dGeomGetBody(dGeomID g) { return g->body; }

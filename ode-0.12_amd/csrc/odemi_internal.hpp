@@ -30,7 +30,8 @@ struct Tape {
   // keep mode: every recorded step's pre-state (x, q, v, w) and contacts,
   // written during the forward pass, so that the backward pass needs no
   // replay (memory per step: 64 B per body plus the contact capacity)
-  bool keep = false;
+  bool keep = false;                             // requested for the next dBatchTapeBegin
+  bool kept = false;                             // mode the current tape was recorded in
   DevBuf kst, kcon, knk, kovf, kcon_off;
   long long kncon = 0;
 };

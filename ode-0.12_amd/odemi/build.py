@@ -41,6 +41,33 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_asan(out_dir: str) -> str:
+    """Host-side AddressSanitizer build of the same sources (device code
+    unchanged; -fsanitize applies to the host pass only), linked into
+    out_dir/libodemi_asan.so for the CPU checks of the API object model."""
+    os.makedirs(out_dir, exist_ok=True)
+
+    def one(src):
+        obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, "-O1", "-g", "-std=c++17", "-fPIC", f"-I{CSRC}", f"-I{INCLUDE}", "-Wno-unused-result",
+               *DEVICE, "-Xarch_host", "-fsanitize=address", "-c", os.path.join(CSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return obj
+
+    srcs = _sources()
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(one, srcs))
+    lib = os.path.join(out_dir, "libodemi_asan.so")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
+           "-shared-libsan", "-o", lib, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return lib
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = _sources()

@@ -130,3 +130,24 @@ def test_api_assertions_follow_the_reference(root, capfd):
     assert "geom must be placeable" in err and "body1==body2" in err, err
     L.dGeomDestroy(p)
     L.dWorldDestroy(w)
+
+
+def test_api_teardown_under_address_sanitizer(root, tmp_path):
+    """The API object model under AddressSanitizer (host-side build of the
+    library, no device work): dBodyDestroy leaves the body's geoms at its last
+    pose (ode.cpp:296-309), and dWorldDestroy before dJointGroupDestroy and
+    dSpaceDestroy (demos/raycar/ode.cpp:26-29) touches no freed memory
+    (tests/capi/teardown.c)."""
+    import glob
+    asan_dir = os.path.join(root, "tests", "_build", "asan")
+    lib = os.path.join(asan_dir, "libodemi_asan.so")
+    assert os.path.exists(lib), "build first (__graft_entry__.build())"
+    rt = glob.glob("/opt/rocm*/lib/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so")
+    assert rt, "clang AddressSanitizer runtime not found"
+    exe = str(tmp_path / "teardown")
+    subprocess.run(["/opt/rocm/llvm/bin/clang", "-g", "-fsanitize=address", "-shared-libsan",
+                    "-I", os.path.join(root, "include"), os.path.join(root, "tests", "capi", "teardown.c"),
+                    "-o", exe, lib, f"-Wl,-rpath,{asan_dir}", f"-Wl,-rpath,{os.path.dirname(rt[0])}"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr[-3000:]
