@@ -349,11 +349,16 @@ ODE_API int dBatchGetState(dBatchID b, float *out);
 ODE_API int dBatchSetState(dBatchID b, const float *in);
 /* Totals of the last step: [pairs, contacts, rows, islands_or_colours, overflow]. */
 ODE_API int dBatchGetCounters(dBatchID b, long long *out5);
-/* Counters of the last step for every world, 6 ints per world: pairs,
+/* Counters for every world, 8 ints per world.  Of the last step: pairs,
  * contacts, rows, islands or colours, overflow bits (1 grid cells, 2 pairs,
  * 4 contacts, 8 rows, 16 non-finite state: the world is frozen), SOR
- * iterations run. */
+ * iterations run.  Since dBatchResetSolveStats (or dBatchCreate): the number
+ * of iterate-to-tolerance solves that ended at the iteration cap with the
+ * residual above the tolerance, and the largest final residual of such a
+ * solve (the bits of a float). */
 ODE_API int dBatchGetWorldCounters(dBatchID b, int *out);
+/* Zeroes the two accumulating counters of dBatchGetWorldCounters. */
+ODE_API int dBatchResetSolveStats(dBatchID b);
 /* Coloured order only: iterate each world's SOR until its LCP residual is
  * at most tol (checked every check_every iterations, at most max_iterations);
  * tol = 0 restores the fixed iteration count of the world.  The reference

@@ -964,6 +964,16 @@ int dBatchGetWorldCounters(dBatchID b, int *out) {
   return hipMemcpy(out, i->d.counters.p, sizeof(int) * CNT_N * (size_t)i->nworlds, hipMemcpyDeviceToHost) ==
          hipSuccess;
 }
+int dBatchResetSolveStats(dBatchID b) {
+  if (!b) return 0;
+  BatchImpl *i = b->impl;
+  if (hipStreamSynchronize(i->stream) != hipSuccess) return 0;
+  std::vector<int> c((size_t)CNT_N * i->nworlds);
+  OD_HIP(hipMemcpy(c.data(), i->d.counters.p, sizeof(int) * c.size(), hipMemcpyDeviceToHost));
+  for (int w = 0; w < i->nworlds; w++) c[(size_t)CNT_N * w + CNT_UNCONV] = c[(size_t)CNT_N * w + CNT_RESMAX] = 0;
+  OD_HIP(hipMemcpy(i->d.counters.p, c.data(), sizeof(int) * c.size(), hipMemcpyHostToDevice));
+  return 1;
+}
 int dBatchSetPersistentGrid(dBatchID b, int grid) {
   if (!b || grid < 0) return 0;
   b->impl->grid = grid;

@@ -32,7 +32,12 @@ enum {
   SM_APPROX1_1 = 0x1000, SM_APPROX1_2 = 0x2000
 };
 // per-world counters written by the step kernels
-enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_ITERS, CNT_N };
+// per-world counters: the first six describe the last step; CNT_UNCONV and
+// CNT_RESMAX accumulate over the steps since dBatchResetSolveStats (or the
+// batch's creation): tolerance-mode solves that ended at the iteration cap
+// above the tolerance, and the largest final residual of a tolerance-mode
+// solve (float bits)
+enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_ITERS, CNT_UNCONV, CNT_RESMAX, CNT_N };
 enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8, OVF_NONFINITE = 16 };
 
 static const int MAX_SLOTS = 8;   // most contacts a collider produces for one pair

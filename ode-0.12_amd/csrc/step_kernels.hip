@@ -1344,6 +1344,13 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   if (threadIdx.x == 0) {
     B.counters[c.w * CNT_N + CNT_ITERS] = it;
     c.at<int>(c.L.misc)[MISC_SOR_RESIDUAL] = __float_as_int(use_tol ? res_last : 0.f);
+    if (use_tol) {
+      int *cw = B.counters + c.w * CNT_N;
+      if (!(res_last <= P.tol)) cw[CNT_UNCONV] += 1;
+      // non-negative floats order like their bit patterns
+      const int rb = __float_as_int(res_last == res_last ? res_last : INFINITY);
+      if (rb > cw[CNT_RESMAX]) cw[CNT_RESMAX] = rb;
+    }
   }
   if (LDS) {
     for (int b = threadIdx.x; b < nb; b += nt)

@@ -108,12 +108,25 @@ class Batch:
         return dict(pairs=c[0], contacts=c[1], rows=c[2], groups=c[3], overflow=c[4])
 
     def world_counters(self) -> np.ndarray:
-        """(nworlds, 6) int32: pairs, contacts, rows, islands/colours, overflow
-        bits, SOR iterations."""
-        out = np.zeros((self.nworlds, 6), np.int32)
+        """(nworlds, 8) int32: of the last step pairs, contacts, rows,
+        islands/colours, overflow bits, SOR iterations; since the last
+        reset_solve_stats() the tolerance-mode solves that ended at the cap
+        above the tolerance and the largest final residual (float bits; see
+        solve_stats())."""
+        out = np.zeros((self.nworlds, 8), np.int32)
         if not self.L.dBatchGetWorldCounters(self.h, out.ctypes.data_as(C.POINTER(C.c_int))):
             raise RuntimeError("dBatchGetWorldCounters failed")
         return out
+
+    def reset_solve_stats(self):
+        if not self.L.dBatchResetSolveStats(self.h):
+            raise RuntimeError("dBatchResetSolveStats failed")
+
+    def solve_stats(self):
+        """Per world since the last reset: (unconverged solves, largest final
+        residual of an iterate-to-tolerance solve)."""
+        wc = self.world_counters()
+        return wc[:, 6].copy(), wc[:, 7].copy().view(np.float32)
 
     def set_tolerance(self, tol: float, max_iterations: int = 400, check_every: int = 10, accel: bool = False):
         """Coloured order: SOR runs until the LCP residual <= tol (0: fixed

@@ -61,6 +61,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchSetLaunchConfig": (C.c_int, [V, C.c_int, C.c_float]),
         "dBatchGetLds": (C.c_int, [V]),
         "dBatchGetWorldCounters": (C.c_int, [V, _ip]),
+        "dBatchResetSolveStats": (C.c_int, [V]),
         "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
         "dBatchSetSolverToleranceAccel": (C.c_int, [V, C.c_float, C.c_int, C.c_int, C.c_int]),
         "dBatchSetWarmStart": (C.c_int, [V, C.c_float]),

@@ -270,8 +270,10 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     w.quickstep((R)s->dt);
     w.empty_contacts();
     auto t2 = clk::now();
-    o->sec_collide += std::chrono::duration<double>(t1 - t0).count();
-    o->sec_step += std::chrono::duration<double>(t2 - t1).count();
+    if (step >= o->time_from) {
+      o->sec_collide += std::chrono::duration<double>(t1 - t0).count();
+      o->sec_step += std::chrono::duration<double>(t2 - t1).count();
+    }
     if (diag) {
       if (o->pair_step_count) o->pair_step_count[step] = (int)pairs.size();
       if (o->contact_step_count) o->contact_step_count[step] = step_contacts;

@@ -47,7 +47,7 @@ class SceneOut(C.Structure):
         ("diag_steps", C.c_int), ("max_pairs", C.c_int), ("pairs", _ip), ("pair_step_count", _ip),
         ("max_contacts", C.c_int), ("contacts", _dp), ("contact_step_count", _ip), ("feedback", _dp),
         ("total_pairs", C.c_longlong), ("total_contacts", C.c_longlong),
-        ("sec_collide", C.c_double), ("sec_step", C.c_double), ("status", C.c_int),
+        ("sec_collide", C.c_double), ("sec_step", C.c_double), ("status", C.c_int), ("time_from", C.c_int),
     ]
 
 
@@ -120,11 +120,13 @@ def ref_lib(precision: str = "double"):
 
 
 def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pairs: int = 0,
-        max_contacts: int = 0, feedback: bool = False) -> dict:
-    """Run `nsteps` of a Scene through `lib.scene_run`; returns numpy arrays."""
+        max_contacts: int = 0, feedback: bool = False, time_from: int = 0) -> dict:
+    """Run `nsteps` of a Scene through `lib.scene_run`; returns numpy arrays.
+    sec_collide / sec_step time the steps from `time_from` on."""
     desc, keep = make_desc(sc)
     out = SceneOut()
     out.nsteps = nsteps
+    out.time_from = time_from
     out.record_every = record_every
     nrec = (nsteps // record_every + 1) if record_every > 0 else 1
     traj = np.zeros((nrec, sc.nb, 13))

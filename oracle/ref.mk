@@ -35,7 +35,12 @@ objs = $(patsubst $(REF)/%.cpp,$(OUT)/obj_$(1)/%.o,$(ODE_CPP)) \
 OBJ_D := $(call objs,double)
 OBJ_S := $(call objs,single)
 
-all: $(OUT)/libode_ref_double.so $(OUT)/libode_ref_single.so
+all: $(OUT)/libode_ref_double.so $(OUT)/libode_ref_single.so $(OUT)/buildinfo.txt
+
+# compiler and flags of the reference build, reported with the CPU baseline
+$(OUT)/buildinfo.txt: oracle/ref.mk
+	@mkdir -p $(OUT)
+	@echo "$$($(CXX) --version | head -1); flags $(OPT) -fPIC -DdNODEBUG -DdSINGLE|-DdDOUBLE" > $@
 
 $(OUT)/libode_ref_double.so: $(OBJ_D)
 	$(CXX) -shared -Wl,-Bsymbolic -o $@ $^

@@ -87,6 +87,7 @@ typedef struct SceneOut {
   long long total_pairs, total_contacts;
   double sec_collide, sec_step;  /* wall time in dSpaceCollide / QuickStep+Empty */
   int status;                    /* 0 ok, else failing step + 1 */
+  int time_from;                 /* steps before this one are not timed */
 } SceneOut;
 
 #ifdef __cplusplus

@@ -268,8 +268,10 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     }
     dJointGroupEmpty(cgroup);
     auto t2 = clk::now();
-    o->sec_collide += std::chrono::duration<double>(t1 - t0).count();
-    o->sec_step += std::chrono::duration<double>(t2 - t1).count();
+    if (step >= o->time_from) {
+      o->sec_collide += std::chrono::duration<double>(t1 - t0).count();
+      o->sec_step += std::chrono::duration<double>(t2 - t1).count();
+    }
     if (!ok && !o->status) o->status = step + 1;
     if (c.diag) {
       if (o->pair_step_count) o->pair_step_count[step] = c.step_pairs;
