@@ -258,6 +258,15 @@ ODE_API void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback);
  * refine nothing, so it changes no result */
 #define CONTACTS_UNIMPORTANT 0x80000000
 ODE_API int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip); /* collision.h:795 */
+/* No counterpart in the reference.  dCollide for callers that cannot
+ * allocate device memory (code inside a serving loop or a harness that
+ * forbids allocation): the same contacts as dCollide, computed in a
+ * library-owned scratch array of one pair's contacts per GPU (set up by
+ * dInitODE2 or on the GPU's first call), on the calling thread's per-thread
+ * stream.  At most 8 contacts may be requested (flags & 0xffff <= 8, the most
+ * any collider here returns); a larger request is refused with an error and
+ * returns 0.  Calls that overlap on one GPU share the array and race. */
+ODE_API int dCollideNoAlloc(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip);
 ODE_API dGeomID dCreateSphere(dSpaceID space, dReal radius);                  /* collision.h:924 */
 ODE_API dGeomID dCreateBox(dSpaceID space, dReal lx, dReal ly, dReal lz);     /* collision.h:1002 */
 ODE_API dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d); /* collision.h:1046 */

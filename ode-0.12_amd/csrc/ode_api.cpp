@@ -22,6 +22,9 @@ namespace odemi {
 unsigned long &global_seed();
 void report_debug(const char *fmt, ...);
 hipError_t launch_collide_one(const GeomView &a, const GeomView &b, int maxc, CPoint *out, int *n, hipStream_t st);
+void *noalloc_scratch(hipError_t *err);
+hipError_t collide_one_noalloc(void *addr, const GeomView &a, const GeomView &b, int maxc, CPoint *host_pts,
+                               int *host_n, hipStream_t st);
 }  // namespace odemi
 
 using namespace odemi;
@@ -385,8 +388,15 @@ void write_contact(dContactGeom *c, const float *pos, const float *n, float dept
 extern "C" {
 
 // ---------------------------------------------------------------- init
-int dInitODE2(unsigned int) { g_init++; return 1; }
-void dInitODE(void) { g_init++; }
+// library set-up: the device address of dCollideNoAlloc's scratch array on
+// the current GPU, when there is one (looked up again on a device's first
+// dCollideNoAlloc call otherwise)
+static void init_device_state() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) == hipSuccess && n > 0) (void)noalloc_scratch(nullptr);
+}
+int dInitODE2(unsigned int) { g_init++; init_device_state(); return 1; }
+void dInitODE(void) { g_init++; init_device_state(); }
 void dCloseODE(void) { if (g_init > 0) g_init--; }
 
 // ---------------------------------------------------------------- world
@@ -1063,10 +1073,28 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
 }
 
 // dCollide (collision_kernel.cpp:292-339)
-int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip) {
-  int maxc = flags & 0xffff;
-  if (maxc <= 0 || o1 == o2) return 0;
-  if (o1->body && o1->body == o2->body) return 0;
+// pose and shape of a geom for the device colliders, from the host copies
+static GeomView geom_view(dxGeom *g) {
+  GeomView v;
+  v.type = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
+  const float *p = g->body ? g->body->pos : g->pos;
+  const float *R = g->body ? g->body->R : g->R;
+  v.p = make_float3(p[0], p[1], p[2]);
+  v.R.r0 = make_float3(R[0], R[1], R[2]);
+  v.R.r1 = make_float3(R[4], R[5], R[6]);
+  v.R.r2 = make_float3(R[8], R[9], R[10]);
+  v.prm = make_float4(g->param[0], g->param[1], g->param[2], g->param[3]);
+  return v;
+}
+
+// The checks and the cached contacts of dCollide (collision_kernel.cpp:
+// 292-339): returns 1 with *n set when the call is answered without the
+// device (no contacts requested, the same geom or body, or the pair the
+// current dSpaceCollide callback was called for), else 0.
+static int collide_host_answer(dGeomID o1, dGeomID o2, int maxc, dContactGeom *contact, int skip, int *n) {
+  *n = 0;
+  if (maxc <= 0 || o1 == o2) return 1;
+  if (o1->body && o1->body == o2->body) return 1;
   if (g_ctx.active && g_ctx.g1 == o1 && g_ctx.g2 == o2 && g_ctx.space) {
     int want = maxc > MAX_SLOTS ? MAX_SLOTS : maxc;
     int have = g_ctx.maxc > MAX_SLOTS ? MAX_SLOTS : g_ctx.maxc;
@@ -1076,41 +1104,69 @@ int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip)
         const ContactRec &c = g_ctx.con[i];
         write_contact((dContactGeom *)((char *)contact + (size_t)skip * i), c.pos, c.normal, c.depth, o1, o2);
       }
-      return g_ctx.n;
+      *n = g_ctx.n;
+      return 1;
     }
   }
-  // one pair on the device with the current poses
-  dxWorld *w = o1->body ? o1->body->world : (o2->body ? o2->body->world : nullptr);
-  if (w) pull_state(w);
-  auto view = [](dxGeom *g) {
-    GeomView v;
-    v.type = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
-    const float *p = g->body ? g->body->pos : g->pos;
-    const float *R = g->body ? g->body->R : g->R;
-    v.p = make_float3(p[0], p[1], p[2]);
-    v.R.r0 = make_float3(R[0], R[1], R[2]);
-    v.R.r1 = make_float3(R[4], R[5], R[6]);
-    v.R.r2 = make_float3(R[8], R[9], R[10]);
-    v.prm = make_float4(g->param[0], g->param[1], g->param[2], g->param[3]);
-    return v;
-  };
-  static DevBuf out, nout;
-  static hipStream_t st = nullptr;
-  if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return 0;
-  if (!out.alloc(sizeof(CPoint) * MAX_SLOTS) || !nout.alloc(16)) return 0;
-  if (launch_collide_one(view(o1), view(o2), maxc, out.as<CPoint>(), nout.as<int>(), st) != hipSuccess) return 0;
-  CPoint pts[MAX_SLOTS];
-  int n = 0;
-  if (hipMemcpyAsync(&n, nout.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(pts, out.p, sizeof(pts), hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return 0;
+  return 0;
+}
+
+static int write_pair_contacts(dGeomID o1, dGeomID o2, int maxc, const CPoint *pts, int n, dContactGeom *contact,
+                               int skip) {
   if (n > maxc) n = maxc;
   for (int i = 0; i < n; i++) {
     float p[3] = {pts[i].pos.x, pts[i].pos.y, pts[i].pos.z}, nn[3] = {pts[i].n.x, pts[i].n.y, pts[i].n.z};
     write_contact((dContactGeom *)((char *)contact + (size_t)skip * i), p, nn, pts[i].depth, o1, o2);
   }
   return n;
+}
+
+int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip) {
+  int maxc = flags & 0xffff, n = 0;
+  if (collide_host_answer(o1, o2, maxc, contact, skip, &n)) return n;
+  // one pair on the device with the current poses
+  dxWorld *w = o1->body ? o1->body->world : (o2->body ? o2->body->world : nullptr);
+  if (w) pull_state(w);
+  static DevBuf out, nout;
+  static hipStream_t st = nullptr;
+  if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return 0;
+  if (!out.alloc(sizeof(CPoint) * MAX_SLOTS) || !nout.alloc(16)) return 0;
+  if (launch_collide_one(geom_view(o1), geom_view(o2), maxc, out.as<CPoint>(), nout.as<int>(), st) != hipSuccess)
+    return 0;
+  CPoint pts[MAX_SLOTS];
+  if (hipMemcpyAsync(&n, nout.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(pts, out.p, sizeof(pts), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return 0;
+  return write_pair_contacts(o1, o2, maxc, pts, n, contact, skip);
+}
+
+// dCollide without device allocation: the one-pair collide runs in the
+// library's per-GPU scratch array (set up by dInitODE2 or on the GPU's first
+// call) on the calling thread's per-thread stream.  At most MAX_SLOTS (8)
+// contacts may be requested, the most any collider of this library returns.
+int dCollideNoAlloc(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip) {
+  int maxc = flags & 0xffff, n = 0;
+  if (maxc > MAX_SLOTS) {
+    report_error("dCollideNoAlloc: %d contacts requested; at most %d per call", maxc, MAX_SLOTS);
+    return 0;
+  }
+  if (collide_host_answer(o1, o2, maxc, contact, skip, &n)) return n;
+  dxWorld *w = o1->body ? o1->body->world : (o2->body ? o2->body->world : nullptr);
+  if (w) pull_state(w);
+  hipError_t err = hipSuccess;
+  void *addr = noalloc_scratch(&err);
+  if (!addr) {
+    report_error("dCollideNoAlloc: the scratch array is not available on this device (%s)", hipGetErrorString(err));
+    return 0;
+  }
+  CPoint pts[MAX_SLOTS];
+  err = collide_one_noalloc(addr, geom_view(o1), geom_view(o2), maxc, pts, &n, hipStreamPerThread);
+  if (err != hipSuccess) {
+    report_error("dCollideNoAlloc: %s", hipGetErrorString(err));
+    return 0;
+  }
+  return write_pair_contacts(o1, o2, maxc, pts, n, contact, skip);
 }
 
 // ---------------------------------------------------------------- adjoint (ODE API form)
