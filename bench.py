@@ -178,6 +178,59 @@ def adjoint_run(odemi, scenes, np, steps, interval, keep=False):
             "worlds_with_finite_gradients": int(fin_w.sum()), "frozen_worlds": int(((wc[:, 4] & 16) != 0).sum())}
 
 
+def _oracle_traj(job):
+    """FP64 (or FP32) CPU restatement of the reference order over one world:
+    positions every `every` steps."""
+    sc, nsteps, every, prec = job
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    return po.run(po.oracle_lib(prec), sc, nsteps, every)["traj"][:, :, :3]
+
+
+def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
+    """Position divergence over `nsteps` steps on a sample of worlds, sampled
+    every 10 steps (max over bodies and samples), between: the GPU reference
+    order and the FP64 build of the reference's arithmetic (the CPU
+    restatement, bit-exact to the reference double build); the reference's
+    own FP32 build and FP64 build (its FP32 envelope); the GPU coloured order
+    and the GPU reference order (both at the quickstep's iteration count);
+    the headline path (coloured, iterated to the tolerance) and the FP64
+    build."""
+    import multiprocessing as mp
+    every = 10
+    has_ctrl = worlds[0].step_ctrl is not None
+    ctrl = np.concatenate([w.step_ctrl[:nsteps] for w in worlds], axis=1) if has_ctrl else None
+
+    def gpu(order, tolerance):
+        b = odemi.Batch(worlds, order)
+        if tolerance:
+            b.set_tolerance(tol, max_iters, 10, accel=True)
+            b.set_warm_start(1.0)
+        out = [b.state()[:, :3].astype(np.float64)]
+        for k in range(nsteps):
+            if has_ctrl:
+                b.set_controls(ctrl[k])
+            b.step(1)
+            if (k + 1) % every == 0:
+                out.append(b.state()[:, :3].astype(np.float64))
+        b.close()
+        return np.stack(out)
+
+    ref = gpu(odemi.ORDER_REFERENCE, False)
+    col = gpu(odemi.ORDER_COLORED, False)
+    head = gpu(odemi.ORDER_COLORED, True)
+    with mp.get_context("spawn").Pool(procs) as pool:
+        f64 = pool.map(_oracle_traj, [(w, nsteps, every, "f64") for w in worlds])
+        f32 = pool.map(_oracle_traj, [(w, nsteps, every, "f32") for w in worlds])
+    f64 = np.concatenate(f64, axis=1)
+    f32 = np.concatenate(f32, axis=1)
+    d = lambda a, b: float(np.abs(a - b).max())
+    return {"sample": f"{label}: {len(worlds)} worlds ({sum(w.nb for w in worlds)} bodies), {nsteps} steps, "
+                      f"positions every {every} steps",
+            "reference_order_vs_fp64_m": d(ref, f64), "reference_fp32_vs_fp64_m": d(f32, f64),
+            "coloured_vs_reference_order_m": d(col, ref), "headline_vs_fp64_m": d(head, f64)}
+
+
 def config_run(args):
     """The single 4096-body pile (500-step config: seed 12345, h=0.005) or the
     8192 vehicle worlds (per-step steer stops and wheel torques applied before
@@ -290,6 +343,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0,
                     help="host processes of the CPU baseline (0: the host cores available, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--divergence-worlds", type=int, default=64,
+                    help="uneven worlds: worlds of the trajectory-divergence sample (C3 and vehicles; 0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device copy of this many bytes first (calibrates HBM byte counters)")
     ap.add_argument("--launch", default=None,
@@ -465,6 +520,17 @@ def main():
         adjoint = adjoint_run(odemi, scenes, np, args.adjoint_steps, 10)
         log(f"adjoint: {adjoint}")
 
+    divergence = None
+    if args.divergence_worlds > 0:
+        nd = args.divergence_worlds
+        procs = args.cpu_procs or host_cores()
+        divergence = {
+            "uneven_worlds": divergence_run(odemi, np, worlds[:nd], ROLL, args.tol, args.max_iters, procs,
+                                            "batched uneven worlds (the first of the workload)"),
+            "vehicles": divergence_run(odemi, np, scenes.vehicle_worlds(nd, ROLL + 1, seed=11), ROLL, args.tol,
+                                       args.max_iters, procs, "vehicle worlds (seed 11)")}
+        log(f"divergence: {divergence}")
+
     line = {
         "metric": "body-steps/sec batched vs plain GPU quickstep at LCP residual <=1e-4",
         "value": value,
@@ -509,6 +575,7 @@ def main():
                      "algorithmic_bytes_per_launch": B, "kernel_ms": kernel_ms},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "trajectory_divergence": divergence,
     }
     print(json.dumps(line), flush=True)
 

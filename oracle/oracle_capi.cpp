@@ -267,7 +267,9 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
       }
     }
     auto t1 = clk::now();
+    w.track_residual = o->residual != nullptr;
     w.quickstep((R)s->dt);
+    if (o->residual) o->residual[step] = (double)(g_colored ? w.last_res : w.step_res);
     w.empty_contacts();
     auto t2 = clk::now();
     if (step >= o->time_from) {

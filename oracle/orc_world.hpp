@@ -918,6 +918,9 @@ template <class R> struct World {
       if (jb[2 * i + 1] != -1) for (int k = 6; k < 12; k++) s += iMJ[12 * i + k] * J[12 * i + k];
       Ad[i] = prm.w / (s + cfm[i]);
     }
+    // unscaled rows and rhs, for the residual after the iterations
+    std::vector<R> J0, b0;
+    if (track_residual) { J0 = J; b0 = b; }
     for (int i = 0; i < m; i++) {
       R a = Ad[i];
       for (int k = 0; k < 12; k++) J[12 * i + k] *= a;
@@ -964,7 +967,30 @@ template <class R> struct World {
         if (f2) for (int k = 0; k < 6; k++) f2[k] += delta * M[6 + k];
       }
     }
+    if (track_residual) {
+      // max_i |lambda_i - clamp(lambda_i - (A lambda - b)_i / A_ii)| with the
+      // unscaled A = J M^-1 J^T + cfm/h and b (not part of the reference,
+      // which computes no residual)
+      for (int r = 0; r < m; r++) {
+        int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+        const R *Jr = &J0[12 * (size_t)r], *M = &iMJ[12 * (size_t)r];
+        R s = 0, d = 0;
+        for (int k = 0; k < 6; k++) { s += Jr[k] * fc[6 * (size_t)b1 + k]; d += Jr[k] * M[k]; }
+        if (b2 != -1)
+          for (int k = 0; k < 6; k++) { s += Jr[6 + k] * fc[6 * (size_t)b2 + k]; d += Jr[6 + k] * M[6 + k]; }
+        R y = s + cfm[r] * lambda[r] - b0[r];
+        R lo_a = lo[r], hi_a = hi[r];
+        if (findex[r] != -1) { hi_a = std::fabs(hi[r] * lambda[findex[r]]); lo_a = -hi_a; }
+        R dd = y / (d + cfm[r]), p = lambda[r] - dd, e;
+        if (p < lo_a) e = lambda[r] - lo_a;
+        else if (p > hi_a) e = hi_a - lambda[r];
+        else e = dd;
+        step_res = std::max(step_res, std::fabs(e));
+      }
+    }
   }
+  bool track_residual = false;   // reference order: record the residual of each solve
+  R step_res = 0;                // largest residual over this step's islands
 
   // Graph-coloured Gauss-Seidel over the island's constraints (the order of
   // the MI355X coloured mode, not a reference order): Jones-Plassmann
@@ -1155,6 +1181,7 @@ template <class R> struct World {
   // dWorldQuickStep: ode.cpp:1807-1825; islands util.cpp:384-568
   void quickstep(R h) {
     moved.clear();
+    step_res = 0;
     int nbod = (int)bodies.size();
     for (Body<R> &b : bodies) b.tag = 0;
     std::vector<char> jtag(joints.size(), 0);

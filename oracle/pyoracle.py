@@ -48,6 +48,7 @@ class SceneOut(C.Structure):
         ("max_contacts", C.c_int), ("contacts", _dp), ("contact_step_count", _ip), ("feedback", _dp),
         ("total_pairs", C.c_longlong), ("total_contacts", C.c_longlong),
         ("sec_collide", C.c_double), ("sec_step", C.c_double), ("status", C.c_int), ("time_from", C.c_int),
+        ("residual", _dp),
     ]
 
 
@@ -120,9 +121,10 @@ def ref_lib(precision: str = "double"):
 
 
 def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pairs: int = 0,
-        max_contacts: int = 0, feedback: bool = False, time_from: int = 0) -> dict:
+        max_contacts: int = 0, feedback: bool = False, time_from: int = 0, residual: bool = False) -> dict:
     """Run `nsteps` of a Scene through `lib.scene_run`; returns numpy arrays.
-    sec_collide / sec_step time the steps from `time_from` on."""
+    sec_collide / sec_step time the steps from `time_from` on; residual=True
+    (CPU restatement only) records every step's LCP residual."""
     desc, keep = make_desc(sc)
     out = SceneOut()
     out.nsteps = nsteps
@@ -144,6 +146,8 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
     out.contacts = contacts.ctypes.data_as(_dp)
     out.contact_step_count = csc.ctypes.data_as(_ip)
     out.feedback = fb.ctypes.data_as(_dp) if feedback else None
+    res = np.zeros(max(nsteps, 1))
+    out.residual = res.ctypes.data_as(_dp) if residual else None
     status = lib.scene_run(C.byref(desc), C.byref(out))
     del keep
     npairs = int(psc[:diag_steps].sum()) if diag_steps else 0
@@ -152,6 +156,7 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
                 pair_step_count=psc[:diag_steps].copy(), contacts=contacts[:min(ncont, max_contacts)],
                 contact_step_count=csc[:diag_steps].copy(), feedback=fb[:min(ncont, max_contacts)],
                 total_pairs=out.total_pairs, total_contacts=out.total_contacts,
+                residual=res[:nsteps].copy() if residual else None,
                 sec_collide=out.sec_collide, sec_step=out.sec_step)
 
 

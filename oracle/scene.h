@@ -88,6 +88,7 @@ typedef struct SceneOut {
   double sec_collide, sec_step;  /* wall time in dSpaceCollide / QuickStep+Empty */
   int status;                    /* 0 ok, else failing step + 1 */
   int time_from;                 /* steps before this one are not timed */
+  double *residual;              /* may be NULL: nsteps LCP residuals (CPU restatement only) */
 } SceneOut;
 
 #ifdef __cplusplus

@@ -326,6 +326,108 @@ def test_dropin_driver_runs_the_reference_frame_loop(odemi_mod):
         assert np.all(err <= 4 * env + 1e-3)
 
 
+def _tie_order(con, ref):
+    """Permutation of this step's GPU contacts onto the reference's: within a
+    pair, contacts whose depths tie come out in an order decided by the last
+    bit of the arithmetic, so they are matched by position."""
+    perm = np.arange(len(ref))
+    i, n = 0, len(ref)
+    while i < n:
+        j = i
+        while j < n and ref[j, 7] == ref[i, 7] and ref[j, 8] == ref[i, 8]:
+            j += 1
+        if j - i > 1:
+            used = set()
+            for a in range(i, j):
+                d = [np.abs(con[c, :3] - ref[a, :3]).max() if c not in used else np.inf for c in range(i, j)]
+                c = i + int(np.argmin(d))
+                used.add(c)
+                perm[a] = c
+        i = j
+    return perm
+
+
+@pytest.mark.parametrize("name", ["box_stack", "chains", "surfaces", "vehicle"])
+def test_dropin_joint_feedback_every_step(odemi_mod, name):
+    """dJointFeedback written back by dWorldQuickStep (quickstep.cpp:918-957)
+    for every contact joint of every diagnostic step of the drop-in run,
+    against the reference double build's: within 4x the distance of the
+    reference's own single build (its CPU restatement) from the double build
+    at the same step, plus 2e-3 of the step's largest force.  A step is
+    compared when its callback pairs and contact counts equal the
+    reference's (both runs then hold the same joints in the same order)."""
+    say("drop-in feedback", name)
+    from test_golden import CASES
+    fac, n, every, diag = CASES[name]
+    sc = fac()
+    g = golden(name)
+    lib = po.load(os.path.join(ROOT, "tests", "_build", "libscene_odemi.so"))
+    kw = dict(diag_steps=diag, max_pairs=400000, max_contacts=400000, feedback=True)
+    r = po.run(lib, sc, diag, 0, **kw)
+    r32 = po.run(po.oracle_lib("f32"), sc, diag, 0, **kw)
+    pc, cc = g["pair_step_count"], g["contact_step_count"]
+    p0 = np.concatenate([[0], np.cumsum(pc)])
+    k0 = np.concatenate([[0], np.cumsum(cc)])
+    rp0 = np.concatenate([[0], np.cumsum(r["pair_step_count"])])
+    rk0 = np.concatenate([[0], np.cumsum(r["contact_step_count"])])
+    sp0 = np.concatenate([[0], np.cumsum(r32["pair_step_count"])])
+    sk0 = np.concatenate([[0], np.cumsum(r32["contact_step_count"])])
+    compared, worst = 0, 0.0
+    for s in range(diag):
+        if cc[s] == 0:
+            continue
+        same = (r["pair_step_count"][s] == pc[s] and r["contact_step_count"][s] == cc[s] and
+                np.array_equal(r["pairs"][rp0[s]:rp0[s + 1]], g["pairs"][p0[s]:p0[s + 1]]))
+        if not same:
+            continue
+        ref = g["contacts"][k0[s]:k0[s + 1]]
+        fref = g["feedback"][k0[s]:k0[s + 1]]
+        perm = _tie_order(r["contacts"][rk0[s]:rk0[s + 1]], ref)
+        fgpu = r["feedback"][rk0[s]:rk0[s + 1]][perm]
+        scale = max(1.0, float(np.abs(fref).max()))
+        err = float(np.abs(fgpu - fref).max())
+        env = 0.0
+        if (r32["contact_step_count"][s] == cc[s] and
+                np.array_equal(r32["pairs"][sp0[s]:sp0[s + 1]], g["pairs"][p0[s]:p0[s + 1]])):
+            p32 = _tie_order(r32["contacts"][sk0[s]:sk0[s + 1]], ref)
+            env = float(np.abs(r32["feedback"][sk0[s]:sk0[s + 1]][p32] - fref).max())
+        tol = 4 * env + 2e-3 * scale
+        worst = max(worst, err / tol)
+        assert err <= tol, (name, s, err, env, scale)
+        compared += 1
+    with_contacts = int((cc > 0).sum())
+    say(name, "steps compared", compared, "of", with_contacts, "worst err/tol", worst)
+    assert compared >= 0.8 * with_contacts
+
+
+@pytest.mark.parametrize("name", ["box_stack", "chains", "pile256_dense", "surfaces"])
+def test_reference_order_residual_matches_restatement(odemi_mod, name):
+    """The LCP residual after each reference-order solve (20 iterations, the
+    reference's row order and shuffle) against the residual of the double
+    restatement's impulses, which equal the reference double build's (the
+    reference computes no residual): within 4x the single restatement's
+    distance from the double one, plus 5 % and 1e-4."""
+    say("reference-order residual", name)
+    from test_golden import CASES
+    sc = CASES[name][0]()
+    nsteps = 20
+    r64 = po.run(po.oracle_lib("f64"), sc, nsteps, 0, residual=True)["residual"]
+    r32 = po.run(po.oracle_lib("f32"), sc, nsteps, 0, residual=True)["residual"]
+    b = _batch(odemi_mod, [sc])
+    b.track_residual(True)
+    got = []
+    for k in range(nsteps):
+        if sc.step_ctrl is not None:
+            b.set_controls(sc.step_ctrl[k])
+        b.step(1)
+        got.append(float(b.residual()[0]))
+    got = np.array(got)
+    tol = 4 * np.abs(r32 - r64) + 0.05 * r64 + 1e-4
+    say(name, "residuals", np.round(got[:5], 6), "restatement", np.round(r64[:5], 6),
+        "worst err/tol", float((np.abs(got - r64) / tol).max()))
+    assert np.all(np.abs(got - r64) <= tol)
+
+
 def test_plain_path_equals_batched_path(odemi_mod):
     say("plain vs batched")
     ws = scenes.uneven_worlds(24, seed=3)
@@ -649,3 +751,41 @@ def test_coloured_order_is_reproducible(odemi_mod):
         b.close()
     assert np.array_equal(outs[0][0], outs[1][0], equal_nan=True)
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_uneven_worlds_rollout_at_tolerance(odemi_mod):
+    """The batched-uneven-worlds configuration at the metric's operating
+    point on 24 worlds: 200 steps, coloured order iterated to an LCP residual
+    of 1e-4 (cap 2000, CG steps, warm start).  Every world stays finite (none
+    frozen); whenever the solver reports a solve converged, the residual
+    recomputed afterwards from the rows and the final impulses is within the
+    tolerance (the stopping test is sound); the fraction of converged solves
+    is reported and must be at least half."""
+    ws = scenes.uneven_worlds(24, seed=5)
+    say("uneven worlds at tolerance:", len(ws), "worlds,", sum(w.nb for w in ws), "bodies, 200 steps")
+    b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    b.set_tolerance(1e-4, 2000, 10, accel=True)
+    b.set_warm_start(1.0)
+    b.track_residual(True)
+    solves = conv = 0
+    worst_conv = 0.0
+    prev = np.zeros(len(ws), np.int64)
+    for k in range(200):
+        b.step(1)
+        res = b.residual()
+        unconv, _ = b.solve_stats()
+        newly = unconv.astype(np.int64) - prev
+        prev = unconv.astype(np.int64)
+        ok = newly == 0
+        solves += len(ws)
+        conv += int(ok.sum())
+        if ok.any():
+            worst_conv = max(worst_conv, float(res[ok].max()))
+        assert (res[ok] <= 1.05e-4).all(), (k, res[ok].max())
+    st = b.state()
+    wc = b.world_counters()
+    assert np.isfinite(st).all()
+    assert not ((wc[:, 4] & 16) != 0).any()
+    say("converged solves", conv, "of", solves, f"({conv / solves:.3f}); largest residual of a converged solve "
+        f"{worst_conv:.3g}")
+    assert conv >= 0.5 * solves

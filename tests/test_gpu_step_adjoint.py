@@ -4,11 +4,14 @@ and restitution, against FP64 central differences through the CPU
 restatement (converged solver) and against the host double adjoint; the
 checkpoint interval must not change the result; the torch.autograd.Function
 wrapper returns the same gradients."""
+import copy
+
 import numpy as np
 import pytest
 
 from adjoint.harness import Rollout, com_loss, contact_scene, fd_grad, host_adjoint
 from conftest import say
+from odemi import scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -121,3 +124,88 @@ def test_torch_autograd_function(odemi_mod):
     assert np.allclose(lv.grad.numpy(), gi[:, 7:10], rtol=1e-4, atol=1e-6)
     assert np.allclose(av.grad.numpy(), gi[:, 10:13], rtol=1e-4, atol=1e-6)
     assert np.allclose(mu.grad.numpy(), gmu, rtol=1e-4, atol=1e-6)
+
+
+def _perturbed(sc, field, idx, delta):
+    s2 = copy.deepcopy(sc)
+    if field in ("mu", "bounce"):
+        setattr(s2, field, getattr(sc, field) + delta)
+    else:
+        a = getattr(s2, field).copy()
+        a[idx] += delta
+        setattr(s2, field, a)
+    return s2
+
+
+def _contact_structure(sc, n):
+    import pyoracle as po
+    r = po.run(po.oracle_lib("f64"), sc, n, n, diag_steps=n, max_pairs=10 ** 6, max_contacts=10 ** 6)
+    return r["contact_step_count"], r["pairs"]
+
+
+def test_differentiable_pile_config(odemi_mod):
+    """The differentiable-pile configuration: worlds of its U{32..128}
+    box/sphere distribution with restitution, h = 0.01, 100 steps, the tape
+    checkpointed every 10 steps, loss = sum of squared centres of mass.  The
+    GPU gradients (FP32, forward iterated to 1e-6) against the host double
+    adjoint and FP64 central differences (h = 1e-6) through the restatement
+    for sampled initial velocities, mu and restitution.  A component whose
+    central difference crosses a change of the contact structure (the
+    contacts or pairs of some step differ between +h and -h) is ill-posed for
+    finite differences: it is reported and not compared."""
+    n = 100
+    ws = sorted(scenes.diff_piles(64, seed=5), key=lambda s: s.nb)[:2]
+    scs = []
+    for sc in ws:
+        sc = copy.deepcopy(sc)
+        sc.iterations = 20000   # the FP64 restatement converges too
+        scs.append(sc)
+    say("differentiable piles:", [s.nb for s in scs], "bodies,", n, "steps, checkpoint 10")
+    b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
+    b.set_tolerance(1e-6, 4000, 10, accel=True)
+    b.tape_options(10)
+    b.tape_begin(n)
+    b.step(n)
+    st = b.state().astype(np.float64)
+    finals, off = [], 0
+    for sc in scs:
+        finals.append(st[off:off + sc.nb])
+        off += sc.nb
+    _, gl = com_loss(scs, finals)
+    gi, gmu, gb = b.step_backward(n, np.concatenate(gl))
+    stats = b.tape_stats()
+    b.close()
+    assert stats["collider_mismatches"] == 0
+    assert np.isfinite(gi).all()
+    off = 0
+    for w, sc in enumerate(scs):
+        g = gi[off:off + sc.nb].astype(np.float64)
+        _, hg, hmu, hb, _ = host_adjoint(sc, n)
+        gmax = float(np.abs(hg[:, 7:13]).max())
+        err_host = float(np.abs(g[:, 7:13] - hg[:, 7:13]).max()) / gmax
+        say(f"world {w} ({sc.nb} bodies): GPU vs host adjoint {err_host:.2e} of max |grad| {gmax:.3g}")
+        assert err_host <= TOL
+        rng = np.random.default_rng(w)
+        comps = [("body_lvel", (int(rng.integers(sc.nb)), int(rng.integers(3)))),
+                 ("body_avel", (int(rng.integers(sc.nb)), int(rng.integers(3)))), ("mu", None), ("bounce", None)]
+        compared = 0
+        for field, idx in comps:
+            a, c = (_contact_structure(_perturbed(sc, field, idx, s * 1e-6), n) for s in (1, -1))
+            if not (np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])):
+                say(f"  {field} {idx}: ill-posed for central differences (contacts switch within +-1e-6)")
+                continue
+            fd = fd_grad(sc, n, field, idx)
+            if field == "mu":
+                got, host = float(gmu[w]), hmu
+            elif field == "bounce":
+                got, host = float(gb[w]), hb
+            else:
+                col = 7 if field == "body_lvel" else 10
+                got, host = float(g[idx[0], col + idx[1]]), float(hg[idx[0], col + idx[1]])
+            scale = max(gmax, abs(fd))
+            say(f"  {field} {idx}: fd {fd:.6g} host {host:.6g} gpu {got:.6g}")
+            assert abs(host - fd) <= 1e-4 * scale
+            assert abs(got - fd) <= TOL * scale
+            compared += 1
+        assert compared >= 2
+        off += sc.nb
