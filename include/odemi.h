@@ -481,8 +481,21 @@ ODE_API int dBatchTapeStats(dBatchID b, long long *out2);
  * state is left at the end of the tape. */
 ODE_API int dBatchStepBackward(dBatchID b, int nsteps, const float *grad_out, float *grad_in,
                                float *grad_mu, float *grad_bounce);
-/* Single-world form of the adjoint for the ODE API path: derivative of the
- * last dWorldQuickStep of `w` (recorded when tape recording is on). */
+/* Single-world form of the adjoint for the ODE API path (no counterpart in
+ * the reference; the batch form is dBatchStepBackward).  With tape recording
+ * on, each dWorldQuickStep of `w` keeps its pre-step state and its contacts
+ * with their impulses; dWorldQuickStepBackward then returns the derivative
+ * of the last such step: grad_out = dL/d(state after), grad_in receives
+ * dL/d(state before), 13 dReals per body in creation order (pos 3, q 4,
+ * lvel 3, avel 3).  The LCP is differentiated implicitly at the step's
+ * impulses on their active set, which is the derivative of the step when
+ * its solve converged (raise dWorldSetQuickStepNumIterations until it does).
+ * Differentiable steps: contact joints only (no ball/hinge/hinge2 joints),
+ * one surface policy shared by every contact (dContactApprox1 friction or
+ * mu = 0, optional dContactBounce), no forces besides gravity, contact geoms
+ * in the world's space.  A step outside these conditions is still taken;
+ * dWorldQuickStepBackward then reports why and returns 0, as it does when
+ * nothing was recorded or the world changed since.  Returns 1 on success. */
 ODE_API int dWorldSetTape(dWorldID w, int on);
 ODE_API int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in);
 

@@ -135,6 +135,42 @@ __global__ __launch_bounds__(256) void adjoint_step_kernel(
   adj::adjoint_step<float>(par, D, Q, G, scr + scr_off[w]);
 }
 
+// After a dWorldQuickStep of the ODE API path (one world, contacts from the
+// host's contact joints): the step's contacts in creation order with the
+// impulses of their rows, for dWorldQuickStepBackward.
+__global__ void api_tape_capture_kernel(DevBatch B, const ContactIn *cin, int nk, TapeContact<float> *con,
+                                        int *nk_out) {
+  const WorldInfo W = B.wi[0];
+  const ScratchLayout L = scratch_layout(W.nb, W.ng, W.nj, W.pcap, W.kcap, W.rcap);
+  const char *S = B.scratch + W.scratch;
+  const Cons *cons = (const Cons *)(S + L.cons);
+  const Row *rows = (const Row *)(S + L.rows);
+  const int flags = B.counters[CNT_OVERFLOW];
+  if (flags & OVF_ROWS) nk = 0;
+  if (nk > W.kcap) nk = W.kcap;
+  for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+    const ContactIn &c = cin[i];
+    TapeContact<float> t;
+    for (int k = 0; k < 3; k++) { t.pos[k] = c.pos[k]; t.normal[k] = c.normal[k]; }
+    t.depth = c.depth;
+    t.g1 = c.g1; t.g2 = c.g2; t.b1 = c.b1; t.b2 = c.b2; t.flags = c.flags;
+    const int m = cons[i].m;
+    t.m = m;
+    for (int r = 0; r < 3; r++) {
+      const int ri = cons[i].row0 + r;
+      t.lam[r] = (r < m && ri < W.rcap) ? rows[ri].lambda : 0.f;
+    }
+    con[i] = t;
+  }
+  if (threadIdx.x == 0) *nk_out = nk;
+}
+
+hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int nk, TapeContact<float> *con,
+                                   int *nk_out, hipStream_t st) {
+  api_tape_capture_kernel<<<1, 128, 0, st>>>(B, cin, nk, con, nk_out);
+  return hipGetLastError();
+}
+
 hipError_t launch_tape_capture(const DevBatch &B, TapeContact<float> *con, const long long *con_off, int *nk,
                                int *ovf, hipStream_t st) {
   if (B.nworlds <= 0) return hipSuccess;

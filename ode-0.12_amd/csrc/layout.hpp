@@ -97,7 +97,8 @@ struct ContactIn {         // a contact joint created on the host (ODE API path)
   float pos[4], normal[4], fdir1[4];
   float depth;
   int b1, b2, flags;       // node order after dJointAttach
-  int seq, mode, fb, pad;
+  int seq, mode, fb, g1;   // g1, g2: mirror indices of the contact's geoms (-1: none)
+  int g2, pad[3];
   float mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm, motion1, motion2, motionN, slip1, slip2, pad2;
 };
 

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdlib>
 
+#include "adj_step.hpp"
 #include "colliders.hpp"
 #include "host_common.hpp"
 #include "odemi.h"
@@ -23,6 +24,13 @@ unsigned long &global_seed();
 void report_debug(const char *fmt, ...);
 hipError_t launch_collide_one(const GeomView &a, const GeomView &b, int maxc, CPoint *out, int *n, hipStream_t st);
 void *noalloc_scratch(hipError_t *err);
+hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int nk, adj::TapeContact<float> *con,
+                                   int *nk_out, hipStream_t st);
+hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], const adj::TapeContact<float> *con,
+                               const long long *con_off, const int *nk, const float4 *const gin[4],
+                               float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
+                               const long long *scr_off, float act_tol, float lin_tol, int lin_maxit, int wg,
+                               hipStream_t st_);
 hipError_t collide_one_noalloc(void *addr, const GeomView &a, const GeomView &b, int maxc, CPoint *host_pts,
                                int *host_n, hipStream_t st);
 }  // namespace odemi
@@ -52,6 +60,14 @@ struct dxWorld {
   bool device_newer = false;
   bool tape = false;
   DevBuf cin, cin_start, cin_count, feedback, pairs_out, pcount_out, contacts_out, one_out, one_n;
+  // the last dWorldQuickStep recorded for dWorldQuickStepBackward: pre-step
+  // state (pos, q, lvel, avel; nb float4 each), contacts with impulses, and
+  // the contact policy the adjoint differentiates
+  DevBuf tape_st, tape_con, tape_nk;
+  bool tape_ok = false;
+  char tape_why[200] = "no step recorded (dWorldSetTape(w, 1) before dWorldQuickStep)";
+  int tape_mode = 0, tape_maxc = 4;
+  float tape_mu = 0, tape_bounce = 0, tape_bounce_vel = 0;
   int pcap = 0, kcap = 0;
 };
 
@@ -192,6 +208,10 @@ bool is_permanent(const dxJoint *j) { return j->type != dJointTypeContact; }
 
 // (re)creates the device mirror of a world with its space
 int build_mirror(dxWorld *w, dxSpace *sp) {
+  if (w->tape_ok) {
+    w->tape_ok = false;
+    std::snprintf(w->tape_why, sizeof(w->tape_why), "the world changed since the recorded step");
+  }
   if (w->mir) {
     pull_state(w);
     batch_destroy(w->mir);
@@ -383,6 +403,42 @@ void write_contact(dContactGeom *c, const float *pos, const float *n, float dept
   c->side2 = -1;
 }
 
+// Checks that the coming step is one dWorldQuickStepBackward differentiates
+// (the conditions of adj_step.hpp: contact joints only, one contact policy
+// -- dContactApprox1 friction or none, optional dContactBounce -- for every
+// contact, no forces but gravity, every contact's geoms in the world's
+// space) and sizes the tape; records the reason when it is not.
+bool tape_prepare(dxWorld *w, const std::vector<ContactIn> &cin) {
+  auto refuse = [&](const char *fmt, auto... a) {
+    std::snprintf(w->tape_why, sizeof(w->tape_why), fmt, a...);
+    w->tape_ok = false;
+    return false;
+  };
+  if (!w->mjoints.empty()) return refuse("the world has %zu ball/hinge/hinge2 joints", w->mjoints.size());
+  for (dxBody *b : w->bodies)
+    for (int k = 0; k < 3; k++)
+      if (b->facc[k] != 0 || b->tacc[k] != 0) return refuse("a body carries a force or torque besides gravity");
+  const int approx = dContactApprox1;
+  for (size_t i = 0; i < cin.size(); i++) {
+    const ContactIn &c = cin[i];
+    if (c.g1 < 0 || c.g2 < 0) return refuse("contact %zu names a geom outside the world's space", i);
+    if (c.mode & ~(approx | dContactBounce)) return refuse("contact %zu uses surface mode bits 0x%x", i, c.mode);
+    if (c.mu > 0 && (c.mode & approx) != approx) return refuse("contact %zu: friction without dContactApprox1", i);
+    if (i == 0) {
+      w->tape_mode = c.mode; w->tape_mu = c.mu; w->tape_bounce = c.bounce; w->tape_bounce_vel = c.bounce_vel;
+    } else if (c.mode != w->tape_mode || c.mu != w->tape_mu || c.bounce != w->tape_bounce ||
+               c.bounce_vel != w->tape_bounce_vel) {
+      return refuse("contacts %d and %zu differ in surface parameters", 0, i);
+    }
+  }
+  w->tape_maxc = w->mir_space ? w->mir_space->cached_maxc : 4;
+  const size_t nb = w->mir->nb, kcap = w->mir->wi[0].kcap;
+  if (!w->tape_st.alloc(4 * sizeof(float4) * (nb + 1)) ||
+      !w->tape_con.alloc(sizeof(adj::TapeContact<float>) * (kcap + 1)) || !w->tape_nk.alloc(16))
+    return refuse("device allocation of the tape failed");
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -464,6 +520,10 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     c.slip2 = k.surface.slip2;
     c.fb = -1;
     if (j->fb) { c.fb = (int)fbs.size(); fbs.push_back(j->fb); }
+    c.g1 = (k.geom.g1 && k.geom.g1->mindex >= 0 && k.geom.g1->mindex < (int)w->mgeoms.size() &&
+            w->mgeoms[k.geom.g1->mindex] == k.geom.g1) ? k.geom.g1->mindex : -1;
+    c.g2 = (k.geom.g2 && k.geom.g2->mindex >= 0 && k.geom.g2->mindex < (int)w->mgeoms.size() &&
+            w->mgeoms[k.geom.g2->mindex] == k.geom.g2) ? k.geom.g2->mindex : -1;
     cin.push_back(c);
   }
   if ((int)cin.size() > m->wi[0].kcap) {
@@ -509,7 +569,19 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   D.cin_count = w->cin_count.as<int>();
   D.feedback = fbs.empty() ? nullptr : w->feedback.as<float>();
   D.ctrl = nullptr;
+  const bool record = w->tape && tape_prepare(w, cin);
+  if (record) {
+    const size_t bytes = sizeof(float4) * (size_t)m->nb;
+    float4 *st = w->tape_st.as<float4>();
+    OD_HIP(hipMemcpyAsync(st, m->d.pos.p, bytes, hipMemcpyDeviceToDevice, m->stream));
+    OD_HIP(hipMemcpyAsync(st + m->nb, m->d.q.p, bytes, hipMemcpyDeviceToDevice, m->stream));
+    OD_HIP(hipMemcpyAsync(st + 2 * (size_t)m->nb, m->d.lvel.p, bytes, hipMemcpyDeviceToDevice, m->stream));
+    OD_HIP(hipMemcpyAsync(st + 3 * (size_t)m->nb, m->d.avel.p, bytes, hipMemcpyDeviceToDevice, m->stream));
+  }
   OD_HIP(launch_world_kernel(KM_API_STEP, D, 0, 1, 0, m->wg, 0, m->stream));
+  if (record)
+    OD_HIP(launch_api_tape_capture(D, w->cin.as<ContactIn>(), nk, w->tape_con.as<adj::TapeContact<float>>(),
+                                   w->tape_nk.as<int>(), m->stream));
   OD_HIP(hipStreamSynchronize(m->stream));
   long long cnt[5];
   batch_counters(m, cnt);
@@ -517,6 +589,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     report_error("dWorldQuickStep: device capacity exceeded (flags %lld)", cnt[4]);
     return 0;
   }
+  w->tape_ok = record;
   // RNG state, feedback, joint limit state, space order
   WorldParams P;
   OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
@@ -1185,13 +1258,100 @@ int dCollideNoAlloc(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, in
 int dWorldSetTape(dWorldID w, int on) {
   if (!w) return 0;
   w->tape = on != 0;
+  if (!w->tape) {
+    w->tape_ok = false;
+    std::snprintf(w->tape_why, sizeof(w->tape_why), "tape recording is off");
+  }
   return 1;
 }
+
+// Reverse step of the last recorded dWorldQuickStep (adj_step.hpp on the
+// world's device mirror): the LCP differentiated implicitly at the step's
+// impulses on their active set (exact when the solve converged: raise
+// dWorldSetQuickStepNumIterations until it does), the colliders and the
+// integrator analytically.  grad_out / grad_in: 13 floats per body in
+// creation order (pos 3, q 4, lvel 3, avel 3).
 int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in) {
-  (void)grad_out; (void)grad_in;
-  if (!w) return 0;
-  report_error("dWorldQuickStepBackward: use the batch form (dBatchStepBackward) in this build");
-  return 0;
+  if (!w || !grad_out || !grad_in) return 0;
+  if (!w->tape_ok || !w->mir) {
+    report_error("dWorldQuickStepBackward: %s", w->tape_why);
+    return 0;
+  }
+  BatchImpl *m = w->mir;
+  const int nb = m->nb, kcap = m->wi[0].kcap;
+  // the contact policy of the recorded step, for the adjoint kernel
+  WorldParams P0, P;
+  OD_HIP(hipMemcpy(&P0, m->d.wp.p, sizeof(P0), hipMemcpyDeviceToHost));
+  P = P0;
+  P.surf_mode = w->tape_mode;
+  P.mu = w->tape_mu;
+  P.bounce = w->tape_bounce;
+  P.bounce_vel = w->tape_bounce_vel;
+  P.max_contacts = w->tape_maxc;
+  OD_HIP(hipMemcpy(m->d.wp.p, &P, sizeof(P), hipMemcpyHostToDevice));
+  DevBuf gbuf, gsc, stats, scr, off;
+  const long long nscr = adj::adj_scratch_elems<float>(nb, kcap);
+  bool ok = gbuf.alloc(8 * sizeof(float4) * (size_t)(nb + 1)) && gsc.alloc(2 * sizeof(float)) &&
+            stats.alloc(2 * sizeof(int)) && scr.alloc(sizeof(float) * (size_t)nscr) &&
+            off.alloc(sizeof(long long));
+  if (!ok) {
+    OD_HIP(hipMemcpy(m->d.wp.p, &P0, sizeof(P0), hipMemcpyHostToDevice));
+    report_error("dWorldQuickStepBackward: device allocation failed");
+    return 0;
+  }
+  float4 *G = gbuf.as<float4>();
+  const float4 *gin[4];
+  float4 *gout[4];
+  for (int k = 0; k < 4; k++) {
+    gin[k] = G + (size_t)k * nb;
+    gout[k] = G + (size_t)(4 + k) * nb;
+  }
+  std::vector<float4> h(4 * (size_t)nb, make_float4(0, 0, 0, 0));
+  for (size_t bi = 0; bi < w->bodies.size(); bi++) {
+    const int i = w->bodies[bi]->mindex;
+    if (i < 0 || i >= nb) continue;
+    const float *g = grad_out + 13 * bi;
+    h[i] = make_float4(g[0], g[1], g[2], 0);
+    h[nb + i] = make_float4(g[3], g[4], g[5], g[6]);
+    h[2 * (size_t)nb + i] = make_float4(g[7], g[8], g[9], 0);
+    h[3 * (size_t)nb + i] = make_float4(g[10], g[11], g[12], 0);
+  }
+  OD_HIP(hipMemcpy(G, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice));
+  OD_HIP(hipMemset(gsc.p, 0, gsc.n));
+  OD_HIP(hipMemset(stats.p, 0, stats.n));
+  OD_HIP(hipMemset(off.p, 0, off.n));
+  const float4 *st[4];
+  const float4 *S = w->tape_st.as<float4>();
+  for (int k = 0; k < 4; k++) st[k] = S + (size_t)k * nb;
+  refresh_dev(m);
+  hipError_t e = launch_adjoint_step(m->dev, st, w->tape_con.as<adj::TapeContact<float>>(), off.as<long long>(),
+                                     w->tape_nk.as<int>(), gin, gout, gsc.as<float>(), gsc.as<float>() + 1,
+                                     stats.as<int>(), scr.as<float>(), off.as<long long>(), 1e-6f, 1e-6f, 500, 256,
+                                     m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  OD_HIP(hipMemcpy(m->d.wp.p, &P0, sizeof(P0), hipMemcpyHostToDevice));
+  if (e != hipSuccess) {
+    report_error("dWorldQuickStepBackward: %s", hipGetErrorString(e));
+    return 0;
+  }
+  OD_HIP(hipMemcpy(h.data(), gout[0], sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+  int st2[2];
+  OD_HIP(hipMemcpy(st2, stats.p, sizeof(st2), hipMemcpyDeviceToHost));
+  if (st2[1] != 0) {
+    report_error("dWorldQuickStepBackward: the colliders returned a different contact set on the recorded poses");
+    return 0;
+  }
+  for (size_t bi = 0; bi < w->bodies.size(); bi++) {
+    const int i = w->bodies[bi]->mindex;
+    float *g = grad_in + 13 * bi;
+    if (i < 0 || i >= nb) { for (int k = 0; k < 13; k++) g[k] = 0; continue; }
+    const float4 a = h[i], q = h[nb + i], v = h[2 * (size_t)nb + i], r = h[3 * (size_t)nb + i];
+    g[0] = a.x; g[1] = a.y; g[2] = a.z;
+    g[3] = q.x; g[4] = q.y; g[5] = q.z; g[6] = q.w;
+    g[7] = v.x; g[8] = v.y; g[9] = v.z;
+    g[10] = r.x; g[11] = r.y; g[12] = r.z;
+  }
+  return 1;
 }
 
 }  // extern "C"

@@ -48,7 +48,7 @@ class SceneOut(C.Structure):
         ("max_contacts", C.c_int), ("contacts", _dp), ("contact_step_count", _ip), ("feedback", _dp),
         ("total_pairs", C.c_longlong), ("total_contacts", C.c_longlong),
         ("sec_collide", C.c_double), ("sec_step", C.c_double), ("status", C.c_int), ("time_from", C.c_int),
-        ("residual", _dp),
+        ("residual", _dp), ("grad_out", _dp), ("grad_in", _dp),
     ]
 
 
@@ -121,10 +121,13 @@ def ref_lib(precision: str = "double"):
 
 
 def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pairs: int = 0,
-        max_contacts: int = 0, feedback: bool = False, time_from: int = 0, residual: bool = False) -> dict:
+        max_contacts: int = 0, feedback: bool = False, time_from: int = 0, residual: bool = False,
+        grad_out=None) -> dict:
     """Run `nsteps` of a Scene through `lib.scene_run`; returns numpy arrays.
     sec_collide / sec_step time the steps from `time_from` on; residual=True
-    (CPU restatement only) records every step's LCP residual."""
+    (CPU restatement only) records every step's LCP residual; grad_out
+    (nb x 13, MI355X library's scene driver only) differentiates the last
+    step through dWorldQuickStepBackward into the result's grad_in."""
     desc, keep = make_desc(sc)
     out = SceneOut()
     out.nsteps = nsteps
@@ -148,6 +151,11 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
     out.feedback = fb.ctypes.data_as(_dp) if feedback else None
     res = np.zeros(max(nsteps, 1))
     out.residual = res.ctypes.data_as(_dp) if residual else None
+    gin = np.zeros((sc.nb, 13))
+    if grad_out is not None:
+        gout = np.ascontiguousarray(grad_out, np.float64).reshape(sc.nb, 13)
+        out.grad_out = gout.ctypes.data_as(_dp)
+        out.grad_in = gin.ctypes.data_as(_dp)
     status = lib.scene_run(C.byref(desc), C.byref(out))
     del keep
     npairs = int(psc[:diag_steps].sum()) if diag_steps else 0
@@ -157,6 +165,7 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
                 contact_step_count=csc[:diag_steps].copy(), feedback=fb[:min(ncont, max_contacts)],
                 total_pairs=out.total_pairs, total_contacts=out.total_contacts,
                 residual=res[:nsteps].copy() if residual else None,
+                grad_in=gin if grad_out is not None else None,
                 sec_collide=out.sec_collide, sec_step=out.sec_step)
 
 

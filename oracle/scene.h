@@ -89,6 +89,12 @@ typedef struct SceneOut {
   int status;                    /* 0 ok, else failing step + 1 */
   int time_from;                 /* steps before this one are not timed */
   double *residual;              /* may be NULL: nsteps LCP residuals (CPU restatement only) */
+  /* may be NULL (MI355X library only): the last step is recorded with
+     dWorldSetTape and differentiated with dWorldQuickStepBackward; grad_out
+     is dL/d(final state), grad_in receives dL/d(state before the last step),
+     nb*13 each (pos 3, q 4, lvel 3, avel 3) */
+  const double *grad_out;
+  double *grad_in;
 } SceneOut;
 
 #ifdef __cplusplus

@@ -250,6 +250,9 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
         }
       }
     }
+#ifndef SCENE_DRIVER_REFERENCE
+    if (o->grad_out && step == o->nsteps - 1) dWorldSetTape(world, 1);
+#endif
     auto t0 = clk::now();
     dSpaceCollide(space, &c, &near_callback);
     auto t1 = clk::now();
@@ -285,6 +288,15 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     }
   }
 
+#ifndef SCENE_DRIVER_REFERENCE
+  if (o->grad_out && o->grad_in && o->nsteps > 0) {
+    // reverse step of the last dWorldQuickStep (an addition of this library)
+    std::vector<dReal> go(13 * (size_t)s->nb), gi(13 * (size_t)s->nb, 0);
+    for (size_t k = 0; k < go.size(); k++) go[k] = (dReal)o->grad_out[k];
+    if (!dWorldQuickStepBackward(world, go.data(), gi.data()) && !o->status) o->status = o->nsteps + 1;
+    for (size_t k = 0; k < gi.size(); k++) o->grad_in[k] = gi[k];
+  }
+#endif
   dJointGroupDestroy(cgroup);
   dSpaceDestroy(space);
   dWorldDestroy(world);
