@@ -251,9 +251,22 @@ ODE_API void dHashSpaceSetLevels(dSpaceID space, int minlevel, int maxlevel); /*
 ODE_API void dSpaceAdd(dSpaceID space, dGeomID g);                            /* collision_space.h:149 */
 ODE_API void dSpaceRemove(dSpaceID space, dGeomID g);                         /* collision_space.h:150 */
 ODE_API int dSpaceGetNumGeoms(dSpaceID space);                                /* collision_space.h:153 */
+ODE_API dGeomID dSpaceGetGeom(dSpaceID space, int i);                         /* collision_space.h:154 */
+ODE_API int dSpaceQuery(dSpaceID space, dGeomID g);                           /* collision_space.h:151 */
+ODE_API void dSpaceSetCleanup(dSpaceID space, int mode);                      /* collision_space.h:75 */
+ODE_API int dSpaceGetCleanup(dSpaceID space);                                 /* collision_space.h:76 */
+ODE_API void dSpaceSetSublevel(dSpaceID space, int sublevel);                 /* collision_space.h:100 */
+ODE_API int dSpaceGetSublevel(dSpaceID space);                                /* collision_space.h:114 */
+ODE_API int dGeomIsSpace(dGeomID geom);                                       /* collision.h:273 */
+ODE_API dSpaceID dGeomGetSpace(dGeomID geom);                                 /* collision.h:283 */
 
 /* collision: include/ode/collision.h:68-1048 */
 ODE_API void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback); /* collision.h:825 */
+/* Spaces are geoms: a space created with a parent sits in that space, and a
+ * dSpaceID is passed where a dGeomID is taken by a cast, as in the reference.
+ * dSpaceCollide on a space that holds spaces is refused in this build (the
+ * error handler says so); dSpaceCollide2 takes spaces at any level. */
+ODE_API void dSpaceCollide2(dGeomID space1, dGeomID space2, void *data, dNearCallback *callback); /* collision.h:868 */
 /* flags bit: any contacts will do (collision.h:746); the colliders here
  * refine nothing, so it changes no result */
 #define CONTACTS_UNIMPORTANT 0x80000000

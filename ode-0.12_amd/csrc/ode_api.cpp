@@ -24,6 +24,8 @@ unsigned long &global_seed();
 void report_debug(const char *fmt, ...);
 hipError_t launch_collide_one(const GeomView &a, const GeomView &b, int maxc, CPoint *out, int *n, hipStream_t st);
 void *noalloc_scratch(hipError_t *err);
+hipError_t launch_collide_list(const GeomView *a, const GeomView *b, int npairs, int maxc, CPoint *out, int *n,
+                               hipStream_t st);
 hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int nk, adj::TapeContact<float> *con,
                                    int *nk_out, hipStream_t st);
 hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], const adj::TapeContact<float> *con,
@@ -84,25 +86,32 @@ struct dxBody {
   int mindex = -1;
 };
 
-struct dxSpace {
-  int cls = dHashSpaceClass;
-  int minlevel = -3, maxlevel = 10;
-  std::vector<dxGeom *> list;              // space list, front first
-  int cached_maxc = 4;
-  dxWorld *world = nullptr;
-};
-
 struct dxGeom {
   int cls;
   dxBody *body = nullptr;
-  dxSpace *space = nullptr;
+  dxSpace *space = nullptr;                // the space holding this geom
   float param[4] = {0, 0, 0, 0};
   float pos[4] = {0, 0, 0, 0}, R[12];
   unsigned long cat = ~0UL, col = ~0UL;
   void *data = nullptr;
   bool dirty = true;
   int mindex = -1;
+  virtual ~dxGeom() {}
 };
+
+// A space is a geom (collision_space.cpp:85-104): it can sit in another
+// space, and its bounding box is the union of its geoms' boxes.
+struct dxSpace : dxGeom {
+  int minlevel = -3, maxlevel = 10;
+  std::vector<dxGeom *> list;              // space list, front first
+  int cached_maxc = 4;
+  dxWorld *world = nullptr;
+  int sublevel = 0;
+  int cleanup = 1;
+  dxSpace() { cls = dHashSpaceClass; }
+};
+
+inline bool is_space(const dxGeom *g) { return g && g->cls >= dFirstSpaceClass && g->cls <= dLastSpaceClass; }
 
 struct dxJoint {
   int type;                                // dJointType*
@@ -144,14 +153,19 @@ void mark_struct(dxWorld *w) { if (w) w->struct_dirty = true; }
 
 // geoms move to the front of their space list when first marked dirty
 // (collision_space.cpp:47-75, 209-213)
+// (collision_space.cpp:47-75): from the geom up the space hierarchy, each
+// clean geom or space becomes dirty and moves to the front of its space
 void geom_moved(dxGeom *g) {
-  if (!g->space || g->dirty) return;
-  auto &L = g->space->list;
-  auto it = std::find(L.begin(), L.end(), g);
-  if (it != L.end()) L.erase(it);
-  L.insert(L.begin(), g);
-  g->dirty = true;
-  if (g->space->world) g->space->world->rank_dirty = true;
+  while (g && g->space && !g->dirty) {
+    auto &L = g->space->list;
+    auto it = std::find(L.begin(), L.end(), g);
+    if (it != L.end()) L.erase(it);
+    L.insert(L.begin(), g);
+    g->dirty = true;
+    if (g->space->world) g->space->world->rank_dirty = true;
+    g = g->space;
+  }
+  for (; g; g = g->space) g->dirty = true;
 }
 void body_moved(dxBody *b) {
   for (dxGeom *g : b->geoms) geom_moved(g);
@@ -354,8 +368,13 @@ int world_params_upload(dxWorld *w, float dt, int maxc) {
 
 dxWorld *space_world(dxSpace *sp) {
   if (sp->world) return sp->world;
-  for (dxGeom *g : sp->list)
+  for (dxGeom *g : sp->list) {
     if (g->body) return sp->world = g->body->world;
+    if (is_space(g)) {
+      dxWorld *w = space_world(static_cast<dxSpace *>(g));
+      if (w) return sp->world = w;
+    }
+  }
   return nullptr;
 }
 
@@ -935,25 +954,36 @@ int dAreConnectedExcluding(dBodyID a, dBodyID b, int joint_type) {
 }
 
 // ---------------------------------------------------------------- spaces
-dSpaceID dSimpleSpaceCreate(dSpaceID) {
+static dxSpace *make_space(dSpaceID parent, int cls) {
   dxSpace *s = new dxSpace();
-  s->cls = dSimpleSpaceClass;
+  s->cls = cls;
+  std::memset(s->R, 0, sizeof(s->R));
+  s->R[0] = s->R[5] = s->R[10] = 1;
   g_spaces.push_back(s);
+  if (parent) dSpaceAdd(parent, s);   // collision_space.cpp:85-93
   return s;
 }
-dSpaceID dHashSpaceCreate(dSpaceID) {
-  dxSpace *s = new dxSpace();
-  g_spaces.push_back(s);
-  return s;
-}
+dSpaceID dSimpleSpaceCreate(dSpaceID parent) { return make_space(parent, dSimpleSpaceClass); }
+dSpaceID dHashSpaceCreate(dSpaceID parent) { return make_space(parent, dHashSpaceClass); }
 void dHashSpaceSetLevels(dSpaceID s, int lo, int hi) {
   s->minlevel = lo;
   s->maxlevel = hi;
 }
+void dSpaceSetCleanup(dSpaceID s, int mode) { s->cleanup = mode ? 1 : 0; }
+int dSpaceGetCleanup(dSpaceID s) { return s->cleanup; }
+void dSpaceSetSublevel(dSpaceID s, int level) { s->sublevel = level; }
+int dSpaceGetSublevel(dSpaceID s) { return s->sublevel; }
+int dGeomIsSpace(dGeomID g) { return is_space(g) ? 1 : 0; }
+dSpaceID dGeomGetSpace(dGeomID g) { return g->space; }
 void dSpaceDestroy(dSpaceID s) {
-  // the space owns its geoms by default (collision_space.cpp:86-104)
+  // the space owns its geoms unless its cleanup mode is off
+  // (collision_space.cpp:96-118); a space leaves the space holding it
   std::vector<dxGeom *> gl = s->list;
-  for (dxGeom *g : gl) dGeomDestroy(g);
+  for (dxGeom *g : gl) {
+    if (s->cleanup) dGeomDestroy(g);
+    else dSpaceRemove(s, g);
+  }
+  if (s->space) dSpaceRemove(s->space, s);
   for (dxWorld *w : std::vector<dxWorld *>{s->world})
     if (w && w->mir_space == s) { w->mir_space = nullptr; mark_struct(w); }
   auto it = std::find(g_spaces.begin(), g_spaces.end(), s);
@@ -961,10 +991,11 @@ void dSpaceDestroy(dSpaceID s) {
   delete s;
 }
 void dSpaceAdd(dSpaceID s, dGeomID g) {
-  if (g->space) return;
+  if (g->space || g == s) return;
   g->space = s;
   s->list.insert(s->list.begin(), g);
   g->dirty = true;
+  geom_moved(s);   // the space, and the spaces above it, become dirty (collision_space.cpp:160-178)
   dxWorld *w = space_world(s);
   if (w) mark_struct(w);
 }
@@ -972,7 +1003,13 @@ void dSpaceRemove(dSpaceID s, dGeomID g) {
   auto it = std::find(s->list.begin(), s->list.end(), g);
   if (it != s->list.end()) s->list.erase(it);
   g->space = nullptr;
+  geom_moved(s);
   if (s->world) mark_struct(s->world);
+}
+int dSpaceQuery(dSpaceID s, dGeomID g) { return g && g->space == s ? 1 : 0; }
+dGeomID dSpaceGetGeom(dSpaceID s, int i) {
+  if (i < 0 || i >= (int)s->list.size()) return nullptr;
+  return s->list[i];
 }
 int dSpaceGetNumGeoms(dSpaceID s) { return (int)s->list.size(); }
 
@@ -1007,6 +1044,10 @@ dGeomID dCreatePlane(dSpaceID s, dReal a, dReal b, dReal c, dReal d) {
   return g;
 }
 void dGeomDestroy(dGeomID g) {
+  if (is_space(g)) {
+    dSpaceDestroy(static_cast<dxSpace *>(g));
+    return;
+  }
   if (g->space) dSpaceRemove(g->space, g);
   if (g->body) {
     auto &L = g->body->geoms;
@@ -1078,6 +1119,22 @@ dReal dGeomSphereGetRadius(dGeomID g) { return g->param[0]; }
 void dGeomBoxGetLengths(dGeomID g, dVector3 r) { for (int k = 0; k < 3; k++) r[k] = g->param[k]; }
 void dGeomPlaneGetParams(dGeomID g, dVector4 r) { for (int k = 0; k < 4; k++) r[k] = g->param[k]; }
 void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
+  if (is_space(g)) {
+    // union of the geoms' boxes, zero for an empty space (collision_space.cpp:120-145)
+    const dxSpace *sp = static_cast<const dxSpace *>(g);
+    if (sp->list.empty()) {
+      for (int k = 0; k < 6; k++) aabb[k] = 0;
+      return;
+    }
+    for (int k = 0; k < 6; k++) aabb[k] = (k & 1) ? -INFINITY : INFINITY;
+    for (dxGeom *m : sp->list) {
+      float b[6];
+      dGeomGetAABB(m, b);
+      for (int k = 0; k < 6; k += 2) if (b[k] < aabb[k]) aabb[k] = b[k];
+      for (int k = 1; k < 6; k += 2) if (b[k] > aabb[k]) aabb[k] = b[k];
+    }
+    return;
+  }
   // box.cpp:60-77, sphere.cpp:59-67, plane.cpp:80-107 on the host copies
   const float *p = dGeomGetPosition(g);
   const float *R = dGeomGetRotation(g);
@@ -1108,6 +1165,11 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
     if (s) for (dxGeom *g : s->list) g->dirty = false;
     return;
   }
+  for (dxGeom *g : s->list)
+    if (is_space(g)) {
+      report_error("dSpaceCollide: the space holds a space; collide the spaces with dSpaceCollide2 in this build");
+      return;
+    }
   dxWorld *w = space_world(s);
   if (!w) {
     report_error("dSpaceCollide: the space holds no geom attached to a body");
@@ -1157,7 +1219,6 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
   }
 }
 
-// dCollide (collision_kernel.cpp:292-339)
 // pose and shape of a geom for the device colliders, from the host copies
 static GeomView geom_view(dxGeom *g) {
   GeomView v;
@@ -1252,6 +1313,162 @@ int dCollideNoAlloc(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, in
     return 0;
   }
   return write_pair_contacts(o1, o2, maxc, pts, n, contact, skip);
+}
+
+// ---------------------------------------------------------------- dSpaceCollide2
+
+namespace {
+
+// a pair dSpaceCollide2 reports, in callback argument order; `space` is the
+// space whose geoms were tested (it keeps the contact count dCollide asks for)
+struct Pair2 {
+  dxGeom *a, *b;
+  dxSpace *space;
+};
+
+// collideAABBs (collision_space_internal.h:48-82) without the callback
+bool aabbs_may_touch(const dxGeom *g1, const dxGeom *g2, const float *b1, const float *b2) {
+  if (g1->body == g2->body && g1->body) return false;
+  if (((g1->cat & g2->col) || (g2->cat & g1->col)) == 0) return false;
+  return !(b1[0] > b2[1] || b1[1] < b2[0] || b1[2] > b2[3] || b1[3] < b2[2] || b1[4] > b2[5] || b1[5] < b2[4]);
+}
+
+// cleanGeoms (collision_space.cpp:239-251): the space's geoms, and those of
+// the spaces it holds, are clean once tested
+void clean_space(dxSpace *s) {
+  for (dxGeom *g : s->list) {
+    if (is_space(g)) clean_space(static_cast<dxSpace *>(g));
+    g->dirty = false;
+  }
+}
+
+// dxHashSpace::collide2 (collision_space.cpp:585-603): every geom of S, in
+// list order, against G; `swap` gives the callback (G, g) instead of (g, G)
+void collide2_space_geom(dxSpace *S, dxGeom *G, bool swap, std::vector<Pair2> &out) {
+  clean_space(S);
+  float bg[6];
+  dGeomGetAABB(G, bg);
+  for (dxGeom *g : S->list) {
+    float b[6];
+    dGeomGetAABB(g, b);
+    if (aabbs_may_touch(g, G, b, bg)) out.push_back(swap ? Pair2{G, g, S} : Pair2{g, G, S});
+  }
+}
+
+void pull_geom_world(dxGeom *g) {
+  if (g->body) pull_state(g->body->world);
+  if (is_space(g))
+    for (dxGeom *m : static_cast<dxSpace *>(g)->list) pull_geom_world(m);
+}
+
+}  // namespace
+
+// dSpaceCollide2 (collision.h:845, collision_space.cpp:768-837): the pairs
+// of geoms of g1 and g2 (either may be a space; between two spaces the one
+// with fewer geoms is iterated outside, the callback keeping the (g1-side,
+// g2-side) argument order) whose boxes touch, reported in the reference's
+// order; a space against itself is dSpaceCollide.  The geom-geom pairs'
+// narrowphase runs in one device launch first, so that dCollide inside the
+// callback returns the pair's contacts; a pair holding a space goes to the
+// callback as it is (the callback descends with dSpaceCollide2).
+void dSpaceCollide2(dGeomID g1, dGeomID g2, void *data, dNearCallback *cb) {
+  if (!g1 || !g2 || !cb) return;
+  dxSpace *s1 = is_space(g1) ? static_cast<dxSpace *>(g1) : nullptr;
+  dxSpace *s2 = is_space(g2) ? static_cast<dxSpace *>(g2) : nullptr;
+  if (s1 && s2 && s1->sublevel != s2->sublevel) {
+    if (s1->sublevel > s2->sublevel) s2 = nullptr;
+    else s1 = nullptr;
+  }
+  if (s1 && s2 && s1 == s2) {
+    dSpaceCollide(s1, data, cb);
+    return;
+  }
+  pull_geom_world(g1);
+  pull_geom_world(g2);
+  std::vector<Pair2> pairs;
+  if (s1 && s2) {
+    if (s1->list.size() < s2->list.size()) {
+      std::vector<dxGeom *> outer = s1->list;
+      for (dxGeom *g : outer) collide2_space_geom(s2, g, true, pairs);
+    } else {
+      std::vector<dxGeom *> outer = s2->list;
+      for (dxGeom *g : outer) collide2_space_geom(s1, g, false, pairs);
+    }
+  } else if (s1) {
+    collide2_space_geom(s1, g2, false, pairs);
+  } else if (s2) {
+    collide2_space_geom(s2, g1, true, pairs);
+  } else {
+    float b1[6], b2[6];
+    dGeomGetAABB(g1, b1);
+    dGeomGetAABB(g2, b2);
+    if (aabbs_may_touch(g1, g2, b1, b2)) pairs.push_back(Pair2{g1, g2, nullptr});
+  }
+  if (pairs.empty()) return;
+  // speculative narrowphase of the geom-geom pairs
+  std::vector<int> slot(pairs.size(), -1);
+  std::vector<GeomView> va, vb;
+  int maxc = 1;
+  for (size_t i = 0; i < pairs.size(); i++) {
+    const Pair2 &p = pairs[i];
+    if (is_space(p.a) || is_space(p.b)) continue;
+    slot[i] = (int)va.size();
+    va.push_back(geom_view(p.a));
+    vb.push_back(geom_view(p.b));
+    const int mc = p.space ? p.space->cached_maxc : 4;
+    if (mc > maxc) maxc = mc;
+  }
+  if (maxc > MAX_SLOTS) maxc = MAX_SLOTS;
+  const int np = (int)va.size();
+  std::vector<int> cnt(np, 0);
+  std::vector<CPoint> pts((size_t)np * MAX_SLOTS);
+  if (np > 0) {
+    static DevBuf dva, dvb, dout, dn;
+    static hipStream_t st = nullptr;
+    if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return;
+    if (!dva.alloc(sizeof(GeomView) * np) || !dvb.alloc(sizeof(GeomView) * np) ||
+        !dout.alloc(sizeof(CPoint) * MAX_SLOTS * (size_t)np) || !dn.alloc(sizeof(int) * np)) {
+      report_error("dSpaceCollide2: device allocation failed");
+      return;
+    }
+    if (hipMemcpyAsync(dva.p, va.data(), sizeof(GeomView) * np, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dvb.p, vb.data(), sizeof(GeomView) * np, hipMemcpyHostToDevice, st) != hipSuccess ||
+        launch_collide_list(dva.as<GeomView>(), dvb.as<GeomView>(), np, maxc, dout.as<CPoint>(), dn.as<int>(), st) !=
+            hipSuccess ||
+        hipMemcpyAsync(cnt.data(), dn.p, sizeof(int) * np, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(pts.data(), dout.p, sizeof(CPoint) * pts.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      report_error("dSpaceCollide2: narrowphase kernel failed");
+      return;
+    }
+  }
+  for (size_t i = 0; i < pairs.size(); i++) {
+    const Pair2 &p = pairs[i];
+    CollideCtx saved = g_ctx;
+    std::vector<ContactRec> cr;
+    if (slot[i] >= 0) {
+      const int k = slot[i];
+      for (int j = 0; j < cnt[k]; j++) {
+        const CPoint &c = pts[(size_t)MAX_SLOTS * k + j];
+        ContactRec r;
+        std::memset(&r, 0, sizeof(r));
+        r.pos[0] = c.pos.x; r.pos[1] = c.pos.y; r.pos[2] = c.pos.z; r.depth = c.depth;
+        r.normal[0] = c.n.x; r.normal[1] = c.n.y; r.normal[2] = c.n.z;
+        cr.push_back(r);
+      }
+      g_ctx.active = p.space != nullptr;
+      g_ctx.space = p.space;
+      g_ctx.g1 = p.a;
+      g_ctx.g2 = p.b;
+      g_ctx.con = cr.data();
+      g_ctx.n = (int)cr.size();
+      g_ctx.maxc = maxc;
+    } else {
+      g_ctx.active = false;
+    }
+    cb(data, p.a, p.b);
+    g_ctx = saved;
+  }
 }
 
 // ---------------------------------------------------------------- adjoint (ODE API form)

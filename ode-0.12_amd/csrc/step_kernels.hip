@@ -2340,6 +2340,19 @@ __global__ void collide_one_kernel(GeomView a, GeomView b, int maxc, CPoint *out
   }
 }
 
+// the narrowphase of a list of geom pairs, one pair per lane (the
+// geom-geom pairs dSpaceCollide2 reports): out holds MAX_SLOTS contacts per
+// pair, n the count of each
+__global__ void collide_list_kernel(const GeomView *a, const GeomView *b, int npairs, int maxc, CPoint *out,
+                                    int *n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  CPoint pts[MAX_SLOTS];
+  const int k = collide_pair(a[i], b[i], maxc > MAX_SLOTS ? MAX_SLOTS : maxc, pts);
+  for (int j = 0; j < k; j++) out[(size_t)MAX_SLOTS * i + j] = pts[j];
+  n[i] = k;
+}
+
 // body-frame setup: inverse inertia is computed on the host; here the
 // rotation matrices are derived from the quaternions
 __global__ void init_rot_kernel(DevBatch B, int nbodies) {
@@ -2437,6 +2450,13 @@ hipError_t launch_world_kernel(int mode, const DevBatch &B0, int first_world, in
     case KM_SOR: sor_kernel<true><<<blocks, wg, lds, st>>>(B, first_world, persistent); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_collide_list(const GeomView *a, const GeomView *b, int npairs, int maxc, CPoint *out, int *n,
+                               hipStream_t st) {
+  if (npairs <= 0) return hipSuccess;
+  collide_list_kernel<<<(npairs + 63) / 64, 64, 0, st>>>(a, b, npairs, maxc, out, n);
   return hipGetLastError();
 }
 

@@ -80,6 +80,7 @@ class Scene:
     joint_axis2: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     joint_param: np.ndarray = field(default_factory=lambda: np.zeros((0, NJP)))
     step_ctrl: np.ndarray | None = None   # (nsteps, nj, 4): lostop, histop, torque1, torque2
+    geom_space: np.ndarray | None = None  # per geom 0 (dynamic space) or 1 (static terrain space)
 
     @property
     def nb(self) -> int:
@@ -352,12 +353,14 @@ def uneven_worlds(n_worlds: int = 4096, seed: int = 7, mean: float = 64.0, lo: i
     return worlds
 
 
-def vehicle_world(rng: np.random.Generator, nsteps: int, n_ramps: int = 32, seed: int = 0) -> Scene:
+def vehicle_world(rng: np.random.Generator, nsteps: int, n_ramps: int = 32, seed: int = 0,
+                  static_space: bool = False) -> Scene:
     """A 4x2x0.5 m, 1000 kg chassis on four r=0.35 m, 20 kg sphere wheels with
     hinge2 joints (axis1 = z steering, axis2 = y spin, as demo_buggy.cpp:
     271-278); suspension ERP/CFM from k=3e4 N/m, c=2e3 N s/m at h=0.01; 32
     static ramp boxes; per-step steer stop U[-0.5,0.5] rad on the front wheels
-    and wheel torque U[-500,500] N m; mu=1.0."""
+    and wheel torque U[-500,500] N m; mu=1.0.  static_space: the plane and
+    the ramps sit in a static space of their own."""
     h = 0.01
     kp, kd = 3e4, 2e3
     susp_erp = h * kp / (h * kp + kd)
@@ -397,6 +400,10 @@ def vehicle_world(rng: np.random.Generator, nsteps: int, n_ramps: int = 32, seed
         ctrl[:, k, 1] = steer if front else 0.0
         ctrl[:, k, 3] = torque
     sc.step_ctrl = ctrl
+    if static_space:
+        # the plane and the ramps in a static terrain space, collided against
+        # the car's space with dSpaceCollide2 (the raycar frame)
+        sc.geom_space = np.where(sc.geom_body < 0, 1, 0).astype(np.int32)
     return sc
 
 

@@ -133,6 +133,7 @@ static void build_world(const SceneDesc *s, World<R> &w) {
     R p[4];
     for (int k = 0; k < 4; k++) p[k] = (R)s->geom_param[4 * (size_t)i + k];
     int g = w.add_geom(s->geom_type[i], p);
+    if (s->geom_space && s->geom_space[i] == 1) w.move_to_space2(g);
     if (s->geom_type[i] != SC_PLANE) {
       if (s->geom_body[i] >= 0) w.geom_set_body(g, s->geom_body[i]);
       else {
@@ -240,6 +241,13 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     auto t0 = clk::now();
     w.hash_pairs(pairs);
     if (g_colored) colored_pair_order(pairs);
+    if (!w.space2.empty()) {
+      // dSpaceCollide2(space 0, space 1) runs first in the frame
+      std::vector<std::pair<int, int>> p2;
+      w.collide2_pairs(p2);
+      p2.insert(p2.end(), pairs.begin(), pairs.end());
+      pairs.swap(p2);
+    }
     int step_contacts = 0;
     for (auto &pr : pairs) {
       int g1 = pr.first, g2 = pr.second;

@@ -361,6 +361,33 @@ template <class R> struct World {
       return false;
     return true;
   }
+  // A second top-level space (geom ids in list order), collided against the
+  // first with dSpaceCollide2 (collision_space.cpp:768-837): the space with
+  // fewer geoms is iterated outside, each of its geoms tested against every
+  // geom of the other in list order (dxHashSpace::collide2,
+  // collision_space.cpp:585-603); the pair keeps the (first space, second
+  // space) argument order.  Appends to out.
+  std::vector<int> space2;
+  void collide2_pairs(std::vector<std::pair<int, int>> &out) {
+    for (int gi : space) geom_aabb(shape(gi), geoms[gi].aabb);
+    for (int gi : space2) geom_aabb(shape(gi), geoms[gi].aabb);
+    if (space.size() < space2.size()) {
+      for (int a : space)
+        for (int b : space2)
+          if (aabb_pass(b, a)) out.push_back({a, b});
+    } else {
+      for (int b : space2)
+        for (int a : space)
+          if (aabb_pass(a, b)) out.push_back({a, b});
+    }
+  }
+  // moves a geom from the first space's list to the front of the second's
+  // (a geom created in the second space: list prepend, collision_kernel.h:166-171)
+  void move_to_space2(int gi) {
+    auto it = std::find(space.begin(), space.end(), gi);
+    if (it != space.end()) space.erase(it);
+    space2.insert(space2.begin(), gi);
+  }
   void hash_pairs(std::vector<std::pair<int, int>> &out) {
     out.clear();
     for (int gi : space) geom_aabb(shape(gi), geoms[gi].aabb);
@@ -1249,10 +1276,11 @@ template <class R> struct World {
     islands_j.clear();
     // new space list: moved geoms in reverse order of their move, then the
     // untouched ones in their old relative order
-    std::vector<char> is_moved(geoms.size(), 0);
+    std::vector<char> is_moved(geoms.size(), 0), in_first(geoms.size(), 0);
+    for (int g : space) in_first[g] = 1;
     std::vector<int> ns;
     for (auto it = moved.rbegin(); it != moved.rend(); ++it) {
-      if (!is_moved[*it]) { is_moved[*it] = 1; ns.push_back(*it); }
+      if (in_first[*it] && !is_moved[*it]) { is_moved[*it] = 1; ns.push_back(*it); }
     }
     for (int g : space) if (!is_moved[g]) ns.push_back(g);
     space.swap(ns);

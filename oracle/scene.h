@@ -68,6 +68,12 @@ typedef struct SceneDesc {
   /* per-step controls, may be NULL: nsteps*nj*4 = lostop, histop,
      torque1, torque2 (NaN lostop/histop = unchanged) */
   const double *step_ctrl;
+  /* may be NULL: per geom 0 or 1.  With a geom in space 1 the run uses two
+     top-level hash spaces (space 1 holding static geoms), and every frame
+     collides them with dSpaceCollide2(space 0, space 1) before
+     dSpaceCollide(space 0), as an application with a static terrain space
+     does (demos/raycar/ode.cpp:118-123) */
+  const int *geom_space;
 } SceneDesc;
 
 typedef struct SceneOut {

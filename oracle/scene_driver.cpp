@@ -39,6 +39,15 @@ struct Ctx {
 inline int geom_index(dGeomID g) { return (int)(intptr_t)dGeomGetData(g) - 1; }
 
 void near_callback(void *data, dGeomID o1, dGeomID o2) {
+  // a space met in dSpaceCollide2: descend (demos/raycar/ode.cpp:50-61)
+  if (dGeomIsSpace(o1)) {
+    dSpaceCollide2(o1, o2, data, &near_callback);
+    return;
+  }
+  if (dGeomIsSpace(o2)) {
+    dSpaceCollide2(o2, o1, data, &near_callback);
+    return;
+  }
   Ctx *c = (Ctx *)data;
   const SceneDesc *s = c->s;
   SceneOut *o = c->o;
@@ -135,6 +144,12 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
   dWorldSetContactSurfaceLayer(world, (dReal)s->min_depth);
   dSpaceID space = dHashSpaceCreate(0);
   dHashSpaceSetLevels(space, s->hash_minlevel, s->hash_maxlevel);
+  // a static terrain space collided against the dynamic one
+  // (demos/raycar/ode.cpp:42-48, 118-123)
+  dSpaceID space2 = 0;
+  for (int i = 0; s->geom_space && i < s->ng && !space2; i++)
+    if (s->geom_space[i] == 1) space2 = dHashSpaceCreate(0);
+  if (space2) dHashSpaceSetLevels(space2, s->hash_minlevel, s->hash_maxlevel);
   dJointGroupID cgroup = dJointGroupCreate(0);
 
   std::vector<dBodyID> bodies(s->nb);
@@ -159,11 +174,12 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
   }
   for (int i = 0; i < s->ng; i++) {
     const double *p = s->geom_param + 4 * (size_t)i;
+    dSpaceID gs = (space2 && s->geom_space[i] == 1) ? space2 : space;
     dGeomID g = 0;
     switch (s->geom_type[i]) {
-      case SC_SPHERE: g = dCreateSphere(space, (dReal)p[0]); break;
-      case SC_BOX: g = dCreateBox(space, (dReal)p[0], (dReal)p[1], (dReal)p[2]); break;
-      case SC_PLANE: g = dCreatePlane(space, (dReal)p[0], (dReal)p[1], (dReal)p[2], (dReal)p[3]); break;
+      case SC_SPHERE: g = dCreateSphere(gs, (dReal)p[0]); break;
+      case SC_BOX: g = dCreateBox(gs, (dReal)p[0], (dReal)p[1], (dReal)p[2]); break;
+      case SC_PLANE: g = dCreatePlane(gs, (dReal)p[0], (dReal)p[1], (dReal)p[2], (dReal)p[3]); break;
     }
     dGeomSetData(g, (void *)(intptr_t)(i + 1));
     if (s->geom_type[i] != SC_PLANE) {
@@ -254,6 +270,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     if (o->grad_out && step == o->nsteps - 1) dWorldSetTape(world, 1);
 #endif
     auto t0 = clk::now();
+    if (space2) dSpaceCollide2((dGeomID)space, (dGeomID)space2, &c, &near_callback);
     dSpaceCollide(space, &c, &near_callback);
     auto t1 = clk::now();
     int ok = dWorldQuickStep(world, (dReal)s->dt);
@@ -299,6 +316,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
 #endif
   dJointGroupDestroy(cgroup);
   dSpaceDestroy(space);
+  if (space2) dSpaceDestroy(space2);
   dWorldDestroy(world);
   dCloseODE();
   return o->status;

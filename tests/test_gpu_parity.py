@@ -347,7 +347,7 @@ def _tie_order(con, ref):
     return perm
 
 
-@pytest.mark.parametrize("name", ["box_stack", "chains", "surfaces", "vehicle"])
+@pytest.mark.parametrize("name", ["box_stack", "chains", "surfaces", "vehicle", "vehicle_spaces"])
 def test_dropin_joint_feedback_every_step(odemi_mod, name):
     """dJointFeedback written back by dWorldQuickStep (quickstep.cpp:918-957)
     for every contact joint of every diagnostic step of the drop-in run,
