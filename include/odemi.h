@@ -283,6 +283,9 @@ ODE_API int dCollideNoAlloc(dGeomID o1, dGeomID o2, int flags, dContactGeom *con
 ODE_API dGeomID dCreateSphere(dSpaceID space, dReal radius);                  /* collision.h:924 */
 ODE_API dGeomID dCreateBox(dSpaceID space, dReal lx, dReal ly, dReal lz);     /* collision.h:1002 */
 ODE_API dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d); /* collision.h:1046 */
+ODE_API dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length);  /* collision.h:1051 */
+ODE_API void dGeomCapsuleSetParams(dGeomID ccylinder, dReal radius, dReal length); /* collision.h:1052 */
+ODE_API void dGeomCapsuleGetParams(dGeomID ccylinder, dReal *radius, dReal *length); /* collision.h:1053 */
 ODE_API void dGeomDestroy(dGeomID g);                                         /* collision.h:68 */
 ODE_API void dGeomSetData(dGeomID g, void *data);                             /* collision.h:78 */
 ODE_API void *dGeomGetData(dGeomID g);                                        /* collision.h:87 */

@@ -27,6 +27,11 @@ bool world_supported(const BatchImpl *b, int w, char *why, size_t n) {
     snprintf(why, n, "world %d has %d permanent joints", w, W.nj);
     return false;
   }
+  for (int g = 0; g < W.ng; g++)
+    if (b->h_gtype[W.g0 + g] == GEOM_CAPSULE) {
+      snprintf(why, n, "world %d has a capsule geom (the adjoint colliders are sphere, box and plane)", w);
+      return false;
+    }
   int other = P.surf_mode & ~(SM_BOUNCE | SM_APPROX1_1 | SM_APPROX1_2);
   if (other) {
     snprintf(why, n, "world %d uses contact surface mode bits 0x%x", w, other);

@@ -284,6 +284,7 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   }
   b->nb = (int)A.pos.size();
   b->ng = (int)A.gtype.size();
+  b->h_gtype = A.gtype;
   b->nj = (int)A.joints.size();
   if (!A.wp.empty()) b->cell_scale = A.wp[0].cell_scale;
   derive_lists(A);
@@ -417,7 +418,7 @@ static bool validate_desc(const dBatchDesc *D) {
       const long long gg = g0 + g;
       const int t = D->geom_type[gg], body = D->geom_body[gg];
       const double *p = D->geom_param + 4 * gg;
-      if (t != GEOM_SPHERE && t != GEOM_BOX && t != GEOM_PLANE) {
+      if (t != GEOM_SPHERE && t != GEOM_BOX && t != GEOM_PLANE && t != GEOM_CAPSULE) {
         report_error("dBatchCreate: world %d geom %d: unknown type %d", w, g, t);
         return false;
       }
@@ -425,7 +426,7 @@ static bool validate_desc(const dBatchDesc *D) {
         report_error("dBatchCreate: world %d geom %d: body %d out of range (planes take none)", w, g, body);
         return false;
       }
-      const int nsz = t == GEOM_SPHERE ? 1 : (t == GEOM_BOX ? 3 : 0);
+      const int nsz = t == GEOM_SPHERE ? 1 : (t == GEOM_BOX ? 3 : (t == GEOM_CAPSULE ? 2 : 0));
       for (int k = 0; k < nsz; k++)
         if (!(p[k] >= 0 && std::isfinite(p[k]))) {
           report_error("dBatchCreate: world %d geom %d: size must be >= 0 and finite", w, g);

@@ -12,7 +12,7 @@ struct GeomView {
   int type;
   float3 p;      // world position (unused for planes)
   Mat3 R;        // world rotation
-  float4 prm;    // radius | lx,ly,lz | plane a,b,c,d (unit normal)
+  float4 prm;    // radius | lx,ly,lz | plane a,b,c,d (unit normal) | capsule radius, length
 };
 
 struct CPoint {
@@ -35,6 +35,14 @@ __device__ inline void geom_aabb(const GeomView &g, float *a) {
     float ex = 0.5f * (fabsf(g.R.r0.x * s.x) + fabsf(g.R.r0.y * s.y) + fabsf(g.R.r0.z * s.z));
     float ey = 0.5f * (fabsf(g.R.r1.x * s.x) + fabsf(g.R.r1.y * s.y) + fabsf(g.R.r1.z * s.z));
     float ez = 0.5f * (fabsf(g.R.r2.x * s.x) + fabsf(g.R.r2.y * s.y) + fabsf(g.R.r2.z * s.z));
+    a[0] = g.p.x - ex; a[1] = g.p.x + ex;
+    a[2] = g.p.y - ey; a[3] = g.p.y + ey;
+    a[4] = g.p.z - ez; a[5] = g.p.z + ez;
+  } else if (g.type == GEOM_CAPSULE) {
+    // capsule.cpp:60-74: the axis is column 2 of R
+    float ex = fabsf(g.R.r0.z * g.prm.y) * 0.5f + g.prm.x;
+    float ey = fabsf(g.R.r1.z * g.prm.y) * 0.5f + g.prm.x;
+    float ez = fabsf(g.R.r2.z * g.prm.y) * 0.5f + g.prm.x;
     a[0] = g.p.x - ex; a[1] = g.p.x + ex;
     a[2] = g.p.y - ey; a[3] = g.p.y + ey;
     a[4] = g.p.z - ez; a[5] = g.p.z + ez;
@@ -434,13 +442,220 @@ __device__ __noinline__ int coll_box_box(float3 p1, const Mat3 &R1, float3 side1
 
 // dCollide dispatch (collision_kernel.cpp:292-339): sphere/box come first in
 // the collider argument order; a reversed call negates the normals.
+// ------------------------------------------------------------ capsules
+// A capsule (prm: radius, length) lies along column 2 of its rotation.
+
+// closest points of segments a1-a2 and b1-b2 (collision_util.cpp:109-240):
+// the vertex-vertex, edge-vertex and edge-edge features in the reference's
+// order of tests
+__device__ inline void closest_segment_points(float3 a1, float3 a2, float3 b1, float3 b2, float3 &cp1,
+                                              float3 &cp2) {
+  const float3 a1a2 = a2 - a1, b1b2 = b2 - b1, a1b1 = b1 - a1;
+  const float da1 = dot(a1a2, a1b1), db1 = dot(b1b2, a1b1);
+  if (da1 <= 0.f && db1 >= 0.f) { cp1 = a1; cp2 = b1; return; }
+  const float3 a1b2 = b2 - a1;
+  const float da2 = dot(a1a2, a1b2), db2 = dot(b1b2, a1b2);
+  if (da2 <= 0.f && db2 <= 0.f) { cp1 = a1; cp2 = b2; return; }
+  const float3 a2b1 = b1 - a2;
+  const float da3 = dot(a1a2, a2b1), db3 = dot(b1b2, a2b1);
+  if (da3 >= 0.f && db3 >= 0.f) { cp1 = a2; cp2 = b1; return; }
+  const float3 a2b2 = b2 - a2;
+  const float da4 = dot(a1a2, a2b2), db4 = dot(b1b2, a2b2);
+  if (da4 >= 0.f && db4 <= 0.f) { cp1 = a2; cp2 = b2; return; }
+  const float la = dot(a1a2, a1a2);
+  if (da1 >= 0.f && da3 <= 0.f) {
+    const float k = da1 / la;
+    if (dot(b1b2, a1b1 - k * a1a2) >= 0.f) { cp1 = a1 + k * a1a2; cp2 = b1; return; }
+  }
+  if (da2 >= 0.f && da4 <= 0.f) {
+    const float k = da2 / la;
+    if (dot(b1b2, a1b2 - k * a1a2) <= 0.f) { cp1 = a1 + k * a1a2; cp2 = b2; return; }
+  }
+  const float lb = dot(b1b2, b1b2);
+  if (db1 <= 0.f && db2 >= 0.f) {
+    const float k = -db1 / lb;
+    if (dot(a1a2, -a1b1 - k * b1b2) >= 0.f) { cp1 = a1; cp2 = b1 + k * b1b2; return; }
+  }
+  if (db3 <= 0.f && db4 >= 0.f) {
+    const float k = -db3 / lb;
+    if (dot(a1a2, -a2b1 - k * b1b2) <= 0.f) { cp1 = a2; cp2 = b1 + k * b1b2; return; }
+  }
+  const float k = dot(a1a2, b1b2);
+  float det = la * lb - k * k;
+  if (det <= 0.f) { cp1 = a1; cp2 = b1; return; }
+  det = 1.f / det;
+  const float alpha = (lb * da1 - k * db1) * det;
+  const float beta = (k * da1 - la * db1) * det;
+  cp1 = a1 + alpha * a1a2;
+  cp2 = b1 + beta * b1b2;
+}
+
+// the point of segment p1-p2 closest to a box and the box point closest to
+// it (collision_util.cpp:247-375): the derivative of the squared distance
+// along the segment is piecewise linear in box coordinates; walk its pieces
+// until it turns non-negative
+__device__ inline void closest_line_box_points(float3 p1, float3 p2, float3 c, const Mat3 &R, float3 side,
+                                               float3 &lret, float3 &bret) {
+  const float3 d = p2 - p1;
+  float s[3], v[3], sg[3], v2[3], h[3], ta[3];
+  int rg[3];
+  {
+    const float3 s3 = R.tmul(p1 - c), v3 = R.tmul(d);
+    s[0] = s3.x; s[1] = s3.y; s[2] = s3.z;
+    v[0] = v3.x; v[1] = v3.y; v[2] = v3.z;
+  }
+  for (int i = 0; i < 3; i++) {
+    if (v[i] < 0.f) { s[i] = -s[i]; v[i] = -v[i]; sg[i] = -1.f; }
+    else sg[i] = 1.f;
+    v2[i] = v[i] * v[i];
+  }
+  h[0] = 0.5f * side.x; h[1] = 0.5f * side.y; h[2] = 0.5f * side.z;
+  for (int i = 0; i < 3; i++) {
+    if (v[i] > 1e-19f) {
+      if (s[i] < -h[i]) { rg[i] = -1; ta[i] = (-h[i] - s[i]) / v[i]; }
+      else { rg[i] = (s[i] > h[i]) ? 1 : 0; ta[i] = (h[i] - s[i]) / v[i]; }
+    } else {
+      rg[i] = 0;
+      ta[i] = 2.f;
+    }
+  }
+  float t = 0.f, dd2dt = 0.f;
+  for (int i = 0; i < 3; i++) dd2dt -= (rg[i] ? v2[i] : 0.f) * ta[i];
+  if (!(dd2dt >= 0.f)) {
+    // each pass crosses at least one clip plane (at most six in all)
+    for (int pass = 0; pass < 8; pass++) {
+      float next_t = 1.f;
+      for (int i = 0; i < 3; i++)
+        if (ta[i] > t && ta[i] < 1.f && ta[i] < next_t) next_t = ta[i];
+      float nd = 0.f;
+      for (int i = 0; i < 3; i++) nd += (rg[i] ? v2[i] : 0.f) * (next_t - ta[i]);
+      if (nd >= 0.f) {
+        const float m = (nd - dd2dt) / (next_t - t);
+        t -= dd2dt / m;
+        break;
+      }
+      for (int i = 0; i < 3; i++)
+        if (ta[i] == next_t) { ta[i] = (h[i] - s[i]) / v[i]; rg[i]++; }
+      t = next_t;
+      dd2dt = nd;
+      if (!(t < 1.f)) { t = 1.f; break; }
+    }
+  }
+  lret = p1 + t * d;
+  float b[3];
+  for (int i = 0; i < 3; i++) {
+    b[i] = sg[i] * (s[i] + t * v[i]);
+    if (b[i] < -h[i]) b[i] = -h[i];
+    else if (b[i] > h[i]) b[i] = h[i];
+  }
+  bret = R.mul(mk3(b[0], b[1], b[2])) + c;
+}
+
+__device__ inline float3 capsule_axis(const Mat3 &R) { return mk3(R.r0.z, R.r1.z, R.r2.z); }
+
+// capsule-sphere: capsule.cpp:130-162
+__device__ inline int coll_capsule_sphere(const GeomView &cap, float3 ps, float rs, CPoint *c) {
+  const float3 ax = capsule_axis(cap.R);
+  float alpha = ax.x * (ps.x - cap.p.x) + ax.y * (ps.y - cap.p.y) + ax.z * (ps.z - cap.p.z);
+  const float lz2 = cap.prm.y * 0.5f;
+  if (alpha > lz2) alpha = lz2;
+  if (alpha < -lz2) alpha = -lz2;
+  return coll_sphere_sphere(cap.p + alpha * ax, cap.prm.x, ps, rs, c);
+}
+
+// capsule-box: capsule.cpp:178-231
+__device__ inline int coll_capsule_box(const GeomView &cap, const GeomView &box, CPoint *c) {
+  const float3 ax = capsule_axis(cap.R);
+  const float clen = cap.prm.y * 0.5f;
+  const float3 p1 = cap.p + clen * ax, p2 = cap.p - clen * ax;
+  float3 pl, pb;
+  closest_line_box_points(p1, p2, box.p, box.R, mk3(box.prm.x, box.prm.y, box.prm.z), pl, pb);
+  const float3 d = pl - pb;
+  const float rad = cap.prm.x;
+  if (sqrtf(d.x * d.x + d.y * d.y + d.z * d.z) < 1e-9f) {
+    // the axis point lies in the box: the normal points from the box
+    // centre (dCollideSpheresZeroDist, capsule.cpp:165-176)
+    c->n = safe_normalize(pb - box.p);
+    c->depth = rad + 0.f;
+    c->pos = pl + (0.5f * (0.f - rad)) * c->n;
+    return 1;
+  }
+  return coll_sphere_sphere(pl, rad, pb, 0.f, c);
+}
+
+// capsule-capsule: capsule.cpp:234-349 -- two contacts along nearly parallel
+// axes when both are made, else one at the middle of the overlap, else the
+// closest points of the axis segments
+__device__ inline int coll_capsule_capsule(const GeomView &g1, const GeomView &g2, int maxc, CPoint *c) {
+  const float tolerance = 1e-5f;
+  const float lz1 = g1.prm.y * 0.5f, lz2 = g2.prm.y * 0.5f;
+  const float3 axis1 = capsule_axis(g1.R);
+  float3 axis2 = capsule_axis(g2.R);
+  const float r1 = g1.prm.x, r2 = g2.prm.x;
+  const float a1a2 = dot(axis1, axis2);
+  const float det = 1.0f - a1a2 * a1a2;
+  if (det < tolerance) {
+    if (a1a2 < 0.f) axis2 = -axis2;
+    const float k = dot(axis1, g1.p - g2.p);
+    const float a1lo = -lz1, a1hi = lz1, a2lo = -lz2 - k, a2hi = lz2 - k;
+    const float lo = (a1lo > a2lo) ? a1lo : a2lo;
+    const float hi = (a1hi < a2hi) ? a1hi : a2hi;
+    if (lo <= hi) {
+      if (maxc >= 2 && lo < hi) {
+        if (coll_sphere_sphere(g1.p + lo * axis1, r1, g2.p + (lo + k) * axis2, r2, c) &&
+            coll_sphere_sphere(g1.p + hi * axis1, r1, g2.p + (hi + k) * axis2, r2, c + 1))
+          return 2;
+      }
+      const float alpha1 = (lo + hi) * 0.5f;
+      const float alpha2 = alpha1 + k;
+      return coll_sphere_sphere(g1.p + alpha1 * axis1, r1, g2.p + alpha2 * axis2, r2, c);
+    }
+  }
+  float3 s1, s2;
+  closest_segment_points(g1.p + lz1 * axis1, g1.p - lz1 * axis1, g2.p + lz2 * axis2, g2.p - lz2 * axis2, s1, s2);
+  return coll_sphere_sphere(s1, r1, s2, r2, c);
+}
+
+// capsule-plane: capsule.cpp:352-409 -- the deeper end cap, then the other
+__device__ inline int coll_capsule_plane(const GeomView &cap, float4 pl, int maxc, CPoint *c) {
+  const float3 n = mk3(pl.x, pl.y, pl.z);
+  const float3 ax = capsule_axis(cap.R);
+  const float sign = (dot(n, ax) > 0.f) ? -1.0f : 1.0f;
+  const float rad = cap.prm.x;
+  float3 p = cap.p + (cap.prm.y * 0.5f * sign) * ax;
+  float depth = pl.w - dot(p, n) + rad;
+  if (depth < 0.f) return 0;
+  c[0].n = n;
+  c[0].pos = p - rad * n;
+  c[0].depth = depth;
+  if (maxc < 2) return 1;
+  p = cap.p - (cap.prm.y * 0.5f * sign) * ax;
+  depth = pl.w - dot(p, n) + rad;
+  if (depth < 0.f) return 1;
+  c[1].n = n;
+  c[1].pos = p - rad * n;
+  c[1].depth = depth;
+  return 2;
+}
+
+// the class that comes first in a collider's arguments (the colliders table,
+// collision_kernel.cpp:181-189): capsule, then sphere, box, plane
+__device__ inline int collider_rank(int type) {
+  return type == GEOM_CAPSULE ? 0 : (type == GEOM_SPHERE ? 1 : (type == GEOM_BOX ? 2 : 3));
+}
+
 __device__ inline int collide_pair(const GeomView &a, const GeomView &b, int maxc, CPoint *c) {
   if (maxc <= 0) return 0;
-  bool rev = a.type > b.type;
+  bool rev = collider_rank(a.type) > collider_rank(b.type);
   const GeomView &x = rev ? b : a;
   const GeomView &y = rev ? a : b;
   int n = 0;
-  if (x.type == GEOM_SPHERE) {
+  if (x.type == GEOM_CAPSULE) {
+    if (y.type == GEOM_SPHERE) n = coll_capsule_sphere(x, y.p, y.prm.x, c);
+    else if (y.type == GEOM_BOX) n = coll_capsule_box(x, y, c);
+    else if (y.type == GEOM_CAPSULE) n = coll_capsule_capsule(x, y, maxc, c);
+    else n = coll_capsule_plane(x, y.prm, maxc, c);
+  } else if (x.type == GEOM_SPHERE) {
     if (y.type == GEOM_SPHERE) n = coll_sphere_sphere(x.p, x.prm.x, y.p, y.prm.x, c);
     else if (y.type == GEOM_BOX) n = coll_sphere_box(x.p, x.prm.x, y.p, y.R, mk3(y.prm.x, y.prm.y, y.prm.z), c);
     else n = coll_sphere_plane(x.p, x.prm.x, y.prm, c);

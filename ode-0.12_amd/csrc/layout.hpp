@@ -21,7 +21,7 @@ struct float4 { float x, y, z, w; };
 
 namespace odemi {
 
-enum { GEOM_SPHERE = 0, GEOM_BOX = 1, GEOM_PLANE = 2 };
+enum { GEOM_SPHERE = 0, GEOM_BOX = 1, GEOM_PLANE = 2, GEOM_CAPSULE = 3 };
 enum { JT_CONTACT = 0, JT_BALL = 1, JT_HINGE = 2, JT_HINGE2 = 3 };
 enum { ORDER_REFERENCE = 0, ORDER_COLORED = 1 };
 enum { JF_REVERSE = 2 };

@@ -113,6 +113,17 @@ struct dxSpace : dxGeom {
 
 inline bool is_space(const dxGeom *g) { return g && g->cls >= dFirstSpaceClass && g->cls <= dLastSpaceClass; }
 
+// device geom type of a geom class (-1: no collider on the device)
+inline int device_type(int cls) {
+  switch (cls) {
+    case dSphereClass: return GEOM_SPHERE;
+    case dBoxClass: return GEOM_BOX;
+    case dPlaneClass: return GEOM_PLANE;
+    case dCapsuleClass: return GEOM_CAPSULE;
+    default: return -1;
+  }
+}
+
 struct dxJoint {
   int type;                                // dJointType*
   dxWorld *world;
@@ -279,8 +290,7 @@ int build_mirror(dxWorld *w, dxSpace *sp) {
   for (size_t i = 0; i < w->mgeoms.size(); i++) {
     dxGeom *g = w->mgeoms[i];
     g->mindex = (int)i;
-    int t = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
-    A.gtype.push_back(t);
+    A.gtype.push_back(device_type(g->cls));
     A.gbody.push_back(g->body ? g->body->mindex : -1);
     A.gparam.push_back(make_float4(g->param[0], g->param[1], g->param[2], g->param[3]));
     A.gpos.push_back(make_float4(g->pos[0], g->pos[1], g->pos[2], 0));
@@ -434,6 +444,8 @@ bool tape_prepare(dxWorld *w, const std::vector<ContactIn> &cin) {
     return false;
   };
   if (!w->mjoints.empty()) return refuse("the world has %zu ball/hinge/hinge2 joints", w->mjoints.size());
+  for (dxGeom *g : w->mgeoms)
+    if (g->cls == dCapsuleClass) return refuse("the world has a capsule geom (the adjoint colliders are sphere, box and plane)");
   for (dxBody *b : w->bodies)
     for (int k = 0; k < 3; k++)
       if (b->facc[k] != 0 || b->tacc[k] != 0) return refuse("a body carries a force or torque besides gravity");
@@ -1043,6 +1055,27 @@ dGeomID dCreatePlane(dSpaceID s, dReal a, dReal b, dReal c, dReal d) {
   }
   return g;
 }
+dGeomID dCreateCapsule(dSpaceID s, dReal radius, dReal length) {   // capsule.cpp:77-80
+  dxGeom *g = make_geom(s, dCapsuleClass);
+  g->param[0] = radius;
+  g->param[1] = length;
+  return g;
+}
+void dGeomCapsuleSetParams(dGeomID g, dReal radius, dReal length) {
+  if (g->cls != dCapsuleClass) {
+    report_debug("argument not a ccylinder");
+    return;
+  }
+  g->param[0] = radius;
+  g->param[1] = length;
+  geom_moved(g);
+  dxWorld *w = g->body ? g->body->world : (g->space ? space_world(g->space) : nullptr);
+  if (w) mark_struct(w);
+}
+void dGeomCapsuleGetParams(dGeomID g, dReal *radius, dReal *length) {
+  *radius = g->param[0];
+  *length = g->param[1];
+}
 void dGeomDestroy(dGeomID g) {
   if (is_space(g)) {
     dSpaceDestroy(static_cast<dxSpace *>(g));
@@ -1147,6 +1180,12 @@ void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
       aabb[2 * k] = p[k] - e;
       aabb[2 * k + 1] = p[k] + e;
     }
+  } else if (g->cls == dCapsuleClass) {   // capsule.cpp:60-74
+    for (int k = 0; k < 3; k++) {
+      float e = std::fabs(R[4 * k + 2] * g->param[1]) * 0.5f + g->param[0];
+      aabb[2 * k] = p[k] - e;
+      aabb[2 * k + 1] = p[k] + e;
+    }
   } else {
     for (int k = 0; k < 6; k++) aabb[k] = (k & 1) ? INFINITY : -INFINITY;
     const float *n = g->param;
@@ -1222,7 +1261,7 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
 // pose and shape of a geom for the device colliders, from the host copies
 static GeomView geom_view(dxGeom *g) {
   GeomView v;
-  v.type = g->cls == dSphereClass ? GEOM_SPHERE : (g->cls == dBoxClass ? GEOM_BOX : GEOM_PLANE);
+  v.type = device_type(g->cls);
   const float *p = g->body ? g->body->pos : g->pos;
   const float *R = g->body ? g->body->R : g->R;
   v.p = make_float3(p[0], p[1], p[2]);
@@ -1240,6 +1279,11 @@ static GeomView geom_view(dxGeom *g) {
 static int collide_host_answer(dGeomID o1, dGeomID o2, int maxc, dContactGeom *contact, int skip, int *n) {
   *n = 0;
   if (maxc <= 0 || o1 == o2) return 1;
+  if (device_type(o1->cls) < 0 || device_type(o2->cls) < 0) {
+    report_error("dCollide: geom classes %d and %d have no collider in this build (spaces are collided with "
+                 "dSpaceCollide2)", o1->cls, o2->cls);
+    return 1;
+  }
   if (o1->body && o1->body == o2->body) return 1;
   if (g_ctx.active && g_ctx.g1 == o1 && g_ctx.g2 == o2 && g_ctx.space) {
     int want = maxc > MAX_SLOTS ? MAX_SLOTS : maxc;

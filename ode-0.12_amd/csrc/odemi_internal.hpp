@@ -50,6 +50,7 @@ struct BatchImpl {
   int order_mode = ORDER_COLORED;
   std::vector<WorldInfo> wi;
   std::vector<WorldParams> wp;
+  std::vector<int> h_gtype;   // geom types (host copy)
   BatchBufs d;
   DevBatch dev{};
   hipStream_t stream = nullptr;

@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-SPHERE, BOX, PLANE = 0, 1, 2
+SPHERE, BOX, PLANE, CAPSULE = 0, 1, 2, 3
 BALL, HINGE, HINGE2 = 1, 2, 3
 
 # contact surface mode bits (include/ode/contact.h:33-49)
@@ -129,6 +129,18 @@ class _Builder:
         self.geom(SPHERE, (r, 0.0, 0.0, 0.0), b)
         return b
 
+    def capsule_body(self, pos, radius, length, density, **kw) -> int:
+        """A capsule along its body z axis; mass and inertia of
+        dMassSetCapsule (mass.cpp:141-162)."""
+        m1 = math.pi * radius * radius * length * density
+        m2 = (4.0 / 3.0) * math.pi * radius ** 3 * density
+        ia = m1 * (0.25 * radius * radius + length * length / 12.0) + \
+            m2 * (0.4 * radius * radius + 0.375 * radius * length + 0.25 * length * length)
+        ib = (m1 * 0.5 + m2 * 0.4) * radius * radius
+        b = self.body(pos, m1 + m2, np.diag([ia, ia, ib]), **kw)
+        self.geom(CAPSULE, (radius, length, 0.0, 0.0), b)
+        return b
+
     def geom(self, gtype, param, body=-1, pos=(0.0, 0.0, 0.0), q=(1.0, 0.0, 0.0, 0.0)) -> int:
         self.g_type.append(gtype)
         self.g_body.append(body)
@@ -199,6 +211,44 @@ def surface_pile(seed: int = 13, mu: float = 0.7, mu2: float = 0.3) -> Scene:
     sc.bounce, sc.bounce_vel = 0.3, 0.05
     sc.soft_erp, sc.soft_cfm = 0.5, 1e-4
     return sc
+
+
+def capsule_pile(n: int = 30, seed: int = 19) -> Scene:
+    """Capsules (radius U[0.1,0.3], length U[0.2,1.0], random orientation)
+    with spheres and boxes dropped on a plane, spawned without overlap of
+    their bounding spheres; h = 0.01, mu = 0.6.  Every capsule collider of the
+    reference (capsule-sphere, -box, -capsule, -plane) meets its pairs."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
+    clear = _Clearance()
+    half = 0.3 * math.sqrt(n)
+    for i in range(n):
+        dens = rng.uniform(0.5, 2.0)
+        kind = i % 3
+        if kind == 0:
+            r, ln = rng.uniform(0.1, 0.3), rng.uniform(0.2, 1.0)
+            rad = r + 0.5 * ln
+        elif kind == 1:
+            r = rng.uniform(0.1, 0.4)
+            rad = r
+        else:
+            he = rng.uniform(0.1, 0.4, size=3)
+            rad = float(np.linalg.norm(he))
+        for _ in range(200):
+            pos = (rng.uniform(-half, half), rng.uniform(-half, half), rng.uniform(0.4, 0.4 + n / 8.0))
+            if clear.free([pos], [rad]):
+                break
+        clear.add([pos], [rad])
+        q = rng.normal(size=4)
+        q = tuple(q / np.linalg.norm(q))
+        if kind == 0:
+            b.capsule_body(pos, r, ln, dens, q=q)
+        elif kind == 1:
+            b.sphere_body(pos, r, dens)
+        else:
+            b.box_body(pos, tuple(2.0 * he), dens, q=q)
+    return b.scene(dt=0.01, mu=0.6, seed=seed)
 
 
 def box_stack(n: int = 10, gap: float = 1e-3, seed: int = 0) -> Scene:

@@ -11,7 +11,7 @@
 
 namespace orc {
 
-enum { G_SPHERE = 0, G_BOX = 1, G_PLANE = 2 };
+enum { G_SPHERE = 0, G_BOX = 1, G_PLANE = 2, G_CAPSULE = 3 };
 static const int CONTACTS_UNIMPORTANT = (int)0x80000000;
 
 template <class R> struct ContactGeom {
@@ -39,6 +39,15 @@ template <class R> void geom_aabb(const GeomShape<R> &g, R *aabb) {
     const R *s = g.param;
     for (int k = 0; k < 3; k++) {
       R ext = R(0.5) * (std::fabs(M[4 * k] * s[0]) + std::fabs(M[4 * k + 1] * s[1]) + std::fabs(M[4 * k + 2] * s[2]));
+      aabb[2 * k] = g.pos[k] - ext;
+      aabb[2 * k + 1] = g.pos[k] + ext;
+    }
+  } else if (g.type == G_CAPSULE) {
+    // capsule.cpp:60-74: half the axis (column 2 of R) times the length,
+    // plus the radius
+    const R *M = g.Rm;
+    for (int k = 0; k < 3; k++) {
+      R ext = std::fabs(M[4 * k + 2] * g.param[1]) * R(0.5) + g.param[0];
       aabb[2 * k] = g.pos[k] - ext;
       aabb[2 * k + 1] = g.pos[k] + ext;
     }
@@ -477,10 +486,277 @@ int collide_box_box(const R *p1, const R *R1, const R *side1, const R *p2, const
 // 167-190): the sphere-box, sphere-plane and box-plane colliders take the
 // sphere/box first; a reversed call swaps the arguments, negates the normals
 // and swaps g1/g2.  g1/g2 are written as the caller's ids.
+// ------------------------------------------------------------ capsules
+// A capsule (param: radius, length) lies along column 2 of its rotation.
+
+// dClosestLineSegmentPoints (collision_util.cpp:109-240): the closest points
+// of segments a1-a2 and b1-b2 by the Voronoi clipping of their vertex and
+// edge features, in the reference's order of tests
+template <class R>
+void closest_segment_points(const R *a1, const R *a2, const R *b1, const R *b2, R *cp1, R *cp2) {
+  R a1a2[3], b1b2[3], a1b1[3], a1b2[3], a2b1[3], a2b2[3], n[3];
+  auto sub = [](R *o, const R *x, const R *y) { for (int k = 0; k < 3; k++) o[k] = x[k] - y[k]; };
+  auto set = [](R *o, const R *x) { for (int k = 0; k < 3; k++) o[k] = x[k]; };
+  sub(a1a2, a2, a1);
+  sub(b1b2, b2, b1);
+  sub(a1b1, b1, a1);
+  R da1 = dot3(a1a2, a1b1), db1 = dot3(b1b2, a1b1);
+  if (da1 <= 0 && db1 >= 0) { set(cp1, a1); set(cp2, b1); return; }
+  sub(a1b2, b2, a1);
+  R da2 = dot3(a1a2, a1b2), db2 = dot3(b1b2, a1b2);
+  if (da2 <= 0 && db2 <= 0) { set(cp1, a1); set(cp2, b2); return; }
+  sub(a2b1, b1, a2);
+  R da3 = dot3(a1a2, a2b1), db3 = dot3(b1b2, a2b1);
+  if (da3 >= 0 && db3 >= 0) { set(cp1, a2); set(cp2, b1); return; }
+  sub(a2b2, b2, a2);
+  R da4 = dot3(a1a2, a2b2), db4 = dot3(b1b2, a2b2);
+  if (da4 >= 0 && db4 <= 0) { set(cp1, a2); set(cp2, b2); return; }
+  R la = dot3(a1a2, a1a2);
+  if (da1 >= 0 && da3 <= 0) {
+    R k = da1 / la;
+    for (int i = 0; i < 3; i++) n[i] = a1b1[i] - k * a1a2[i];
+    if (dot3(b1b2, n) >= 0) {
+      for (int i = 0; i < 3; i++) cp1[i] = a1[i] + k * a1a2[i];
+      set(cp2, b1);
+      return;
+    }
+  }
+  if (da2 >= 0 && da4 <= 0) {
+    R k = da2 / la;
+    for (int i = 0; i < 3; i++) n[i] = a1b2[i] - k * a1a2[i];
+    if (dot3(b1b2, n) <= 0) {
+      for (int i = 0; i < 3; i++) cp1[i] = a1[i] + k * a1a2[i];
+      set(cp2, b2);
+      return;
+    }
+  }
+  R lb = dot3(b1b2, b1b2);
+  if (db1 <= 0 && db2 >= 0) {
+    R k = -db1 / lb;
+    for (int i = 0; i < 3; i++) n[i] = -a1b1[i] - k * b1b2[i];
+    if (dot3(a1a2, n) >= 0) {
+      set(cp1, a1);
+      for (int i = 0; i < 3; i++) cp2[i] = b1[i] + k * b1b2[i];
+      return;
+    }
+  }
+  if (db3 <= 0 && db4 >= 0) {
+    R k = -db3 / lb;
+    for (int i = 0; i < 3; i++) n[i] = -a2b1[i] - k * b1b2[i];
+    if (dot3(a1a2, n) <= 0) {
+      set(cp1, a2);
+      for (int i = 0; i < 3; i++) cp2[i] = b1[i] + k * b1b2[i];
+      return;
+    }
+  }
+  R k = dot3(a1a2, b1b2);
+  R det = la * lb - k * k;
+  if (det <= 0) { set(cp1, a1); set(cp2, b1); return; }
+  det = recip(det);
+  R alpha = (lb * da1 - k * db1) * det;
+  R beta = (k * da1 - la * db1) * det;
+  for (int i = 0; i < 3; i++) cp1[i] = a1[i] + alpha * a1a2[i];
+  for (int i = 0; i < 3; i++) cp2[i] = b1[i] + beta * b1b2[i];
+}
+
+// dClosestLineBoxPoints (collision_util.cpp:247-375): the point of segment
+// p1-p2 closest to the box (c, Rb, side) and the box point closest to it, by
+// walking the piecewise-linear derivative of the squared distance along the
+// segment in box coordinates
+template <class R>
+void closest_line_box_points(const R *p1, const R *p2, const R *c, const R *Rb, const R *side, R *lret, R *bret) {
+  R tmp[3], s[3], v[3], sign[3], v2[3], h[3], tanchor[3];
+  int region[3];
+  for (int i = 0; i < 3; i++) tmp[i] = p1[i] - c[i];
+  mtv(s, Rb, tmp);
+  for (int i = 0; i < 3; i++) tmp[i] = p2[i] - p1[i];
+  mtv(v, Rb, tmp);
+  for (int i = 0; i < 3; i++) {
+    if (v[i] < 0) { s[i] = -s[i]; v[i] = -v[i]; sign[i] = -1; }
+    else sign[i] = 1;
+  }
+  for (int i = 0; i < 3; i++) v2[i] = v[i] * v[i];
+  for (int i = 0; i < 3; i++) h[i] = R(0.5) * side[i];
+  const R eps = sizeof(R) == 4 ? R(1e-19) : R(1e-307);
+  for (int i = 0; i < 3; i++) {
+    if (v[i] > eps) {
+      if (s[i] < -h[i]) { region[i] = -1; tanchor[i] = (-h[i] - s[i]) / v[i]; }
+      else { region[i] = (s[i] > h[i]); tanchor[i] = (h[i] - s[i]) / v[i]; }
+    } else {
+      region[i] = 0;
+      tanchor[i] = 2;
+    }
+  }
+  R t = 0, dd2dt = 0;
+  for (int i = 0; i < 3; i++) dd2dt -= (region[i] ? v2[i] : R(0)) * tanchor[i];
+  bool done = dd2dt >= 0;
+  while (!done) {
+    R next_t = 1;
+    for (int i = 0; i < 3; i++)
+      if (tanchor[i] > t && tanchor[i] < 1 && tanchor[i] < next_t) next_t = tanchor[i];
+    R next_dd2dt = 0;
+    for (int i = 0; i < 3; i++) next_dd2dt += (region[i] ? v2[i] : R(0)) * (next_t - tanchor[i]);
+    if (next_dd2dt >= 0) {
+      R m = (next_dd2dt - dd2dt) / (next_t - t);
+      t -= dd2dt / m;
+      done = true;
+      break;
+    }
+    for (int i = 0; i < 3; i++)
+      if (tanchor[i] == next_t) { tanchor[i] = (h[i] - s[i]) / v[i]; region[i]++; }
+    t = next_t;
+    dd2dt = next_dd2dt;
+    if (!(t < 1)) { t = 1; break; }
+  }
+  for (int i = 0; i < 3; i++) lret[i] = p1[i] + t * tmp[i];
+  for (int i = 0; i < 3; i++) {
+    tmp[i] = sign[i] * (s[i] + t * v[i]);
+    if (tmp[i] < -h[i]) tmp[i] = -h[i];
+    else if (tmp[i] > h[i]) tmp[i] = h[i];
+  }
+  mv(s, Rb, tmp);
+  for (int i = 0; i < 3; i++) bret[i] = s[i] + c[i];
+}
+
+// capsule-sphere: capsule.cpp:130-162 (the sphere against the closest point
+// of the capsule's axis segment)
+template <class R>
+int collide_capsule_sphere(const R *pc, const R *Rc, R rad, R len, const R *ps, R rs, ContactGeom<R> *c) {
+  R alpha = Rc[2] * (ps[0] - pc[0]) + Rc[6] * (ps[1] - pc[1]) + Rc[10] * (ps[2] - pc[2]);
+  R lz2 = len * R(0.5);
+  if (alpha > lz2) alpha = lz2;
+  if (alpha < -lz2) alpha = -lz2;
+  R p[3] = {pc[0] + alpha * Rc[2], pc[1] + alpha * Rc[6], pc[2] + alpha * Rc[10]};
+  return collide_sphere_sphere(p, rad, ps, rs, c);
+}
+
+// capsule-box: capsule.cpp:178-231
+template <class R>
+int collide_capsule_box(const R *pc, const R *Rc, R rad, R len, const R *pb, const R *Rb, const R *side,
+                        ContactGeom<R> *c) {
+  R clen = len * R(0.5);
+  R p1[3] = {pc[0] + clen * Rc[2], pc[1] + clen * Rc[6], pc[2] + clen * Rc[10]};
+  R p2[3] = {pc[0] - clen * Rc[2], pc[1] - clen * Rc[6], pc[2] - clen * Rc[10]};
+  R pl[3], pbx[3];
+  closest_line_box_points(p1, p2, pb, Rb, side, pl, pbx);
+  const R mindist = sizeof(R) == 4 ? R(1e-9) : R(1e-18);
+  R d[3] = {pl[0] - pbx[0], pl[1] - pbx[1], pl[2] - pbx[2]};
+  if (std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) < mindist) {
+    // the axis point is inside the box: the normal points from the box
+    // centre (dCollideSpheresZeroDist, capsule.cpp:165-176)
+    R n[3] = {pbx[0] - pb[0], pbx[1] - pb[1], pbx[2] - pb[2]};
+    normalize3(n);
+    for (int k = 0; k < 3; k++) c->normal[k] = n[k];
+    c->depth = rad + R(0);
+    R k2 = R(0.5) * (R(0) - rad);
+    for (int k = 0; k < 3; k++) c->pos[k] = pl[k] + c->normal[k] * k2;
+    return 1;
+  }
+  return collide_sphere_sphere(pl, rad, pbx, R(0), c);
+}
+
+// capsule-capsule: capsule.cpp:234-349 (up to two contacts along nearly
+// parallel axes, else the closest points of the axis segments)
+template <class R>
+int collide_capsule_capsule(const R *pos1, const R *R1, R rad1, R len1, const R *pos2, const R *R2, R rad2, R len2,
+                            int flags, ContactGeom<R> *c) {
+  const R tolerance = R(1e-5);
+  R lz1 = len1 * R(0.5), lz2 = len2 * R(0.5);
+  R axis1[3] = {R1[2], R1[6], R1[10]}, axis2[3] = {R2[2], R2[6], R2[10]};
+  R sphere1[3], sphere2[3];
+  R a1a2 = dot3(axis1, axis2);
+  R det = R(1.0) - a1a2 * a1a2;
+  if (det < tolerance) {
+    if (a1a2 < 0) for (int i = 0; i < 3; i++) axis2[i] = -axis2[i];
+    R q[3];
+    for (int i = 0; i < 3; i++) q[i] = pos1[i] - pos2[i];
+    R k = dot3(axis1, q);
+    R a1lo = -lz1, a1hi = lz1, a2lo = -lz2 - k, a2hi = lz2 - k;
+    R lo = (a1lo > a2lo) ? a1lo : a2lo;
+    R hi = (a1hi < a2hi) ? a1hi : a2hi;
+    if (lo <= hi) {
+      int num = flags & 0xffff;
+      if (num >= 2 && lo < hi) {
+        for (int i = 0; i < 3; i++) sphere1[i] = pos1[i] + lo * axis1[i];
+        for (int i = 0; i < 3; i++) sphere2[i] = pos2[i] + (lo + k) * axis2[i];
+        int n1 = collide_sphere_sphere(sphere1, rad1, sphere2, rad2, c);
+        if (n1) {
+          for (int i = 0; i < 3; i++) sphere1[i] = pos1[i] + hi * axis1[i];
+          for (int i = 0; i < 3; i++) sphere2[i] = pos2[i] + (hi + k) * axis2[i];
+          int n2 = collide_sphere_sphere(sphere1, rad1, sphere2, rad2, c + 1);
+          if (n2) return 2;
+        }
+      }
+      R alpha1 = (lo + hi) * R(0.5);
+      R alpha2 = alpha1 + k;
+      for (int i = 0; i < 3; i++) sphere1[i] = pos1[i] + alpha1 * axis1[i];
+      for (int i = 0; i < 3; i++) sphere2[i] = pos2[i] + alpha2 * axis2[i];
+      return collide_sphere_sphere(sphere1, rad1, sphere2, rad2, c);
+    }
+  }
+  R a1[3], a2[3], b1[3], b2[3];
+  for (int i = 0; i < 3; i++) {
+    a1[i] = pos1[i] + axis1[i] * lz1;
+    a2[i] = pos1[i] - axis1[i] * lz1;
+    b1[i] = pos2[i] + axis2[i] * lz2;
+    b2[i] = pos2[i] - axis2[i] * lz2;
+  }
+  closest_segment_points(a1, a2, b1, b2, sphere1, sphere2);
+  return collide_sphere_sphere(sphere1, rad1, sphere2, rad2, c);
+}
+
+// capsule-plane: capsule.cpp:352-409 (the deeper end cap, then the other)
+template <class R>
+int collide_capsule_plane(const R *pc, const R *Rc, R rad, R len, const R *plane, int flags, ContactGeom<R> *c) {
+  R ax[3] = {Rc[2], Rc[6], Rc[10]};
+  R sign = (dot3(plane, ax) > 0) ? R(-1.0) : R(1.0);
+  R p[3];
+  for (int i = 0; i < 3; i++) p[i] = pc[i] + ax[i] * len * R(0.5) * sign;
+  R k = dot3(p, plane);
+  R depth = plane[3] - k + rad;
+  if (depth < 0) return 0;
+  for (int i = 0; i < 3; i++) {
+    c[0].normal[i] = plane[i];
+    c[0].pos[i] = p[i] - plane[i] * rad;
+  }
+  c[0].depth = depth;
+  int n = 1;
+  if ((flags & 0xffff) >= 2) {
+    for (int i = 0; i < 3; i++) p[i] = pc[i] - ax[i] * len * R(0.5) * sign;
+    k = dot3(p, plane);
+    depth = plane[3] - k + rad;
+    if (depth >= 0) {
+      for (int i = 0; i < 3; i++) {
+        c[1].normal[i] = plane[i];
+        c[1].pos[i] = p[i] - plane[i] * rad;
+      }
+      c[1].depth = depth;
+      n = 2;
+    }
+  }
+  return n;
+}
+
+// the class that comes first in a collider's arguments (the colliders table,
+// collision_kernel.cpp:181-189): capsule, then sphere, box, plane
+inline int collider_rank(int type) {
+  return type == G_CAPSULE ? 0 : (type == G_SPHERE ? 1 : (type == G_BOX ? 2 : 3));
+}
+
 template <class R>
 int collide_geoms(const GeomShape<R> &a, int ida, const GeomShape<R> &b, int idb, int flags, ContactGeom<R> *c) {
   if ((flags & 0xffff) == 0) return 0;
   auto run = [&](const GeomShape<R> &x, const GeomShape<R> &y) -> int {
+    if (x.type == G_CAPSULE) {
+      if (y.type == G_SPHERE) return collide_capsule_sphere(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.param[0], c);
+      if (y.type == G_BOX)
+        return collide_capsule_box(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.Rm, y.param, c);
+      if (y.type == G_CAPSULE)
+        return collide_capsule_capsule(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.Rm, y.param[0], y.param[1], flags,
+                                       c);
+      if (y.type == G_PLANE) return collide_capsule_plane(x.pos, x.Rm, x.param[0], x.param[1], y.param, flags, c);
+      return 0;
+    }
     if (x.type == G_SPHERE && y.type == G_SPHERE) return collide_sphere_sphere(x.pos, x.param[0], y.pos, y.param[0], c);
     if (x.type == G_SPHERE && y.type == G_BOX) return collide_sphere_box(x.pos, x.param[0], y.pos, y.Rm, y.param, c);
     if (x.type == G_SPHERE && y.type == G_PLANE) return collide_sphere_plane(x.pos, x.param[0], y.param, c);
@@ -488,7 +764,7 @@ int collide_geoms(const GeomShape<R> &a, int ida, const GeomShape<R> &b, int idb
     if (x.type == G_BOX && y.type == G_PLANE) return collide_box_plane(x.pos, x.Rm, x.param, y.param, flags, c);
     return 0;
   };
-  bool reverse = (a.type > b.type);
+  bool reverse = collider_rank(a.type) > collider_rank(b.type);
   int n = reverse ? run(b, a) : run(a, b);
   for (int i = 0; i < n; i++) {
     if (reverse) for (int k = 0; k < 3; k++) c[i].normal[k] = -c[i].normal[k];

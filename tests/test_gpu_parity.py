@@ -35,6 +35,7 @@ class Api:
         V, F = C.c_void_p, C.c_float
         for name, res, args in [
             ("dCreateBox", V, [V, F, F, F]), ("dCreateSphere", V, [V, F]), ("dCreatePlane", V, [V, F, F, F, F]),
+            ("dCreateCapsule", V, [V, F, F]),
             ("dGeomSetPosition", None, [V, F, F, F]), ("dGeomSetRotation", None, [V, C.POINTER(F)]),
             ("dCollide", C.c_int, [V, V, C.c_int, V, C.c_int]), ("dGeomDestroy", None, [V])]:
             f = getattr(L, name)
@@ -48,6 +49,8 @@ class Api:
             g = L.dCreateSphere(None, prm[0])
         elif t == 1:
             g = L.dCreateBox(None, prm[0], prm[1], prm[2])
+        elif t == 3:
+            g = L.dCreateCapsule(None, prm[0], prm[1])
         else:
             return L.dCreatePlane(None, *prm)
         L.dGeomSetPosition(g, *p)
@@ -122,6 +125,62 @@ def test_narrowphase_pairs_vs_oracle(odemi_mod):
     assert n_cmp > 150 and n_disc < 10
 
 
+def test_capsule_narrowphase_vs_oracle(odemi_mod):
+    """The four capsule colliders (capsule-sphere, -box, -capsule, -plane,
+    capsule.cpp:130-409) through dCollide on random overlapping poses, both
+    argument orders, including capsules with parallel axes (the two-contact
+    branch), against the restatement: the bar of the narrowphase test above."""
+    from test_oracle_pins import collide as orc_collide
+    api = Api()
+    rng = np.random.default_rng(17)
+    combos = [(3, 0), (0, 3), (3, 1), (1, 3), (3, 3), (3, 2), (2, 3)]
+    n_cmp = n_disc = n_two = 0
+    for t1, t2 in combos:
+        say("capsule narrowphase case", t1, t2)
+        for trial in range(60):
+            def shape(t):
+                if t == 0:
+                    return [rng.uniform(0.1, 0.6), 0, 0, 0]
+                if t == 1:
+                    return [*rng.uniform(0.2, 1.2, 3), 0]
+                if t == 3:
+                    return [rng.uniform(0.1, 0.4), rng.uniform(0.2, 1.0), 0, 0]
+                n = rng.normal(size=3)
+                n /= np.linalg.norm(n)
+                return [*n, rng.uniform(-0.3, 0.3)]
+            s1, s2 = shape(t1), shape(t2)
+            q1, q2 = rng.normal(size=4), rng.normal(size=4)
+            if t1 == 3 and t2 == 3 and trial % 3 == 0:
+                q2 = q1.copy() * (1 if trial % 2 else -1)   # parallel or antiparallel axes
+            p1, p2 = rng.uniform(-0.4, 0.4, 3), rng.uniform(-0.4, 0.4, 3)
+            if t1 == 2:
+                p1 = np.zeros(3); q1 = np.array([1.0, 0, 0, 0])
+            if t2 == 2:
+                p2 = np.zeros(3); q2 = np.array([1.0, 0, 0, 0])
+            maxc = 4
+            g1 = api.geom(t1, s1, p1, _rot(q1))
+            g2 = api.geom(t2, s2, p2, _rot(q2))
+            got = api.collide(g1, g2, maxc)
+            api.L.dGeomDestroy(g1)
+            api.L.dGeomDestroy(g2)
+            ref32 = orc_collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc, prec="f32")
+            ref64 = orc_collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc, prec="f64")
+            if len(ref32) != len(ref64):
+                n_disc += 1
+                continue
+            assert len(got) == len(ref64), (t1, t2, trial, got, ref64)
+            if len(got):
+                scale = np.maximum(1.0, np.abs(ref64))
+                err = np.abs(got - ref64) / scale
+                e32 = np.abs(ref32 - ref64) / scale
+                tol = max(1e-5, 4.0 * float(e32.max()))
+                assert err.max() <= tol, (t1, t2, trial, err.max(), tol)
+                n_cmp += 1
+                n_two += int(len(got) == 2 and t1 == 3 and t2 == 3)
+    say("capsule narrowphase compared", n_cmp, "two-contact capsule pairs", n_two, "precision ties skipped", n_disc)
+    assert n_cmp > 150 and n_two > 0 and n_disc < 10
+
+
 def test_dcollide_refusals_match_the_reference(odemi_mod):
     """dCollide returns 0 for a request of no contacts, a geom against itself
     and two geoms on one body (collision_kernel.cpp:306-311); the flag bits
@@ -160,7 +219,7 @@ def _batch(odemi_mod, scs, order=None):
     return odemi_mod.Batch(scs, odemi_mod.ORDER_REFERENCE if order is None else order)
 
 
-@pytest.mark.parametrize("name", ["pile4096", "pile256_dense", "chains", "vehicle", "surfaces"])
+@pytest.mark.parametrize("name", ["pile4096", "pile256_dense", "chains", "vehicle", "surfaces", "capsules"])
 def test_first_step_pairs_and_contacts_match_reference(odemi_mod, name):
     say("first-step parity", name)
     from test_golden import CASES
@@ -281,7 +340,7 @@ def _envelope(sc, n, every):
 
 
 @pytest.mark.parametrize("name", ["box_stack", "chains", "vehicle", "chains200", "vehicle200", "joints", "surfaces",
-                                  "frictionless"])
+                                  "frictionless", "capsules"])
 def test_trajectory_within_fp32_envelope(odemi_mod, name):
     say("trajectory", name)
     from test_golden import CASES
@@ -347,7 +406,7 @@ def _tie_order(con, ref):
     return perm
 
 
-@pytest.mark.parametrize("name", ["box_stack", "chains", "surfaces", "vehicle", "vehicle_spaces"])
+@pytest.mark.parametrize("name", ["box_stack", "chains", "surfaces", "vehicle", "vehicle_spaces", "capsules"])
 def test_dropin_joint_feedback_every_step(odemi_mod, name):
     """dJointFeedback written back by dWorldQuickStep (quickstep.cpp:918-957)
     for every contact joint of every diagnostic step of the drop-in run,
