@@ -208,6 +208,10 @@ ODE_API void dMassSetSphereTotal(dMass *m, dReal total_mass, dReal radius);   /*
 ODE_API void dMassSetBox(dMass *m, dReal density, dReal lx, dReal ly, dReal lz);          /* mass.h:65 */
 ODE_API void dMassSetBoxTotal(dMass *m, dReal total_mass, dReal lx, dReal ly, dReal lz);  /* mass.h:67 */
 ODE_API void dMassAdjust(dMass *m, dReal newmass);                            /* mass.h:74 */
+ODE_API void dMassSetCapsule(dMass *m, dReal density, int direction, dReal radius, dReal length);      /* mass.h:55 */
+ODE_API void dMassSetCapsuleTotal(dMass *m, dReal total_mass, int direction, dReal radius, dReal length); /* mass.h:57 */
+ODE_API void dMassSetCylinder(dMass *m, dReal density, int direction, dReal radius, dReal length);     /* mass.h:60 */
+ODE_API void dMassSetCylinderTotal(dMass *m, dReal total_mass, int direction, dReal radius, dReal length); /* mass.h:62 */
 
 /* random numbers: include/ode/misc.h:42-51 */
 ODE_API unsigned long dRand(void);                                            /* misc.h:42 */
@@ -286,6 +290,31 @@ ODE_API dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d)
 ODE_API dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length);  /* collision.h:1051 */
 ODE_API void dGeomCapsuleSetParams(dGeomID ccylinder, dReal radius, dReal length); /* collision.h:1052 */
 ODE_API void dGeomCapsuleGetParams(dGeomID ccylinder, dReal *radius, dReal *length); /* collision.h:1053 */
+/* Cylinders (cylinder.cpp:47-103): the axis is column 2 of the rotation.
+ * Their colliders are those of the reference without libccd: cylinder-sphere,
+ * cylinder-plane, cylinder-box and ray-cylinder; a cylinder against a
+ * cylinder or a capsule gives no contacts, as there. */
+ODE_API dGeomID dCreateCylinder(dSpaceID space, dReal radius, dReal length);  /* collision.h:1063 */
+ODE_API void dGeomCylinderSetParams(dGeomID cylinder, dReal radius, dReal length); /* collision.h:1064 */
+ODE_API void dGeomCylinderGetParams(dGeomID cylinder, dReal *radius, dReal *length); /* collision.h:1065 */
+/* Rays (ray.cpp:47-176): a ray starts at its geom position and points along
+ * column 2 of its rotation; its contacts have depth = distance from the start
+ * and the normal of the surface hit (reversed when the ray starts inside).
+ * A ray attached to a body follows the body; dGeomRaySet on such a ray moves
+ * the body (its position) and is refused for a direction other than the
+ * body's own z axis, since the body's rotation here comes from its
+ * quaternion.  The FirstContact / BackfaceCull / ClosestHit flags are stored
+ * and returned; no collider here reads them (in the reference only the
+ * trimesh and heightfield colliders do). */
+ODE_API dGeomID dCreateRay(dSpaceID space, dReal length);                     /* collision.h:1067 */
+ODE_API void dGeomRaySetLength(dGeomID ray, dReal length);                    /* collision.h:1068 */
+ODE_API dReal dGeomRayGetLength(dGeomID ray);                                 /* collision.h:1069 */
+ODE_API void dGeomRaySet(dGeomID ray, dReal px, dReal py, dReal pz, dReal dx, dReal dy, dReal dz); /* collision.h:1070 */
+ODE_API void dGeomRayGet(dGeomID ray, dVector3 start, dVector3 dir);          /* collision.h:1072 */
+ODE_API void dGeomRaySetParams(dGeomID g, int FirstContact, int BackfaceCull); /* collision.h:1079 */
+ODE_API void dGeomRayGetParams(dGeomID g, int *FirstContact, int *BackfaceCull); /* collision.h:1080 */
+ODE_API void dGeomRaySetClosestHit(dGeomID g, int closestHit);                /* collision.h:1081 */
+ODE_API int dGeomRayGetClosestHit(dGeomID g);                                 /* collision.h:1082 */
 ODE_API void dGeomDestroy(dGeomID g);                                         /* collision.h:68 */
 ODE_API void dGeomSetData(dGeomID g, void *data);                             /* collision.h:78 */
 ODE_API void *dGeomGetData(dGeomID g);                                        /* collision.h:87 */

@@ -28,8 +28,9 @@ bool world_supported(const BatchImpl *b, int w, char *why, size_t n) {
     return false;
   }
   for (int g = 0; g < W.ng; g++)
-    if (b->h_gtype[W.g0 + g] == GEOM_CAPSULE) {
-      snprintf(why, n, "world %d has a capsule geom (the adjoint colliders are sphere, box and plane)", w);
+    if (b->h_gtype[W.g0 + g] != GEOM_SPHERE && b->h_gtype[W.g0 + g] != GEOM_BOX &&
+        b->h_gtype[W.g0 + g] != GEOM_PLANE) {
+      snprintf(why, n, "world %d has a capsule, cylinder or ray geom (the adjoint colliders are sphere, box and plane)", w);
       return false;
     }
   int other = P.surf_mode & ~(SM_BOUNCE | SM_APPROX1_1 | SM_APPROX1_2);

@@ -418,7 +418,12 @@ static bool validate_desc(const dBatchDesc *D) {
       const long long gg = g0 + g;
       const int t = D->geom_type[gg], body = D->geom_body[gg];
       const double *p = D->geom_param + 4 * gg;
-      if (t != GEOM_SPHERE && t != GEOM_BOX && t != GEOM_PLANE && t != GEOM_CAPSULE) {
+      if (t == GEOM_RAY) {
+        report_error("dBatchCreate: world %d geom %d: a ray (its contacts are reported through dSpaceCollide / "
+                     "dCollide; every contact of a batch world becomes a joint)", w, g);
+        return false;
+      }
+      if (t != GEOM_SPHERE && t != GEOM_BOX && t != GEOM_PLANE && t != GEOM_CAPSULE && t != GEOM_CYLINDER) {
         report_error("dBatchCreate: world %d geom %d: unknown type %d", w, g, t);
         return false;
       }
@@ -426,7 +431,7 @@ static bool validate_desc(const dBatchDesc *D) {
         report_error("dBatchCreate: world %d geom %d: body %d out of range (planes take none)", w, g, body);
         return false;
       }
-      const int nsz = t == GEOM_SPHERE ? 1 : (t == GEOM_BOX ? 3 : (t == GEOM_CAPSULE ? 2 : 0));
+      const int nsz = t == GEOM_SPHERE ? 1 : (t == GEOM_BOX ? 3 : ((t == GEOM_CAPSULE || t == GEOM_CYLINDER) ? 2 : 0));
       for (int k = 0; k < nsz; k++)
         if (!(p[k] >= 0 && std::isfinite(p[k]))) {
           report_error("dBatchCreate: world %d geom %d: size must be >= 0 and finite", w, g);

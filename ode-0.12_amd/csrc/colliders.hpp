@@ -1,5 +1,5 @@
-// colliders.hpp -- narrowphase for the path's five geom pairs, one pair per
-// lane.  Each collider writes up to `maxc` contacts (position, normal into
+// colliders.hpp -- narrowphase for the path's geom pairs (sphere, box, plane,
+// capsule, ray, cylinder), one pair per lane.  Each collider writes up to `maxc` contacts (position, normal into
 // body 1, depth) and returns their count; the contact normal convention and
 // the contact selection are those of the reference colliders cited below.
 #pragma once
@@ -12,7 +12,7 @@ struct GeomView {
   int type;
   float3 p;      // world position (unused for planes)
   Mat3 R;        // world rotation
-  float4 prm;    // radius | lx,ly,lz | plane a,b,c,d (unit normal) | capsule radius, length
+  float4 prm;    // radius | lx,ly,lz | plane a,b,c,d (unit normal) | capsule or cylinder radius, length | ray length
 };
 
 struct CPoint {
@@ -35,6 +35,23 @@ __device__ inline void geom_aabb(const GeomView &g, float *a) {
     float ex = 0.5f * (fabsf(g.R.r0.x * s.x) + fabsf(g.R.r0.y * s.y) + fabsf(g.R.r0.z * s.z));
     float ey = 0.5f * (fabsf(g.R.r1.x * s.x) + fabsf(g.R.r1.y * s.y) + fabsf(g.R.r1.z * s.z));
     float ez = 0.5f * (fabsf(g.R.r2.x * s.x) + fabsf(g.R.r2.y * s.y) + fabsf(g.R.r2.z * s.z));
+    a[0] = g.p.x - ex; a[1] = g.p.x + ex;
+    a[2] = g.p.y - ey; a[3] = g.p.y + ey;
+    a[4] = g.p.z - ez; a[5] = g.p.z + ez;
+  } else if (g.type == GEOM_RAY) {
+    // ray.cpp:56-88: the box of the start point and the end point (the
+    // direction is column 2 of R)
+    const float3 e = mk3(g.p.x + g.R.r0.z * g.prm.x, g.p.y + g.R.r1.z * g.prm.x, g.p.z + g.R.r2.z * g.prm.x);
+    if (g.p.x < e.x) { a[0] = g.p.x; a[1] = e.x; } else { a[0] = e.x; a[1] = g.p.x; }
+    if (g.p.y < e.y) { a[2] = g.p.y; a[3] = e.y; } else { a[2] = e.y; a[3] = g.p.y; }
+    if (g.p.z < e.z) { a[4] = g.p.z; a[5] = e.z; } else { a[4] = e.z; a[5] = g.p.z; }
+  } else if (g.type == GEOM_CYLINDER) {
+    // cylinder.cpp:58-74: the radius along columns 0 and 1 of R, half the
+    // length along column 2
+    const float r = g.prm.x, l = g.prm.y;
+    float ex = fabsf(g.R.r0.x * r) + fabsf(g.R.r0.y * r) + 0.5f * fabsf(g.R.r0.z * l);
+    float ey = fabsf(g.R.r1.x * r) + fabsf(g.R.r1.y * r) + 0.5f * fabsf(g.R.r1.z * l);
+    float ez = fabsf(g.R.r2.x * r) + fabsf(g.R.r2.y * r) + 0.5f * fabsf(g.R.r2.z * l);
     a[0] = g.p.x - ex; a[1] = g.p.x + ex;
     a[2] = g.p.y - ey; a[3] = g.p.y + ey;
     a[4] = g.p.z - ez; a[5] = g.p.z + ez;
@@ -638,10 +655,533 @@ __device__ inline int coll_capsule_plane(const GeomView &cap, float4 pl, int max
   return 2;
 }
 
+// ------------------------------------------------------------ rays
+// A ray (prm: length) starts at its position along column 2 of its rotation;
+// the contact depth is the distance along the ray.
+
+// ray.cpp:196-232: where the ray enters a sphere (mode 1 from outside: where
+// it leaves)
+__device__ inline int ray_sphere_hit(const GeomView &ray, float3 ps, float rad, int mode, CPoint *c) {
+  const float3 dir = capsule_axis(ray.R);
+  const float3 q = ray.p - ps;
+  const float B = q.x * dir.x + q.y * dir.y + q.z * dir.z;
+  const float C = dot(q, q) - rad * rad;
+  float k = B * B - C;
+  if (k < 0.f) return 0;
+  k = sqrtf(k);
+  float alpha;
+  if (mode && C >= 0.f) {
+    alpha = -B + k;
+    if (alpha < 0.f) return 0;
+  } else {
+    alpha = -B - k;
+    if (alpha < 0.f) {
+      alpha = -B + k;
+      if (alpha < 0.f) return 0;
+    }
+  }
+  if (alpha > ray.prm.x) return 0;
+  c->pos = ray.p + alpha * dir;
+  const float nsign = (C < 0.f || mode) ? -1.0f : 1.0f;
+  c->n = safe_normalize(nsign * (c->pos - ps));
+  c->depth = alpha;
+  return 1;
+}
+
+// ray-box: ray.cpp:253-349 -- slab intersection in box coordinates
+__device__ inline int coll_ray_box(const GeomView &ray, const GeomView &box, CPoint *c) {
+  const float3 dir = capsule_axis(ray.R);
+  float s[3], v[3], sg[3], h[3];
+  {
+    const float3 s3 = box.R.tmul(ray.p - box.p), v3 = box.R.tmul(dir);
+    s[0] = s3.x; s[1] = s3.y; s[2] = s3.z;
+    v[0] = v3.x; v[1] = v3.y; v[2] = v3.z;
+  }
+  for (int i = 0; i < 3; i++) {
+    if (v[i] < 0.f) { s[i] = -s[i]; v[i] = -v[i]; sg[i] = 1.f; }
+    else sg[i] = -1.f;
+  }
+  h[0] = 0.5f * box.prm.x; h[1] = 0.5f * box.prm.y; h[2] = 0.5f * box.prm.z;
+  if ((s[0] < -h[0] && v[0] <= 0.f) || s[0] > h[0] || (s[1] < -h[1] && v[1] <= 0.f) || s[1] > h[1] ||
+      (s[2] < -h[2] && v[2] <= 0.f) || s[2] > h[2] || (v[0] == 0.f && v[1] == 0.f && v[2] == 0.f))
+    return 0;
+  const float inf = __int_as_float(0x7f800000);
+  float lo = -inf, hi = inf;
+  int nlo = 0, nhi = 0;
+  for (int i = 0; i < 3; i++) {
+    if (v[i] != 0.f) {
+      float k = (-h[i] - s[i]) / v[i];
+      if (k > lo) { lo = k; nlo = i; }
+      k = (h[i] - s[i]) / v[i];
+      if (k < hi) { hi = k; nhi = i; }
+    }
+  }
+  if (lo > hi) return 0;
+  const float alpha = lo >= 0.f ? lo : hi;
+  const int n = lo >= 0.f ? nlo : nhi;
+  if (alpha < 0.f || alpha > ray.prm.x) return 0;
+  c->pos = ray.p + alpha * dir;
+  c->n = sg[n] * box.R.col(n);
+  c->depth = alpha;
+  return 1;
+}
+
+// ray-capsule: ray.cpp:352-464 -- the infinite cylinder, then the end caps
+__device__ inline int coll_ray_capsule(const GeomView &ray, const GeomView &cap, CPoint *c) {
+  const float3 dir = capsule_axis(ray.R), ax = capsule_axis(cap.R);
+  const float lz2 = cap.prm.y * 0.5f, rad = cap.prm.x;
+  const float3 cs = ray.p - cap.p;
+  float k = ax.x * cs.x + ax.y * cs.y + ax.z * cs.z;
+  float3 q = k * ax - cs;
+  const float C = dot(q, q) - rad * rad;
+  int inside = 0;
+  if (C < 0.f) {
+    if (k < -lz2) k = -lz2;
+    else if (k > lz2) k = lz2;
+    const float3 r = cap.p + k * ax;
+    const float3 d = ray.p - r;
+    if (d.x * d.x + d.y * d.y + d.z * d.z < rad * rad) inside = 1;
+  }
+  if (!inside && C < 0.f) {
+    k = (k < 0.f) ? -lz2 : lz2;
+  } else {
+    const float uv = ax.x * dir.x + ax.y * dir.y + ax.z * dir.z;
+    const float3 r = uv * ax - dir;
+    float A = dot(r, r);
+    const float B = 2.f * dot(q, r);
+    k = B * B - 4.f * A * C;
+    if (k < 0.f) {
+      if (!inside) return 0;
+      k = (uv < 0.f) ? -lz2 : lz2;
+    } else {
+      k = sqrtf(k);
+      A = 1.f / (2.f * A);
+      float alpha = (-B - k) * A;
+      if (alpha < 0.f) {
+        alpha = (-B + k) * A;
+        if (alpha < 0.f) return 0;
+      }
+      if (alpha > ray.prm.x) return 0;
+      c->pos = ray.p + alpha * dir;
+      q = c->pos - cap.p;
+      k = q.x * ax.x + q.y * ax.y + q.z * ax.z;
+      const float nsign = inside ? -1.0f : 1.0f;
+      if (k >= -lz2 && k <= lz2) {
+        c->n = safe_normalize(nsign * (c->pos - (cap.p + k * ax)));
+        c->depth = alpha;
+        return 1;
+      }
+      k = (k < 0.f) ? -lz2 : lz2;
+    }
+  }
+  return ray_sphere_hit(ray, cap.p + k * ax, rad, inside, c);
+}
+
+// ray-plane: ray.cpp:467-497
+__device__ inline int coll_ray_plane(const GeomView &ray, float4 pl, CPoint *c) {
+  const float3 n = mk3(pl.x, pl.y, pl.z), dir = capsule_axis(ray.R);
+  float alpha = pl.w - dot(n, ray.p);
+  const float nsign = (alpha > 0.f) ? -1.0f : 1.0f;
+  const float k = n.x * dir.x + n.y * dir.y + n.z * dir.z;
+  if (k == 0.f) return 0;
+  alpha /= k;
+  if (alpha < 0.f || alpha > ray.prm.x) return 0;
+  c->pos = ray.p + alpha * dir;
+  c->n = nsign * n;
+  c->depth = alpha;
+  return 1;
+}
+
+// ------------------------------------------------------------ cylinders
+// A cylinder (prm: radius, length) has its axis along column 2 of R.  The
+// reference without libccd has cylinder-sphere, cylinder-plane,
+// cylinder-box and ray-cylinder colliders; cylinder-cylinder and
+// capsule-cylinder pairs give no contacts.  The parallel-test tolerance is
+// the single-precision build's (collision_cylinder_sphere.cpp:66-71).
+constexpr float CYL_TOL = 0.0001f;
+
+// cylinder-sphere: collision_cylinder_sphere.cpp:57-278 -- s along the axis
+// from the first disc, t from the axis; the side, a disc or a disc's rim
+__device__ inline int coll_cylinder_sphere(const GeomView &cyl, float3 ps, float r2, CPoint *c) {
+  const float radius = cyl.prm.x, length = cyl.prm.y;
+  const float3 ax = capsule_axis(cyl.R);
+  const float h = length * 0.5f;
+  const float3 top = mk3(ax.x * h + cyl.p.x, ax.y * h + cyl.p.y, ax.z * h + cyl.p.z);
+  const float3 bot = mk3(ax.x * -h + cyl.p.x, ax.y * -h + cyl.p.y, ax.z * -h + cyl.p.z);
+  const float s = (ps.x - bot.x) * ax.x - (bot.y - ps.y) * ax.y - (bot.z - ps.z) * ax.z;
+  if (s < -r2 || s > length + r2) return 0;
+  float3 C = mk3(s * ax.x + bot.x - ps.x, s * ax.y + bot.y - ps.y, s * ax.z + bot.z - ps.z);
+  const float t = sqrtf(C.x * C.x + C.y * C.y + C.z * C.z);
+  if (t > radius + r2) return 0;
+  if (t > radius && (s < 0.f || s > length)) {
+    const float3 disc = (s <= 0.f) ? bot : top;
+    const float e = (s <= 0.f) ? s : s - length;
+    c->depth = r2 - sqrtf(e * e + (t - radius) * (t - radius));
+    if (c->depth < 0.f) return 0;
+    c->pos = mk3(C.x / t * -radius + disc.x, C.y / t * -radius + disc.y, C.z / t * -radius + disc.z);
+    const float den = r2 - c->depth;
+    c->n = mk3((c->pos.x - ps.x) / den, (c->pos.y - ps.y) / den, (c->pos.z - ps.z) / den);
+    return 1;
+  }
+  if ((radius - t) <= s && (radius - t) <= (length - s)) {
+    c->depth = (r2 + radius) - t;
+    if (c->depth < 0.f) return 0;
+    if (t > r2 + CYL_TOL) {
+      C = mk3(C.x / t, C.y / t, C.z / t);
+      c->pos = mk3(C.x * r2 + ps.x, C.y * r2 + ps.y, C.z * r2 + ps.z);
+      c->n = C;
+    } else {
+      c->pos = C + ps;
+      c->n = mk3(C.x / t, C.y / t, C.z / t);
+    }
+    return 1;
+  }
+  if (s <= length * 0.5f) {
+    c->depth = s + r2;
+    if (c->depth < 0.f) return 0;
+    c->pos = mk3(r2 * ax.x + ps.x, r2 * ax.y + ps.y, r2 * ax.z + ps.z);
+    c->n = ax;
+  } else {
+    c->depth = r2 + length - s;
+    if (c->depth < 0.f) return 0;
+    c->pos = mk3(r2 * -ax.x + ps.x, r2 * -ax.y + ps.y, r2 * -ax.z + ps.z);
+    c->n = -ax;
+  }
+  return 1;
+}
+
+// cylinder-plane: collision_cylinder_plane.cpp:40-267 -- a disc parallel to
+// the plane gives up to four rim points, otherwise the deepest rim point of
+// each disc
+__device__ inline int coll_cylinder_plane(const GeomView &cyl, float4 pl, int maxc, CPoint *c) {
+  const float radius = cyl.prm.x, length = cyl.prm.y;
+  const float3 ax = capsule_axis(cyl.R), pn = mk3(pl.x, pl.y, pl.z);
+  const float h = length * 0.5f;
+  const float3 top = mk3(ax.x * h + cyl.p.x, ax.y * h + cyl.p.y, ax.z * h + cyl.p.z);
+  const float3 bot = mk3(ax.x * -h + cyl.p.x, ax.y * -h + cyl.p.y, ax.z * -h + cyl.p.z);
+  int n = 0;
+  // a candidate point; `plain`: depth from the dot product (else the
+  // expanded form the reference writes for the second disc)
+  auto emit = [&](float3 q, bool strict, bool plain) -> bool {
+    const float d = plain ? pl.w - dot(pn, q) : pl.w - pl.x * q.x - pl.y * q.y - pl.z * q.z;
+    if (strict ? d > 0.f : d >= 0.f) {
+      c[n].pos = q;
+      c[n].n = pn;
+      c[n].depth = d;
+      n++;
+      return n >= maxc;
+    }
+    return false;
+  };
+  float s = ax.x * pl.x + ax.y * pl.y + ax.z * pl.z;
+  if (s < 0.f) s += 1.0f;
+  else s -= 1.0f;
+  if (s < CYL_TOL && s > -CYL_TOL) {
+    const float d1 = pl.w - dot(pn, bot), d2 = pl.w - dot(pn, top);
+    float3 P;
+    if (d1 >= d2) {
+      if (!(d1 >= 0.f)) return 0;
+      P = bot;
+    } else {
+      if (!(d2 >= 0.f)) return 0;
+      P = top;
+    }
+    float3 V1 = (ax.x < CYL_TOL && ax.x > -CYL_TOL) ? mk3(ax.x + 1.0f, ax.y, ax.z) : mk3(ax.x, ax.y + 1.0f, ax.z);
+    float3 V2 = cross(V1, ax);
+    float l = sqrtf(V2.x * V2.x + V2.y * V2.y + V2.z * V2.z);
+    l = radius / l;
+    V2 = mk3(V2.x * l, V2.y * l, V2.z * l);
+    V1 = cross(V2, ax);
+    if (emit(P + V1, true, true)) return n;
+    if (emit(P - V1, true, true)) return n;
+    if (emit(P + V2, true, true)) return n;
+    emit(P - V2, true, true);
+    return n;
+  }
+  const float t = dot(pn, ax);
+  float3 C = mk3(ax.x * t - pl.x, ax.y * t - pl.y, ax.z * t - pl.z);
+  float l = sqrtf(C.x * C.x + C.y * C.y + C.z * C.z);
+  l = radius / l;
+  C = mk3(C.x * l, C.y * l, C.z * l);
+  if (emit(C + bot, false, true)) return n;
+  emit(C + top, false, false);
+  return n;
+}
+
+// cylinder-box: collision_cylinder_box.cpp -- separating axes over the box
+// faces, the cylinder axis, their cross products, the box vertices and the
+// twelve box edges against both rims; then the cylinder's side segment
+// clipped by the box, or the box face clipped to an 8-gon of the disc
+struct CylBoxDev {
+  float3 pc, pb, ax, half, diff, nrm;
+  Mat3 Rc, Rb;
+  float radius, size, best_depth, best_rb, best_rc;
+  int best_axis;
+  float3 vert[8];
+
+  __device__ CylBoxDev(const GeomView &cyl, const GeomView &box) {
+    pc = cyl.p; pb = box.p; Rc = cyl.R; Rb = box.R;
+    radius = cyl.prm.x; size = cyl.prm.y;
+    ax = capsule_axis(Rc);
+    half = mk3(box.prm.x * 0.5f, box.prm.y * 0.5f, box.prm.z * 0.5f);
+    // the vertex numbering of collision_cylinder_box.cpp:130-180
+    const float sx[8] = {-1, 1, -1, 1, 1, 1, -1, -1}, sy[8] = {1, 1, -1, -1, 1, -1, -1, 1},
+                sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+#pragma unroll
+    for (int v = 0; v < 8; v++) vert[v] = Rb.mul(mk3(sx[v] * half.x, sy[v] * half.y, sz[v] * half.z)) + pb;
+    diff = pc - pb;
+    nrm = mk3(0.f, 0.f, 0.f);
+    best_depth = __int_as_float(0x7f800000);
+    best_rb = best_rc = 0.f;
+    best_axis = 0;
+  }
+
+  // collision_cylinder_box.cpp:206-272
+  __device__ bool test_axis(float3 n, int id) {
+    const float l = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+    if (l < 1e-5f) return true;
+    n = safe_normalize_div(n);
+    const float d1 = dot(ax, n);
+    float rc;
+    if (d1 > 1.0f || d1 < -1.0f) rc = size * 0.5f;
+    else rc = fabsf(d1 * (size * 0.5f)) + radius * sqrtf(1.0f - d1 * d1);
+    float rb = fabsf(dot(Rb.col(0), n)) * half.x;
+    rb += fabsf(dot(Rb.col(1), n)) * half.y;
+    rb += fabsf(dot(Rb.col(2), n)) * half.z;
+    const float fd = dot(diff, n);
+    float depth = rc + rb;
+    if (fabsf(fd) > depth) return false;
+    depth -= fabsf(fd);
+    if (depth < best_depth) {
+      best_depth = depth;
+      nrm = fd > 0.f ? -n : n;
+      best_axis = id;
+      best_rb = rb;
+      best_rc = rc;
+    }
+    return true;
+  }
+
+  // collision_cylinder_box.cpp:275-320
+  __device__ bool test_edge_rim(float3 centre, float3 v0, float3 v1, int id) {
+    const float3 dir = safe_normalize_div(v1 - v0);
+    const float d2 = dot(dir, ax);
+    if (fabsf(d2) < 1e-5f) return true;
+    const float d1 = dot(centre - v0, ax);
+    const float k = d1 / d2;
+    const float3 p = mk3(v0.x + dir.x * k, v0.y + dir.y * k, v0.z + dir.z * k);
+    const float3 tan = cross(centre - p, ax);
+    return test_axis(cross(tan, dir), id);
+  }
+
+  // collision_cylinder_box.cpp:323-560
+  __device__ bool separated() {
+    for (int k = 0; k < 3; k++)
+      if (!test_axis(Rb.col(k), 1 + k)) return true;
+    if (!test_axis(ax, 4)) return true;
+    for (int k = 0; k < 3; k++) {
+      const float3 a = cross(ax, Rb.col(k));
+      if (dot(a, a) > 1e-6f && !test_axis(a, 5 + k)) return true;
+    }
+    for (int v = 0; v < 8; v++) {
+      const float3 a = cross(ax, cross(ax, vert[v] - pc));
+      if (dot(a, a) > 1e-6f && !test_axis(a, 8 + v)) return true;
+    }
+    const int e0[12] = {1, 1, 2, 2, 4, 4, 0, 5, 5, 2, 4, 6}, e1[12] = {0, 3, 3, 0, 1, 7, 7, 3, 6, 6, 5, 7};
+    const float h = size * 0.5f;
+    for (int side = 0; side < 2; side++) {
+      const float3 cc = side == 0 ? mk3(pc.x + ax.x * h, pc.y + ax.y * h, pc.z + ax.z * h)
+                                  : mk3(pc.x - ax.x * h, pc.y - ax.y * h, pc.z - ax.z * h);
+      for (int e = 0; e < 12; e++)
+        if (!test_edge_rim(cc, vert[e0[e]], vert[e1[e]], 16 + 12 * side + e)) return true;
+    }
+    return false;
+  }
+
+  // collision_util.cpp:471-509
+  __device__ static bool clip_edge(float3 &a, float3 &b, float4 pl) {
+    const float d0 = pl.x * a.x + pl.y * a.y + pl.z * a.z + pl.w;
+    const float d1 = pl.x * b.x + pl.y * b.y + pl.z * b.z + pl.w;
+    if (d0 < 0.f && d1 < 0.f) return false;
+    if (d0 > 0.f && d1 > 0.f) return true;
+    if ((d0 > 0.f && d1 < 0.f) || (d0 < 0.f && d1 > 0.f)) {
+      const float3 x = mk3(a.x - (a.x - b.x) * d0 / (d0 - d1), a.y - (a.y - b.y) * d0 / (d0 - d1),
+                           a.z - (a.z - b.z) * d0 / (d0 - d1));
+      if (d0 < 0.f) a = x;
+      else b = x;
+    }
+    return true;
+  }
+
+  // collision_util.cpp:512-555 (at most 16 points: 4 corners + one per plane)
+  __device__ static void clip_poly(const float3 *in, int nin, float3 *out, int &nout, float4 pl) {
+    nout = 0;
+    for (int i1 = 0, i0 = nin - 1; i1 < nin; i0 = i1, i1++) {
+      const float d0 = pl.x * in[i0].x + pl.y * in[i0].y + pl.z * in[i0].z + pl.w;
+      const float d1 = pl.x * in[i1].x + pl.y * in[i1].y + pl.z * in[i1].z + pl.w;
+      if (d0 >= 0.f) out[nout++] = in[i0];
+      if ((d0 > 0.f && d1 < 0.f) || (d0 < 0.f && d1 > 0.f))
+        out[nout++] = mk3(in[i0].x - (in[i0].x - in[i1].x) * d0 / (d0 - d1),
+                          in[i0].y - (in[i0].y - in[i1].y) * d0 / (d0 - d1),
+                          in[i0].z - (in[i0].z - in[i1].z) * d0 / (d0 - d1));
+    }
+  }
+
+  // collision_cylinder_box.cpp:563-680
+  __device__ int clip_side(int maxc, CPoint *c) {
+    const float t = dot(ax, nrm);
+    const float3 vn = safe_normalize_div(mk3(nrm.x - ax.x * t, nrm.y - ax.y * t, nrm.z - ax.z * t));
+    const float h = size * 0.5f;
+    const float3 q = mk3(pc.x + vn.x * radius, pc.y + vn.y * radius, pc.z + vn.z * radius);
+    float3 a = mk3(q.x + ax.x * h, q.y + ax.y * h, q.z + ax.z * h) - pb;
+    float3 b = mk3(q.x - ax.x * h, q.y - ax.y * h, q.z - ax.z * h) - pb;
+    for (int sg = 0; sg < 2; sg++)
+      for (int k = 0; k < 3; k++) {
+        float3 col = Rb.col(k);
+        if (sg) col = -col;
+        if (!clip_edge(a, b, make_float4(col.x, col.y, col.z, comp(half, k)))) return 0;
+      }
+    float dep0 = best_rb + dot(a, nrm), dep1 = best_rb + dot(b, nrm);
+    if (dep0 < 0.f) dep0 = 0.f;
+    if (dep1 < 0.f) dep1 = 0.f;
+    int n = 0;
+    c[n].pos = a + pb; c[n].n = -nrm; c[n].depth = dep0; n++;
+    if (n != maxc) { c[n].pos = b + pb; c[n].n = -nrm; c[n].depth = dep1; n++; }
+    return n;
+  }
+
+  // collision_util.h:176-210, the inverse as written there (the division
+  // binds to the second product of the diagonal terms)
+  __device__ static Mat3 inv3(const Mat3 &M) {
+    const float m0 = M.r0.x, m1 = M.r0.y, m2 = M.r0.z, m4 = M.r1.x, m5 = M.r1.y, m6 = M.r1.z, m8 = M.r2.x,
+                m9 = M.r2.y, m10 = M.r2.z;
+    const float det = m0 * (m5 * m10 - m9 * m6) - m1 * (m4 * m10 - m8 * m6) + m2 * (m4 * m9 - m8 * m5);
+    Mat3 d;
+    if (fabsf(det) < 0.0005f) {
+      d.r0 = mk3(1.f, 0.f, 0.f); d.r1 = mk3(0.f, 1.f, 0.f); d.r2 = mk3(0.f, 0.f, 1.f);
+      return d;
+    }
+    d.r0 = mk3(m5 * m10 - m6 * m9 / det, -(m1 * m10 - m9 * m2) / det, m1 * m6 - m5 * m2 / det);
+    d.r1 = mk3(-(m4 * m10 - m6 * m8) / det, m0 * m10 - m8 * m2 / det, -(m0 * m6 - m4 * m2) / det);
+    d.r2 = mk3(m4 * m9 - m8 * m5 / det, -(m0 * m9 - m8 * m1) / det, m0 * m5 - m1 * m4 / det);
+    return d;
+  }
+
+  // collision_cylinder_box.cpp:683-1000
+  __device__ int clip_face(int maxc, CPoint *c) {
+    const float h = size * 0.5f;
+    float3 cpos;
+    float cz;
+    if (dot(ax, nrm) > 0.f) { cpos = mk3(pc.x + ax.x * h, pc.y + ax.y * h, pc.z + ax.z * h); cz = -1.0f; }
+    else { cpos = mk3(pc.x - ax.x * h, pc.y - ax.y * h, pc.z - ax.z * h); cz = 1.0f; }
+    const float3 nr = inv3(Rb).mul(nrm);
+    const float a0 = fabsf(nr.x), a1 = fabsf(nr.y), a2 = fabsf(nr.z);
+    int b0, b1, b2;
+    if (a1 > a0) {
+      if (a0 > a2) { b0 = 1; b1 = 0; b2 = 2; }
+      else if (a1 > a2) { b0 = 1; b1 = 2; b2 = 0; }
+      else { b0 = 2; b1 = 1; b2 = 0; }
+    } else {
+      if (a1 > a2) { b0 = 0; b1 = 1; b2 = 2; }
+      else if (a0 > a2) { b0 = 0; b1 = 2; b2 = 1; }
+      else { b0 = 2; b1 = 0; b2 = 1; }
+    }
+    const float3 c0 = Rb.col(b0);
+    const float hb0 = comp(half, b0);
+    const float3 ctr = comp(nr, b0) > 0.f ? mk3(pb.x - hb0 * c0.x, pb.y - hb0 * c0.y, pb.z - hb0 * c0.z)
+                                          : mk3(pb.x + hb0 * c0.x, pb.y + hb0 * c0.y, pb.z + hb0 * c0.z);
+    const float3 u1 = comp(half, b1) * Rb.col(b1), u2 = comp(half, b2) * Rb.col(b2);
+    float3 A[16], B[16];
+    float3 pts[4];
+    pts[0] = (ctr + u1) - u2;
+    pts[1] = (ctr - u1) - u2;
+    pts[2] = (ctr - u1) + u2;
+    pts[3] = (ctr + u1) + u2;
+    const Mat3 ci = inv3(Rc);
+    for (int k = 0; k < 4; k++) pts[k] = ci.mul(pts[k] - cpos);
+    int n1 = 0, n2 = 0;
+    clip_poly(pts, 4, A, n1, make_float4(0.f, 0.f, cz, 0.f));
+    float ang = (float)(3.14159265358979323846 / 8);
+    const float inc = ang * 2.0f;
+    for (int seg = 0; seg < 8; seg++) {
+      const float4 pl = make_float4(-cosf(ang), -sinf(ang), 0.f, radius);
+      ang += inc;
+      if (seg % 2 == 0) clip_poly(A, n1, B, n2, pl);
+      else clip_poly(B, n2, A, n1, pl);
+    }
+    int n = 0;
+    for (int k = 0; k < n1; k++) {
+      const float3 p = Rc.mul(A[k]) + cpos;
+      const float d = best_rc - dot(p - pc, nrm);
+      if (d > 0.f) {
+        c[n].pos = p; c[n].n = -nrm; c[n].depth = d;
+        if (++n == maxc) break;
+      }
+    }
+    return n;
+  }
+};
+
+__device__ __attribute__((noinline)) int coll_cylinder_box(const GeomView &cyl, const GeomView &box, int maxc,
+                                                          CPoint *c) {
+  CylBoxDev S(cyl, box);
+  if (S.separated() || S.best_axis == 0) return 0;
+  if (fabsf(dot(S.nrm, S.ax)) < 0.9f) return S.clip_side(maxc, c);
+  return S.clip_face(maxc, c);
+}
+
+// ray-cylinder: ray.cpp:500-656 -- the flat caps when the ray runs inside
+// the infinite cylinder parallel to the axis, else the curved side
+__device__ inline int coll_ray_cylinder(const GeomView &ray, const GeomView &cyl, CPoint *c) {
+  const float len = ray.prm.x, radius = cyl.prm.x, hl = cyl.prm.y * 0.5f;
+  const float3 dir = capsule_axis(ray.R), ax = capsule_axis(cyl.R);
+  float3 r = ray.p - cyl.p;
+  float d = ax.x * r.x + ax.y * r.y + ax.z * r.z;
+  float3 q = mk3(d * ax.x - r.x, d * ax.y - r.y, d * ax.z - r.z);
+  const float C = dot(q, q) - radius * radius;
+  const float uv = ax.x * dir.x + ax.y * dir.y + ax.z * dir.z;
+  r = mk3(uv * ax.x - dir.x, uv * ax.y - dir.y, uv * ax.z - dir.z);
+  float A = dot(r, r);
+  const float B = 2.f * dot(q, r);
+  float k = B * B - 4.f * A * C;
+  if (k < 1.1920929e-07f && C <= 0.f) {
+    const float uvsign = (uv < 0.f) ? -1.0f : 1.0f;
+    const float internal = (d >= -hl && d <= +hl) ? -1.0f : 1.0f;
+    if ((uv > 0.f && d + uvsign * len < hl * internal) || (uv < 0.f && d + uvsign * len > hl * internal)) return 0;
+    c->depth = (-uvsign * d) - (internal * hl);
+    c->pos = mk3(ray.p.x + c->depth * dir.x, ray.p.y + c->depth * dir.y, ray.p.z + c->depth * dir.z);
+    c->n = uvsign * ax;
+    return 1;
+  }
+  if (k > 0.f) {
+    k = sqrtf(k);
+    A = 1.f / (2.f * A);
+    float alpha = (-B - k) * A;
+    if (alpha < 0.f) alpha = (-B + k) * A;
+    if (alpha >= 0.f && alpha <= len) {
+      c->pos = ray.p + alpha * dir;
+      q = c->pos - cyl.p;
+      d = q.x * ax.x + q.y * ax.y + q.z * ax.z;
+      if (d >= -hl && d <= +hl) {
+        const float nsign = (C < 0.f) ? -1.0f : 1.0f;
+        c->n = safe_normalize(nsign * (c->pos - (cyl.p + d * ax)));
+        c->depth = alpha;
+        return 1;
+      }
+    }
+  }
+  return 0;
+}
+
 // the class that comes first in a collider's arguments (the colliders table,
-// collision_kernel.cpp:181-189): capsule, then sphere, box, plane
+// collision_kernel.cpp:181-219): ray, then capsule, cylinder, sphere, box,
+// plane
 __device__ inline int collider_rank(int type) {
-  return type == GEOM_CAPSULE ? 0 : (type == GEOM_SPHERE ? 1 : (type == GEOM_BOX ? 2 : 3));
+  return type == GEOM_RAY ? 0
+         : type == GEOM_CAPSULE ? 1
+         : type == GEOM_CYLINDER ? 2
+         : type == GEOM_SPHERE ? 3
+         : type == GEOM_BOX ? 4 : 5;
 }
 
 __device__ inline int collide_pair(const GeomView &a, const GeomView &b, int maxc, CPoint *c) {
@@ -650,11 +1190,21 @@ __device__ inline int collide_pair(const GeomView &a, const GeomView &b, int max
   const GeomView &x = rev ? b : a;
   const GeomView &y = rev ? a : b;
   int n = 0;
-  if (x.type == GEOM_CAPSULE) {
+  if (x.type == GEOM_RAY) {
+    if (y.type == GEOM_SPHERE) n = ray_sphere_hit(x, y.p, y.prm.x, 0, c);
+    else if (y.type == GEOM_BOX) n = coll_ray_box(x, y, c);
+    else if (y.type == GEOM_CAPSULE) n = coll_ray_capsule(x, y, c);
+    else if (y.type == GEOM_PLANE) n = coll_ray_plane(x, y.prm, c);
+    else if (y.type == GEOM_CYLINDER) n = coll_ray_cylinder(x, y, c);
+  } else if (x.type == GEOM_CAPSULE) {
     if (y.type == GEOM_SPHERE) n = coll_capsule_sphere(x, y.p, y.prm.x, c);
     else if (y.type == GEOM_BOX) n = coll_capsule_box(x, y, c);
     else if (y.type == GEOM_CAPSULE) n = coll_capsule_capsule(x, y, maxc, c);
-    else n = coll_capsule_plane(x, y.prm, maxc, c);
+    else if (y.type == GEOM_PLANE) n = coll_capsule_plane(x, y.prm, maxc, c);
+  } else if (x.type == GEOM_CYLINDER) {
+    if (y.type == GEOM_SPHERE) n = coll_cylinder_sphere(x, y.p, y.prm.x, c);
+    else if (y.type == GEOM_PLANE) n = coll_cylinder_plane(x, y.prm, maxc, c);
+    else if (y.type == GEOM_BOX) n = coll_cylinder_box(x, y, maxc, c);
   } else if (x.type == GEOM_SPHERE) {
     if (y.type == GEOM_SPHERE) n = coll_sphere_sphere(x.p, x.prm.x, y.p, y.prm.x, c);
     else if (y.type == GEOM_BOX) n = coll_sphere_box(x.p, x.prm.x, y.p, y.R, mk3(y.prm.x, y.prm.y, y.prm.z), c);

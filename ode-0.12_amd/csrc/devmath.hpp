@@ -79,6 +79,17 @@ __device__ __forceinline__ float3 safe_normalize(float3 a) {
   a = (1.f / m) * a;
   return rsqrtf(dot(a, a)) * a;
 }
+// the same in the reference's own operations (divide by the largest
+// magnitude, then multiply by the reciprocal square root), for colliders whose
+// contacts follow the reference's rounding closely
+__device__ __forceinline__ float3 safe_normalize_div(float3 a) {
+  float ax = fabsf(a.x), ay = fabsf(a.y), az = fabsf(a.z);
+  float m = (ay > ax) ? ((az > ay) ? az : ay) : ((az > ax) ? az : ax);
+  if (!(m > 0.f)) return mk3(1.f, 0.f, 0.f);
+  a = mk3(a.x / m, a.y / m, a.z / m);
+  const float l = 1.f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+  return mk3(a.x * l, a.y * l, a.z * l);
+}
 __device__ __forceinline__ float4 quat_normalize(float4 q) {
   float l = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
   if (l > 0.f) {

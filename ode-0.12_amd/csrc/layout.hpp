@@ -21,7 +21,7 @@ struct float4 { float x, y, z, w; };
 
 namespace odemi {
 
-enum { GEOM_SPHERE = 0, GEOM_BOX = 1, GEOM_PLANE = 2, GEOM_CAPSULE = 3 };
+enum { GEOM_SPHERE = 0, GEOM_BOX = 1, GEOM_PLANE = 2, GEOM_CAPSULE = 3, GEOM_RAY = 4, GEOM_CYLINDER = 5 };
 enum { JT_CONTACT = 0, JT_BALL = 1, JT_HINGE = 2, JT_HINGE2 = 3 };
 enum { ORDER_REFERENCE = 0, ORDER_COLORED = 1 };
 enum { JF_REVERSE = 2 };
@@ -219,6 +219,13 @@ OD_HD inline long long sor_lds_bytes(int nb, int nrows, int ncons, bool accel = 
                 ((4LL * (2LL * ncons + 8) + 15) & ~15LL) + ((4LL * nrows + 15) & ~15LL);
   if (accel) b += (long long)sizeof(SAccBody) * nb + 8LL * nrows;   // conjugate-gradient state
   return b;
+}
+
+// the part of it held in LDS when the rows stay in global memory (bodies,
+// constraints, colour starts, impulses, conjugate-gradient state): the
+// placement of a world whose whole working set exceeds the pool
+OD_HD inline long long sor_lds_bytes_rows_global(int nb, int nrows, int ncons, bool accel = false) {
+  return sor_lds_bytes(nb, nrows, ncons, accel) - (long long)sizeof(SRow) * nrows;
 }
 
 // the same working set in the world's scratch

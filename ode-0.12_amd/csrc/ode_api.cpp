@@ -96,6 +96,7 @@ struct dxGeom {
   void *data = nullptr;
   bool dirty = true;
   int mindex = -1;
+  int ray_flags = 0;                       // ray FirstContact / BackfaceCull / ClosestHit bits
   virtual ~dxGeom() {}
 };
 
@@ -120,6 +121,8 @@ inline int device_type(int cls) {
     case dBoxClass: return GEOM_BOX;
     case dPlaneClass: return GEOM_PLANE;
     case dCapsuleClass: return GEOM_CAPSULE;
+    case dRayClass: return GEOM_RAY;
+    case dCylinderClass: return GEOM_CYLINDER;
     default: return -1;
   }
 }
@@ -445,7 +448,8 @@ bool tape_prepare(dxWorld *w, const std::vector<ContactIn> &cin) {
   };
   if (!w->mjoints.empty()) return refuse("the world has %zu ball/hinge/hinge2 joints", w->mjoints.size());
   for (dxGeom *g : w->mgeoms)
-    if (g->cls == dCapsuleClass) return refuse("the world has a capsule geom (the adjoint colliders are sphere, box and plane)");
+    if (g->cls != dSphereClass && g->cls != dBoxClass && g->cls != dPlaneClass)
+      return refuse("the world has a capsule, cylinder or ray geom (the adjoint colliders are sphere, box and plane)");
   for (dxBody *b : w->bodies)
     for (int k = 0; k < 3; k++)
       if (b->facc[k] != 0 || b->tacc[k] != 0) return refuse("a body carries a force or torque besides gravity");
@@ -797,6 +801,45 @@ void dMassSetBoxTotal(dMass *m, dReal total, dReal lx, dReal ly, dReal lz) {  //
 void dMassSetBox(dMass *m, dReal density, dReal lx, dReal ly, dReal lz) {
   dMassSetBoxTotal(m, lx * ly * lz * density, lx, ly, lz);
 }
+// mass.cpp:141-162: a cylinder of the given length plus two hemispherical
+// caps, axis along `direction` (1..3)
+void dMassSetCapsule(dMass *m, dReal density, int direction, dReal radius, dReal length) {
+  if (direction < 1 || direction > 3) {
+    report_debug("bad direction number");
+    return;
+  }
+  dMassSetZero(m);
+  const double pi = 3.14159265358979323846;
+  dReal M1 = (dReal)(pi * radius * radius * length * density);
+  dReal M2 = (dReal)((4.0f / 3.0f) * pi * radius * radius * radius * density);
+  m->mass = M1 + M2;
+  dReal Ia = M1 * (0.25f * radius * radius + (1.0f / 12.0f) * length * length) +
+             M2 * (0.4f * radius * radius + 0.375f * radius * length + 0.25f * length * length);
+  dReal Ib = (M1 * 0.5f + M2 * 0.4f) * radius * radius;
+  m->I[0] = m->I[5] = m->I[10] = Ia;
+  m->I[5 * (direction - 1)] = Ib;
+}
+void dMassSetCapsuleTotal(dMass *m, dReal total, int direction, dReal radius, dReal length) {
+  dMassSetCapsule(m, 1.0f, direction, radius, length);
+  dMassAdjust(m, total);
+}
+// mass.cpp:173-198: a solid cylinder, axis along `direction` (1..3)
+void dMassSetCylinderTotal(dMass *m, dReal total, int direction, dReal radius, dReal length) {
+  if (direction < 1 || direction > 3) {
+    report_debug("bad direction number");
+    return;
+  }
+  dMassSetZero(m);
+  dReal r2 = radius * radius;
+  m->mass = total;
+  dReal I = total * (0.25f * r2 + (1.0f / 12.0f) * length * length);
+  m->I[0] = m->I[5] = m->I[10] = I;
+  m->I[5 * (direction - 1)] = total * 0.5f * r2;
+}
+void dMassSetCylinder(dMass *m, dReal density, int direction, dReal radius, dReal length) {
+  dMassSetCylinderTotal(m, (dReal)(3.14159265358979323846 * radius * radius * length * density), direction, radius,
+                        length);
+}
 void dMassAdjust(dMass *m, dReal newmass) {  // mass.cpp:441-452
   dReal s = newmass / m->mass;
   m->mass = newmass;
@@ -1076,6 +1119,106 @@ void dGeomCapsuleGetParams(dGeomID g, dReal *radius, dReal *length) {
   *radius = g->param[0];
   *length = g->param[1];
 }
+dGeomID dCreateCylinder(dSpaceID s, dReal radius, dReal length) {   // cylinder.cpp:47-56, 80-83
+  dxGeom *g = make_geom(s, dCylinderClass);
+  g->param[0] = radius;
+  g->param[1] = length;
+  return g;
+}
+void dGeomCylinderSetParams(dGeomID g, dReal radius, dReal length) {
+  if (g->cls != dCylinderClass) {
+    report_debug("argument not a ccylinder");
+    return;
+  }
+  g->param[0] = radius;
+  g->param[1] = length;
+  geom_moved(g);
+  dxWorld *w = g->body ? g->body->world : (g->space ? space_world(g->space) : nullptr);
+  if (w) mark_struct(w);
+}
+void dGeomCylinderGetParams(dGeomID g, dReal *radius, dReal *length) {
+  *radius = g->param[0];
+  *length = g->param[1];
+}
+
+// ---------------------------------------------------------------- rays
+// ray.cpp:47-176; the length is param[0], the direction column 2 of R
+dGeomID dCreateRay(dSpaceID s, dReal length) {
+  dxGeom *g = make_geom(s, dRayClass);
+  g->param[0] = length;
+  return g;
+}
+static bool is_ray(dGeomID g) {
+  if (g && g->cls == dRayClass) return true;
+  report_debug("argument not a ray");
+  return false;
+}
+static dxWorld *geom_world(dxGeom *g) {
+  return g->body ? g->body->world : (g->space ? space_world(g->space) : nullptr);
+}
+void dGeomRaySetLength(dGeomID g, dReal length) {
+  if (!is_ray(g)) return;
+  g->param[0] = length;
+  geom_moved(g);
+  if (dxWorld *w = geom_world(g)) mark_struct(w);
+}
+dReal dGeomRayGetLength(dGeomID g) { return is_ray(g) ? g->param[0] : 0; }
+void dGeomRaySet(dGeomID g, dReal px, dReal py, dReal pz, dReal dx, dReal dy, dReal dz) {
+  if (!is_ray(g)) return;
+  float n[3] = {dx, dy, dz};
+  // dNormalize3 (odemath.h:383): scaled by the largest component first
+  float a0 = std::fabs(n[0]), a1 = std::fabs(n[1]), a2 = std::fabs(n[2]);
+  int k = a1 > a0 ? (a2 > a1 ? 2 : 1) : (a2 > a0 ? 2 : 0);
+  float amax = std::fabs(n[k]);
+  if (amax > 0) {
+    for (int i = 0; i < 3; i++) n[i] /= amax;
+    float l = 1.0f / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int i = 0; i < 3; i++) n[i] *= l;
+  } else {
+    n[0] = 1; n[1] = 0; n[2] = 0;
+  }
+  if (g->body) {
+    // the reference writes the body's own pose here (its position and the
+    // third column of its rotation)
+    const float *R = g->body->R;
+    if (n[0] != R[2] || n[1] != R[6] || n[2] != R[10]) {
+      report_error("dGeomRaySet: the ray is attached to a body; its direction is the body's z axis here");
+      return;
+    }
+    dBodySetPosition(g->body, px, py, pz);
+    return;
+  }
+  g->pos[0] = px; g->pos[1] = py; g->pos[2] = pz;
+  g->R[2] = n[0]; g->R[6] = n[1]; g->R[10] = n[2];
+  geom_moved(g);
+  if (dxWorld *w = geom_world(g)) mark_struct(w);
+}
+void dGeomRayGet(dGeomID g, dVector3 start, dVector3 dir) {
+  if (!is_ray(g)) return;
+  const float *p = dGeomGetPosition(g), *R = dGeomGetRotation(g);
+  for (int i = 0; i < 3; i++) {
+    start[i] = p[i];
+    dir[i] = R[4 * i + 2];
+  }
+}
+enum { RAY_FIRST_CONTACT = 1, RAY_BACKFACE_CULL = 2, RAY_CLOSEST_HIT = 4 };
+void dGeomRaySetParams(dGeomID g, int first_contact, int backface_cull) {
+  if (!is_ray(g)) return;
+  g->ray_flags = (g->ray_flags & RAY_CLOSEST_HIT) | (first_contact ? RAY_FIRST_CONTACT : 0) |
+                 (backface_cull ? RAY_BACKFACE_CULL : 0);
+}
+void dGeomRayGetParams(dGeomID g, int *first_contact, int *backface_cull) {
+  if (!is_ray(g)) return;
+  *first_contact = (g->ray_flags & RAY_FIRST_CONTACT) != 0;
+  *backface_cull = (g->ray_flags & RAY_BACKFACE_CULL) != 0;
+}
+void dGeomRaySetClosestHit(dGeomID g, int closest_hit) {
+  if (!is_ray(g)) return;
+  if (closest_hit) g->ray_flags |= RAY_CLOSEST_HIT;
+  else g->ray_flags &= ~RAY_CLOSEST_HIT;
+}
+int dGeomRayGetClosestHit(dGeomID g) { return is_ray(g) ? (g->ray_flags & RAY_CLOSEST_HIT) != 0 : 0; }
+
 void dGeomDestroy(dGeomID g) {
   if (is_space(g)) {
     dSpaceDestroy(static_cast<dxSpace *>(g));
@@ -1185,6 +1328,19 @@ void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
       float e = std::fabs(R[4 * k + 2] * g->param[1]) * 0.5f + g->param[0];
       aabb[2 * k] = p[k] - e;
       aabb[2 * k + 1] = p[k] + e;
+    }
+  } else if (g->cls == dCylinderClass) {  // cylinder.cpp:58-74
+    for (int k = 0; k < 3; k++) {
+      float e = std::fabs(R[4 * k] * g->param[0]) + std::fabs(R[4 * k + 1] * g->param[0]) +
+                0.5f * std::fabs(R[4 * k + 2] * g->param[1]);
+      aabb[2 * k] = p[k] - e;
+      aabb[2 * k + 1] = p[k] + e;
+    }
+  } else if (g->cls == dRayClass) {       // ray.cpp:56-88
+    for (int k = 0; k < 3; k++) {
+      float e = p[k] + R[4 * k + 2] * g->param[0];
+      aabb[2 * k] = p[k] < e ? p[k] : e;
+      aabb[2 * k + 1] = p[k] < e ? e : p[k];
     }
   } else {
     for (int k = 0; k < 6; k++) aabb[k] = (k & 1) ? INFINITY : -INFINITY;

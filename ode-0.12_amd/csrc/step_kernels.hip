@@ -1183,22 +1183,31 @@ __device__ void sor_prepare(const DevBatch &B, const Ctx &c, const WorldParams &
   __syncthreads();
 }
 
-// The sweeps.  LDS: the working set after the rows is copied to the LDS pool
-// first and back at the end; the choice is a template argument so that every
-// access compiles to LDS (or global) instructions rather than generic flat
-// ones.  Writes the iteration count and leaves the last residual in the
-// world's misc block.
-template <bool TOL, bool LDS>
+// Placement of the coloured solver's working set during the sweeps: all in
+// global scratch, all in the LDS pool, or everything but the (read-only)
+// rows in the pool.
+enum { SOR_IN_GLOBAL = 0, SOR_IN_LDS = 1, SOR_ROWS_GLOBAL = 2 };
+
+// The sweeps.  The working set (rows included, or all but the rows) is
+// copied to the LDS pool first and back at the end; the placement is a
+// template argument so that every access compiles to LDS (or global)
+// instructions rather than generic flat ones.  Writes the iteration count
+// and leaves the last residual in the world's misc block.
+template <bool TOL, int PLACE>
 __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, const Ctx &c, const WorldParams &P,
                                                          int nrows, int ngroups, int nact, bool host_contacts,
                                                          Shared &sh) {
+  constexpr bool LDS = PLACE != SOR_IN_GLOBAL;
+  constexpr bool ROWS_LDS = PLACE == SOR_IN_LDS;
   const int nb = c.W.nb;
   const bool use_tol = TOL && P.tol > 0.f;
   const bool accel = use_tol && P.accel != 0;
   const bool warm = P.warm != 0.f && !host_contacts && B.warm_n != nullptr && B.warm_n[c.w] >= 0;
   const int nt = blockDim.x;
   const SorSet G = sor_global(c, nrows, nact);
-  const SorSet S = LDS ? sor_set((char *)((SRow *)od_pool + nrows), (SRow *)od_pool, nb, nrows, nact) : G;
+  const SorSet S = ROWS_LDS ? sor_set((char *)((SRow *)od_pool + nrows), (SRow *)od_pool, nb, nrows, nact)
+                 : LDS      ? sor_set((char *)od_pool, G.sr, nb, nrows, nact)
+                            : G;
   SRow *__restrict__ sr = S.sr;
   SBody *__restrict__ sb = S.sb;
   SCons *__restrict__ sc = S.sc;
@@ -1207,7 +1216,8 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   SAccBody *__restrict__ sa = S.sa;
   float2 *__restrict__ ld = S.ld;
   if (LDS) {
-    for (int r = threadIdx.x; r < nrows; r += nt) sr[r] = G.sr[r];
+    if (ROWS_LDS)
+      for (int r = threadIdx.x; r < nrows; r += nt) sr[r] = G.sr[r];
     for (int b = threadIdx.x; b < nb; b += nt) sb[b] = G.sb[b];
     for (int x = threadIdx.x; x < nact; x += nt) sc[x] = G.sc[x];
     for (int k = threadIdx.x; k <= ngroups; k += nt) sstart[k] = G.sstart[k];
@@ -1366,7 +1376,16 @@ template <bool TOL>
 __device__ __attribute__((noinline)) void sor_solve_global(const DevBatch &B, const Ctx &c, const WorldParams &P,
                                                            int nrows, int ngroups, int nact, bool host_contacts,
                                                            Shared &sh) {
-  sor_solve<TOL, false>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  sor_solve<TOL, SOR_IN_GLOBAL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+}
+
+// the sweeps with the rows read from global scratch and the rest of the
+// working set in the pool (worlds whose rows do not fit); out of line too
+template <bool TOL>
+__device__ __attribute__((noinline)) void sor_solve_rows_global(const DevBatch &B, const Ctx &c,
+                                                                const WorldParams &P, int nrows, int ngroups,
+                                                                int nact, bool host_contacts, Shared &sh) {
+  sor_solve<TOL, SOR_ROWS_GLOBAL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
 }
 
 template <bool TOL>
@@ -1374,7 +1393,9 @@ __device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const Worl
                                      int nact, bool host_contacts, Shared &sh) {
   const bool accel = TOL && P.tol > 0.f && P.accel != 0;
   if (sor_lds_bytes(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
-    sor_solve<TOL, true>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+    sor_solve<TOL, SOR_IN_LDS>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  else if (sor_lds_bytes_rows_global(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
+    sor_solve_rows_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
   else
     sor_solve_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
 }
@@ -2304,7 +2325,7 @@ __device__ void sor_world(const DevBatch &B, int w, Shared &sh) {
   // integration kernel, in global scratch
   if (sor_lds_bytes(c.W.nb, nrows, nact, accel) > B.lds_bytes) return;
   c.clk.start(B, w);
-  sor_solve<TOL, true>(B, c, P, nrows, ngroups, nact, false, sh);
+  sor_solve<TOL, SOR_IN_LDS>(B, c, P, nrows, ngroups, nact, false, sh);
   c.clk.lap(PH_SOR);
 }
 

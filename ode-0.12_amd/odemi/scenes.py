@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-SPHERE, BOX, PLANE, CAPSULE = 0, 1, 2, 3
+SPHERE, BOX, PLANE, CAPSULE, CYLINDER = 0, 1, 2, 3, 5
 BALL, HINGE, HINGE2 = 1, 2, 3
 
 # contact surface mode bits (include/ode/contact.h:33-49)
@@ -141,6 +141,15 @@ class _Builder:
         self.geom(CAPSULE, (radius, length, 0.0, 0.0), b)
         return b
 
+    def cylinder_body(self, pos, radius, length, density, **kw) -> int:
+        """A solid cylinder along its body z axis; mass and inertia of
+        dMassSetCylinder (mass.cpp:173-198)."""
+        m = math.pi * radius * radius * length * density
+        i = m * (0.25 * radius * radius + length * length / 12.0)
+        b = self.body(pos, m, np.diag([i, i, m * 0.5 * radius * radius]), **kw)
+        self.geom(CYLINDER, (radius, length, 0.0, 0.0), b)
+        return b
+
     def geom(self, gtype, param, body=-1, pos=(0.0, 0.0, 0.0), q=(1.0, 0.0, 0.0, 0.0)) -> int:
         self.g_type.append(gtype)
         self.g_body.append(body)
@@ -244,6 +253,50 @@ def capsule_pile(n: int = 30, seed: int = 19) -> Scene:
         q = tuple(q / np.linalg.norm(q))
         if kind == 0:
             b.capsule_body(pos, r, ln, dens, q=q)
+        elif kind == 1:
+            b.sphere_body(pos, r, dens)
+        else:
+            b.box_body(pos, tuple(2.0 * he), dens, q=q)
+    return b.scene(dt=0.01, mu=0.6, seed=seed)
+
+
+def cylinder_pile(n: int = 32, seed: int = 31) -> Scene:
+    """Cylinders (radius U[0.15,0.35], length U[0.2,0.8]; half of them upright
+    or on their side, the rest at random orientations) with spheres and boxes
+    dropped on a plane, spawned without overlap of their bounding spheres;
+    h = 0.01, mu = 0.6.  Every cylinder collider of the reference without
+    libccd (cylinder-sphere, -box, -plane) meets its pairs; two cylinders have
+    no collider there and pass through each other, here as in the reference."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
+    clear = _Clearance()
+    half = 0.3 * math.sqrt(n)
+    for i in range(n):
+        dens = rng.uniform(0.5, 2.0)
+        kind = i % 3
+        if kind == 0:
+            r, ln = rng.uniform(0.15, 0.35), rng.uniform(0.2, 0.8)
+            rad = math.hypot(r, 0.5 * ln)
+        elif kind == 1:
+            r = rng.uniform(0.1, 0.4)
+            rad = r
+        else:
+            he = rng.uniform(0.1, 0.4, size=3)
+            rad = float(np.linalg.norm(he))
+        for _ in range(200):
+            pos = (rng.uniform(-half, half), rng.uniform(-half, half), rng.uniform(0.4, 0.4 + n / 8.0))
+            if clear.free([pos], [rad]):
+                break
+        clear.add([pos], [rad])
+        q = rng.normal(size=4)
+        q = tuple(q / np.linalg.norm(q))
+        if kind == 0:
+            if i % 4 == 0:
+                q = (1.0, 0.0, 0.0, 0.0)                       # upright: a disc on the plane
+            elif i % 4 == 2:
+                q = (math.sqrt(0.5), math.sqrt(0.5), 0.0, 0.0)  # on its side
+            b.cylinder_body(pos, r, ln, dens, q=q)
         elif kind == 1:
             b.sphere_body(pos, r, dens)
         else:

@@ -1,6 +1,6 @@
 // orc_collide.hpp -- CPU restatement of the narrowphase colliders on the path
-// (sphere-sphere, sphere-box, sphere-plane, box-box, box-plane), the geom
-// AABBs and the collider dispatch.
+// (sphere-sphere, sphere-box, sphere-plane, box-box, box-plane, the capsule,
+// ray and cylinder colliders), the geom AABBs and the collider dispatch.
 //
 // TEST INFRASTRUCTURE (oracle).  Each function cites the reference code whose
 // behaviour it restates.  The contact arrays hold up to `maxc` contacts of
@@ -11,7 +11,7 @@
 
 namespace orc {
 
-enum { G_SPHERE = 0, G_BOX = 1, G_PLANE = 2, G_CAPSULE = 3 };
+enum { G_SPHERE = 0, G_BOX = 1, G_PLANE = 2, G_CAPSULE = 3, G_RAY = 4, G_CYLINDER = 5 };
 static const int CONTACTS_UNIMPORTANT = (int)0x80000000;
 
 template <class R> struct ContactGeom {
@@ -50,6 +50,23 @@ template <class R> void geom_aabb(const GeomShape<R> &g, R *aabb) {
       R ext = std::fabs(M[4 * k + 2] * g.param[1]) * R(0.5) + g.param[0];
       aabb[2 * k] = g.pos[k] - ext;
       aabb[2 * k + 1] = g.pos[k] + ext;
+    }
+  } else if (g.type == G_CYLINDER) {
+    // cylinder.cpp:58-74: the radius along columns 0 and 1 of R, half the
+    // length along column 2
+    const R *M = g.Rm;
+    for (int k = 0; k < 3; k++) {
+      R ext = std::fabs(M[4 * k] * g.param[0]) + std::fabs(M[4 * k + 1] * g.param[0]) +
+              R(0.5) * std::fabs(M[4 * k + 2] * g.param[1]);
+      aabb[2 * k] = g.pos[k] - ext;
+      aabb[2 * k + 1] = g.pos[k] + ext;
+    }
+  } else if (g.type == G_RAY) {
+    // ray.cpp:56-88: the box of the start point and the end point
+    for (int k = 0; k < 3; k++) {
+      R e = g.pos[k] + g.Rm[4 * k + 2] * g.param[0];
+      if (g.pos[k] < e) { aabb[2 * k] = g.pos[k]; aabb[2 * k + 1] = e; }
+      else { aabb[2 * k] = e; aabb[2 * k + 1] = g.pos[k]; }
     }
   } else {
     const R *p = g.param;
@@ -737,16 +754,645 @@ int collide_capsule_plane(const R *pc, const R *Rc, R rad, R len, const R *plane
   return n;
 }
 
+// ------------------------------------------------------------ rays
+// A ray (param: length) starts at its position along column 2 of its
+// rotation; the contact depth is the distance along the ray.
+
+// ray.cpp:196-232: the entry point (or, for mode 1 from outside, the exit
+// point) of the ray through a sphere
+template <class R>
+int ray_sphere_hit(const R *pr, const R *Rr, R len, const R *ps, R rad, int mode, ContactGeom<R> *c) {
+  R q[3] = {pr[0] - ps[0], pr[1] - ps[1], pr[2] - ps[2]};
+  R B = q[0] * Rr[2] + q[1] * Rr[6] + q[2] * Rr[10];
+  R C = dot3(q, q) - rad * rad;
+  R k = B * B - C;
+  if (k < 0) return 0;
+  k = std::sqrt(k);
+  R alpha;
+  if (mode && C >= 0) {
+    alpha = -B + k;
+    if (alpha < 0) return 0;
+  } else {
+    alpha = -B - k;
+    if (alpha < 0) {
+      alpha = -B + k;
+      if (alpha < 0) return 0;
+    }
+  }
+  if (alpha > len) return 0;
+  for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + alpha * Rr[4 * i + 2];
+  R nsign = (C < 0 || mode) ? R(-1.0) : R(1.0);
+  for (int i = 0; i < 3; i++) c->normal[i] = nsign * (c->pos[i] - ps[i]);
+  normalize3(c->normal);
+  c->depth = alpha;
+  return 1;
+}
+
+// ray-box: ray.cpp:253-349 (slab intersection in box coordinates)
+template <class R>
+int collide_ray_box(const R *pr, const R *Rr, R len, const R *pb, const R *Rb, const R *side, ContactGeom<R> *c) {
+  R tmp[3], s[3], v[3], sign[3], h[3];
+  for (int i = 0; i < 3; i++) tmp[i] = pr[i] - pb[i];
+  mtv(s, Rb, tmp);
+  for (int i = 0; i < 3; i++) tmp[i] = Rr[4 * i + 2];
+  mtv(v, Rb, tmp);
+  for (int i = 0; i < 3; i++) {
+    if (v[i] < 0) { s[i] = -s[i]; v[i] = -v[i]; sign[i] = 1; }
+    else sign[i] = -1;
+  }
+  for (int i = 0; i < 3; i++) h[i] = R(0.5) * side[i];
+  if ((s[0] < -h[0] && v[0] <= 0) || s[0] > h[0] || (s[1] < -h[1] && v[1] <= 0) || s[1] > h[1] ||
+      (s[2] < -h[2] && v[2] <= 0) || s[2] > h[2] || (v[0] == 0 && v[1] == 0 && v[2] == 0))
+    return 0;
+  R lo = -INFINITY, hi = INFINITY;
+  int nlo = 0, nhi = 0;
+  for (int i = 0; i < 3; i++) {
+    if (v[i] != 0) {
+      R k = (-h[i] - s[i]) / v[i];
+      if (k > lo) { lo = k; nlo = i; }
+      k = (h[i] - s[i]) / v[i];
+      if (k < hi) { hi = k; nhi = i; }
+    }
+  }
+  if (lo > hi) return 0;
+  R alpha;
+  int n;
+  if (lo >= 0) { alpha = lo; n = nlo; }
+  else { alpha = hi; n = nhi; }
+  if (alpha < 0 || alpha > len) return 0;
+  for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + alpha * Rr[4 * i + 2];
+  for (int i = 0; i < 3; i++) c->normal[i] = Rb[4 * i + n] * sign[n];
+  c->depth = alpha;
+  return 1;
+}
+
+// ray-capsule: ray.cpp:352-464 (the infinite cylinder, then the end caps)
+template <class R>
+int collide_ray_capsule(const R *pr, const R *Rr, R len, const R *pc, const R *Rc, R rad, R clen,
+                        ContactGeom<R> *c) {
+  R lz2 = clen * R(0.5);
+  R cs[3] = {pr[0] - pc[0], pr[1] - pc[1], pr[2] - pc[2]};
+  R k = Rc[2] * cs[0] + Rc[6] * cs[1] + Rc[10] * cs[2];
+  R q[3], r[3];
+  for (int i = 0; i < 3; i++) q[i] = k * Rc[4 * i + 2] - cs[i];
+  R C = dot3(q, q) - rad * rad;
+  int inside = 0;
+  if (C < 0) {
+    if (k < -lz2) k = -lz2;
+    else if (k > lz2) k = lz2;
+    for (int i = 0; i < 3; i++) r[i] = pc[i] + k * Rc[4 * i + 2];
+    if ((pr[0] - r[0]) * (pr[0] - r[0]) + (pr[1] - r[1]) * (pr[1] - r[1]) + (pr[2] - r[2]) * (pr[2] - r[2]) <
+        rad * rad)
+      inside = 1;
+  }
+  if (!inside && C < 0) {
+    k = (k < 0) ? -lz2 : lz2;
+  } else {
+    R uv = Rc[2] * Rr[2] + Rc[6] * Rr[6] + Rc[10] * Rr[10];
+    for (int i = 0; i < 3; i++) r[i] = uv * Rc[4 * i + 2] - Rr[4 * i + 2];
+    R A = dot3(r, r);
+    R B = 2 * dot3(q, r);
+    k = B * B - 4 * A * C;
+    if (k < 0) {
+      if (!inside) return 0;
+      k = (uv < 0) ? -lz2 : lz2;
+    } else {
+      k = std::sqrt(k);
+      A = recip(2 * A);
+      R alpha = (-B - k) * A;
+      if (alpha < 0) {
+        alpha = (-B + k) * A;
+        if (alpha < 0) return 0;
+      }
+      if (alpha > len) return 0;
+      for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + alpha * Rr[4 * i + 2];
+      for (int i = 0; i < 3; i++) q[i] = c->pos[i] - pc[i];
+      k = q[0] * Rc[2] + q[1] * Rc[6] + q[2] * Rc[10];
+      R nsign = inside ? R(-1.0) : R(1.0);
+      if (k >= -lz2 && k <= lz2) {
+        for (int i = 0; i < 3; i++) c->normal[i] = nsign * (c->pos[i] - (pc[i] + k * Rc[4 * i + 2]));
+        normalize3(c->normal);
+        c->depth = alpha;
+        return 1;
+      }
+      k = (k < 0) ? -lz2 : lz2;
+    }
+  }
+  for (int i = 0; i < 3; i++) q[i] = pc[i] + k * Rc[4 * i + 2];
+  return ray_sphere_hit(pr, Rr, len, q, rad, inside, c);
+}
+
+// ray-plane: ray.cpp:467-497
+template <class R>
+int collide_ray_plane(const R *pr, const R *Rr, R len, const R *plane, ContactGeom<R> *c) {
+  R alpha = plane[3] - dot3(plane, pr);
+  R nsign = (alpha > 0) ? R(-1.0) : R(1.0);
+  R k = plane[0] * Rr[2] + plane[1] * Rr[6] + plane[2] * Rr[10];
+  if (k == 0) return 0;
+  alpha /= k;
+  if (alpha < 0 || alpha > len) return 0;
+  for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + alpha * Rr[4 * i + 2];
+  for (int i = 0; i < 3; i++) c->normal[i] = nsign * plane[i];
+  c->depth = alpha;
+  return 1;
+}
+
+// ------------------------------------------------------------ cylinders
+// A cylinder (param: radius, length) has its axis along column 2 of its
+// rotation.  Without libccd (the reference's config.h) the colliders are
+// cylinder-sphere, cylinder-plane, cylinder-box and ray-cylinder;
+// cylinder-cylinder and capsule-cylinder pairs have none (no contacts).
+
+// the tolerance of the parallel tests, per precision (collision_cylinder_
+// sphere.cpp:66-71, collision_cylinder_plane.cpp:51-56)
+template <class R> inline R cyl_tol() { return sizeof(R) == 4 ? R(0.0001) : R(0.0000001); }
+
+// cylinder-sphere: collision_cylinder_sphere.cpp:57-278 -- s along the axis
+// from the first disc, t from the axis; the side (nappe), a disc or a disc's
+// rim
+template <class R>
+int collide_cylinder_sphere(const R *pc, const R *Rc, R radius, R length, const R *ps, R radius2,
+                            ContactGeom<R> *c) {
+  const R tol = cyl_tol<R>();
+  R ax[3] = {Rc[2], Rc[6], Rc[10]};
+  R h = length * R(0.5);
+  R top[3], bot[3];
+  for (int i = 0; i < 3; i++) {
+    top[i] = ax[i] * h + pc[i];
+    bot[i] = ax[i] * -h + pc[i];
+  }
+  R s = (ps[0] - bot[0]) * ax[0] - (bot[1] - ps[1]) * ax[1] - (bot[2] - ps[2]) * ax[2];
+  if (s < -radius2 || s > length + radius2) return 0;
+  R C[3];
+  for (int i = 0; i < 3; i++) C[i] = s * ax[i] + bot[i] - ps[i];
+  R t = std::sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+  if (t > radius + radius2) return 0;
+  if (t > radius && (s < 0 || s > length)) {
+    // the rim of a disc
+    const R *disc = (s <= 0) ? bot : top;
+    R e = (s <= 0) ? s : s - length;
+    c->depth = radius2 - std::sqrt(e * e + (t - radius) * (t - radius));
+    if (c->depth < 0) return 0;
+    for (int i = 0; i < 3; i++) c->pos[i] = C[i] / t * -radius + disc[i];
+    for (int i = 0; i < 3; i++) c->normal[i] = (c->pos[i] - ps[i]) / (radius2 - c->depth);
+    return 1;
+  }
+  if ((radius - t) <= s && (radius - t) <= (length - s)) {
+    // the side
+    c->depth = (radius2 + radius) - t;
+    if (c->depth < 0) return 0;
+    if (t > radius2 + tol) {
+      for (int i = 0; i < 3; i++) C[i] /= t;
+      for (int i = 0; i < 3; i++) c->pos[i] = C[i] * radius2 + ps[i];
+      for (int i = 0; i < 3; i++) c->normal[i] = C[i];
+    } else {
+      for (int i = 0; i < 3; i++) c->pos[i] = C[i] + ps[i];
+      for (int i = 0; i < 3; i++) c->normal[i] = C[i] / t;
+    }
+    return 1;
+  }
+  // a disc
+  if (s <= length * R(0.5)) {
+    c->depth = s + radius2;
+    if (c->depth < 0) return 0;
+    for (int i = 0; i < 3; i++) c->pos[i] = radius2 * ax[i] + ps[i];
+    for (int i = 0; i < 3; i++) c->normal[i] = ax[i];
+  } else {
+    c->depth = radius2 + length - s;
+    if (c->depth < 0) return 0;
+    for (int i = 0; i < 3; i++) c->pos[i] = radius2 * -ax[i] + ps[i];
+    for (int i = 0; i < 3; i++) c->normal[i] = -ax[i];
+  }
+  return 1;
+}
+
+// cylinder-plane: collision_cylinder_plane.cpp:40-267 -- a disc parallel to
+// the plane gives up to four rim points, otherwise the deepest rim point of
+// each disc
+template <class R>
+int collide_cylinder_plane(const R *pc, const R *Rc, R radius, R length, const R *pl, int flags,
+                           ContactGeom<R> *c) {
+  const R tol = cyl_tol<R>();
+  const int maxc = flags & 0xffff;
+  R ax[3] = {Rc[2], Rc[6], Rc[10]};
+  R h = length * R(0.5);
+  R top[3], bot[3];
+  for (int i = 0; i < 3; i++) {
+    top[i] = ax[i] * h + pc[i];
+    bot[i] = ax[i] * -h + pc[i];
+  }
+  int n = 0;
+  auto emit = [&](const R *p, bool strict, bool plain_dot) -> bool {
+    ContactGeom<R> &k = c[n];
+    for (int i = 0; i < 3; i++) k.pos[i] = p[i];
+    k.depth = plain_dot ? pl[3] - dot3(pl, k.pos) : pl[3] - pl[0] * k.pos[0] - pl[1] * k.pos[1] - pl[2] * k.pos[2];
+    if (strict ? k.depth > 0 : k.depth >= 0) {
+      for (int i = 0; i < 3; i++) k.normal[i] = pl[i];
+      n++;
+      return n >= maxc;
+    }
+    return false;
+  };
+  R s = ax[0] * pl[0] + ax[1] * pl[1] + ax[2] * pl[2];
+  if (s < 0) s += R(1.0);
+  else s -= R(1.0);
+  if (s < tol && s > -tol) {
+    R d1 = pl[3] - dot3(pl, bot);
+    R d2 = pl[3] - dot3(pl, top);
+    const R *P;
+    if (d1 >= d2) {
+      if (!(d1 >= 0)) return 0;
+      P = bot;
+    } else {
+      if (!(d2 >= 0)) return 0;
+      P = top;
+    }
+    R V1[3], V2[3];
+    if (ax[0] < tol && ax[0] > -tol) { V1[0] = ax[0] + R(1.0); V1[1] = ax[1]; V1[2] = ax[2]; }
+    else { V1[0] = ax[0]; V1[1] = ax[1] + R(1.0); V1[2] = ax[2]; }
+    cross3(V2, V1, ax);
+    R l = std::sqrt(V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]);
+    l = radius / l;
+    for (int i = 0; i < 3; i++) V2[i] *= l;
+    cross3(V1, V2, ax);
+    R q[3];
+    for (int i = 0; i < 3; i++) q[i] = P[i] + V1[i];
+    if (emit(q, true, true)) return n;
+    for (int i = 0; i < 3; i++) q[i] = P[i] - V1[i];
+    if (emit(q, true, true)) return n;
+    for (int i = 0; i < 3; i++) q[i] = P[i] + V2[i];
+    if (emit(q, true, true)) return n;
+    for (int i = 0; i < 3; i++) q[i] = P[i] - V2[i];
+    emit(q, true, true);
+    return n;
+  }
+  R t = dot3(pl, ax);
+  R C[3];
+  for (int i = 0; i < 3; i++) C[i] = ax[i] * t - pl[i];
+  R l = std::sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+  l = radius / l;
+  for (int i = 0; i < 3; i++) C[i] *= l;
+  R q[3];
+  for (int i = 0; i < 3; i++) q[i] = C[i] + bot[i];
+  if (emit(q, false, true)) return n;
+  for (int i = 0; i < 3; i++) q[i] = C[i] + top[i];
+  emit(q, false, false);
+  return n;
+}
+
+// cylinder-box: collision_cylinder_box.cpp (separating axes over the box
+// faces, the cylinder axis, their cross products, the box vertices and the
+// box edges against both rims; then the cylinder's side segment clipped to
+// the box, or the box face clipped to an 8-gon of the disc)
+template <class R> struct CylBox {
+  const R *Rc, *pc, *Rb, *pb;
+  R radius, size, ax[3], half[3], vert[8][3], diff[3], normal[3], best_depth, best_rb, best_rc;
+  int best_axis;
+
+  CylBox(const R *pc_, const R *Rc_, R radius_, R size_, const R *pb_, const R *Rb_, const R *side)
+      : Rc(Rc_), pc(pc_), Rb(Rb_), pb(pb_), radius(radius_), size(size_) {
+    for (int i = 0; i < 3; i++) { ax[i] = Rc[4 * i + 2]; half[i] = side[i] * R(0.5); }
+    // box vertices (collision_cylinder_box.cpp:130-180 numbering) in world space
+    static const int sg[8][3] = {{-1, 1, -1}, {1, 1, -1}, {-1, -1, -1}, {1, -1, -1},
+                                 {1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {-1, 1, 1}};
+    for (int v = 0; v < 8; v++) {
+      R loc[3] = {sg[v][0] * half[0], sg[v][1] * half[1], sg[v][2] * half[2]}, w[3];
+      mv(w, Rb, loc);
+      for (int i = 0; i < 3; i++) vert[v][i] = w[i] + pb[i];
+    }
+    for (int i = 0; i < 3; i++) { diff[i] = pc[i] - pb[i]; normal[i] = 0; }
+    best_depth = INFINITY;
+    best_rb = best_rc = 0;
+    best_axis = 0;
+  }
+
+  // collision_cylinder_box.cpp:206-272
+  bool test_axis(R *n, int id) {
+    R l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l < R(1e-5)) return true;
+    normalize3(n);
+    R d1 = dot3(ax, n);
+    R rc;
+    if (d1 > R(1.0) || d1 < R(-1.0)) rc = size * R(0.5);
+    else rc = std::fabs(d1 * (size * R(0.5))) + radius * std::sqrt(R(1.0) - d1 * d1);
+    R rb = std::fabs(dot3(Rb, n, 4, 1)) * half[0];
+    rb += std::fabs(dot3(Rb + 1, n, 4, 1)) * half[1];
+    rb += std::fabs(dot3(Rb + 2, n, 4, 1)) * half[2];
+    R fd = dot3(diff, n);
+    R depth = rc + rb;
+    if (std::fabs(fd) > depth) return false;
+    depth -= std::fabs(fd);
+    if (depth < best_depth) {
+      best_depth = depth;
+      for (int i = 0; i < 3; i++) normal[i] = n[i];
+      best_axis = id;
+      best_rb = rb;
+      best_rc = rc;
+      if (fd > 0) for (int i = 0; i < 3; i++) normal[i] = -normal[i];
+    }
+    return true;
+  }
+
+  // collision_cylinder_box.cpp:275-320: the axis normal to a box edge and the
+  // rim's tangent where the edge's line meets the rim's plane
+  bool test_edge_rim(const R *centre, const R *v0, const R *v1, int id) {
+    R dir[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    normalize3(dir);
+    R d2 = dot3(dir, ax);
+    if (std::fabs(d2) < R(1e-5)) return true;
+    R t[3] = {centre[0] - v0[0], centre[1] - v0[1], centre[2] - v0[2]};
+    R d1 = dot3(t, ax);
+    R p[3];
+    for (int i = 0; i < 3; i++) p[i] = v0[i] + dir[i] * (d1 / d2);
+    for (int i = 0; i < 3; i++) t[i] = centre[i] - p[i];
+    R tan[3], a[3];
+    cross3(tan, t, ax);
+    cross3(a, tan, dir);
+    return test_axis(a, id);
+  }
+
+  // collision_cylinder_box.cpp:323-560
+  bool separated() {
+    R a[3];
+    for (int k = 0; k < 3; k++) {
+      for (int i = 0; i < 3; i++) a[i] = Rb[4 * i + k];
+      if (!test_axis(a, 1 + k)) return true;
+    }
+    for (int i = 0; i < 3; i++) a[i] = ax[i];
+    if (!test_axis(a, 4)) return true;
+    const R eps = R(1e-6);
+    for (int k = 0; k < 3; k++) {
+      cross3(a, ax, Rb + k, 1, 1, 4);
+      if (dot3(a, a) > eps && !test_axis(a, 5 + k)) return true;
+    }
+    for (int v = 0; v < 8; v++) {
+      R t1[3] = {vert[v][0] - pc[0], vert[v][1] - pc[1], vert[v][2] - pc[2]}, t2[3];
+      cross3(t2, ax, t1);
+      cross3(a, ax, t2);
+      if (dot3(a, a) > eps && !test_axis(a, 8 + v)) return true;
+    }
+    // the twelve box edges against the top rim (ids 16-27), then the bottom
+    // rim (ids 28-39), in the reference's edge order
+    static const int edges[12][2] = {{1, 0}, {1, 3}, {2, 3}, {2, 0}, {4, 1}, {4, 7},
+                                     {0, 7}, {5, 3}, {5, 6}, {2, 6}, {4, 5}, {6, 7}};
+    for (int side = 0; side < 2; side++) {
+      R h = size * R(0.5), cc[3];
+      for (int i = 0; i < 3; i++) cc[i] = side == 0 ? pc[i] + ax[i] * h : pc[i] - ax[i] * h;
+      for (int e = 0; e < 12; e++)
+        if (!test_edge_rim(cc, vert[edges[e][0]], vert[edges[e][1]], 16 + 12 * side + e)) return true;
+    }
+    return false;
+  }
+
+  // collision_util.cpp:471-509
+  static bool clip_edge(R *e0, R *e1, const R *pl) {
+    R d0 = pl[0] * e0[0] + pl[1] * e0[1] + pl[2] * e0[2] + pl[3];
+    R d1 = pl[0] * e1[0] + pl[1] * e1[1] + pl[2] * e1[2] + pl[3];
+    if (d0 < 0 && d1 < 0) return false;
+    if (d0 > 0 && d1 > 0) return true;
+    if ((d0 > 0 && d1 < 0) || (d0 < 0 && d1 > 0)) {
+      R x[3];
+      for (int i = 0; i < 3; i++) x[i] = e0[i] - (e0[i] - e1[i]) * d0 / (d0 - d1);
+      R *dst = d0 < 0 ? e0 : e1;
+      for (int i = 0; i < 3; i++) dst[i] = x[i];
+    }
+    return true;
+  }
+
+  // collision_util.cpp:512-555
+  static void clip_poly(const R (*in)[3], int nin, R (*out)[3], int &nout, const R *pl) {
+    nout = 0;
+    for (int i1 = 0, i0 = nin - 1; i1 < nin; i0 = i1, i1++) {
+      R d0 = pl[0] * in[i0][0] + pl[1] * in[i0][1] + pl[2] * in[i0][2] + pl[3];
+      R d1 = pl[0] * in[i1][0] + pl[1] * in[i1][1] + pl[2] * in[i1][2] + pl[3];
+      if (d0 >= 0) {
+        for (int i = 0; i < 3; i++) out[nout][i] = in[i0][i];
+        nout++;
+      }
+      if ((d0 > 0 && d1 < 0) || (d0 < 0 && d1 > 0)) {
+        for (int i = 0; i < 3; i++) out[nout][i] = in[i0][i] - (in[i0][i] - in[i1][i]) * d0 / (d0 - d1);
+        nout++;
+      }
+    }
+  }
+
+  void put(ContactGeom<R> *c, int &n, const R *p, R depth) {
+    for (int i = 0; i < 3; i++) { c[n].pos[i] = p[i]; c[n].normal[i] = -normal[i]; }
+    c[n].depth = depth;
+    n++;
+  }
+
+  // collision_cylinder_box.cpp:563-680: the side segment of the cylinder
+  // facing the box, clipped by the six face planes
+  int clip_side(int maxc, ContactGeom<R> *c) {
+    R t = dot3(ax, normal), vn[3];
+    for (int i = 0; i < 3; i++) vn[i] = normal[i] - ax[i] * t;
+    normalize3(vn);
+    R h = size * R(0.5), e0[3], e1[3];
+    for (int i = 0; i < 3; i++) {
+      R q = pc[i] + vn[i] * radius;
+      e0[i] = q + ax[i] * h;
+      e1[i] = q - ax[i] * h;
+    }
+    for (int i = 0; i < 3; i++) { e0[i] -= pb[i]; e1[i] -= pb[i]; }
+    for (int sgn = 0; sgn < 2; sgn++)
+      for (int k = 0; k < 3; k++) {
+        R pl[4] = {Rb[k], Rb[4 + k], Rb[8 + k], half[k]};
+        if (sgn) { pl[0] = -pl[0]; pl[1] = -pl[1]; pl[2] = -pl[2]; }
+        if (!clip_edge(e0, e1, pl)) return 0;
+      }
+    R dep0 = best_rb + dot3(e0, normal), dep1 = best_rb + dot3(e1, normal);
+    if (dep0 < 0) dep0 = 0;
+    if (dep1 < 0) dep1 = 0;
+    for (int i = 0; i < 3; i++) { e0[i] += pb[i]; e1[i] += pb[i]; }
+    int n = 0;
+    put(c, n, e0, dep0);
+    if (n != maxc) put(c, n, e1, dep1);
+    return n;
+  }
+
+  // collision_util.h:176-210 (the determinant and the inverse as written
+  // there, division binding to the second product of the diagonal and
+  // off-diagonal terms the way the expression parses)
+  static void inv3(const R *m, R *d) {
+    R det = m[0] * (m[5] * m[10] - m[9] * m[6]) - m[1] * (m[4] * m[10] - m[8] * m[6]) +
+            m[2] * (m[4] * m[9] - m[8] * m[5]);
+    for (int i = 0; i < 12; i++) d[i] = 0;
+    if (std::fabs(det) < R(0.0005)) {
+      d[0] = d[5] = d[10] = 1;
+      return;
+    }
+    d[0] = m[5] * m[10] - m[6] * m[9] / det;
+    d[1] = -(m[1] * m[10] - m[9] * m[2]) / det;
+    d[2] = m[1] * m[6] - m[5] * m[2] / det;
+    d[4] = -(m[4] * m[10] - m[6] * m[8]) / det;
+    d[5] = m[0] * m[10] - m[8] * m[2] / det;
+    d[6] = -(m[0] * m[6] - m[4] * m[2]) / det;
+    d[8] = m[4] * m[9] - m[8] * m[5] / det;
+    d[9] = -(m[0] * m[9] - m[8] * m[1]) / det;
+    d[10] = m[0] * m[5] - m[1] * m[4] / det;
+  }
+
+  // collision_cylinder_box.cpp:683-1000: the box face most facing the
+  // normal, in the disc's frame, clipped by the disc plane and the eight
+  // planes of an 8-gon around the rim
+  int clip_face(int maxc, ContactGeom<R> *c) {
+    R h = size * R(0.5), cpos[3], cn[3] = {0, 0, 0};
+    if (dot3(ax, normal) > R(0.0)) {
+      for (int i = 0; i < 3; i++) cpos[i] = pc[i] + ax[i] * h;
+      cn[2] = R(-1.0);
+    } else {
+      for (int i = 0; i < 3; i++) cpos[i] = pc[i] - ax[i] * h;
+      cn[2] = R(1.0);
+    }
+    R binv[12], nr[3], an[3];
+    inv3(Rb, binv);
+    mv(nr, binv, normal);
+    for (int i = 0; i < 3; i++) an[i] = std::fabs(nr[i]);
+    int b0, b1, b2;
+    if (an[1] > an[0]) {
+      if (an[0] > an[2]) { b0 = 1; b1 = 0; b2 = 2; }
+      else if (an[1] > an[2]) { b0 = 1; b1 = 2; b2 = 0; }
+      else { b0 = 2; b1 = 1; b2 = 0; }
+    } else {
+      if (an[1] > an[2]) { b0 = 0; b1 = 1; b2 = 2; }
+      else if (an[0] > an[2]) { b0 = 0; b1 = 2; b2 = 1; }
+      else { b0 = 2; b1 = 0; b2 = 1; }
+    }
+    R ctr[3];
+    for (int i = 0; i < 3; i++)
+      ctr[i] = nr[b0] > 0 ? pb[i] - half[b0] * Rb[4 * i + b0] : pb[i] + half[b0] * Rb[4 * i + b0];
+    R pts[4][3], A[16][3], B[16][3];
+    for (int k = 0; k < 16; k++)
+      for (int i = 0; i < 3; i++) A[k][i] = B[k][i] = 0;
+    static const int s1[4] = {1, -1, -1, 1}, s2[4] = {-1, -1, 1, 1};
+    for (int k = 0; k < 4; k++)
+      for (int i = 0; i < 3; i++) {
+        R u = half[b1] * Rb[4 * i + b1], v = half[b2] * Rb[4 * i + b2];
+        R x = s1[k] > 0 ? ctr[i] + u : ctr[i] - u;
+        pts[k][i] = s2[k] > 0 ? x + v : x - v;
+      }
+    R cinv[12];
+    inv3(Rc, cinv);
+    for (int k = 0; k < 4; k++) {
+      R t[3] = {pts[k][0] - cpos[0], pts[k][1] - cpos[1], pts[k][2] - cpos[2]};
+      mv(pts[k], cinv, t);
+    }
+    int n1 = 0, n2 = 0;
+    R pl[4] = {cn[0], cn[1], cn[2], R(0.0)};
+    clip_poly(pts, 4, A, n1, pl);
+    R ang = (R)(M_PI / 8);
+    const R inc = ang * R(2.0);
+    int seg = 0;
+    for (; seg < 8; seg++) {
+      R rn[4] = {-std::cos(ang), -std::sin(ang), 0, radius};
+      ang += inc;
+      if (seg % 2 == 0) clip_poly(A, n1, B, n2, rn);
+      else clip_poly(B, n2, A, n1, rn);
+    }
+    // seg == 8: the last clip wrote A
+    int n = 0;
+    for (int k = 0; k < n1; k++) {
+      R p[3], t[3];
+      mv(p, Rc, A[k]);
+      for (int i = 0; i < 3; i++) p[i] += cpos[i];
+      for (int i = 0; i < 3; i++) t[i] = p[i] - pc[i];
+      R d = best_rc - dot3(t, normal);
+      if (d > R(0.0)) {
+        put(c, n, p, d);
+        if (n == maxc) break;
+      }
+    }
+    return n;
+  }
+
+  int run(int flags, ContactGeom<R> *c) {
+    if (separated() || best_axis == 0) return 0;
+    const int maxc = flags & 0xffff;
+    R d = dot3(normal, ax);
+    if (std::fabs(d) < R(0.9)) return clip_side(maxc, c);
+    return clip_face(maxc, c);
+  }
+};
+
+// ray-cylinder: ray.cpp:500-656 -- the flat caps when the ray runs inside
+// the infinite cylinder parallel to the axis, else the curved side
+template <class R>
+int collide_ray_cylinder(const R *pr, const R *Rr, R len, const R *pc, const R *Rc, R radius, R clen,
+                         ContactGeom<R> *c) {
+  const R eps = sizeof(R) == 4 ? R(1.1920929e-07) : R(2.220446049250313e-16);
+  R hl = clen * R(0.5);
+  R r[3] = {pr[0] - pc[0], pr[1] - pc[1], pr[2] - pc[2]};
+  R d = dot3(Rc + 2, r, 4, 1);
+  R q[3];
+  for (int i = 0; i < 3; i++) q[i] = d * Rc[4 * i + 2] - r[i];
+  R C = dot3(q, q) - radius * radius;
+  R uv = dot3(Rc + 2, Rr + 2, 4, 4);
+  for (int i = 0; i < 3; i++) r[i] = uv * Rc[4 * i + 2] - Rr[4 * i + 2];
+  R A = dot3(r, r);
+  R B = 2 * dot3(q, r);
+  R k = B * B - 4 * A * C;
+  if (k < eps && C <= 0) {
+    R uvsign = (uv < 0) ? R(-1.0) : R(1.0);
+    R internal = (d >= -hl && d <= +hl) ? R(-1.0) : R(1.0);
+    if ((uv > 0 && d + uvsign * len < hl * internal) || (uv < 0 && d + uvsign * len > hl * internal)) return 0;
+    c->depth = (-uvsign * d) - (internal * hl);
+    for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + c->depth * Rr[4 * i + 2];
+    for (int i = 0; i < 3; i++) c->normal[i] = uvsign * Rc[4 * i + 2];
+    return 1;
+  }
+  if (k > 0) {
+    k = std::sqrt(k);
+    A = recip(2 * A);
+    R alpha = (-B - k) * A;
+    if (alpha < 0) alpha = (-B + k) * A;
+    if (alpha >= 0 && alpha <= len) {
+      for (int i = 0; i < 3; i++) c->pos[i] = pr[i] + alpha * Rr[4 * i + 2];
+      for (int i = 0; i < 3; i++) q[i] = c->pos[i] - pc[i];
+      d = dot3(q, Rc + 2, 1, 4);
+      if (d >= -hl && d <= +hl) {
+        R nsign = (C < 0) ? R(-1.0) : R(1.0);
+        for (int i = 0; i < 3; i++) c->normal[i] = nsign * (c->pos[i] - (pc[i] + d * Rc[4 * i + 2]));
+        normalize3(c->normal);
+        c->depth = alpha;
+        return 1;
+      }
+    }
+  }
+  return 0;
+}
+
 // the class that comes first in a collider's arguments (the colliders table,
-// collision_kernel.cpp:181-189): capsule, then sphere, box, plane
+// collision_kernel.cpp:181-219): ray, then capsule, cylinder, sphere, box,
+// plane
 inline int collider_rank(int type) {
-  return type == G_CAPSULE ? 0 : (type == G_SPHERE ? 1 : (type == G_BOX ? 2 : 3));
+  return type == G_RAY ? 0
+         : type == G_CAPSULE ? 1
+         : type == G_CYLINDER ? 2
+         : type == G_SPHERE ? 3
+         : type == G_BOX ? 4 : 5;
 }
 
 template <class R>
 int collide_geoms(const GeomShape<R> &a, int ida, const GeomShape<R> &b, int idb, int flags, ContactGeom<R> *c) {
   if ((flags & 0xffff) == 0) return 0;
   auto run = [&](const GeomShape<R> &x, const GeomShape<R> &y) -> int {
+    if (x.type == G_RAY) {
+      if (y.type == G_SPHERE) return ray_sphere_hit(x.pos, x.Rm, x.param[0], y.pos, y.param[0], 0, c);
+      if (y.type == G_BOX) return collide_ray_box(x.pos, x.Rm, x.param[0], y.pos, y.Rm, y.param, c);
+      if (y.type == G_CAPSULE)
+        return collide_ray_capsule(x.pos, x.Rm, x.param[0], y.pos, y.Rm, y.param[0], y.param[1], c);
+      if (y.type == G_PLANE) return collide_ray_plane(x.pos, x.Rm, x.param[0], y.param, c);
+      if (y.type == G_CYLINDER)
+        return collide_ray_cylinder(x.pos, x.Rm, x.param[0], y.pos, y.Rm, y.param[0], y.param[1], c);
+      return 0;
+    }
+    if (x.type == G_CYLINDER) {
+      if (y.type == G_SPHERE) return collide_cylinder_sphere(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.param[0], c);
+      if (y.type == G_PLANE) return collide_cylinder_plane(x.pos, x.Rm, x.param[0], x.param[1], y.param, flags, c);
+      if (y.type == G_BOX) return CylBox<R>(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.Rm, y.param).run(flags, c);
+      return 0;
+    }
     if (x.type == G_CAPSULE) {
       if (y.type == G_SPHERE) return collide_capsule_sphere(x.pos, x.Rm, x.param[0], x.param[1], y.pos, y.param[0], c);
       if (y.type == G_BOX)

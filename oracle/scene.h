@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-enum { SC_SPHERE = 0, SC_BOX = 1, SC_PLANE = 2, SC_CAPSULE = 3 };
+enum { SC_SPHERE = 0, SC_BOX = 1, SC_PLANE = 2, SC_CAPSULE = 3, SC_CYLINDER = 5 };
 enum { SC_BALL = 1, SC_HINGE = 2, SC_HINGE2 = 3 };
 
 /* joint_param layout (per joint, SC_NJP doubles) */
@@ -53,7 +53,7 @@ typedef struct SceneDesc {
   int ng;
   const int *geom_type;          /* SC_SPHERE / SC_BOX / SC_PLANE */
   const int *geom_body;          /* body index or -1 (static) */
-  const double *geom_param;      /* ng*4: radius | lx,ly,lz | a,b,c,d | capsule radius, length */
+  const double *geom_param;      /* ng*4: radius | lx,ly,lz | a,b,c,d | capsule or cylinder radius, length */
   const double *geom_pos;        /* ng*3 pose of static placeable geoms */
   const double *geom_q;          /* ng*4 */
   /* joints */

@@ -181,6 +181,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
       case SC_BOX: g = dCreateBox(gs, (dReal)p[0], (dReal)p[1], (dReal)p[2]); break;
       case SC_PLANE: g = dCreatePlane(gs, (dReal)p[0], (dReal)p[1], (dReal)p[2], (dReal)p[3]); break;
       case SC_CAPSULE: g = dCreateCapsule(gs, (dReal)p[0], (dReal)p[1]); break;
+      case SC_CYLINDER: g = dCreateCylinder(gs, (dReal)p[0], (dReal)p[1]); break;
     }
     dGeomSetData(g, (void *)(intptr_t)(i + 1));
     if (s->geom_type[i] != SC_PLANE) {

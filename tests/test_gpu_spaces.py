@@ -125,3 +125,134 @@ def test_dropin_static_space_frame(odemi_mod):
     err = np.abs(r["traj"][:, :, :3] - g["traj"][:, :, :3]).max(axis=(1, 2))
     say("static-space frame: pairs", len(r["pairs"]), "max traj err", float(err.max()))
     assert np.all(err <= 4 * env + 1e-3)
+
+
+def _declare_rays(L):
+    _declare(L)
+    for name, res, args in [
+            ("dCreateRay", V, [V, F]), ("dGeomRaySet", None, [V, F, F, F, F, F, F]),
+            ("dGeomRayGet", None, [V, C.POINTER(F), C.POINTER(F)]), ("dGeomRaySetLength", None, [V, F]),
+            ("dGeomRayGetLength", F, [V]), ("dGeomRaySetParams", None, [V, I, I]),
+            ("dGeomRayGetParams", None, [V, C.POINTER(I), C.POINTER(I)]),
+            ("dGeomRaySetClosestHit", None, [V, I]), ("dGeomRayGetClosestHit", I, [V]),
+            ("dCreateCapsule", V, [V, F, F]), ("dGeomSetQuaternion", None, [V, C.POINTER(F)]),
+            ("dGeomDestroy", None, [V]), ("dGeomGetAABB", None, [V, C.POINTER(F)]),
+            ("dGeomGetBody", V, [V])]:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def _wheel_rays(L):
+    """A car's wheel sensing (demos/raycar/car.cpp:351-380, 451-480): one ray
+    geom outside every space, set per wheel with dGeomRaySet, collided with
+    dSpaceCollide2 against a static terrain space (plane, boxes, capsules and
+    a nested space); the callback skips the car's own body, descends into
+    spaces and keeps the nearest contact of dCollide.  Returns the callback
+    log, the nearest contact per wheel and the ray getters' answers."""
+    L.dInitODE2(0)
+    w = L.dWorldCreate()
+    stat = L.dHashSpaceCreate(None)
+    sub = L.dSimpleSpaceCreate(stat)
+    names = {}
+
+    def tag(g, name):
+        names[g] = name
+        L.dGeomSetData(g, C.c_void_p(len(names)))
+        return g
+
+    names[stat], names[sub] = "stat", "sub"
+    rng = np.random.default_rng(31)
+    tag(L.dCreatePlane(stat, 0.0, 0.0, 1.0, 0.0), "plane")
+    for i in range(6):
+        g = tag(L.dCreateBox(stat, *map(float, rng.uniform(0.5, 2.0, 3))), f"s{i}")
+        L.dGeomSetPosition(g, *map(float, [*rng.uniform(-3, 3, 2), rng.uniform(-0.2, 0.4)]))
+        q = rng.normal(size=4)
+        L.dGeomSetQuaternion(g, (F * 4)(*map(float, q / np.linalg.norm(q))))
+    for i in range(3):
+        g = tag(L.dCreateCapsule(sub, float(rng.uniform(0.2, 0.5)), float(rng.uniform(0.5, 2.0))), f"c{i}")
+        L.dGeomSetPosition(g, *map(float, [*rng.uniform(-3, 3, 2), 0.3]))
+        q = rng.normal(size=4)
+        L.dGeomSetQuaternion(g, (F * 4)(*map(float, q / np.linalg.norm(q))))
+    car = L.dBodyCreate(w)
+    L.dBodySetPosition(car, 0.0, 0.0, 1.0)
+    chassis = tag(L.dCreateBox(stat, 1.0, 2.0, 0.4), "chassis")   # the car's own geom in the terrain space
+    L.dGeomSetBody(chassis, car)
+    ray = tag(L.dCreateRay(None, 1.0), "ray")
+    log, nearest, getters = [], [], []
+    buf = (C.c_byte * (128 * 20))()
+    state = {}
+
+    def near(data, o1, o2):
+        log.append((names[o1], names[o2]))
+        if L.dGeomGetBody(o1) == car or L.dGeomGetBody(o2) == car:
+            return
+        if L.dGeomIsSpace(o1) or L.dGeomIsSpace(o2):
+            L.dSpaceCollide2(o1, o2, None, cb)
+            return
+        n = L.dCollide(o1, o2, 10, buf, 128)
+        log.append(("contacts", n))
+        raw = np.frombuffer(bytes(buf), dtype=np.uint8)
+        for i in range(n):
+            c = raw[128 * i:128 * i + 128]
+            if state["dbl"]:
+                v = np.frombuffer(c[:72].tobytes(), dtype=np.float64)
+                pos, nrm = v[0:3], v[4:7]
+            else:
+                v = np.frombuffer(c[:36].tobytes(), dtype=np.float32).astype(np.float64)
+                pos, nrm = v[0:3], v[4:7]
+            dist = float(np.linalg.norm(pos - state["origin"]))
+            if dist < state["best"][0]:
+                state["best"] = (dist, pos.copy(), nrm.copy())
+
+    cb = CB(near)
+    state["dbl"] = False
+    for frame in range(12):
+        origin_c = np.array([*rng.uniform(-2.5, 2.5, 2), rng.uniform(0.6, 1.8)])
+        up = rng.normal(size=3) * 0.25 + np.array([0, 0, 1.0])
+        for wheel in range(4):
+            pos = origin_c + np.array([(-1) ** wheel * 0.8, (-1) ** (wheel // 2) * 1.2, 0.0])
+            state["origin"] = pos
+            state["best"] = (1e4, None, None)
+            L.dGeomRaySet(ray, *map(float, pos), *map(float, -up))
+            L.dGeomRaySetClosestHit(ray, 1)
+            length = float(rng.uniform(0.5, 2.5))
+            L.dGeomRaySetLength(ray, length)
+            L.dGeomRaySetParams(ray, 0, 0)
+            log.append(("frame", frame, wheel))
+            L.dSpaceCollide2(stat, ray, None, cb)
+            nearest.append(state["best"])
+            s, d = (F * 4)(), (F * 4)()
+            fc, bc = I(), I()
+            L.dGeomRayGetParams(ray, C.byref(fc), C.byref(bc))
+            L.dGeomRayGet(ray, s, d)
+            getters.append((L.dGeomRayGetLength(ray), L.dGeomRayGetClosestHit(ray), fc.value, bc.value,
+                            *s[:3], *d[:3]))
+    L.dGeomDestroy(ray)
+    L.dSpaceDestroy(stat)
+    L.dWorldDestroy(w)
+    return log, nearest, getters
+
+
+def test_wheel_rays_match_reference_build(odemi_mod):
+    """Raycar's wheel rays through this library and through the reference's
+    single-precision build with the same calls: identical callback logs
+    (pairs in order, dCollide counts), identical hits and misses, nearest
+    contacts within the FP32 tolerance, and the ray getters' answers."""
+    assert po.have_ref("single"), "the reference single build (oracle/_ref) is needed"
+    ref_log, ref_near, ref_get = _wheel_rays(_declare_rays(po.ref_lib("single")))
+    got_log, got_near, got_get = _wheel_rays(_declare_rays(odemi_mod.lib()))
+    hits = sum(1 for b in ref_near if b[1] is not None)
+    say("wheel rays: callback entries", len(ref_log), "wheel casts", len(ref_near), "hits", hits)
+    assert got_log == ref_log
+    ref_get, got_get = np.array(ref_get), np.array(got_get)
+    assert np.array_equal(got_get[:, :4], ref_get[:, :4])            # length, closest hit, params
+    np.testing.assert_allclose(got_get[:, 4:], ref_get[:, 4:], atol=1e-6)   # start, unit direction
+    assert hits >= 12
+    for (dr, pr, nr), (dg, pg, ng) in zip(ref_near, got_near):
+        assert (pr is None) == (pg is None)
+        if pr is not None:
+            assert abs(dr - dg) < 2e-4
+            np.testing.assert_allclose(pg, pr, atol=2e-4)
+            np.testing.assert_allclose(ng, nr, atol=2e-4)

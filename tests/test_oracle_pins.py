@@ -3,6 +3,7 @@ known-answer tests of the reference, closed forms, invariants and brute force
 on small inputs, each chosen so that a plausible mistake fails it."""
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -355,3 +356,135 @@ def test_warm_start_changes_only_the_starting_point():
     sw_c, sw_w = out[1e-6, 0.0][1], out[1e-6, 1.0][1]
     assert out[1e-6, 0.0][2] == out[1e-6, 1.0][2] == 0
     assert sw_w < 0.85 * sw_c, (sw_w, sw_c)
+
+
+RAY, CAPSULE = 4, 3
+
+
+def _ray_fixture():
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rays.npz"))
+    return z["cases"], z["count"], z["contacts"]
+
+
+def _case(row):
+    t1, t2 = int(row[0]), int(row[1])
+    return t1, row[2:5], row[5:9], row[9:13], t2, row[13:16], row[16:20], row[20:24]
+
+
+def test_ray_colliders_match_the_reference_build():
+    """The restatement's ray colliders (ray-sphere, -box, -capsule, -plane in
+    both argument orders, ray.cpp:196-497) against the reference's own
+    dCollide in double precision (tests/golden/rays.npz, written by
+    make_rays.py): the same hit or miss in every case, contacts equal to
+    rounding."""
+    cases, count, contacts = _ray_fixture()
+    for i, row in enumerate(cases):
+        t1, p1, q1, s1, t2, p2, q2, s2 = _case(row)
+        got = collide(t1, p1, q1, s1, t2, p2, q2, s2, 4, "f64")
+        assert len(got) == count[i], (i, t1, t2, len(got), count[i])
+        if count[i]:
+            np.testing.assert_allclose(got[0, :7], contacts[i], rtol=0, atol=1e-9, err_msg=str(i))
+
+
+def test_ray_contacts_lie_on_the_surface_hit():
+    """What the geometry fixes for every reference ray hit: the contact is
+    start + depth * direction with 0 <= depth <= length, it lies on the
+    surface of the shape (sphere: |x - c| = r; plane: n.x = d; box: on a face;
+    capsule: at distance r from its segment), and the normal is a unit vector
+    (the surface normal, reversed when the ray starts inside)."""
+    cases, count, contacts = _ray_fixture()
+    n_hit = 0
+    for i, row in enumerate(cases):
+        if not count[i]:
+            continue
+        t1, p1, q1, s1, t2, p2, q2, s2 = _case(row)
+        if t1 == RAY:
+            pr, qr, length, t, po_, qo, s = p1, q1, s1[0], t2, p2, q2, s2
+        else:
+            pr, qr, length, t, po_, qo, s = p2, q2, s2[0], t1, p1, q1, s1
+        _, Rr = rot(qr)
+        _, Ro = rot(qo)
+        x, nrm, depth = contacts[i, :3], contacts[i, 3:6], contacts[i, 6]
+        assert -1e-12 <= depth <= length + 1e-12
+        np.testing.assert_allclose(x, pr + depth * Rr[:, 2], atol=1e-9)
+        assert abs(np.linalg.norm(nrm) - 1) < 1e-9
+        if t == 0:
+            assert abs(np.linalg.norm(x - po_) - s[0]) < 1e-9
+        elif t == 2:
+            assert abs(np.dot(s[:3], x) - s[3]) < 1e-9
+            assert abs(abs(np.dot(nrm, s[:3])) - 1) < 1e-9
+        elif t == 1:
+            loc = Ro.T @ (x - po_)
+            h = np.asarray(s[:3]) / 2
+            assert np.all(np.abs(loc) <= h + 1e-9)
+            assert np.min(h - np.abs(loc)) < 1e-9
+        else:
+            ax = Ro[:, 2]
+            k = np.clip(np.dot(x - po_, ax), -s[1] / 2, s[1] / 2)
+            assert abs(np.linalg.norm(x - (po_ + k * ax)) - s[0]) < 1e-9
+        n_hit += 1
+    assert n_hit > 300
+
+
+def _cyl_fixture():
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cylinder_colliders.npz"))
+    return z["cases"], z["count"], z["contacts"]
+
+
+def _cyl_case(row):
+    t1, t2, maxc = int(row[0]), int(row[1]), int(row[2])
+    return t1, row[3:6], row[6:10], row[10:14], t2, row[14:17], row[17:21], row[21:25], maxc
+
+
+def test_cylinder_colliders_match_the_reference_build():
+    """The restatement's cylinder colliders (cylinder-sphere, -plane, -box,
+    ray-cylinder, both argument orders, maxc 1-8; cylinder-cylinder and
+    capsule-cylinder without a collider) against the reference's own dCollide
+    in double precision (tests/golden/cylinder_colliders.npz, written by
+    make_cylinders.py): the same number of contacts in every case and the
+    same contacts to rounding."""
+    cases, count, contacts = _cyl_fixture()
+    for i, row in enumerate(cases):
+        t1, p1, q1, s1, t2, p2, q2, s2, maxc = _cyl_case(row)
+        got = collide(t1, p1, q1, s1, t2, p2, q2, s2, maxc, "f64")
+        assert len(got) == count[i], (i, t1, t2, maxc, len(got), count[i])
+        if count[i]:
+            np.testing.assert_allclose(got[:, :7], contacts[i, :count[i]], rtol=0, atol=1e-9, err_msg=str(i))
+
+
+def test_cylinder_contacts_against_the_geometry():
+    """What the geometry fixes for the reference's cylinder contacts: a
+    sphere's contact depth equals the sphere radius minus the distance from
+    its centre to the cylinder surface (side, disc or rim, computed here from
+    the closest point of the solid cylinder), a plane contact's depth is the
+    point's depth below the plane, and every normal is a unit vector."""
+    cases, count, contacts = _cyl_fixture()
+    n_sph = n_pl = 0
+    for i, row in enumerate(cases):
+        if not count[i]:
+            continue
+        t1, p1, q1, s1, t2, p2, q2, s2, maxc = _cyl_case(row)
+        for k in range(count[i]):
+            assert abs(np.linalg.norm(contacts[i, k, 3:6]) - 1) < 1e-6
+        other = t2 if t1 == 5 else t1
+        pc, qc, sc = (p1, q1, s1) if t1 == 5 else (p2, q2, s2)
+        po_, so = (p2, s2) if t1 == 5 else (p1, s1)
+        if other == 0:
+            _, Rc = rot(qc)
+            loc = Rc.T @ (po_ - pc)
+            rad, hl = sc[0], sc[1] / 2
+            rr = np.hypot(loc[0], loc[1])
+            inside = rr <= rad and abs(loc[2]) <= hl
+            if inside:
+                continue
+            cl = np.array([loc[0], loc[1], 0.0]) * (min(rr, rad) / rr if rr > 0 else 0.0)
+            cl[2] = np.clip(loc[2], -hl, hl)
+            dist = np.linalg.norm(loc - cl)
+            assert abs(contacts[i, 0, 6] - (so[0] - dist)) < 1e-9, i
+            n_sph += 1
+        elif other == 2:
+            for k in range(count[i]):
+                x = contacts[i, k, :3]
+                assert abs(contacts[i, k, 6] - (so[3] - np.dot(so[:3], x))) < 1e-9
+            n_pl += 1
+    assert n_sph > 60 and n_pl > 60
