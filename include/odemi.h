@@ -175,6 +175,14 @@ ODE_API dReal dWorldGetQuickStepW(dWorldID w);
 ODE_API void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel);         /* objects.h:420 */
 ODE_API void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth);           /* objects.h:440 */
 ODE_API int dWorldQuickStep(dWorldID w, dReal stepsize);                      /* objects.h:352 */
+/* dWorldStep: the same step with each island's LCP solved exactly instead of
+ * by the 20 SOR iterations -- the answer dSolveLCP (lcp.cpp) defines: the
+ * rows without a friction index solved first, then the friction bounds fixed
+ * from those normal impulses and all rows solved (block principal pivoting
+ * on a Cholesky factorisation of the free rows, one workgroup per world).
+ * The workspace holds an island of every row of the world (2 m^2 floats for
+ * m rows).  Returns 0 with an error when the solve does not finish. */
+ODE_API int dWorldStep(dWorldID w, dReal stepsize);                           /* objects.h:309 */
 
 /* bodies: include/ode/objects.h:856-1075 */
 ODE_API dBodyID dBodyCreate(dWorldID w);                                      /* objects.h:856 */

@@ -26,6 +26,9 @@ void destroy_graph(BatchImpl *b) {
 void refresh_dev(BatchImpl *b) {
   DevBatch &D = b->dev;
   D.nworlds = b->nworlds;
+  D.lcp_ws = nullptr;       // set per call by dWorldStep
+  D.lcp_ws_stride = 0;
+  D.lcp_ws_rows = 0;
   D.order_mode = b->order_mode;
   D.wi = b->d.wi.as<WorldInfo>();
   D.wp = b->d.wp.as<WorldParams>();

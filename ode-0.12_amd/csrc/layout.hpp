@@ -38,7 +38,7 @@ enum {
 // above the tolerance, and the largest final residual of a tolerance-mode
 // solve (float bits)
 enum { CNT_PAIRS = 0, CNT_CONTACTS, CNT_ROWS, CNT_GROUPS, CNT_OVERFLOW, CNT_ITERS, CNT_UNCONV, CNT_RESMAX, CNT_N };
-enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8, OVF_NONFINITE = 16 };
+enum { OVF_CELLS = 1, OVF_PAIRS = 2, OVF_CONTACTS = 4, OVF_ROWS = 8, OVF_NONFINITE = 16, OVF_LCP = 32 };
 
 static const int MAX_SLOTS = 8;   // most contacts a collider produces for one pair
 
@@ -75,6 +75,7 @@ struct WorldParams {
   int accel;               // with tol: nonsmooth nonlinear conjugate-gradient steps between sweeps
   int dead;                // set by the step when the world's state became non-finite; the world is frozen
   float warm;              // coloured solver: start from warm x the previous step's impulses (0: from zero)
+  int exact;               // reference order: each island's LCP solved exactly (dWorldStep)
 };
 
 struct Limot {             // ode/src/joints/joint.h:172-190
@@ -211,6 +212,10 @@ OD_HD inline long long od_pow2(long long n) {
 // permanent joint (impulses of up to 6 rows, then the row count as int bits)
 OD_HD inline long long warm_bytes(int kcap, int nj) { return od_align(24LL * kcap) + 32LL * nj; }
 
+// floats of the exact-LCP workspace of one world for islands of up to m
+// rows: the matrix and its factor (m*m each), then seven vectors of m
+OD_HD inline long long lcp_ws_floats(int m) { return 2LL * m * m + 7LL * m + 64; }
+
 // bytes of the coloured solver's working set: rows, bodies, constraints in
 // colour order, colour starts, row impulses and (accelerated mode) the
 // conjugate-gradient state; copied to the LDS pool for the sweeps when it fits
@@ -314,6 +319,11 @@ struct DevBatch {
   // tape for the adjoint (null when not recording)
   char *tape;
   const long long *tape_off;   // per world byte offset into the tape
+  // exact LCP (dWorldStep): per world lcp_ws_stride floats of workspace,
+  // for islands of at most lcp_ws_rows rows (null when not stepping exactly)
+  float *lcp_ws;
+  long long lcp_ws_stride;
+  int lcp_ws_rows;
 };
 
 }  // namespace odemi

@@ -66,6 +66,7 @@ struct dxWorld {
   // state (pos, q, lvel, avel; nb float4 each), contacts with impulses, and
   // the contact policy the adjoint differentiates
   DevBuf tape_st, tape_con, tape_nk;
+  DevBuf lcp_ws;                           // dWorldStep's exact-LCP workspace (grown on demand)
   bool tape_ok = false;
   char tape_why[200] = "no step recorded (dWorldSetTape(w, 1) before dWorldQuickStep)";
   int tape_mode = 0, tape_maxc = 4;
@@ -364,7 +365,7 @@ int sync_mirror(dxWorld *w, dxSpace *sp) {
   return 1;
 }
 
-int world_params_upload(dxWorld *w, float dt, int maxc) {
+int world_params_upload(dxWorld *w, float dt, int maxc, int exact) {
   BatchImpl *m = w->mir;
   WorldParams P;
   OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
@@ -375,6 +376,7 @@ int world_params_upload(dxWorld *w, float dt, int maxc) {
   if (maxc > 0) P.max_contacts = maxc;
   P.seed = (unsigned)global_seed();
   if (w->mir_space) { P.hash_min = w->mir_space->minlevel; P.hash_max = w->mir_space->maxlevel; }
+  P.exact = exact;
   OD_HIP(hipMemcpy(m->d.wp.p, &P, sizeof(P), hipMemcpyHostToDevice));
   return 1;
 }
@@ -530,7 +532,10 @@ void dWorldSetContactSurfaceLayer(dWorldID w, dReal d) { w->min_depth = d; }
 
 // dWorldQuickStep (ode.cpp:1807-1825): uploads the contact joints and runs
 // the step kernel on the world's mirror.
-int dWorldQuickStep(dWorldID w, dReal stepsize) {
+// One step of the world on its device mirror: dWorldQuickStep (exact =
+// false: the reference-order SOR) or dWorldStep (exact: each island's LCP
+// solved exactly).
+static int world_step_impl(dWorldID w, dReal stepsize, bool exact, const char *who) {
   if (!w || !(stepsize > 0)) return 0;
   if (!sync_mirror(w, nullptr)) return 0;
   BatchImpl *m = w->mir;
@@ -562,7 +567,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     cin.push_back(c);
   }
   if ((int)cin.size() > m->wi[0].kcap) {
-    report_error("dWorldQuickStep: %zu contact joints exceed the capacity %d", cin.size(), m->wi[0].kcap);
+    report_error("%s: %zu contact joints exceed the capacity %d", who, cin.size(), m->wi[0].kcap);
     return 0;
   }
   // feedback slots of permanent joints
@@ -596,7 +601,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
       }
     OD_HIP(hipMemcpy(w->feedback.p, f0.data(), sizeof(float) * f0.size(), hipMemcpyHostToDevice));
   }
-  if (!world_params_upload(w, stepsize, 0)) return 0;
+  if (!world_params_upload(w, stepsize, 0, exact ? 1 : 0)) return 0;
   refresh_dev(m);
   DevBatch D = m->dev;
   D.cin = w->cin.as<ContactIn>();
@@ -604,7 +609,23 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   D.cin_count = w->cin_count.as<int>();
   D.feedback = fbs.empty() ? nullptr : w->feedback.as<float>();
   D.ctrl = nullptr;
-  const bool record = w->tape && tape_prepare(w, cin);
+  if (exact) {
+    // workspace for an island of every row of the world (a contact has at
+    // most 3 rows, a joint at most 6)
+    const int rows_ub = 3 * (int)cin.size() + 6 * (int)w->mjoints.size() + 1;
+    if (!w->lcp_ws.alloc(sizeof(float) * (size_t)lcp_ws_floats(rows_ub))) {
+      report_error("%s: device allocation of the LCP workspace (%d rows) failed", who, rows_ub);
+      return 0;
+    }
+    D.lcp_ws = w->lcp_ws.as<float>();
+    D.lcp_ws_stride = lcp_ws_floats(rows_ub);
+    D.lcp_ws_rows = rows_ub;
+  }
+  const bool record = !exact && w->tape && tape_prepare(w, cin);
+  if (exact && w->tape) {
+    std::snprintf(w->tape_why, sizeof(w->tape_why), "the last step was a dWorldStep (the adjoint is of dWorldQuickStep)");
+    w->tape_ok = false;
+  }
   if (record) {
     const size_t bytes = sizeof(float4) * (size_t)m->nb;
     float4 *st = w->tape_st.as<float4>();
@@ -621,10 +642,11 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   long long cnt[5];
   batch_counters(m, cnt);
   if (cnt[4]) {
-    report_error("dWorldQuickStep: device capacity exceeded (flags %lld)", cnt[4]);
+    report_error("%s: %s (flags %lld)", who,
+                 (cnt[4] & OVF_LCP) ? "the LCP solve did not finish" : "device capacity exceeded", cnt[4]);
     return 0;
   }
-  w->tape_ok = record;
+  if (!exact) w->tape_ok = record;
   // RNG state, feedback, joint limit state, space order
   WorldParams P;
   OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
@@ -663,6 +685,8 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
   w->device_newer = true;
   return 1;
 }
+int dWorldQuickStep(dWorldID w, dReal stepsize) { return world_step_impl(w, stepsize, false, "dWorldQuickStep"); }
+int dWorldStep(dWorldID w, dReal stepsize) { return world_step_impl(w, stepsize, true, "dWorldStep"); }
 
 // ---------------------------------------------------------------- bodies
 dBodyID dBodyCreate(dWorldID w) {
@@ -1372,7 +1396,7 @@ void dSpaceCollide(dSpaceID s, void *data, dNearCallback *cb) {
   }
   if (!sync_mirror(w, s)) return;
   BatchImpl *m = w->mir;
-  if (!world_params_upload(w, 0, s->cached_maxc)) return;
+  if (!world_params_upload(w, 0, s->cached_maxc, 0)) return;
   refresh_dev(m);
   DevBatch D = m->dev;
   D.pairs_out = w->pairs_out.as<PairRec>();

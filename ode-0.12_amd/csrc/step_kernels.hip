@@ -1612,6 +1612,8 @@ __device__ void ref_order_lds(const DevBatch &B, const Ctx &c, WorldParams &P, C
   __syncthreads();
 }
 
+#include "lcp_exact.hpp"
+
 // STAGE 0: the whole step; 1: up to the coloured solver's set-up (the sweeps
 // and the rest run in later kernels); 2: after the sweeps (write-back,
 // velocities, integration)
@@ -2037,7 +2039,9 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
   const int iters = P.iterations;
   if (ref) {
     int *ord = corder + ncons + nb;
-    if (ref_sor_lds_bytes(nrows, nb) <= B.lds_bytes)
+    if (P.exact && B.lcp_ws)
+      lcp_exact_world(B, c.w, rows, bt, isl, ngroups, nb, sh);
+    else if (ref_sor_lds_bytes(nrows, nb) <= B.lds_bytes)
       sor_reference<1>(rows, bt, isl, ord, nrows, ngroups, nb, iters);
     else if (ref_sor_lds_bytes_streamed(nrows, nb) <= B.lds_bytes)
       sor_reference<2>(rows, bt, isl, ord, nrows, ngroups, nb, iters);

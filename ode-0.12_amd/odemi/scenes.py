@@ -81,6 +81,7 @@ class Scene:
     joint_param: np.ndarray = field(default_factory=lambda: np.zeros((0, NJP)))
     step_ctrl: np.ndarray | None = None   # (nsteps, nj, 4): lostop, histop, torque1, torque2
     geom_space: np.ndarray | None = None  # per geom 0 (dynamic space) or 1 (static terrain space)
+    world_step: int = 0                   # 1: dWorldStep (exact LCP) instead of dWorldQuickStep
 
     @property
     def nb(self) -> int:
@@ -258,6 +259,13 @@ def capsule_pile(n: int = 30, seed: int = 19) -> Scene:
         else:
             b.box_body(pos, tuple(2.0 * he), dens, q=q)
     return b.scene(dt=0.01, mu=0.6, seed=seed)
+
+
+def with_world_step(sc: Scene) -> Scene:
+    """The scene stepped with dWorldStep (the island's LCP solved exactly)
+    instead of dWorldQuickStep."""
+    sc.world_step = 1
+    return sc
 
 
 def cylinder_pile(n: int = 32, seed: int = 31) -> Scene:

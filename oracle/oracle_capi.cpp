@@ -103,6 +103,7 @@ extern "C" void orc_set_state_override(const double *st, int nb) {
 // Builds the world exactly as the scene driver does through the ODE API.
 static void build_world(const SceneDesc *s, World<R> &w) {
   w.prm.colored = g_colored;
+  w.prm.exact = s->world_step ? 1 : 0;
   w.prm.tol = (R)g_tol;
   w.prm.max_iterations = g_maxit;
   w.prm.check_every = g_check;
@@ -432,4 +433,17 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
   }
   record(w, states + (size_t)nsteps * s->nb * 13);
   return total;
+}
+
+// The exact box-constrained LCP of the dWorldStep restatement on one dense
+// problem (A n x n row-major, symmetric positive definite): x in [lo,hi],
+// w = A x - b complementary to the bounds.  Returns the pivoting rounds, or
+// -1 at the cap or for an A that is not positive definite.
+extern "C" int orc_blcp(int n, const double *A, const double *b, const double *lo, const double *hi, double *x) {
+  std::vector<R> a(A, A + (size_t)n * n), bb(b, b + n), l(lo, lo + n), h(hi, hi + n), xx(n, 0);
+  std::vector<char> in(n, 1);
+  std::vector<int> st(n, 0);
+  int r = World<R>::blcp(n, a, bb, l, h, in, xx, st);
+  for (int i = 0; i < n; i++) x[i] = (double)xx[i];
+  return r;
 }

@@ -129,6 +129,7 @@ template <class R> struct WorldParams {
   R max_vel = INFINITY, min_depth = 0;
   int hash_min = -3, hash_max = 10;
   int colored = 0;   // 1: graph-coloured Gauss-Seidel order (no shuffle), as the MI355X coloured mode
+  int exact = 0;     // 1: dWorldStep -- the island's LCP solved exactly (step.cpp, lcp.cpp)
   // coloured mode only: iterate until the LCP residual <= tol (checked every
   // check_every sweeps, at most max_iterations); accel = 1 adds the
   // nonsmooth nonlinear conjugate-gradient step between sweeps
@@ -875,6 +876,8 @@ template <class R> struct World {
           }
         }
         sor_colored(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex, row0, jm, cid);
+      } else if (prm.exact) {
+        lcp_exact(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex);
       } else {
         sor_lcp(m, nb, J, jb, ib, invI, lambda, cforce, rhs, lo, hi, cfm, findex);
       }
@@ -920,6 +923,169 @@ template <class R> struct World {
       zero(b.tacc, 3);
     }
   }
+
+  // ---------------------------------------------------------- dWorldStep
+  // The island's LCP of step.cpp:420-710 -- A = J M^-1 J^T + cfm/h, b = rhs,
+  // bounds lo/hi, friction rows bounded by findex -- solved as dSolveLCP
+  // (lcp.cpp:784-1100) defines the answer: first the LCP of the rows without
+  // a findex (friction impulses zero), then every friction row's bounds fixed
+  // at +-|hi * lambda_normal| from that solution (0 when the normal impulse is
+  // 0) and the LCP of all rows solved with those bounds (lcp.cpp:826-842).
+  // With cfm > 0, A is positive definite and each of the two box-constrained
+  // LCPs has exactly one solution, which this restatement computes by block
+  // principal pivoting (Judice & Pires 1994) with a least-index single pivot
+  // as the fallback that guarantees termination (Murty 1974), on a dense
+  // Cholesky factorisation of the free rows.  Dantzig's pivoting in the
+  // reference reaches the same solution in a different sequence of
+  // arithmetic.
+
+  // x for the rows of `free` (indices into 0..m-1): A_FF x_F = rhs_F by
+  // Cholesky; false if A_FF is not positive definite
+  static bool chol_solve(int m, const std::vector<R> &A, const std::vector<int> &fr, std::vector<R> &rhs) {
+    const int n = (int)fr.size();
+    std::vector<R> L((size_t)n * n, 0);
+    for (int j = 0; j < n; j++) {
+      R d = A[(size_t)fr[j] * m + fr[j]];
+      for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+      if (!(d > 0)) return false;
+      d = std::sqrt(d);
+      L[(size_t)j * n + j] = d;
+      for (int i = j + 1; i < n; i++) {
+        R a = A[(size_t)fr[i] * m + fr[j]];
+        for (int k = 0; k < j; k++) a -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+        L[(size_t)i * n + j] = a / d;
+      }
+    }
+    for (int i = 0; i < n; i++) {
+      R a = rhs[i];
+      for (int k = 0; k < i; k++) a -= L[(size_t)i * n + k] * rhs[k];
+      rhs[i] = a / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      R a = rhs[i];
+      for (int k = i + 1; k < n; k++) a -= L[(size_t)k * n + i] * rhs[k];
+      rhs[i] = a / L[(size_t)i * n + i];
+    }
+    return true;
+  }
+
+  // box-constrained LCP over the rows with in[i]: x in [lo,hi], w = A x - b,
+  // x = lo => w >= 0, x = hi => w <= 0, else w = 0; rows outside `in` stay at
+  // x = 0.  st: 0 free, 1 at lo, 2 at hi (start and result).  Returns the
+  // number of pivoting rounds, or -1 when the cap is reached.
+  static int blcp(int m, const std::vector<R> &A, const std::vector<R> &b, const std::vector<R> &lo,
+                  const std::vector<R> &hi, const std::vector<char> &in, std::vector<R> &x, std::vector<int> &st) {
+    int best = m + 1, backup = 3;
+    for (int round = 0; round < 50 * m + 50; round++) {
+      std::vector<int> fr;
+      for (int i = 0; i < m; i++) {
+        if (!in[i]) { x[i] = 0; continue; }
+        if (st[i] == 0) fr.push_back(i);
+        else x[i] = st[i] == 1 ? lo[i] : hi[i];
+      }
+      std::vector<R> r(fr.size());
+      for (size_t a = 0; a < fr.size(); a++) {
+        R v = b[fr[a]];
+        for (int j = 0; j < m; j++)
+          if (in[j] && st[j] != 0) v -= A[(size_t)fr[a] * m + j] * x[j];
+        r[a] = v;
+      }
+      if (!chol_solve(m, A, fr, r)) return -1;
+      for (size_t a = 0; a < fr.size(); a++) x[fr[a]] = r[a];
+      std::vector<int> bad;
+      for (int i = 0; i < m; i++) {
+        if (!in[i]) continue;
+        if (st[i] == 0) {
+          if (x[i] < lo[i] || x[i] > hi[i]) bad.push_back(i);
+        } else {
+          R w = -b[i];
+          for (int j = 0; j < m; j++)
+            if (in[j]) w += A[(size_t)i * m + j] * x[j];
+          if ((st[i] == 1 && w < 0) || (st[i] == 2 && w > 0)) bad.push_back(i);
+        }
+      }
+      if (bad.empty()) return round;
+      auto flip = [&](int i) {
+        if (st[i] == 0) st[i] = x[i] < lo[i] ? 1 : 2;
+        else st[i] = 0;
+      };
+      if ((int)bad.size() < best) {
+        best = (int)bad.size();
+        backup = 3;
+        for (int i : bad) flip(i);
+      } else if (backup > 0) {
+        backup--;
+        for (int i : bad) flip(i);
+      } else {
+        flip(bad.back());
+      }
+    }
+    return -1;
+  }
+
+  void lcp_exact(int m, int nb, std::vector<R> &J, const std::vector<int> &jb, const std::vector<int> &ib,
+                 const std::vector<R> &invI, std::vector<R> &lambda, std::vector<R> &fc, const std::vector<R> &b,
+                 std::vector<R> lo, std::vector<R> hi, const std::vector<R> &cfm, const std::vector<int> &findex) {
+    std::vector<R> iMJ(12 * (size_t)m, 0);
+    for (int i = 0; i < m; i++) {
+      int b1 = jb[2 * i], b2 = jb[2 * i + 1];
+      R k1 = bodies[ib[b1]].invMass;
+      for (int k = 0; k < 3; k++) iMJ[12 * i + k] = k1 * J[12 * i + k];
+      mv(&iMJ[12 * i + 3], &invI[12 * b1], &J[12 * i + 3]);
+      if (b2 != -1) {
+        R k2 = bodies[ib[b2]].invMass;
+        for (int k = 0; k < 3; k++) iMJ[12 * i + 6 + k] = k2 * J[12 * i + 6 + k];
+        mv(&iMJ[12 * i + 9], &invI[12 * b2], &J[12 * i + 9]);
+      }
+    }
+    // A_ij = J_i . M^-1 J_j^T over the bodies the two rows share
+    std::vector<R> A((size_t)m * m, 0);
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) {
+        R s = 0;
+        for (int si = 0; si < 2; si++) {
+          const int bi = jb[2 * i + si];
+          if (bi < 0) continue;
+          for (int sj = 0; sj < 2; sj++)
+            if (jb[2 * j + sj] == bi)
+              for (int k = 0; k < 6; k++) s += J[12 * i + 6 * si + k] * iMJ[12 * j + 6 * sj + k];
+        }
+        A[(size_t)i * m + j] = s + (i == j ? cfm[i] : R(0));
+      }
+    std::vector<R> x(m, 0);
+    std::vector<int> st(m, 0);
+    std::vector<char> in(m);
+    // rows with lo = hi (= 0) are held there whatever w is (lcp.cpp:832-838)
+    for (int i = 0; i < m; i++) in[i] = findex[i] < 0 && lo[i] != hi[i];
+    lcp_rounds_last = blcp(m, A, b, lo, hi, in, x, st);
+    bool fric = false;
+    for (int i = 0; i < m; i++) fric |= findex[i] >= 0;
+    if (fric && lcp_rounds_last >= 0) {
+      for (int i = 0; i < m; i++) {
+        if (findex[i] < 0) continue;
+        const R xn = x[findex[i]];
+        if (xn == 0) {
+          lo[i] = hi[i] = 0;
+          in[i] = 0;       // held at zero (lcp.cpp:832-835 with w of either sign)
+        } else {
+          hi[i] = std::fabs(hi[i] * xn);
+          lo[i] = -hi[i];
+          in[i] = 1;
+          st[i] = 0;
+        }
+      }
+      lcp_rounds_last = blcp(m, A, b, lo, hi, in, x, st);
+    }
+    for (int i = 0; i < m; i++) lambda[i] = x[i];
+    std::fill(fc.begin(), fc.end(), R(0));
+    for (int i = 0; i < m; i++) {
+      const int b1 = jb[2 * i], b2 = jb[2 * i + 1];
+      for (int k = 0; k < 6; k++) fc[6 * (size_t)b1 + k] += iMJ[12 * i + k] * x[i];
+      if (b2 != -1)
+        for (int k = 0; k < 6; k++) fc[6 * (size_t)b2 + k] += iMJ[12 * i + 6 + k] * x[i];
+    }
+  }
+  int lcp_rounds_last = 0;   // pivoting rounds of the last exact solve (-1: cap reached)
 
   void sor_lcp(int m, int nb, std::vector<R> &J, const std::vector<int> &jb, const std::vector<int> &ib,
                const std::vector<R> &invI, std::vector<R> &lambda, std::vector<R> &fc, std::vector<R> &b,

@@ -37,7 +37,7 @@ class SceneDesc(C.Structure):
         ("geom_q", _dp),
         ("nj", C.c_int), ("joint_type", _ip), ("joint_b1", _ip), ("joint_b2", _ip), ("joint_anchor", _dp),
         ("joint_axis1", _dp), ("joint_axis2", _dp), ("joint_param", _dp),
-        ("step_ctrl", _dp), ("geom_space", _ip),
+        ("step_ctrl", _dp), ("geom_space", _ip), ("world_step", C.c_int),
     ]
 
 
@@ -85,6 +85,7 @@ def make_desc(sc):
         a, p = _d(sc.step_ctrl)
         keep.append(a)
         d.step_ctrl = p
+    d.world_step = int(getattr(sc, "world_step", 0))
     if getattr(sc, "geom_space", None) is not None:
         a, p = _i(sc.geom_space)
         keep.append(a)

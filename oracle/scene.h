@@ -74,6 +74,9 @@ typedef struct SceneDesc {
      dSpaceCollide(space 0), as an application with a static terrain space
      does (demos/raycar/ode.cpp:118-123) */
   const int *geom_space;
+  /* 1: step with dWorldStep (the big-matrix LCP solved exactly) instead of
+     dWorldQuickStep */
+  int world_step;
 } SceneDesc;
 
 typedef struct SceneOut {

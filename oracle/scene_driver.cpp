@@ -275,7 +275,7 @@ extern "C" int scene_run(const SceneDesc *s, SceneOut *o) {
     if (space2) dSpaceCollide2((dGeomID)space, (dGeomID)space2, &c, &near_callback);
     dSpaceCollide(space, &c, &near_callback);
     auto t1 = clk::now();
-    int ok = dWorldQuickStep(world, (dReal)s->dt);
+    int ok = s->world_step ? dWorldStep(world, (dReal)s->dt) : dWorldQuickStep(world, (dReal)s->dt);
     if (c.diag && o->feedback) {
       for (size_t k = 0; k < c.fbjoints.size(); k++) {
         const dJointFeedback &f = c.fb[fb_start + k];

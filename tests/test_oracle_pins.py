@@ -488,3 +488,65 @@ def test_cylinder_contacts_against_the_geometry():
                 assert abs(contacts[i, k, 6] - (so[3] - np.dot(so[:3], x))) < 1e-9
             n_pl += 1
     assert n_sph > 60 and n_pl > 60
+
+
+def _blcp(A, b, lo, hi, prec="f64"):
+    L = lib(prec)
+    L.orc_blcp.restype = C.c_int
+    L.orc_blcp.argtypes = [C.c_int, DP, DP, DP, DP, DP]
+    n = len(b)
+    bufs = [d(A), d(b), d(lo), d(hi)]
+    x = np.zeros(n)
+    r = L.orc_blcp(n, *[q[1] for q in bufs], x.ctypes.data_as(DP))
+    return r, x
+
+
+def test_exact_lcp_against_enumeration():
+    """The dWorldStep restatement's box-constrained LCP solver on random
+    positive definite problems (n = 1..6, some rows unbounded, some with
+    lo = 0, some with finite two-sided bounds): its answer satisfies the
+    complementarity conditions, and equals the one solution found by
+    enumerating all 3^n assignments of each row to its lower bound, its
+    upper bound or the free set (each solved by a linear solve)."""
+    import itertools
+    rng = np.random.default_rng(5)
+    for trial in range(150):
+        n = int(rng.integers(1, 7))
+        M = rng.normal(size=(n, n))
+        A = M @ M.T + 0.05 * np.eye(n)
+        b = rng.normal(size=n) * 3
+        kind = rng.integers(3, size=n)
+        lo = np.where(kind == 0, -np.inf, np.where(kind == 1, 0.0, -rng.uniform(0.1, 2, n)))
+        hi = np.where(kind == 0, np.inf, np.where(kind == 1, np.inf, rng.uniform(0.1, 2, n)))
+        r, x = _blcp(A, b, lo, hi)
+        assert r >= 0, trial
+        w = A @ x - b
+        tol = 1e-9 * (1 + np.abs(b).max())
+        assert np.all(x >= lo - tol) and np.all(x <= hi + tol)
+        for i in range(n):
+            if x[i] <= lo[i] + tol:
+                assert w[i] >= -1e-7
+            elif x[i] >= hi[i] - tol:
+                assert w[i] <= 1e-7
+            else:
+                assert abs(w[i]) <= 1e-7
+        sols = []
+        for assign in itertools.product(range(3), repeat=n):
+            if any((a == 1 and not np.isfinite(lo[i])) or (a == 2 and not np.isfinite(hi[i]))
+                   for i, a in enumerate(assign)):
+                continue
+            xs = np.where(np.array(assign) == 1, lo, np.where(np.array(assign) == 2, hi, 0.0))
+            fr = [i for i in range(n) if assign[i] == 0]
+            fixed = [i for i in range(n) if assign[i] != 0]
+            if fr:
+                rhs = b[fr] - A[np.ix_(fr, fixed)] @ xs[fixed] if fixed else b[fr]
+                xs[fr] = np.linalg.solve(A[np.ix_(fr, fr)], rhs)
+            ws = A @ xs - b
+            ok = all((assign[i] == 0 and lo[i] - 1e-9 <= xs[i] <= hi[i] + 1e-9) or
+                     (assign[i] == 1 and ws[i] >= -1e-9) or (assign[i] == 2 and ws[i] <= 1e-9) for i in range(n))
+            if ok:
+                sols.append(xs)
+        assert sols, trial
+        for s in sols:
+            np.testing.assert_allclose(s, sols[0], atol=1e-7)
+        np.testing.assert_allclose(x, sols[0], atol=1e-7)
