@@ -440,6 +440,15 @@ ODE_API int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterati
  * recreated every step.  dBatchSetState and this call clear the stored
  * impulses.  warm = 0 (the default) restores the reference's zero start. */
 ODE_API int dBatchSetWarmStart(dBatchID b, float warm);
+/* dWorldStep for every world of a batch (no counterpart in the reference's
+ * batch-less API; the single-world form is dWorldStep): each island's LCP
+ * solved exactly instead of by SOR iterations, in the reference order
+ * (ODEMI_ORDER_REFERENCE; refused for the coloured order).  Each world gets
+ * a device workspace for islands of up to max_island_rows rows
+ * (4 (2 m^2 + 7 m) bytes, m = max_island_rows); a larger island flags
+ * OVF_LCP (32) in the world's overflow counter and keeps the clamped last
+ * iterate.  0 switches back to the SOR iterations and frees the workspaces. */
+ODE_API int dBatchSetExactLCP(dBatchID b, int max_island_rows);
 /* Coloured order at a tolerance: on = 1 runs each step as three kernels --
  * collide and the solver set-up, the sweeps, the integration -- so that the
  * sweeps (the bulk of the work at a tolerance) get a lean kernel with its own

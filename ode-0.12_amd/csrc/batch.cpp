@@ -26,9 +26,11 @@ void destroy_graph(BatchImpl *b) {
 void refresh_dev(BatchImpl *b) {
   DevBatch &D = b->dev;
   D.nworlds = b->nworlds;
-  D.lcp_ws = nullptr;       // set per call by dWorldStep
-  D.lcp_ws_stride = 0;
-  D.lcp_ws_rows = 0;
+  // exact LCP workspaces (dBatchSetExactLCP; the ODE API's dWorldStep sets
+  // its own per call)
+  D.lcp_ws = b->lcp_rows > 0 ? b->lcp_ws.as<float>() : nullptr;
+  D.lcp_ws_stride = b->lcp_rows > 0 ? lcp_ws_floats(b->lcp_rows) : 0;
+  D.lcp_ws_rows = b->lcp_rows;
   D.order_mode = b->order_mode;
   D.wi = b->d.wi.as<WorldInfo>();
   D.wp = b->d.wp.as<WorldParams>();
@@ -966,6 +968,30 @@ int dBatchSetSolverToleranceAccel(dBatchID b, float tol, int max_iterations, int
   return b ? batch_set_tolerance(b->impl, tol, max_iterations, check_every, accel) : 0;
 }
 int dBatchSetWarmStart(dBatchID b, float warm) { return b ? batch_set_warm_start(b->impl, warm) : 0; }
+int dBatchSetExactLCP(dBatchID b, int max_island_rows) {
+  if (!b || max_island_rows < 0) return 0;
+  BatchImpl *i = b->impl;
+  if (max_island_rows > 0 && i->order_mode != ORDER_REFERENCE) {
+    report_error("dBatchSetExactLCP: the exact solve runs per island, in the reference order");
+    return 0;
+  }
+  if (hipStreamSynchronize(i->stream) != hipSuccess) return 0;
+  if (max_island_rows > 0) {
+    const size_t bytes = sizeof(float) * (size_t)lcp_ws_floats(max_island_rows) * (size_t)i->nworlds;
+    if (!i->lcp_ws.alloc(bytes)) {
+      report_error("dBatchSetExactLCP: device allocation of %zu bytes failed", bytes);
+      return 0;
+    }
+  } else {
+    i->lcp_ws.release();
+  }
+  i->lcp_rows = max_island_rows;
+  const int on = max_island_rows > 0 ? 1 : 0;
+  if (!patch_params(i, [&](WorldParams &p) { p.exact = on; })) return 0;
+  destroy_graph(i);
+  refresh_dev(i);
+  return 1;
+}
 int dBatchGetWorldCounters(dBatchID b, int *out) {
   if (!b || !out) return 0;
   BatchImpl *i = b->impl;

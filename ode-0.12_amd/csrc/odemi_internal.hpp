@@ -73,6 +73,8 @@ struct BatchImpl {
   int sor_wg = 128, sor_lds = 0, sor_grid = 0;
   int wall_khz = 100000;  // wall clock rate
   Tape tape;              // forward tape for the adjoint
+  DevBuf lcp_ws;          // exact LCP (dWorldStep form): per-world workspaces
+  int lcp_rows = 0;       // largest island (rows) they hold; 0: off
 };
 
 // host arrays of a batch before upload

@@ -141,6 +141,13 @@ class Batch:
         if not self.L.dBatchSetWarmStart(self.h, warm):
             raise ValueError("bad warm-start factor")
 
+    def set_exact_lcp(self, max_island_rows: int):
+        """dWorldStep for every world (reference order): each island's LCP
+        solved exactly, with workspaces for islands of up to
+        max_island_rows rows; 0 switches back to the SOR iterations."""
+        if not self.L.dBatchSetExactLCP(self.h, int(max_island_rows)):
+            raise ValueError("exact LCP needs the reference order and a workspace allocation")
+
     # ---- adjoint
     def tape_options(self, checkpoint_interval: int = 10, act_tol: float = 1e-6, lin_tol: float = 1e-6,
                      lin_maxit: int = 500):

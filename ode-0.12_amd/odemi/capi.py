@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchSetSolverTolerance": (C.c_int, [V, C.c_float, C.c_int, C.c_int]),
         "dBatchSetSolverToleranceAccel": (C.c_int, [V, C.c_float, C.c_int, C.c_int, C.c_int]),
         "dBatchSetWarmStart": (C.c_int, [V, C.c_float]),
+        "dBatchSetExactLCP": (C.c_int, [V, C.c_int]),
         "dBatchSetSorLaunch": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
         "dBatchGetSorLaunch": (C.c_int, [V, _ip, _ip, _ip, _ip]),
         "dBatchGetSplit": (C.c_int, [V]),

@@ -484,9 +484,8 @@ def test_dropin_world_step_matches_reference(odemi_mod, name):
     reference double build's dWorldStep (Dantzig's LCP solver, lcp.cpp):
     every step's callback pairs bit-exact; trajectories within 4x the FP32
     restatement's distance from the reference + 1e-3 m; the contact joints'
-    feedback on every diagnostic step within 4x the FP32 restatement's
-    distance + 2e-3 of the step's largest force (contacts matched inside a
-    pair by position)."""
+    feedback summed per geom pair on every diagnostic step within 4x the
+    FP32 restatement's distance + 2e-3 of the step's largest force."""
     say("drop-in dWorldStep", name)
     from test_golden import CASES
     fac, n, every, diag = CASES[name]
@@ -521,12 +520,69 @@ def test_dropin_world_step_matches_reference(odemi_mod, name):
         perm = _tie_order(r["contacts"][k0[s]:k0[s + 1]], ref)
         p32 = _tie_order(r32["contacts"][k0[s]:k0[s + 1]], ref)
         fref = g["feedback"][k0[s]:k0[s + 1]]
-        fe = float(np.abs(r["feedback"][k0[s]:k0[s + 1]][perm] - fref).max())
-        env_f = float(np.abs(r32["feedback"][k0[s]:k0[s + 1]][p32] - fref).max())
-        tol = 4 * env_f + 2e-3 * max(1.0, float(np.abs(fref).max()))
+        # per geom pair: the sum over its contacts (boxes resting on four
+        # coplanar contacts split their load as FP32 rounding decides -- the
+        # matrix is singular up to cfm along those directions -- while the
+        # sum is determined)
+        keys = ref[:, 7] * 100003 + ref[:, 8]
+        _, grp = np.unique(keys, return_inverse=True)
+        def pair_sum(f):
+            out = np.zeros((grp.max() + 1, f.shape[1]))
+            np.add.at(out, grp, f)
+            return out
+        fg = pair_sum(r["feedback"][k0[s]:k0[s + 1]][perm])
+        f32 = pair_sum(r32["feedback"][k0[s]:k0[s + 1]][p32])
+        fr = pair_sum(fref)
+        fe = float(np.abs(fg - fr).max())
+        env_f = float(np.abs(f32 - fr).max())
+        tol = 4 * env_f + 2e-3 * max(1.0, float(np.abs(fr).max()))
         worst = max(worst, fe / tol)
         assert fe <= tol, (name, s, fe, env_f)
-    say(name, "feedback worst err/tol", worst)
+    say(name, "feedback (per pair) worst err/tol", worst)
+
+
+def test_batch_exact_lcp_matches_world_step(odemi_mod):
+    """dBatchSetExactLCP: the box stack, chains in a pile and the vehicle
+    (per-step controls) stepped together in one batch with each island's LCP
+    solved exactly -- the batch form of dWorldStep -- against the reference
+    double build's dWorldStep fixtures: trajectories within 4x the FP32
+    restatement's distance + 1e-3 m, no world flagged; and each world of the
+    batch bit-identical to the same world stepped alone."""
+    from test_golden import CASES
+    names = ["box_stack_ws", "chains_ws", "vehicle_ws"]
+    scs = [CASES[k][0]() for k in names]
+    n, every = 60, 10
+    b = _batch(odemi_mod, scs, odemi_mod.ORDER_REFERENCE)
+    b.set_exact_lcp(256)
+    nb = [sc.nb for sc in scs]
+    off = np.concatenate([[0], np.cumsum(nb)])
+    trajs = [[] for _ in scs]
+    for k in range(n + 1):
+        if k % every == 0:
+            st = b.state().reshape(-1, 13)
+            for w in range(len(scs)):
+                trajs[w].append(st[off[w]:off[w + 1]].copy())
+        if k == n:
+            break
+        # per joint: lostop, histop (NaN: unchanged), torque1, torque2
+        b.set_controls(np.concatenate([sc.step_ctrl[k].reshape(-1) if sc.step_ctrl is not None
+                                       else np.tile([np.nan, np.nan, 0.0, 0.0], sc.nj) for sc in scs]))
+        b.step(1)
+    cnt = b.world_counters()
+    assert np.all((cnt[:, 4] & 32) == 0), cnt[:, 4]
+    for w, name in enumerate(names):
+        g = golden(name)
+        got = np.array(trajs[w])[:, :, :3]
+        ref = g["traj"][: n // every + 1, :, :3]
+        r32 = po.run(po.oracle_lib("f32"), scs[w], n, every)["traj"][:, :, :3]
+        env = np.abs(r32 - ref).max(axis=(1, 2))
+        err = np.abs(got - ref).max(axis=(1, 2))
+        say(name, "batch exact LCP traj err", float(err.max()), "envelope", float(env.max()))
+        assert np.all(err <= 4 * env + 1e-3), name
+    alone = _batch(odemi_mod, [scs[0]], odemi_mod.ORDER_REFERENCE)
+    alone.set_exact_lcp(256)
+    alone.step(n)
+    assert np.array_equal(alone.state().reshape(-1, 13), b.state().reshape(-1, 13)[off[0]:off[1]])
 
 
 def _tie_order(con, ref):
@@ -563,8 +619,9 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
     compared when its callback pairs and contact counts equal the
     reference's in this run and in the single build's (the runs then hold
     the same joints in the same order) and its contacts are as close to the
-    reference's as the single build's, on at least half of the steps with
-    contacts or 25 steps."""
+    reference's as the single build's and in the same order inside each pair
+    (the 20 sweeps relax rows in that order), on at least half of the steps
+    with contacts or 25 steps."""
     say("drop-in feedback", name)
     from test_golden import CASES
     fac, n, every, diag = CASES[name]
@@ -581,7 +638,7 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
     rk0 = np.concatenate([[0], np.cumsum(r["contact_step_count"])])
     sp0 = np.concatenate([[0], np.cumsum(r32["pair_step_count"])])
     sk0 = np.concatenate([[0], np.cumsum(r32["contact_step_count"])])
-    compared, drifted, worst = 0, 0, 0.0
+    compared, drifted, reordered, worst = 0, 0, 0, 0.0
     for s in range(diag):
         if cc[s] == 0:
             continue
@@ -592,6 +649,11 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
         ref = g["contacts"][k0[s]:k0[s + 1]]
         fref = g["feedback"][k0[s]:k0[s + 1]]
         perm = _tie_order(r["contacts"][rk0[s]:rk0[s + 1]], ref)
+        if not np.array_equal(perm, np.arange(len(perm))):
+            # a pair's contacts came out in another order (a tie decided by
+            # rounding): the 20 sweeps relax the rows in another order
+            reordered += 1
+            continue
         fgpu = r["feedback"][rk0[s]:rk0[s + 1]][perm]
         scale = max(1.0, float(np.abs(fref).max()))
         err = float(np.abs(fgpu - fref).max())
@@ -615,8 +677,8 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
         assert err <= tol, (name, s, err, env, scale, fgpu[:, :3].tolist(), fref[:, :3].tolist())
         compared += 1
     with_contacts = int((cc > 0).sum())
-    say(name, "steps compared", compared, "of", with_contacts, "(contacts drifted apart on", drifted, ") worst err/tol",
-        worst)
+    say(name, "steps compared", compared, "of", with_contacts, "(contacts drifted apart on", drifted,
+        ", contacts of a pair reordered on", reordered, ") worst err/tol", worst)
     # chaotic piles (cylinders pass through each other here as in the
     # reference) leave the reference's pair sets after a few dozen steps
     assert compared >= min(0.5 * with_contacts, 25)
