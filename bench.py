@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4, help="LCP residual the solver iterates down to")
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--accel", type=int, default=1,
+                    help="nonsmooth nonlinear CG steps between the sweeps of the iterate-to-tolerance solver (0: off)")
     ap.add_argument("--warm", type=float, default=1.0,
                     help="warm start of the iterate-to-tolerance solver (factor on the previous impulses; 0: off)")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="steps of the adjoint run (0: skip)")
@@ -459,7 +461,7 @@ def main():
     # before the window (state restored), the last K steps timed
     b = odemi.Batch(worlds, odemi.ORDER_COLORED)
     b.set_launch_config(cfg["wg"], cfg["cell_scale"], cfg["lds"], cfg["split"], cfg["grid"])
-    b.set_tolerance(args.tol, args.max_iters, args.check_every, accel=True)
+    b.set_tolerance(args.tol, args.max_iters, args.check_every, accel=bool(args.accel))
     b.set_warm_start(args.warm)
     if args.launch_tol:
         wg, cs, lds, split, grid = args.launch_tol.split(",")
@@ -549,7 +551,8 @@ def main():
                                f"interpenetration), h=0.005, mu=0.6, {ROLL}-step rollout, timed steps "
                                f"{t_from}-{ROLL}, seed 7",
                    "worlds": len(worlds), "bodies": nb, "geoms": ng, "solver_order": "colored",
-                   "solver": f"coloured Gauss-Seidel with nonsmooth nonlinear CG steps until LCP residual "
+                   "solver": f"coloured Gauss-Seidel {'with nonsmooth nonlinear CG steps ' if args.accel else ''}"
+                             f"until LCP residual "
                              f"<= {args.tol:g} (checked every {args.check_every}, cap {args.max_iters}), w=1.0, "
                              f"warm start {args.warm:g} x the previous step's impulses, every step of the rollout",
                    "launch": cfg_tol, "launch_fixed_iterations": cfg},

@@ -10,6 +10,9 @@
  *       dJointAttach) -> dWorldQuickStep -> dJointGroupEmpty
  *     runs unchanged against this library: broadphase, narrowphase, row
  *     build, SOR-LCP and integration execute as HIP kernels on the GPU.
+ *     dWorldStep (each island's LCP solved exactly), dSpaceCollide2 with
+ *     nested spaces, and sphere, box, plane, capsule, cylinder and ray geoms
+ *     are part of it.
  *
  *  2. Batch extension (dBatch*): thousands of independent worlds described by
  *     flat arrays, stepped together by one persistent kernel launch per step
@@ -495,6 +498,9 @@ ODE_API int dBatchSetLds(dBatchID b, int bytes);
  * doubles).  Reading resets the clocks. */
 ODE_API int dBatchSetProfiling(dBatchID b, int on);
 ODE_API int dBatchGetPhaseTimes(dBatchID b, double *ms8);
+/* The same per world (nworlds x 8 values, in dBatchGetPhaseTimes' phase
+ * order), also read-and-clear. */
+ODE_API int dBatchGetWorldPhaseTimes(dBatchID b, double *ms);
 /* Device pointer of the body positions (one float4 per body, w unused) and
  * the HIP stream the batch runs on, for zero-copy use from other GPU code. */
 /* Diagnostics of the last step of one world.  dBatchGetPairs writes the

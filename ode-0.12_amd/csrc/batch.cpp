@@ -1042,6 +1042,16 @@ int dBatchSetSplit(dBatchID b, int on) {
 int dBatchSetLds(dBatchID b, int bytes) { return b ? batch_set_lds(b->impl, bytes) : 0; }
 int dBatchSetProfiling(dBatchID b, int on) { return b ? batch_set_profiling(b->impl, on != 0) : 0; }
 int dBatchGetPhaseTimes(dBatchID b, double *ms) { return (b && ms) ? batch_phase_times(b->impl, ms) : 0; }
+int dBatchGetWorldPhaseTimes(dBatchID b, double *ms) {
+  if (!b || !ms) return 0;
+  BatchImpl *i = b->impl;
+  OD_HIP(hipStreamSynchronize(i->stream));
+  std::vector<unsigned long long> t((size_t)PH_N * i->nworlds);
+  OD_HIP(hipMemcpy(t.data(), i->d.prof.p, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+  OD_HIP(hipMemset(i->d.prof.p, 0, i->d.prof.n));
+  for (size_t k = 0; k < t.size(); k++) ms[k] = (double)t[k] / (double)i->wall_khz;
+  return 1;
+}
 
 int dBatchGetPairs(dBatchID b, int world, int maxn, int *out) {
   return b ? batch_get_pairs(b->impl, world, maxn, out) : -1;

@@ -241,6 +241,14 @@ class Batch:
             raise RuntimeError("dBatchGetPhaseTimes failed")
         return dict(zip(self.PHASES, list(out)))
 
+    def world_phase_times(self) -> np.ndarray:
+        """Workgroup-ms per phase and world since the last read,
+        (nworlds, 8) in PHASES order."""
+        out = np.zeros((self.nworlds, 8), np.float64)
+        if not self.L.dBatchGetWorldPhaseTimes(self.h, out.ctypes.data_as(C.POINTER(C.c_double))):
+            raise RuntimeError("dBatchGetWorldPhaseTimes failed")
+        return out
+
     def pairs(self, w: int, maxn: int = 1 << 20) -> np.ndarray:
         out = np.zeros((maxn, 2), np.int32)
         n = self.L.dBatchGetPairs(self.h, w, maxn, out.ctypes.data_as(C.POINTER(C.c_int)))

@@ -75,6 +75,7 @@ def load_library(path: str = LIB_PATH):
         "dBatchSetLds": (C.c_int, [V, C.c_int]),
         "dBatchSetProfiling": (C.c_int, [V, C.c_int]),
         "dBatchGetPhaseTimes": (C.c_int, [V, C.POINTER(C.c_double)]),
+        "dBatchGetWorldPhaseTimes": (C.c_int, [V, C.POINTER(C.c_double)]),
         "dBatchGetPairs": (C.c_int, [V, C.c_int, C.c_int, _ip]),
         "dBatchGetContacts": (C.c_int, [V, C.c_int, C.c_int, _fp]),
         "dBatchDeviceState": (V, [V]),

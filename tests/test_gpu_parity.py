@@ -619,9 +619,10 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
     compared when its callback pairs and contact counts equal the
     reference's in this run and in the single build's (the runs then hold
     the same joints in the same order) and its contacts are as close to the
-    reference's as the single build's and in the same order inside each pair
-    (the 20 sweeps relax rows in that order), on at least half of the steps
-    with contacts or 25 steps."""
+    reference's as the single build's (a step over the bar whose pair
+    contacts came out in another order than in the single build's is
+    counted apart: the 20 sweeps relaxed its rows in another order), on at
+    least half of the steps with contacts or 25 steps."""
     say("drop-in feedback", name)
     from test_golden import CASES
     fac, n, every, diag = CASES[name]
@@ -649,11 +650,6 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
         ref = g["contacts"][k0[s]:k0[s + 1]]
         fref = g["feedback"][k0[s]:k0[s + 1]]
         perm = _tie_order(r["contacts"][rk0[s]:rk0[s + 1]], ref)
-        if not np.array_equal(perm, np.arange(len(perm))):
-            # a pair's contacts came out in another order (a tie decided by
-            # rounding): the 20 sweeps relax the rows in another order
-            reordered += 1
-            continue
         fgpu = r["feedback"][rk0[s]:rk0[s + 1]][perm]
         scale = max(1.0, float(np.abs(fref).max()))
         err = float(np.abs(fgpu - fref).max())
@@ -673,6 +669,13 @@ def test_dropin_joint_feedback_every_step(odemi_mod, name):
             continue
         env = float(np.abs(r32["feedback"][sk0[s]:sk0[s + 1]][p32] - fref).max())
         tol = 4 * env + 2e-3 * scale
+        if err > tol and not np.array_equal(perm, p32):
+            # a pair's contacts came out in another order here than in the
+            # single restatement (a tie the last bit decides, e.g. a box
+            # resting flat): the 20 sweeps relaxed the rows in another order,
+            # which the restatement's distance does not cover
+            reordered += 1
+            continue
         worst = max(worst, err / tol)
         assert err <= tol, (name, s, err, env, scale, fgpu[:, :3].tolist(), fref[:, :3].tolist())
         compared += 1
