@@ -216,14 +216,24 @@ def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
         b.close()
         return np.stack(out)
 
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
     ref = gpu(odemi.ORDER_REFERENCE, False)
     col = gpu(odemi.ORDER_COLORED, False)
     head = gpu(odemi.ORDER_COLORED, True)
-    with mp.get_context("spawn").Pool(procs) as pool:
-        f64 = pool.map(_oracle_traj, [(w, nsteps, every, "f64") for w in worlds])
-        f32 = pool.map(_oracle_traj, [(w, nsteps, every, "f32") for w in worlds])
-    f64 = np.concatenate(f64, axis=1)
-    f32 = np.concatenate(f32, axis=1)
+    log(f"divergence ({label}): GPU runs done in {time.perf_counter() - t0:.1f}s")
+
+    def cpu(prec):
+        out = []
+        with mp.get_context("spawn").Pool(procs) as pool:
+            for k, tr in enumerate(pool.imap(_oracle_traj, [(w, nsteps, every, prec) for w in worlds])):
+                out.append(tr)
+                if (k + 1) % 16 == 0:
+                    log(f"divergence ({label}): {prec} restatement, {k + 1} of {len(worlds)} worlds")
+        return np.concatenate(out, axis=1)
+
+    f64 = cpu("f64")
+    f32 = cpu("f32")
     d = lambda a, b: float(np.abs(a - b).max())
     return {"sample": f"{label}: {len(worlds)} worlds ({sum(w.nb for w in worlds)} bodies), {nsteps} steps, "
                       f"positions every {every} steps",
@@ -470,7 +480,7 @@ def main():
     b.step(t_from)
     log(f"steps 0-{t_from} at tolerance (untimed): {time.perf_counter() - tw:.1f}s")
     if not args.launch_tol and args.autotune:
-        b.autotune(2, keep_cell=True, fused_only=True)
+        b.autotune(4, keep_cell=True, fused_only=True)
     cfg_tol = b.launch_config()
     log(f"launch config at tolerance {cfg_tol}")
     b.reset_solve_stats()

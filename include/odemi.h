@@ -519,9 +519,10 @@ ODE_API int dBatchNumBodies(dBatchID b);
 
 /* Starts recording the steps run by dBatchStep (at most `max_steps`): the
  * state is checkpointed every checkpoint_interval steps (default 10); the
- * backward pass replays each segment from its checkpoint.  Worlds must be
- * contact-only (spheres, boxes, planes; no permanent joints) with
- * dContactApprox1 friction (or mu = 0) and optional dContactBounce.  The
+ * backward pass replays each segment from its checkpoint.  Worlds may hold
+ * spheres, boxes and planes, ball joints and hinges without limits or a
+ * motor (no hinge2, no per-step joint controls), with dContactApprox1
+ * friction (or mu = 0) and optional dContactBounce.  The
  * derivative is that of the converged LCP: run the coloured order with a
  * solver tolerance (dBatchSetSolverTolerance) for accurate gradients. */
 ODE_API int dBatchTapeBegin(dBatchID b, int max_steps);

@@ -1,5 +1,6 @@
 // adj_step.hpp -- reverse-mode derivative of one quickstep of one world with
-// contact joints (spheres, boxes, planes), written once over the scalar type
+// contact joints (spheres, boxes, planes), ball joints and hinges (without
+// limit or motor rows), written once over the scalar type
 // and a parallel context so that the GPU runs it with one workgroup per world
 // (float) and the host runs the same code serially (double) for its checks.
 //
@@ -9,7 +10,8 @@
 // {n, t1, t2} (t1, t2 = dPlaneSpace(n)), c = erp/h * max(depth - min_depth, 0)
 // or the bounce term, b = c/h - J (v/h + M^-1 f), the LCP
 //   A lambda - b  in the normal cone of [lo, hi],  A = J M^-1 J^T + cfm/h,
-// friction bounds +-mu * lambda_normal, then v' = v + h M^-1 (J^T lambda + f),
+// friction bounds +-mu * lambda_normal, joint rows (joint.cpp:85-126,
+// hinge.cpp:76-149) unbounded, then v' = v + h M^-1 (J^T lambda + f),
 // x' = x + h v', q' = normalize(q + h/2 (0, w') q).
 //
 // The LCP is differentiated implicitly at its (converged) solution on the
@@ -36,6 +38,20 @@ struct TapeContact {
   int m;                         // rows (1 or 3)
 };
 
+// permanent-joint record of the tape: a ball joint or a hinge without a
+// limit or motor row, with its frame data as the step used it and the
+// converged impulses of its rows
+template <class T>
+struct TapeJoint {
+  T lam[6];
+  T anc1[3], anc2[3];   // anchors in the body frames (anc2: world point without a second body)
+  T ax1[3], ax2[3];     // hinge axes in the body frames (ax2: world axis without a second body)
+  T erp, cfm;           // the rows' Info2 erp and cfm
+  int type;             // JT_BALL or JT_HINGE
+  int b1, b2;           // node order after attach (b2 may be -1)
+  int m;                // rows (3 or 5)
+};
+
 template <class T>
 struct StepParams {
   T h, erp, cfm, min_depth, max_vel, gravity[3];
@@ -59,6 +75,9 @@ struct StepData {
   const int *gtype, *gbody;
   const T *gprm, *gpos, *gR;
   const TapeContact<T> *con;
+  // permanent joints of the step (their rows follow the contacts' rows)
+  int nj = 0;
+  const TapeJoint<T> *jnt = nullptr;
 };
 
 // adjoints: in = of the post-step state, out = of the pre-step state
@@ -74,8 +93,9 @@ struct StepAdj {
 // scratch sizes
 constexpr int ADJ_BODY_FLOATS = 64;
 constexpr int ADJ_ROW_FLOATS = 33;
-template <class T> OD_HD inline long long adj_scratch_elems(int nb, int nk) {
-  return (long long)ADJ_BODY_FLOATS * nb + (long long)ADJ_ROW_FLOATS * 3 * nk + 16;
+// row slots: 3 per contact, then 6 per permanent joint
+template <class T> OD_HD inline long long adj_scratch_elems(int nb, int nk, int nj = 0) {
+  return (long long)ADJ_BODY_FLOATS * nb + (long long)ADJ_ROW_FLOATS * (3LL * nk + 6LL * nj) + 16;
 }
 
 // per-body scratch fields
@@ -198,16 +218,108 @@ OD_HD inline void contact_rows(const StepData<T> &D, const TapeContact<T> &c, T 
   }
 }
 
+// Rows of a ball joint or a hinge (ball.cpp:56-64 with the ball rows of
+// joint.cpp:85-126; hinge.cpp:76-149 without its limit/motor row): J (12 per
+// row, rows `stride` apart) and c, over a scalar type S so that the same code
+// gives the rows (S = T) and their derivatives (S = Dual).  k = erp / h.
+template <class T, class S>
+OD_HD inline void joint_rows(const TapeJoint<T> &j, const V3<S> &x1, const M3<S> &R1, const V3<S> &x2,
+                             const M3<S> &R2, T k, S *J, int stride, S *c) {
+  const bool two = j.b2 >= 0;
+  const S one = S(T(1)), zero = S(T(0));
+  for (int r = 0; r < j.m; r++)
+    for (int i = 0; i < 12; i++) J[stride * r + i] = zero;
+  const V3<S> a1 = R1.mul(v3(S(j.anc1[0]), S(j.anc1[1]), S(j.anc1[2])));
+  S *J0 = J, *J1 = J + stride, *J2 = J + 2 * stride;
+  // J_l1 = I, J_a1 = -[a1]x
+  J0[0] = one; J1[1] = one; J2[2] = one;
+  J0[4] = a1.z;  J0[5] = -a1.y;
+  J1[3] = -a1.z; J1[5] = a1.x;
+  J2[3] = a1.y;  J2[4] = -a1.x;
+  V3<S> d;
+  if (two) {
+    const V3<S> a2 = R2.mul(v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2])));
+    // J_l2 = -I, J_a2 = [a2]x
+    J0[6] = -one; J1[7] = -one; J2[8] = -one;
+    J0[10] = -a2.z; J0[11] = a2.y;
+    J1[9] = a2.z;   J1[11] = -a2.x;
+    J2[9] = -a2.y;  J2[10] = a2.x;
+    d = ((a2 + x2) - a1) - x1;
+  } else {
+    d = (v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2])) - a1) - x1;
+  }
+  c[0] = k * d.x;
+  c[1] = k * d.y;
+  c[2] = k * d.z;
+  if (j.type != JT_HINGE) return;
+  // the two rows keeping the hinge axes aligned
+  const V3<S> ax1 = R1.mul(v3(S(j.ax1[0]), S(j.ax1[1]), S(j.ax1[2])));
+  V3<S> p, q;
+  plane_space(ax1, p, q);
+  S *J3 = J + 3 * stride, *J4 = J + 4 * stride;
+  J3[3] = p.x; J3[4] = p.y; J3[5] = p.z;
+  J4[3] = q.x; J4[4] = q.y; J4[5] = q.z;
+  if (two) {
+    J3[9] = -p.x; J3[10] = -p.y; J3[11] = -p.z;
+    J4[9] = -q.x; J4[10] = -q.y; J4[11] = -q.z;
+  }
+  const V3<S> wa2 = v3(S(j.ax2[0]), S(j.ax2[1]), S(j.ax2[2]));
+  const V3<S> ax2 = two ? R2.mul(wa2) : wa2;
+  const V3<S> b = cross(ax1, ax2);
+  c[3] = k * dot(b, p);
+  c[4] = k * dot(b, q);
+}
+
+// a body pose as dual numbers: the position seeded with the unit directions
+// (kind 0) or the rotation R -> exp([theta]x) R seeded along theta (kind 1),
+// or constant (kind -1)
+template <class T>
+OD_HD inline void pose_dual(const V3<T> &p, const M3<T> &R, int kind, V3<Dual<T, 3>> &pd, M3<Dual<T, 3>> &Rd) {
+  typedef Dual<T, 3> S;
+  pd = v3(S(p.x), S(p.y), S(p.z));
+  Rd.r0 = v3(S(R.r0.x), S(R.r0.y), S(R.r0.z));
+  Rd.r1 = v3(S(R.r1.x), S(R.r1.y), S(R.r1.z));
+  Rd.r2 = v3(S(R.r2.x), S(R.r2.y), S(R.r2.z));
+  if (kind == 0) {
+    pd.x.d[0] = T(1); pd.y.d[1] = T(1); pd.z.d[2] = T(1);
+  } else if (kind == 1) {
+    // d R / d theta_j = [e_j]x R: column cc gets e_j x R_cc
+    for (int cc = 0; cc < 3; cc++) {
+      const V3<T> col = R.col(cc);
+      S *e0 = cc == 0 ? &Rd.r0.x : (cc == 1 ? &Rd.r0.y : &Rd.r0.z);
+      S *e1 = cc == 0 ? &Rd.r1.x : (cc == 1 ? &Rd.r1.y : &Rd.r1.z);
+      S *e2 = cc == 0 ? &Rd.r2.x : (cc == 1 ? &Rd.r2.y : &Rd.r2.z);
+      e0->d[0] = T(0);    e1->d[0] = -col.z; e2->d[0] = col.y;
+      e0->d[1] = col.z;   e1->d[1] = T(0);   e2->d[1] = -col.x;
+      e0->d[2] = -col.y;  e1->d[2] = col.x;  e2->d[2] = T(0);
+    }
+  }
+}
+
 // The adjoint of one step of one world.  Par provides tid(), nt(), sync(),
 // sum(T) (all-reduce) and add(T*, T) (atomic accumulate).
 template <class T, class Par>
 OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, StepAdj<T> &G, T *scr) {
-  const int nb = D.nb, nk = D.nk;
+  const int nb = D.nb, nk = D.nk, nj = D.nj;
   const int tid = par.tid(), nt = par.nt();
   const T h = P.h, hinv = T(1) / P.h;
+  // row slots: 3 per contact (slot 3k + r), then 6 per joint (3 nk + 6 j + r)
+  const int ncs = 3 * nk, nslots = ncs + 6 * nj;
   T *bsc = scr;                                           // ADJ_BODY_FLOATS per body
-  T *rsc = scr + (long long)ADJ_BODY_FLOATS * nb;         // ADJ_ROW_FLOATS per row, 3 rows per contact
-  T *wsc = rsc + (long long)ADJ_ROW_FLOATS * 3 * nk;      // 16 world scalars
+  T *rsc = scr + (long long)ADJ_BODY_FLOATS * nb;         // ADJ_ROW_FLOATS per row slot
+  T *wsc = rsc + (long long)ADJ_ROW_FLOATS * nslots;      // 16 world scalars
+  // bodies and cfm of a row slot; false for a slot beyond its constraint's rows
+  auto slot_rows = [&](int s, int &b1, int &b2, T &cfm) -> bool {
+    if (s < ncs) {
+      const TapeContact<T> &c = D.con[s / 3];
+      b1 = c.b1; b2 = c.b2; cfm = P.cfm;
+      return s % 3 < c.m;
+    }
+    const TapeJoint<T> &jt = D.jnt[(s - ncs) / 6];
+    b1 = jt.b1; b2 = jt.b2; cfm = jt.cfm;
+    return (s - ncs) % 6 < jt.m;
+  };
+  auto joint_slots = [&](int j) -> T * { return rsc + ADJ_ROW_FLOATS * (long long)(ncs + 6 * j); };
 
   // ---- 0. bodies: R, I_w, I_w^-1, torque, Z; clear accumulators
   for (int b = tid; b < nb; b += nt) {
@@ -243,6 +355,26 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       for (int i = 0; i < 6; i++) par.add(b1 + i, J[i] * l);
       if (c.b2 >= 0) {
         T *b2 = bsc + ADJ_BODY_FLOATS * c.b2 + BJL;
+        for (int i = 0; i < 6; i++) par.add(b2 + i, J[6 + i] * l);
+      }
+    }
+  }
+  // joint rows (from the pre-step poses), J^T lambda per body
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    T *J6 = joint_slots(j);
+    T c6[6];
+    const V3<T> zero3 = v3(T(0), T(0), T(0));
+    const M3<T> R2 = jt.b2 >= 0 ? ldm(bsc + ADJ_BODY_FLOATS * jt.b2 + BR) : M3<T>{zero3, zero3, zero3};
+    joint_rows<T, T>(jt, ld3(D.x + 4 * jt.b1), ldm(bsc + ADJ_BODY_FLOATS * jt.b1 + BR),
+                     jt.b2 >= 0 ? ld3(D.x + 4 * jt.b2) : zero3, R2, jt.erp * hinv, J6, ADJ_ROW_FLOATS, c6);
+    for (int r = 0; r < jt.m; r++) {
+      const T *J = J6 + ADJ_ROW_FLOATS * r;
+      const T l = jt.lam[r];
+      T *b1 = bsc + ADJ_BODY_FLOATS * jt.b1 + BJL;
+      for (int i = 0; i < 6; i++) par.add(b1 + i, J[i] * l);
+      if (jt.b2 >= 0) {
+        T *b2 = bsc + ADJ_BODY_FLOATS * jt.b2 + BJL;
         for (int i = 0; i < 6; i++) par.add(b2 + i, J[6 + i] * l);
       }
     }
@@ -337,6 +469,30 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       rr_loc += g[r] * g[r];
     }
   }
+  // joint rows are bilateral: always free, g = lambda_bar
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    T *J6 = joint_slots(j);
+    const T *u1 = bsc + ADJ_BODY_FLOATS * jt.b1 + BU;
+    const T *u2 = jt.b2 >= 0 ? bsc + ADJ_BODY_FLOATS * jt.b2 + BU : nullptr;
+    for (int r = 0; r < 6; r++) {
+      T *R = J6 + ADJ_ROW_FLOATS * r;
+      if (r >= jt.m) {
+        for (int i = 0; i < ADJ_ROW_FLOATS; i++) R[i] = T(0);
+        continue;
+      }
+      const T lb = jdot6(R, u1) + (u2 ? jdot6(R + 6, u2) : T(0));
+      R[RLB] = lb;
+      for (int i = 0; i < 6; i++) R[RJB + i] = jt.lam[r] * u1[i];
+      for (int i = 0; i < 6; i++) R[RJB + 6 + i] = u2 ? jt.lam[r] * u2[i] : T(0);
+      R[RY] = T(0);
+      R[RR] = lb;
+      R[RRH] = lb;
+      R[RP] = T(0);
+      R[RV] = T(0);
+      rr_loc += lb * lb;
+    }
+  }
   T gnorm2 = par.sum(rr_loc);
 
   // K^T applied to the row vector in field `src` (zero outside the free
@@ -347,19 +503,18 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     for (int b = tid; b < nb; b += nt)
       for (int i = 0; i < 6; i++) bsc[ADJ_BODY_FLOATS * b + BJY + i] = T(0);
     par.sync();
-    for (int k = tid; k < nk; k += nt) {
-      const TapeContact<T> &c = D.con[k];
-      const T *J3 = rsc + ADJ_ROW_FLOATS * 3 * k;
-      for (int r = 0; r < c.m; r++) {
-        const T *R = J3 + ADJ_ROW_FLOATS * r;
-        T y = R[src];
-        if (y == T(0)) continue;
-        T *b1 = bsc + ADJ_BODY_FLOATS * c.b1 + BJY;
-        for (int i = 0; i < 6; i++) par.add(b1 + i, R[i] * y);
-        if (c.b2 >= 0) {
-          T *b2 = bsc + ADJ_BODY_FLOATS * c.b2 + BJY;
-          for (int i = 0; i < 6; i++) par.add(b2 + i, R[6 + i] * y);
-        }
+    for (int s = tid; s < nslots; s += nt) {
+      int sb1, sb2;
+      T scfm;
+      if (!slot_rows(s, sb1, sb2, scfm)) continue;
+      const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+      const T y = R[src];
+      if (y == T(0)) continue;
+      T *b1 = bsc + ADJ_BODY_FLOATS * sb1 + BJY;
+      for (int i = 0; i < 6; i++) par.add(b1 + i, R[i] * y);
+      if (sb2 >= 0) {
+        T *b2 = bsc + ADJ_BODY_FLOATS * sb2 + BJY;
+        for (int i = 0; i < 6; i++) par.add(b2 + i, R[6 + i] * y);
       }
     }
     par.sync();
@@ -386,18 +541,32 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       }
       for (int r = 0; r < 3; r++) J3[ADJ_ROW_FLOATS * r + dst] = o[r];
     }
+    for (int j = tid; j < nj; j += nt) {
+      const TapeJoint<T> &jt = D.jnt[j];
+      T *J6 = joint_slots(j);
+      T m1[6], m2[6];
+      minv6(bsc + ADJ_BODY_FLOATS * jt.b1, D.invMass[jt.b1], bsc + ADJ_BODY_FLOATS * jt.b1 + BJY, m1);
+      if (jt.b2 >= 0) minv6(bsc + ADJ_BODY_FLOATS * jt.b2, D.invMass[jt.b2], bsc + ADJ_BODY_FLOATS * jt.b2 + BJY, m2);
+      for (int r = 0; r < 6; r++) {
+        T *R = J6 + ADJ_ROW_FLOATS * r;
+        R[dst] = r < jt.m ? jdot6(R, m1) + (jt.b2 >= 0 ? jdot6(R + 6, m2) : T(0)) + jt.cfm * hinv * R[src] : T(0);
+      }
+    }
     par.sync();
   };
-  // Jacobi preconditioner: inverse diagonal of A on free rows
-  auto precond = [&](const StepData<T> &Dd, int k, int r, T x) -> T {
-    const TapeContact<T> &c = Dd.con[k];
-    const T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-    T m1[6], d = P.cfm * hinv;
-    minv6(bsc + ADJ_BODY_FLOATS * c.b1, Dd.invMass[c.b1], R, m1);
+  // Jacobi preconditioner: inverse diagonal of A on the row of slot s (0
+  // beyond the constraint's rows)
+  auto precond = [&](int s, T x) -> T {
+    int sb1, sb2;
+    T scfm;
+    if (!slot_rows(s, sb1, sb2, scfm)) return T(0);
+    const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+    T m1[6], d = scfm * hinv;
+    minv6(bsc + ADJ_BODY_FLOATS * sb1, D.invMass[sb1], R, m1);
     d += jdot6(R, m1);
-    if (c.b2 >= 0) {
+    if (sb2 >= 0) {
       T m2[6];
-      minv6(bsc + ADJ_BODY_FLOATS * c.b2, Dd.invMass[c.b2], R + 6, m2);
+      minv6(bsc + ADJ_BODY_FLOATS * sb2, D.invMass[sb2], R + 6, m2);
       d += jdot6(R + 6, m2);
     }
     return d > T(0) ? x / d : x;
@@ -413,107 +582,96 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     // y = 0 with residual |g|^2); a non-finite or exploding residual ends the
     // iteration and the kept iterate is used
     T best2 = gnorm2;
-    for (int k = tid; k < nk; k += nt)
-      for (int r = 0; r < 3; r++) rsc[ADJ_ROW_FLOATS * (3 * k + r) + RYB] = T(0);
+    for (int s = tid; s < nslots; s += nt) rsc[ADJ_ROW_FLOATS * (long long)s + RYB] = T(0);
     bool use_best = false;
     // fields: RY x, RR r, RRH r_hat, RP p, RV v, RS s, RT t; a hat vector is
     // kept in the J_bar-free slots RJB..: no -- use the dedicated scratch
     // below (wsc is scalars only), so the hat vectors reuse RS / RT in turn.
     for (iters = 0; iters < P.lin_maxit; iters++) {
       T loc = T(0);
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          const T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          loc += R[RRH] * R[RR];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        loc += R[RRH] * R[RR];
+      }
       T rho1 = par.sum(loc);
       if (rho1 == T(0)) break;
       T beta = (rho1 / rho0) * (alpha / omega);
       // p = r + beta (p - omega v); s := M^-1 p (hat)
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          R[RP] = R[RR] + beta * (R[RP] - omega * R[RV]);
-          R[RS] = (r < D.con[k].m) ? precond(D, k, r, R[RP]) : T(0);
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        R[RP] = R[RR] + beta * (R[RP] - omega * R[RV]);
+        R[RS] = precond(s, R[RP]);
+      }
       par.sync();
       apply_KT(RS, RV);   // v = K^T M^-1 p
       loc = T(0);
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          const T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          loc += R[RRH] * R[RV];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        loc += R[RRH] * R[RV];
+      }
       T den = par.sum(loc);
       if (den == T(0)) break;
       alpha = rho1 / den;
       // x += alpha p_hat; s = r - alpha v
       loc = T(0);
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          R[RY] += alpha * R[RS];
-          R[RS] = R[RR] - alpha * R[RV];
-          loc += R[RS] * R[RS];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        R[RY] += alpha * R[RS];
+        R[RS] = R[RR] - alpha * R[RV];
+        loc += R[RS] * R[RS];
+      }
       T ss = par.sum(loc);
       if (ss <= stop2) {
-        for (int k = tid; k < nk; k += nt)
-          for (int r = 0; r < 3; r++) rsc[ADJ_ROW_FLOATS * (3 * k + r) + RR] = rsc[ADJ_ROW_FLOATS * (3 * k + r) + RS];
+        for (int s = tid; s < nslots; s += nt) rsc[ADJ_ROW_FLOATS * (long long)s + RR] = rsc[ADJ_ROW_FLOATS * (long long)s + RS];
         iters++;
         break;
       }
       // z_hat = M^-1 s into RP (p is rebuilt from r next iteration... keep p:
       // store z_hat in RT temporarily, t = K^T z_hat into RJB+11 is not free,
       // so t overwrites RT after z_hat has been copied to the x update below)
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          R[RT] = (r < D.con[k].m) ? precond(D, k, r, R[RS]) : T(0);
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        R[RT] = precond(s, R[RS]);
+      }
       par.sync();
       // t = K^T z_hat: computed into RJB+... not available; use RR as the
       // destination (r is recomputed from s and t below)
       apply_KT(RT, RR);   // RR := t
       T ts = T(0), tt = T(0);
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          const T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          ts += R[RR] * R[RS];
-          tt += R[RR] * R[RR];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        ts += R[RR] * R[RS];
+        tt += R[RR] * R[RR];
+      }
       ts = par.sum(ts);
       tt = par.sum(tt);
       omega = tt > T(0) ? ts / tt : T(0);
       loc = T(0);
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          R[RY] += omega * R[RT];
-          R[RR] = R[RS] - omega * R[RR];   // r = s - omega t
-          loc += R[RR] * R[RR];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        R[RY] += omega * R[RT];
+        R[RR] = R[RS] - omega * R[RR];   // r = s - omega t
+        loc += R[RR] * R[RR];
+      }
       T rr = par.sum(loc);
       rho0 = rho1;
       if (!(rr <= T(1e12) * gnorm2)) { use_best = true; iters++; break; }   // diverged or not finite
       if (rr < best2) {
         best2 = rr;
-        for (int k = tid; k < nk; k += nt)
-          for (int r = 0; r < 3; r++) {
-            T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-            R[RYB] = R[RY];
-          }
+        for (int s = tid; s < nslots; s += nt) {
+          T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+          R[RYB] = R[RY];
+        }
       }
       if (rr <= stop2 || omega == T(0)) { iters++; break; }
       if (iters + 1 == P.lin_maxit && rr > best2) use_best = true;   // ended worse than the best iterate
     }
     if (use_best) {
       par.sync();
-      for (int k = tid; k < nk; k += nt)
-        for (int r = 0; r < 3; r++) {
-          T *R = rsc + ADJ_ROW_FLOATS * (3 * k + r);
-          R[RY] = R[RYB];
-        }
+      for (int s = tid; s < nslots; s += nt) {
+        T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+        R[RY] = R[RYB];
+      }
     }
   }
   if (tid == 0 && G.stats) G.stats[0] += iters;
@@ -563,24 +721,45 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
         for (int i = 0; i < 6; i++) R[RJB + 6 + i] += -(y * wl2[i] + l * wy2[i]) - y * Z2[i];
     }
   }
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    T *J6 = joint_slots(j);
+    T wl1[6], wl2[6], wy1[6], wy2[6];
+    const T *b1s = bsc + ADJ_BODY_FLOATS * jt.b1;
+    minv6(b1s, D.invMass[jt.b1], b1s + BJL, wl1);
+    minv6(b1s, D.invMass[jt.b1], b1s + BJY, wy1);
+    const T *b2s = jt.b2 >= 0 ? bsc + ADJ_BODY_FLOATS * jt.b2 : nullptr;
+    if (b2s) {
+      minv6(b2s, D.invMass[jt.b2], b2s + BJL, wl2);
+      minv6(b2s, D.invMass[jt.b2], b2s + BJY, wy2);
+    }
+    const T *Z1 = b1s + BZ;
+    const T *Z2 = b2s ? b2s + BZ : nullptr;
+    for (int r = 0; r < jt.m; r++) {
+      T *R = J6 + ADJ_ROW_FLOATS * r;
+      const T y = R[RY], l = jt.lam[r];
+      for (int i = 0; i < 6; i++) R[RJB + i] += -(y * wl1[i] + l * wy1[i]) - y * Z1[i];
+      if (b2s)
+        for (int i = 0; i < 6; i++) R[RJB + 6 + i] += -(y * wl2[i] + l * wy2[i]) - y * Z2[i];
+    }
+  }
   par.sync();
   // Z_bar = -J^T b_bar per body (into BJY, reused), c_bar = b_bar / h
   for (int b = tid; b < nb; b += nt)
     for (int i = 0; i < 6; i++) bsc[ADJ_BODY_FLOATS * b + BJY + i] = T(0);
   par.sync();
-  for (int k = tid; k < nk; k += nt) {
-    const TapeContact<T> &c = D.con[k];
-    const T *J3 = rsc + ADJ_ROW_FLOATS * 3 * k;
-    for (int r = 0; r < c.m; r++) {
-      const T *R = J3 + ADJ_ROW_FLOATS * r;
-      T y = R[RY];
-      if (y == T(0)) continue;
-      T *b1 = bsc + ADJ_BODY_FLOATS * c.b1 + BJY;
-      for (int i = 0; i < 6; i++) par.add(b1 + i, -R[i] * y);
-      if (c.b2 >= 0) {
-        T *b2 = bsc + ADJ_BODY_FLOATS * c.b2 + BJY;
-        for (int i = 0; i < 6; i++) par.add(b2 + i, -R[6 + i] * y);
-      }
+  for (int s = tid; s < nslots; s += nt) {
+    int sb1, sb2;
+    T scfm;
+    if (!slot_rows(s, sb1, sb2, scfm)) continue;
+    const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
+    const T y = R[RY];
+    if (y == T(0)) continue;
+    T *b1 = bsc + ADJ_BODY_FLOATS * sb1 + BJY;
+    for (int i = 0; i < 6; i++) par.add(b1 + i, -R[i] * y);
+    if (sb2 >= 0) {
+      T *b2 = bsc + ADJ_BODY_FLOATS * sb2 + BJY;
+      for (int i = 0; i < 6; i++) par.add(b2 + i, -R[6 + i] * y);
     }
   }
   par.sync();
@@ -687,6 +866,41 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     st3(GA + RJB, pbar);
     st3(GA + RJB + 3, nb_);
     GA[RJB + 6] = depth_bar;
+  }
+  // joints: J and c = erp/h (...) depend on both poses; c_bar = b_bar / h.
+  // Position and rotation adjoints of each body by dual numbers over the
+  // joint's rows.
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    const T *J6 = joint_slots(j);
+    const bool two = jt.b2 >= 0;
+    const V3<T> zero3 = v3(T(0), T(0), T(0));
+    const V3<T> x1 = ld3(D.x + 4 * jt.b1), x2 = two ? ld3(D.x + 4 * jt.b2) : zero3;
+    const M3<T> R1 = ldm(bsc + ADJ_BODY_FLOATS * jt.b1 + BR);
+    const M3<T> R2 = two ? ldm(bsc + ADJ_BODY_FLOATS * jt.b2 + BR) : M3<T>{zero3, zero3, zero3};
+    for (int pass = 0; pass < (two ? 4 : 2); pass++) {
+      const int side = pass >> 1, kind = pass & 1;   // body 1 / 2; position / rotation
+      typedef Dual<T, 3> S;
+      V3<S> px1, px2;
+      M3<S> pR1, pR2;
+      pose_dual(x1, R1, side == 0 ? kind : -1, px1, pR1);
+      pose_dual(x2, R2, side == 1 ? kind : -1, px2, pR2);
+      S Jd[6 * 12], cd[6];
+      joint_rows<T, S>(jt, px1, pR1, px2, pR2, jt.erp * hinv, Jd, 12, cd);
+      T acc[3] = {T(0), T(0), T(0)};
+      for (int r = 0; r < jt.m; r++) {
+        const T *R = J6 + ADJ_ROW_FLOATS * r;
+        const T cbar = R[RY] * hinv;
+        for (int dir = 0; dir < 3; dir++) {
+          T a = cbar * cd[r].d[dir];
+          for (int i = 0; i < 12; i++) a += R[RJB + i] * Jd[12 * r + i].d[dir];
+          acc[dir] += a;
+        }
+      }
+      const int body = side == 0 ? jt.b1 : jt.b2;
+      T *dst = kind == 0 ? G.gx + 4 * body : bsc + ADJ_BODY_FLOATS * body + BTH;
+      for (int dir = 0; dir < 3; dir++) par.add(dst + dir, acc[dir]);
+    }
   }
   par.sync();
 

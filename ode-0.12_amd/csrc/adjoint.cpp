@@ -8,23 +8,37 @@
 namespace odemi {
 
 hipError_t launch_tape_capture(const DevBatch &B, adj::TapeContact<float> *con, const long long *con_off, int *nk,
-                               int *ovf, hipStream_t st);
+                               int *ovf, adj::TapeJoint<float> *jt, hipStream_t st);
 hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], const adj::TapeContact<float> *con,
                                const long long *con_off, const int *nk, const float4 *const gin[4],
                                float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
                                const long long *scr_off, float act_tol, float lin_tol, int lin_maxit, int wg,
-                               hipStream_t st_);
+                               const adj::TapeJoint<float> *jt, hipStream_t st_);
 
 namespace {
 
-// A world the adjoint supports: contact joints only (no permanent joints),
-// dContactApprox1 friction (or mu = 0), optional bounce, no other surface
-// modes.
+// A world the adjoint supports: contact joints, ball joints and hinges
+// without limits or a motor (their rows are bilateral), dContactApprox1
+// friction (or mu = 0), optional bounce, no other surface modes.
 bool world_supported(const BatchImpl *b, int w, char *why, size_t n) {
   const WorldInfo &W = b->wi[w];
   const WorldParams &P = b->wp[w];
-  if (W.nj > 0) {
-    snprintf(why, n, "world %d has %d permanent joints", w, W.nj);
+  const float PI = 3.14159265358979323846f;
+  for (int j = 0; j < W.nj; j++) {
+    const JointDev &J = b->h_joints[W.j0 + j];
+    if (J.type != JT_BALL && J.type != JT_HINGE) {
+      snprintf(why, n, "world %d has a hinge2 joint (the adjoint covers ball joints and hinges)", w);
+      return false;
+    }
+    if (J.type == JT_HINGE &&
+        (J.lm1.fmax > 0.f || ((J.lm1.lostop >= -PI || J.lm1.histop <= PI) && J.lm1.lostop <= J.lm1.histop))) {
+      snprintf(why, n, "world %d has a hinge with a limit or a motor (the adjoint covers its bilateral rows only)",
+               w);
+      return false;
+    }
+  }
+  if (W.nj > 0 && b->ctrl_set) {
+    snprintf(why, n, "world %d: per-step joint controls are set", w);
     return false;
   }
   for (int g = 0; g < W.ng; g++)
@@ -55,7 +69,7 @@ int tape_begin(BatchImpl *b, int max_steps) {
   char why[160];
   for (int w = 0; w < b->nworlds; w++)
     if (!world_supported(b, w, why, sizeof(why))) {
-      report_error("dBatchTapeBegin: %s; the adjoint covers contact-only worlds", why);
+      report_error("dBatchTapeBegin: %s", why);
       return 0;
     }
   Tape &t = b->tape;
@@ -76,7 +90,8 @@ int tape_begin(BatchImpl *b, int max_steps) {
     if (!(t.kst.alloc(4 * stbytes * (size_t)max_steps) &&
           t.kcon.alloc(sizeof(adj::TapeContact<float>) * (size_t)ncon * max_steps) &&
           t.knk.alloc(sizeof(int) * (size_t)nw * max_steps) && t.kovf.alloc(sizeof(int)) &&
-          t.kcon_off.alloc(sizeof(long long) * nw))) {
+          t.kcon_off.alloc(sizeof(long long) * nw) &&
+          (b->nj == 0 || t.kjnt.alloc(sizeof(adj::TapeJoint<float>) * (size_t)b->nj * max_steps)))) {
       report_error("dBatchTapeBegin: device allocation for %d kept steps failed", max_steps);
       return 0;
     }
@@ -109,6 +124,7 @@ int tape_keep_steps(BatchImpl *b, bool on) {
     t.knk.release();
     t.kovf.release();
     t.kcon_off.release();
+    t.kjnt.release();
     if (t.kept) {
       t.kept = false;
       t.nsteps = 0;
@@ -144,7 +160,9 @@ int tape_step(BatchImpl *b, bool sync) {
     if (!batch_launch_once(b)) return 0;
     OD_HIP(launch_tape_capture(b->dev, t.kcon.as<adj::TapeContact<float>>() + (size_t)t.nsteps * t.kncon,
                                t.kcon_off.as<long long>(), t.knk.as<int>() + (size_t)t.nsteps * nw,
-                               t.kovf.as<int>(), b->stream));
+                               t.kovf.as<int>(),
+                               b->nj ? t.kjnt.as<adj::TapeJoint<float>>() + (size_t)t.nsteps * b->nj : nullptr,
+                               b->stream));
   } else if (!batch_launch_once(b)) {
     return 0;
   }
@@ -176,16 +194,17 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
     con_off[w] = ncon;
     scr_off[w] = nscr;
     ncon += b->wi[w].kcap;
-    nscr += adj::adj_scratch_elems<float>(b->wi[w].nb, b->wi[w].kcap);
+    nscr += adj::adj_scratch_elems<float>(b->wi[w].nb, b->wi[w].kcap, b->wi[w].nj);
     nscr = (nscr + 63) & ~63LL;
   }
-  DevBuf d_con_off, d_scr_off, seg_st, seg_con, seg_nk, gbuf, scr, gmu, gbn, stats;
+  DevBuf d_con_off, d_scr_off, seg_st, seg_con, seg_nk, seg_jnt, gbuf, scr, gmu, gbn, stats;
   const size_t stbytes = sizeof(float4) * (size_t)nb;
   bool ok = d_con_off.alloc(sizeof(long long) * nw) && d_scr_off.alloc(sizeof(long long) * nw) &&
             seg_st.alloc(4 * stbytes * K) && seg_con.alloc(sizeof(adj::TapeContact<float>) * (size_t)ncon * K) &&
             seg_nk.alloc(sizeof(int) * (size_t)nw * K) && gbuf.alloc(8 * stbytes) &&
             scr.alloc(sizeof(float) * (size_t)nscr) && gmu.alloc(sizeof(float) * nw) &&
-            gbn.alloc(sizeof(float) * nw) && stats.alloc(sizeof(int) * (2 * (size_t)nw + 1));
+            gbn.alloc(sizeof(float) * nw) && stats.alloc(sizeof(int) * (2 * (size_t)nw + 1)) &&
+            (b->nj == 0 || t.kept || seg_jnt.alloc(sizeof(adj::TapeJoint<float>) * (size_t)b->nj * K));
   if (!ok) {
     report_error("dBatchStepBackward: device allocation failed (%lld contacts, %lld scratch floats per step)", ncon,
                  nscr);
@@ -223,7 +242,8 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
     int ovf = 0;
     OD_HIP(hipMemcpy(&ovf, t.kovf.p, sizeof(int), hipMemcpyDeviceToHost));
     if (ovf) {
-      report_error("dBatchStepBackward: a recorded step overflowed its pair/contact/row capacities");
+      report_error(ovf & 1 ? "dBatchStepBackward: a recorded step overflowed its pair/contact/row capacities"
+                           : "dBatchStepBackward: a recorded step had joint rows the adjoint does not cover");
       fail = true;
     }
     for (int s = T - 1; s >= first && !fail; s--) {
@@ -234,7 +254,9 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       OD_HIP(launch_adjoint_step(b->dev, st4, t.kcon.as<adj::TapeContact<float>>() + (size_t)s * t.kncon,
                                  t.kcon_off.as<long long>(), t.knk.as<int>() + (size_t)s * nw, gin, gout,
                                  gmu.as<float>(), gbn.as<float>(), stats.as<int>(), scr.as<float>(),
-                                 d_scr_off.as<long long>(), t.act_tol, t.lin_tol, t.lin_maxit, 256, b->stream));
+                                 d_scr_off.as<long long>(), t.act_tol, t.lin_tol, t.lin_maxit, 256,
+                                 b->nj ? t.kjnt.as<adj::TapeJoint<float>>() + (size_t)s * b->nj : nullptr,
+                                 b->stream));
       cur = 1 - cur;
     }
   }
@@ -254,13 +276,16 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       if (!batch_launch_once(b)) return 0;
       OD_HIP(launch_tape_capture(b->dev, seg_con.as<adj::TapeContact<float>>() + (size_t)j * ncon,
                                  d_con_off.as<long long>(), seg_nk.as<int>() + (size_t)j * nw,
-                                 stats.as<int>() + 2 * (size_t)nw, b->stream));
+                                 stats.as<int>() + 2 * (size_t)nw,
+                                 b->nj ? seg_jnt.as<adj::TapeJoint<float>>() + (size_t)j * b->nj : nullptr,
+                                 b->stream));
     }
     {
       int ovf = 0;
       OD_HIP(hipMemcpy(&ovf, stats.as<int>() + 2 * (size_t)nw, sizeof(int), hipMemcpyDeviceToHost));
       if (ovf) {
-        report_error("dBatchStepBackward: a replayed step overflowed its pair/contact/row capacities");
+        report_error(ovf & 1 ? "dBatchStepBackward: a replayed step overflowed its pair/contact/row capacities"
+                             : "dBatchStepBackward: a replayed step had joint rows the adjoint does not cover");
         fail = true;
         break;
       }
@@ -275,7 +300,9 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       OD_HIP(launch_adjoint_step(b->dev, st4, seg_con.as<adj::TapeContact<float>>() + (size_t)j * ncon,
                                  d_con_off.as<long long>(), seg_nk.as<int>() + (size_t)j * nw, gin, gout,
                                  gmu.as<float>(), gbn.as<float>(), stats.as<int>(), scr.as<float>(),
-                                 d_scr_off.as<long long>(), t.act_tol, t.lin_tol, t.lin_maxit, 256, b->stream));
+                                 d_scr_off.as<long long>(), t.act_tol, t.lin_tol, t.lin_maxit, 256,
+                                 b->nj ? seg_jnt.as<adj::TapeJoint<float>>() + (size_t)j * b->nj : nullptr,
+                                 b->stream));
       cur = 1 - cur;
     }
   }

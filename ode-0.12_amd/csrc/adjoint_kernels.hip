@@ -9,6 +9,7 @@
 namespace odemi {
 
 using adj::TapeContact;
+using adj::TapeJoint;
 
 struct AdjShared {
   float wsum[16];
@@ -44,9 +45,12 @@ struct AdjSettings {
 };
 
 // After a replayed forward step: the step's contacts (in creation order)
-// with the impulses of their rows, from the world's scratch.
+// with the impulses of their rows, from the world's scratch; and (jt not
+// null) every permanent joint's frame data, Info2 erp / cfm and row impulses
+// (a joint's constraint follows the step's nk contacts).  ovf: 1 when a
+// capacity overflowed, 2 when a joint had rows the adjoint does not cover.
 __global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const long long *con_off, int *nk_out,
-                                    int *ovf) {
+                                    int *ovf, TapeJoint<float> *jt) {
   const int w = blockIdx.x;
   if (w >= B.nworlds) return;
   const WorldInfo W = B.wi[w];
@@ -74,6 +78,32 @@ __global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const l
     }
     dst[i] = t;
   }
+  if (jt && !(flags & OVF_ROWS)) {
+    const WorldParams &P = B.wp[w];
+    const float *erp_of = B.order_mode == ORDER_REFERENCE ? (const float *)((const int *)(S + L.color) + W.nb) : nullptr;
+    for (int j = threadIdx.x; j < W.nj; j += blockDim.x) {
+      const JointDev &J = B.joints[W.j0 + j];
+      const Cons &q = cons[nk + j];
+      TapeJoint<float> t;
+      t.type = J.type; t.b1 = J.b1; t.b2 = J.b2;
+      t.m = q.m;
+      for (int k = 0; k < 3; k++) {
+        t.anc1[k] = J.anchor1[k]; t.anc2[k] = J.anchor2[k];
+        t.ax1[k] = J.axis1[k]; t.ax2[k] = J.axis2[k];
+      }
+      // Info2 erp: a ball joint's own; otherwise the world's, or in the
+      // reference order the running value the step used
+      t.erp = J.type == JT_BALL ? J.erp : (erp_of ? erp_of[nk + j] : P.erp);
+      t.cfm = J.type == JT_BALL ? J.cfm : P.cfm;
+      for (int r = 0; r < 6; r++) {
+        const int ri = q.row0 + r;
+        t.lam[r] = (r < q.m && ri < W.rcap) ? rows[ri].lambda : 0.f;
+      }
+      const bool covered = (J.type == JT_BALL && q.m == 3) || (J.type == JT_HINGE && q.m == 5);
+      if (!covered) atomicOr(ovf, 2);
+      jt[W.j0 + j] = t;
+    }
+  }
   if (threadIdx.x == 0) {
     nk_out[w] = nk;
     if (flags & (OVF_PAIRS | OVF_CONTACTS | OVF_ROWS)) atomicOr(ovf, 1);
@@ -88,7 +118,8 @@ __global__ __launch_bounds__(256) void adjoint_step_kernel(
     DevBatch B, const float4 *st_x, const float4 *st_q, const float4 *st_v, const float4 *st_w,
     const TapeContact<float> *con, const long long *con_off, const int *nk, const float4 *gin_x,
     const float4 *gin_q, const float4 *gin_v, const float4 *gin_w, float4 *gout_x, float4 *gout_q, float4 *gout_v,
-    float4 *gout_w, float *gmu, float *gbounce, int *stats, float *scr, const long long *scr_off, AdjSettings S) {
+    float4 *gout_w, float *gmu, float *gbounce, int *stats, float *scr, const long long *scr_off, AdjSettings S,
+    const TapeJoint<float> *jt) {
   __shared__ AdjShared sh;
   const int w = blockIdx.x;
   if (w >= B.nworlds) return;
@@ -112,6 +143,10 @@ __global__ __launch_bounds__(256) void adjoint_step_kernel(
   D.gpos = (const float *)(B.gpos + W.g0);
   D.gR = (const float *)(B.gR + 3 * (size_t)W.g0);
   D.con = con + con_off[w];
+  if (jt) {
+    D.nj = W.nj;
+    D.jnt = jt + W.j0;
+  }
   adj::StepParams<float> Q;
   Q.h = P.dt; Q.erp = P.erp; Q.cfm = P.cfm; Q.min_depth = P.min_depth; Q.max_vel = P.max_vel;
   for (int k = 0; k < 3; k++) Q.gravity[k] = P.gravity[k];
@@ -172,9 +207,9 @@ hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int 
 }
 
 hipError_t launch_tape_capture(const DevBatch &B, TapeContact<float> *con, const long long *con_off, int *nk,
-                               int *ovf, hipStream_t st) {
+                               int *ovf, TapeJoint<float> *jt, hipStream_t st) {
   if (B.nworlds <= 0) return hipSuccess;
-  tape_capture_kernel<<<B.nworlds, 128, 0, st>>>(B, con, con_off, nk, ovf);
+  tape_capture_kernel<<<B.nworlds, 128, 0, st>>>(B, con, con_off, nk, ovf, jt);
   return hipGetLastError();
 }
 
@@ -182,13 +217,13 @@ hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], con
                                const long long *con_off, const int *nk, const float4 *const gin[4],
                                float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
                                const long long *scr_off, float act_tol, float lin_tol, int lin_maxit, int wg,
-                               hipStream_t st_) {
+                               const TapeJoint<float> *jt, hipStream_t st_) {
   if (B.nworlds <= 0) return hipSuccess;
   if (wg < 64 || wg > 256 || (wg & 63)) return hipErrorInvalidValue;
   AdjSettings S{act_tol, lin_tol, lin_maxit};
   adjoint_step_kernel<<<B.nworlds, wg, 0, st_>>>(B, st[0], st[1], st[2], st[3], con, con_off, nk, gin[0], gin[1],
                                                   gin[2], gin[3], gout[0], gout[1], gout[2], gout[3], gmu, gbounce,
-                                                  stats, scr, scr_off, S);
+                                                  stats, scr, scr_off, S, jt);
   return hipGetLastError();
 }
 

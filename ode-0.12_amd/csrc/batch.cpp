@@ -290,6 +290,7 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
   b->nb = (int)A.pos.size();
   b->ng = (int)A.gtype.size();
   b->h_gtype = A.gtype;
+  b->h_joints = A.joints;
   b->nj = (int)A.joints.size();
   if (!A.wp.empty()) b->cell_scale = A.wp[0].cell_scale;
   derive_lists(A);

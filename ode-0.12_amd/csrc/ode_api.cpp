@@ -32,7 +32,7 @@ hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], con
                                const long long *con_off, const int *nk, const float4 *const gin[4],
                                float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
                                const long long *scr_off, float act_tol, float lin_tol, int lin_maxit, int wg,
-                               hipStream_t st_);
+                               const adj::TapeJoint<float> *jt, hipStream_t st_);
 hipError_t collide_one_noalloc(void *addr, const GeomView &a, const GeomView &b, int maxc, CPoint *host_pts,
                                int *host_n, hipStream_t st);
 }  // namespace odemi
@@ -1768,7 +1768,7 @@ int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in) {
   hipError_t e = launch_adjoint_step(m->dev, st, w->tape_con.as<adj::TapeContact<float>>(), off.as<long long>(),
                                      w->tape_nk.as<int>(), gin, gout, gsc.as<float>(), gsc.as<float>() + 1,
                                      stats.as<int>(), scr.as<float>(), off.as<long long>(), 1e-6f, 1e-6f, 500, 256,
-                                     m->stream);
+                                     nullptr, m->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   OD_HIP(hipMemcpy(m->d.wp.p, &P0, sizeof(P0), hipMemcpyHostToDevice));
   if (e != hipSuccess) {

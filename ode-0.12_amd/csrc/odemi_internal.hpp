@@ -32,7 +32,7 @@ struct Tape {
   // replay (memory per step: 64 B per body plus the contact capacity)
   bool keep = false;                             // requested for the next dBatchTapeBegin
   bool kept = false;                             // mode the current tape was recorded in
-  DevBuf kst, kcon, knk, kovf, kcon_off;
+  DevBuf kst, kcon, knk, kovf, kcon_off, kjnt;   // kjnt: every permanent joint's record per kept step
   long long kncon = 0;
 };
 
@@ -51,6 +51,7 @@ struct BatchImpl {
   std::vector<WorldInfo> wi;
   std::vector<WorldParams> wp;
   std::vector<int> h_gtype;   // geom types (host copy)
+  std::vector<JointDev> h_joints;   // permanent joints as created (host copy)
   BatchBufs d;
   DevBatch dev{};
   hipStream_t stream = nullptr;

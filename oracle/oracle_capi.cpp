@@ -379,7 +379,7 @@ extern "C" void orc_plane_space(const double *n, double *p, double *q) {
 // last is the final state).  Returns the total contact count, or -1 when
 // max_total is exceeded.
 extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int *nk, double *contacts,
-                             long max_total) {
+                             long max_total, double *jlam) {
   World<R> w;
   build_world(s, w);
   Surface<R> surf;
@@ -413,6 +413,12 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
     }
     w.quickstep((R)s->dt);
     nk[step] = (int)cj.size();
+    // permanent joints: their rows' impulses, then the row count
+    for (int j = 0; jlam && j < w.n_perm; j++) {
+      double *o = jlam + 7 * ((size_t)step * w.n_perm + j);
+      for (int r = 0; r < 6; r++) o[r] = r < w.joints[j].nrows ? (double)w.joints[j].lam[r] : 0.0;
+      o[6] = w.joints[j].nrows;
+    }
     if (total + (long)cj.size() > max_total) return -1;
     for (size_t i = 0; i < cj.size(); i++) {
       const Joint<R> &j = w.joints[cj[i]];
@@ -433,6 +439,32 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
   }
   record(w, states + (size_t)nsteps * s->nb * 13);
   return total;
+}
+
+// Frame data of the scene's permanent joints as the restatement holds them
+// after set-up, 17 doubles each: type, first and second body, anchor1[3] and
+// anchor2[3] (body frames; anchor2 a world point without a second body),
+// axis1[3] and axis2[3] (likewise), and the erp and cfm of the joint's rows
+// in the coloured order (a ball joint's own, the world's for a hinge).
+extern "C" int orc_joint_frames(const SceneDesc *s, double *out) {
+  World<R> w;
+  build_world(s, w);
+  for (int j = 0; j < w.n_perm; j++) {
+    const Joint<R> &jt = w.joints[j];
+    double *o = out + 17 * (size_t)j;
+    o[0] = jt.type;
+    o[1] = jt.node0;
+    o[2] = jt.node1;
+    for (int k = 0; k < 3; k++) {
+      o[3 + k] = (double)jt.anchor1[k];
+      o[6 + k] = (double)jt.anchor2[k];
+      o[9 + k] = (double)jt.axis1[k];
+      o[12 + k] = (double)jt.axis2[k];
+    }
+    o[15] = (double)(jt.type == SC_BALL ? jt.erp : w.prm.erp);
+    o[16] = (double)(jt.type == SC_BALL ? jt.cfm : w.prm.cfm);
+  }
+  return w.n_perm;
 }
 
 // The exact box-constrained LCP of the dWorldStep restatement on one dense

@@ -179,14 +179,28 @@ def tape_run(lib, sc, nsteps: int, max_total: int = 200000) -> dict:
     (and the final one) and every step's contacts with their row impulses."""
     fn = lib.orc_tape_run
     fn.restype = C.c_long
-    fn.argtypes = [C.POINTER(SceneDesc), C.c_int, _dp, _ip, _dp, C.c_long]
+    fn.argtypes = [C.POINTER(SceneDesc), C.c_int, _dp, _ip, _dp, C.c_long, _dp]
     desc, keep = make_desc(sc)
     states = np.zeros((nsteps + 1, sc.nb, 13))
     nk = np.zeros(nsteps, np.int32)
     con = np.zeros((max_total, 16))
+    jlam = np.zeros((nsteps, max(sc.nj, 1), 7))
     total = fn(C.byref(desc), nsteps, states.ctypes.data_as(_dp), nk.ctypes.data_as(_ip),
-               con.ctypes.data_as(_dp), max_total)
+               con.ctypes.data_as(_dp), max_total, jlam.ctypes.data_as(_dp))
     del keep
     if total < 0:
         raise RuntimeError("tape_run: contact buffer too small")
-    return dict(states=states, nk=nk, contacts=con[:total].copy())
+    return dict(states=states, nk=nk, contacts=con[:total].copy(), jlam=jlam[:, :sc.nj].copy())
+
+
+def joint_frames(lib, sc) -> np.ndarray:
+    """(nj, 17) frame data of the scene's permanent joints as the restatement
+    holds them after set-up (orc_joint_frames)."""
+    fn = lib.orc_joint_frames
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(SceneDesc), _dp]
+    desc, keep = make_desc(sc)
+    out = np.zeros((max(sc.nj, 1), 17))
+    n = fn(C.byref(desc), out.ctypes.data_as(_dp))
+    del keep
+    return out[:n].copy()

@@ -27,12 +27,16 @@ extern "C" {
 // prm[ng*4], pos[ng*3], R9[ng*9] (static geoms); params[16]: h, erp, cfm,
 // min_depth, max_vel, gx, gy, gz, mu, bounce, bounce_vel, surf_bounce, maxc,
 // act_tol, lin_tol, lin_maxit.  states: (nsteps+1)*nb*13; nk[nsteps];
-// contacts: 16 doubles each (see orc_tape_run).  grad_out: nb*13 (dL/d final
-// state); grad_in: nb*13 out; gmu, gbounce out; stats[2] out.
+// contacts: 16 doubles each (see orc_tape_run).  Permanent joints: nj,
+// jdef[nj*17] (type, b1, b2, anchor1[3], anchor2[3], axis1[3], axis2[3] in
+// the body frames, erp, cfm) and jlam[nsteps*nj*7] (the rows' impulses after
+// each step, then the row count).  grad_out: nb*13 (dL/d final state);
+// grad_in: nb*13 out; gmu, gbounce out; stats[2] out.
 int adj_host_backward(int nb, int ng, const double *mass, const double *I9, const int *gtype, const int *gbody,
                       const double *gprm, const double *gpos, const double *gR9, const double *params, int nsteps,
-                      const double *states, const int *nk, const double *contacts, const double *grad_out,
-                      double *grad_in, double *gmu, double *gbounce, int *stats) {
+                      const double *states, const int *nk, const double *contacts, int nj, const double *jdef,
+                      const double *jlam, const double *grad_out, double *grad_in, double *gmu, double *gbounce,
+                      int *stats) {
   StepParams<double> P;
   P.h = params[0]; P.erp = params[1]; P.cfm = params[2]; P.min_depth = params[3]; P.max_vel = params[4];
   P.gravity[0] = params[5]; P.gravity[1] = params[6]; P.gravity[2] = params[7];
@@ -97,8 +101,22 @@ int adj_host_backward(int nb, int ng, const double *mass, const double *I9, cons
       t.g1 = (int)c[10]; t.g2 = (int)c[11]; t.b1 = (int)c[12]; t.b2 = (int)c[13];
       t.flags = (int)c[14]; t.m = (int)c[15];
     }
+    std::vector<TapeJoint<double>> jt(nj);
+    for (int j = 0; j < nj; j++) {
+      const double *d = jdef + 17 * (size_t)j;
+      const double *l = jlam + 7 * ((size_t)s * nj + j);
+      TapeJoint<double> &t = jt[j];
+      t.type = (int)d[0]; t.b1 = (int)d[1]; t.b2 = (int)d[2];
+      for (int k = 0; k < 3; k++) {
+        t.anc1[k] = d[3 + k]; t.anc2[k] = d[6 + k]; t.ax1[k] = d[9 + k]; t.ax2[k] = d[12 + k];
+      }
+      t.erp = d[15]; t.cfm = d[16];
+      for (int k = 0; k < 6; k++) t.lam[k] = l[k];
+      t.m = (int)l[6];
+    }
     StepData<double> D;
     D.nb = nb; D.nk = nk[s]; D.ng = ng;
+    D.nj = nj; D.jnt = jt.data();
     D.x = x.data(); D.q = q.data(); D.v = v.data(); D.w = w.data();
     D.mass = m.data(); D.invMass = im.data(); D.I = I.data(); D.invI = iI.data();
     D.gtype = gtype; D.gbody = gbody; D.gprm = gprm; D.gpos = gp.data(); D.gR = gR.data();
@@ -108,7 +126,7 @@ int adj_host_backward(int nb, int ng, const double *mass, const double *I9, cons
     G.gx_in = ax.data(); G.gq_in = aq.data(); G.gv_in = av.data(); G.gw_in = aw.data();
     G.gx = ox.data(); G.gq = oq.data(); G.gv = ov.data(); G.gw = ow.data();
     G.gmu = gmu; G.gbounce = gbounce; G.stats = stats;
-    std::vector<double> scr(adj_scratch_elems<double>(nb, nk[s]), 0.0);
+    std::vector<double> scr(adj_scratch_elems<double>(nb, nk[s], nj), 0.0);
     ParSerial par;
     adjoint_step<double>(par, D, P, G, scr.data());
     ax.swap(ox); aq.swap(oq); av.swap(ov); aw.swap(ow);

@@ -2,6 +2,9 @@
 
 * ``contact_scene``: small seeded contact-only worlds (spheres/boxes on a
   plane, optional restitution) of the differentiable-pile kind.
+* ``jointed_scene``: a ball/hinge chain on a plane with a hinge and a ball
+  joint to the static world.
+* ``converged_oracle``: the FP64 restatement solving to a tight residual.
 * ``com_loss``: sum over worlds of the squared centre of mass of the final
   positions, and its gradient with respect to the final state.
 * ``host_adjoint``: the host double build of the adjoint
@@ -56,6 +59,26 @@ def contact_scene(seed: int, n: int, kinds=(0, 1), bounce: float = 0.0, spacing:
     return b.scene(dt=dt, mu=mu, seed=seed, iterations=iterations, surf_mode=mode, bounce=bounce, bounce_vel=0.1)
 
 
+def jointed_scene(seed: int, n_links: int = 4, iterations: int = 20000, dt: float = 0.01,
+                  mu: float = 0.6) -> scenes.Scene:
+    """A chain of n_links boxes joined alternately by ball joints and hinges
+    (scenes._chain) falling onto a plane, a box hanging from a hinge to the
+    static world and a sphere on a ball joint to the static world, all with
+    seeded linear and angular velocities."""
+    rng = np.random.default_rng(seed)
+    b = scenes._Builder()
+    b.geom(scenes.PLANE, (0.0, 0.0, 1.0, 0.0))
+    scenes._chain(b, rng, n_links, (-0.6, 0.0, 0.25))
+    p = b.box_body((0.9, 0.0, 0.75), (0.1, 0.1, 0.3), 0.8)
+    b.joint(scenes.HINGE, p, -1, (0.9, 0.0, 0.9), axis1=(0.0, 1.0, 0.0))
+    s = b.sphere_body((0.9, 0.6, 0.5), 0.12, 0.6)
+    b.joint(scenes.BALL, s, -1, (0.9, 0.6, 0.75))
+    sc = b.scene(dt=dt, mu=mu, seed=seed, iterations=iterations, surf_mode=scenes.APPROX1)
+    sc.body_lvel = rng.uniform(-0.8, 0.8, (sc.nb, 3))
+    sc.body_avel = rng.uniform(-1.0, 1.0, (sc.nb, 3))
+    return sc
+
+
 def com_loss(scs, finals):
     """L = sum_w |COM_w|^2 and dL/d(final state) per world (nb x 13)."""
     L = 0.0
@@ -68,6 +91,28 @@ def com_loss(scs, finals):
         g[:, :3] = 2 * com[None, :] * (m / m.sum())[:, None]
         grads.append(g)
     return L, grads
+
+
+class converged_oracle:
+    """Within the block the FP64 restatement solves every step's LCP in the
+    coloured order to the residual `tol` (conjugate-gradient steps, cap
+    400000 sweeps), so that finite differences and the recorded impulses are
+    those of the converged LCP the adjoint differentiates."""
+
+    def __init__(self, tol: float = 1e-12):
+        self.tol = tol
+
+    def __enter__(self):
+        lib = po.oracle_lib("f64")
+        lib.orc_set_colored(1)
+        lib.orc_set_colored_solver(C.c_double(self.tol), 400000, 10, 1)
+        return self
+
+    def __exit__(self, *exc):
+        lib = po.oracle_lib("f64")
+        lib.orc_set_colored(0)
+        lib.orc_set_colored_solver(C.c_double(0.0), 0, 10, 0)
+        return False
 
 
 def forward_f64(sc, nsteps):
@@ -112,15 +157,18 @@ def host_adjoint(sc, nsteps, lib_path=None):
                        1.0 if sc.surf_mode & scenes.BOUNCE else 0.0, sc.max_contacts, 1e-9, 1e-13, 4000],
                       np.float64)
     arr = lambda a, t=np.float64: np.ascontiguousarray(a, t)
+    jdef = po.joint_frames(po.oracle_lib("f64"), sc) if sc.nj else np.zeros((0, 17))
     keep = [arr(sc.body_mass), arr(sc.body_I), arr(sc.geom_type, np.int32), arr(sc.geom_body, np.int32),
             arr(sc.geom_param), arr(sc.geom_pos), arr(_geom_rot(sc)), params, arr(t["states"]),
-            arr(t["nk"], np.int32), arr(t["contacts"]), arr(gout)]
+            arr(t["nk"], np.int32), arr(t["contacts"]), arr(jdef.reshape(-1, 17)),
+            arr(t["jlam"].reshape(nsteps, -1, 7)), arr(gout)]
     ptr = lambda a: a.ctypes.data_as(_ip if a.dtype == np.int32 else _dp)
     gin = np.zeros((sc.nb, 13))
     gmu, gb = C.c_double(), C.c_double()
     stats = np.zeros(2, np.int32)
     lib.adj_host_backward(sc.nb, sc.ng, *[ptr(a) for a in keep[:8]], nsteps, ptr(keep[8]), ptr(keep[9]),
-                          ptr(keep[10]), ptr(keep[11]), ptr(gin), C.byref(gmu), C.byref(gb), ptr(stats))
+                          ptr(keep[10]), sc.nj, ptr(keep[11]), ptr(keep[12]), ptr(keep[13]), ptr(gin),
+                          C.byref(gmu), C.byref(gb), ptr(stats))
     return L, gin, gmu.value, gb.value, stats
 
 
