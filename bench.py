@@ -223,17 +223,17 @@ def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
     head = gpu(odemi.ORDER_COLORED, True)
     log(f"divergence ({label}): GPU runs done in {time.perf_counter() - t0:.1f}s")
 
-    def cpu(prec):
-        out = []
-        with mp.get_context("spawn").Pool(procs) as pool:
-            for k, tr in enumerate(pool.imap(_oracle_traj, [(w, nsteps, every, prec) for w in worlds])):
-                out.append(tr)
-                if (k + 1) % 16 == 0:
-                    log(f"divergence ({label}): {prec} restatement, {k + 1} of {len(worlds)} worlds")
-        return np.concatenate(out, axis=1)
-
-    f64 = cpu("f64")
-    f32 = cpu("f32")
+    # both restatements in one pool of host processes, in order
+    jobs = [(w, nsteps, every, prec) for prec in ("f64", "f32") for w in worlds]
+    out = []
+    with mp.get_context("spawn").Pool(procs) as pool:
+        log(f"divergence ({label}): {len(jobs)} restatement runs on {procs} host processes")
+        for k, tr in enumerate(pool.imap(_oracle_traj, jobs)):
+            out.append(tr)
+            if (k + 1) % 8 == 0:
+                log(f"divergence ({label}): {k + 1} of {len(jobs)} restatement runs")
+    f64 = np.concatenate(out[:len(worlds)], axis=1)
+    f32 = np.concatenate(out[len(worlds):], axis=1)
     d = lambda a, b: float(np.abs(a - b).max())
     return {"sample": f"{label}: {len(worlds)} worlds ({sum(w.nb for w in worlds)} bodies), {nsteps} steps, "
                       f"positions every {every} steps",

@@ -519,10 +519,13 @@ ODE_API int dBatchNumBodies(dBatchID b);
 
 /* Starts recording the steps run by dBatchStep (at most `max_steps`): the
  * state is checkpointed every checkpoint_interval steps (default 10); the
- * backward pass replays each segment from its checkpoint.  Worlds may hold
- * spheres, boxes and planes, ball joints and hinges without limits or a
- * motor (no hinge2, no per-step joint controls), with dContactApprox1
- * friction (or mu = 0) and optional dContactBounce.  The
+ * backward pass replays each segment from its checkpoint (with the joint
+ * controls each step ran with).  Worlds may hold spheres, boxes and planes,
+ * ball joints, hinges without limits, hinge2 joints (suspension, steering
+ * limits, motors, dBatchSetControls torques), with dContactApprox1 friction
+ * (or mu = 0) and optional dContactBounce; a recorded step with a row the
+ * adjoint does not cover (a bouncy or motor-loaded one-sided limit) makes
+ * dBatchStepBackward fail with an error.  The
  * derivative is that of the converged LCP: run the coloured order with a
  * solver tolerance (dBatchSetSolverTolerance) for accurate gradients. */
 ODE_API int dBatchTapeBegin(dBatchID b, int max_steps);
@@ -552,15 +555,17 @@ ODE_API int dBatchStepBackward(dBatchID b, int nsteps, const float *grad_out, fl
                                float *grad_mu, float *grad_bounce);
 /* Single-world form of the adjoint for the ODE API path (no counterpart in
  * the reference; the batch form is dBatchStepBackward).  With tape recording
- * on, each dWorldQuickStep of `w` keeps its pre-step state and its contacts
- * with their impulses; dWorldQuickStepBackward then returns the derivative
+ * on, each dWorldQuickStep of `w` keeps its pre-step state, its contacts
+ * and its joints with their impulses; dWorldQuickStepBackward then returns
+ * the derivative
  * of the last such step: grad_out = dL/d(state after), grad_in receives
  * dL/d(state before), 13 dReals per body in creation order (pos 3, q 4,
  * lvel 3, avel 3).  The LCP is differentiated implicitly at the step's
  * impulses on their active set, which is the derivative of the step when
  * its solve converged (raise dWorldSetQuickStepNumIterations until it does).
- * Differentiable steps: contact joints only (no ball/hinge/hinge2 joints),
- * one surface policy shared by every contact (dContactApprox1 friction or
+ * Differentiable steps: contact joints, ball joints, hinges without limits
+ * and hinge2 joints (not a bouncy or motor-loaded one-sided limit), one
+ * surface policy shared by every contact (dContactApprox1 friction or
  * mu = 0, optional dContactBounce), no forces besides gravity, contact geoms
  * in the world's space.  A step outside these conditions is still taken;
  * dWorldQuickStepBackward then reports why and returns 0, as it does when

@@ -85,9 +85,18 @@ OD_DUAL Dual<T, N> sqrt_(const Dual<T, N> &a) {
   return r;
 }
 OD_DUAL Dual<T, N> fabs_(const Dual<T, N> &a) { return a.v < T(0) ? -a : a; }
+OD_DUAL Dual<T, N> atan2_(const Dual<T, N> &y, const Dual<T, N> &x) {
+  Dual<T, N> r;
+  r.v = std::atan2(y.v, x.v);
+  const T den = x.v * x.v + y.v * y.v;
+  const T k = den > T(0) ? T(1) / den : T(0);
+  for (int i = 0; i < N; i++) r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) * k;
+  return r;
+}
 #undef OD_DUAL
 template <class T> OD_HD inline T sqrt_(T a) { return std::sqrt(a); }
 template <class T> OD_HD inline T fabs_(T a) { return std::fabs(a); }
+template <class T> OD_HD inline T atan2_(T y, T x) { return std::atan2(y, x); }
 
 // ------------------------------------------------------------------ vectors
 

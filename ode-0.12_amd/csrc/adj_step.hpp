@@ -1,6 +1,7 @@
 // adj_step.hpp -- reverse-mode derivative of one quickstep of one world with
-// contact joints (spheres, boxes, planes), ball joints and hinges (without
-// limit or motor rows), written once over the scalar type
+// contact joints (spheres, boxes, planes), ball joints, hinges (motor rows
+// included, not limit rows) and hinge2 joints (suspension, steering limit,
+// motors, added torques), written once over the scalar type
 // and a parallel context so that the GPU runs it with one workgroup per world
 // (float) and the host runs the same code serially (double) for its checks.
 //
@@ -10,8 +11,9 @@
 // {n, t1, t2} (t1, t2 = dPlaneSpace(n)), c = erp/h * max(depth - min_depth, 0)
 // or the bounce term, b = c/h - J (v/h + M^-1 f), the LCP
 //   A lambda - b  in the normal cone of [lo, hi],  A = J M^-1 J^T + cfm/h,
-// friction bounds +-mu * lambda_normal, joint rows (joint.cpp:85-126,
-// hinge.cpp:76-149) unbounded, then v' = v + h M^-1 (J^T lambda + f),
+// friction bounds +-mu * lambda_normal, joint rows (joint.cpp:85-196 and
+// 556-733, hinge.cpp:76-149, hinge2.cpp:127-180) with their own bounds, then
+// v' = v + h M^-1 (J^T lambda + f),
 // x' = x + h v', q' = normalize(q + h/2 (0, w') q).
 //
 // The LCP is differentiated implicitly at its (converged) solution on the
@@ -38,18 +40,30 @@ struct TapeContact {
   int m;                         // rows (1 or 3)
 };
 
-// permanent-joint record of the tape: a ball joint or a hinge without a
-// limit or motor row, with its frame data as the step used it and the
-// converged impulses of its rows
+// permanent-joint record of the tape: a ball joint, a hinge (its motor row
+// included; not its limit row) or a hinge2 (suspension, steering limit and
+// motors), with its frame data as the step used it, each row's bounds and
+// cfm, and the converged impulses of its rows
 template <class T>
 struct TapeJoint {
   T lam[6];
+  T lo[6], hi[6];       // row bounds
+  T rcfm[6];            // row cfm / h
   T anc1[3], anc2[3];   // anchors in the body frames (anc2: world point without a second body)
-  T ax1[3], ax2[3];     // hinge axes in the body frames (ax2: world axis without a second body)
-  T erp, cfm;           // the rows' Info2 erp and cfm
-  int type;             // JT_BALL or JT_HINGE
+  T ax1[3], ax2[3];     // axes in the body frames (ax2: world axis without a second body)
+  T erp;                // the joint's Info2 erp
+  T susp_erp;           // hinge2: erp of the suspension row
+  T c0, s0;             // hinge2: sine and cosine terms of the rest angle
+  T v1[3], v2[3];       // hinge2: steering-angle frame in body 1
+  T lerp[2];            // limit row of limot k: the stop erp, c = -erp/h (angle - lstop)
+  T lstop[2];
+  T lc[2];              // motor row of limot k: c (the target velocity)
+  T torque[2];          // hinge2 torques added this step about axis 1 and axis 2
+  int type;             // JT_BALL, JT_HINGE or JT_HINGE2
   int b1, b2;           // node order after attach (b2 may be -1)
-  int m;                // rows (3 or 5)
+  int m;                // rows
+  int lrow[2];          // row of limot 1 / 2, -1 without one
+  int lkind[2];         // 1: limit row, 2: motor row
 };
 
 template <class T>
@@ -218,56 +232,108 @@ OD_HD inline void contact_rows(const StepData<T> &D, const TapeContact<T> &c, T 
   }
 }
 
-// Rows of a ball joint or a hinge (ball.cpp:56-64 with the ball rows of
-// joint.cpp:85-126; hinge.cpp:76-149 without its limit/motor row): J (12 per
-// row, rows `stride` apart) and c, over a scalar type S so that the same code
-// gives the rows (S = T) and their derivatives (S = Dual).  k = erp / h.
+// Rows of a permanent joint (ball.cpp:56-64 with the ball rows of
+// joint.cpp:85-126; hinge.cpp:76-149; hinge2.cpp:127-180 with the aligned
+// ball rows of joint.cpp:134-196; the limit/motor rows of joint.cpp:556-733):
+// J (12 per row, rows `stride` apart) and c, over a scalar type S so that the
+// same code gives the rows (S = T) and their derivatives (S = Dual).
 template <class T, class S>
 OD_HD inline void joint_rows(const TapeJoint<T> &j, const V3<S> &x1, const M3<S> &R1, const V3<S> &x2,
-                             const M3<S> &R2, T k, S *J, int stride, S *c) {
+                             const M3<S> &R2, T hinv, S *J, int stride, S *c) {
   const bool two = j.b2 >= 0;
   const S one = S(T(1)), zero = S(T(0));
-  for (int r = 0; r < j.m; r++)
+  for (int r = 0; r < j.m; r++) {
     for (int i = 0; i < 12; i++) J[stride * r + i] = zero;
+    c[r] = zero;
+  }
+  const T k = j.erp * hinv;
   const V3<S> a1 = R1.mul(v3(S(j.anc1[0]), S(j.anc1[1]), S(j.anc1[2])));
-  S *J0 = J, *J1 = J + stride, *J2 = J + 2 * stride;
-  // J_l1 = I, J_a1 = -[a1]x
-  J0[0] = one; J1[1] = one; J2[2] = one;
-  J0[4] = a1.z;  J0[5] = -a1.y;
-  J1[3] = -a1.z; J1[5] = a1.x;
-  J2[3] = a1.y;  J2[4] = -a1.x;
-  V3<S> d;
-  if (two) {
-    const V3<S> a2 = R2.mul(v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2])));
-    // J_l2 = -I, J_a2 = [a2]x
-    J0[6] = -one; J1[7] = -one; J2[8] = -one;
-    J0[10] = -a2.z; J0[11] = a2.y;
-    J1[9] = a2.z;   J1[11] = -a2.x;
-    J2[9] = -a2.y;  J2[10] = a2.x;
-    d = ((a2 + x2) - a1) - x1;
-  } else {
-    d = (v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2])) - a1) - x1;
-  }
-  c[0] = k * d.x;
-  c[1] = k * d.y;
-  c[2] = k * d.z;
-  if (j.type != JT_HINGE) return;
-  // the two rows keeping the hinge axes aligned
+  const V3<S> a2 = two ? R2.mul(v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2]))) : v3(zero, zero, zero);
+  // separation of the anchors: second anchor (or the world point) - first
+  const V3<S> d = two ? ((a2 + x2) - a1) - x1 : (v3(S(j.anc2[0]), S(j.anc2[1]), S(j.anc2[2])) - a1) - x1;
   const V3<S> ax1 = R1.mul(v3(S(j.ax1[0]), S(j.ax1[1]), S(j.ax1[2])));
-  V3<S> p, q;
-  plane_space(ax1, p, q);
-  S *J3 = J + 3 * stride, *J4 = J + 4 * stride;
-  J3[3] = p.x; J3[4] = p.y; J3[5] = p.z;
-  J4[3] = q.x; J4[4] = q.y; J4[5] = q.z;
-  if (two) {
-    J3[9] = -p.x; J3[10] = -p.y; J3[11] = -p.z;
-    J4[9] = -q.x; J4[10] = -q.y; J4[11] = -q.z;
-  }
   const V3<S> wa2 = v3(S(j.ax2[0]), S(j.ax2[1]), S(j.ax2[2]));
   const V3<S> ax2 = two ? R2.mul(wa2) : wa2;
-  const V3<S> b = cross(ax1, ax2);
-  c[3] = k * dot(b, p);
-  c[4] = k * dot(b, q);
+  // an angular row along u: J_a1 = u, J_a2 = -u
+  auto ang_row = [&](int r, const V3<S> &u) {
+    S *Jr = J + stride * r;
+    Jr[3] = u.x; Jr[4] = u.y; Jr[5] = u.z;
+    if (two) { Jr[9] = -u.x; Jr[10] = -u.y; Jr[11] = -u.z; }
+  };
+  int ax_rows = 0;   // first row of the limit / motor rows
+  if (j.type == JT_HINGE2) {
+    // three ball rows along ax1 and its plane space, the first with the
+    // suspension erp
+    V3<S> q1, q2;
+    plane_space(ax1, q1, q2);
+    const V3<S> dirs[3] = {ax1, q1, q2};
+    for (int r = 0; r < 3; r++) {
+      S *Jr = J + stride * r;
+      const V3<S> u = dirs[r], c1 = cross(a1, u);
+      Jr[0] = u.x; Jr[1] = u.y; Jr[2] = u.z;
+      Jr[3] = c1.x; Jr[4] = c1.y; Jr[5] = c1.z;
+      if (two) {
+        const V3<S> c2 = cross(a2, u);
+        Jr[6] = -u.x; Jr[7] = -u.y; Jr[8] = -u.z;
+        Jr[9] = -c2.x; Jr[10] = -c2.y; Jr[11] = -c2.z;
+      }
+      c[r] = (r == 0 ? j.susp_erp * hinv : k) * dot(u, d);
+    }
+    // the row holding the angle between the axes: along ax1 x ax2 normalised
+    const V3<S> axc = cross(ax1, ax2);
+    const S sn = sqrt_(dot(axc, axc)), cs = dot(ax1, ax2);
+    ang_row(3, (one / sn) * axc);
+    c[3] = k * (j.c0 * sn - j.s0 * cs);
+    ax_rows = 4;
+  } else {
+    S *J0 = J, *J1 = J + stride, *J2 = J + 2 * stride;
+    // J_l1 = I, J_a1 = -[a1]x
+    J0[0] = one; J1[1] = one; J2[2] = one;
+    J0[4] = a1.z;  J0[5] = -a1.y;
+    J1[3] = -a1.z; J1[5] = a1.x;
+    J2[3] = a1.y;  J2[4] = -a1.x;
+    if (two) {
+      // J_l2 = -I, J_a2 = [a2]x
+      J0[6] = -one; J1[7] = -one; J2[8] = -one;
+      J0[10] = -a2.z; J0[11] = a2.y;
+      J1[9] = a2.z;   J1[11] = -a2.x;
+      J2[9] = -a2.y;  J2[10] = a2.x;
+    }
+    c[0] = k * d.x;
+    c[1] = k * d.y;
+    c[2] = k * d.z;
+    if (j.type != JT_HINGE) return;
+    // the two rows keeping the hinge axes aligned
+    V3<S> p, q;
+    plane_space(ax1, p, q);
+    ang_row(3, p);
+    ang_row(4, q);
+    const V3<S> b = cross(ax1, ax2);
+    c[3] = k * dot(b, p);
+    c[4] = k * dot(b, q);
+    ax_rows = 5;
+  }
+  (void)ax_rows;
+  for (int l = 0; l < 2; l++) {
+    const int r = j.lrow[l];
+    if (r < 0) continue;
+    ang_row(r, l == 0 ? ax1 : ax2);
+    if (j.lkind[l] == 2) {
+      c[r] = S(j.lc[l]);
+    } else {
+      // hinge2 steering angle (hinge2.cpp:34-43): -atan2(v2 . a, v1 . a)
+      // with a = R1^T R2 axis2
+      const V3<S> a = R1.tmul(R2.mul(wa2));
+      const S ang = -atan2_(dot(v3(S(j.v2[0]), S(j.v2[1]), S(j.v2[2])), a),
+                            dot(v3(S(j.v1[0]), S(j.v1[1]), S(j.v1[2])), a));
+      c[r] = -(j.lerp[l] * hinv) * (ang - j.lstop[l]);
+    }
+  }
+}
+
+// row r of a joint is free (strictly inside its bounds)
+template <class T> OD_HD inline bool joint_row_free(const TapeJoint<T> &j, int r, T tol) {
+  return j.lam[r] > j.lo[r] + tol && j.lam[r] < j.hi[r] - tol;
 }
 
 // a body pose as dual numbers: the position seeded with the unit directions
@@ -308,16 +374,18 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
   T *bsc = scr;                                           // ADJ_BODY_FLOATS per body
   T *rsc = scr + (long long)ADJ_BODY_FLOATS * nb;         // ADJ_ROW_FLOATS per row slot
   T *wsc = rsc + (long long)ADJ_ROW_FLOATS * nslots;      // 16 world scalars
-  // bodies and cfm of a row slot; false for a slot beyond its constraint's rows
-  auto slot_rows = [&](int s, int &b1, int &b2, T &cfm) -> bool {
+  // bodies and cfm / h of a row slot; false for a slot beyond its
+  // constraint's rows
+  auto slot_rows = [&](int s, int &b1, int &b2, T &cfm_h) -> bool {
     if (s < ncs) {
       const TapeContact<T> &c = D.con[s / 3];
-      b1 = c.b1; b2 = c.b2; cfm = P.cfm;
+      b1 = c.b1; b2 = c.b2; cfm_h = P.cfm * hinv;
       return s % 3 < c.m;
     }
     const TapeJoint<T> &jt = D.jnt[(s - ncs) / 6];
-    b1 = jt.b1; b2 = jt.b2; cfm = jt.cfm;
-    return (s - ncs) % 6 < jt.m;
+    const int r = (s - ncs) % 6;
+    b1 = jt.b1; b2 = jt.b2; cfm_h = r < jt.m ? jt.rcfm[r] : T(0);
+    return r < jt.m;
   };
   auto joint_slots = [&](int j) -> T * { return rsc + ADJ_ROW_FLOATS * (long long)(ncs + 6 * j); };
 
@@ -333,14 +401,32 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     V3<T> w = ld3(D.w + 4 * b);
     V3<T> tau = -cross(w, Iw.mul(w));
     st3(bs + BTAU, tau);
-    V3<T> v = ld3(D.v + 4 * b);
-    V3<T> g = v3(P.gravity[0], P.gravity[1], P.gravity[2]);
-    st3(bs + BZ, hinv * v + g);
-    st3(bs + BZ + 3, hinv * w + iIw.mul(tau));
     for (int k = BJL; k < ADJ_BODY_FLOATS; k++) bs[k] = T(0);
   }
   if (tid == 0)
     for (int k = 0; k < 16; k++) wsc[k] = T(0);
+  par.sync();
+  // hinge2 torques of the step (hinge2.cpp:393-410): t = t1 R1 axis1 + t2 R2
+  // axis2 on body 1, -t on body 2
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    if (jt.type != JT_HINGE2 || jt.b2 < 0 || (jt.torque[0] == T(0) && jt.torque[1] == T(0))) continue;
+    const V3<T> a1 = ldm(bsc + ADJ_BODY_FLOATS * jt.b1 + BR).mul(ld3(jt.ax1));
+    const V3<T> a2 = ldm(bsc + ADJ_BODY_FLOATS * jt.b2 + BR).mul(ld3(jt.ax2));
+    const V3<T> t = jt.torque[0] * a1 + jt.torque[1] * a2;
+    for (int i = 0; i < 3; i++) {
+      par.add(bsc + ADJ_BODY_FLOATS * jt.b1 + BTAU + i, comp(t, i));
+      par.add(bsc + ADJ_BODY_FLOATS * jt.b2 + BTAU + i, -comp(t, i));
+    }
+  }
+  par.sync();
+  for (int b = tid; b < nb; b += nt) {
+    T *bs = bsc + ADJ_BODY_FLOATS * b;
+    const V3<T> v = ld3(D.v + 4 * b), w = ld3(D.w + 4 * b);
+    const V3<T> g = v3(P.gravity[0], P.gravity[1], P.gravity[2]);
+    st3(bs + BZ, hinv * v + g);
+    st3(bs + BZ + 3, hinv * w + ldm(bs + BIIW).mul(ld3(bs + BTAU)));
+  }
   par.sync();
 
   // ---- 1. contact rows, J^T lambda per body
@@ -367,7 +453,7 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     const V3<T> zero3 = v3(T(0), T(0), T(0));
     const M3<T> R2 = jt.b2 >= 0 ? ldm(bsc + ADJ_BODY_FLOATS * jt.b2 + BR) : M3<T>{zero3, zero3, zero3};
     joint_rows<T, T>(jt, ld3(D.x + 4 * jt.b1), ldm(bsc + ADJ_BODY_FLOATS * jt.b1 + BR),
-                     jt.b2 >= 0 ? ld3(D.x + 4 * jt.b2) : zero3, R2, jt.erp * hinv, J6, ADJ_ROW_FLOATS, c6);
+                     jt.b2 >= 0 ? ld3(D.x + 4 * jt.b2) : zero3, R2, hinv, J6, ADJ_ROW_FLOATS, c6);
     for (int r = 0; r < jt.m; r++) {
       const T *J = J6 + ADJ_ROW_FLOATS * r;
       const T l = jt.lam[r];
@@ -469,7 +555,8 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       rr_loc += g[r] * g[r];
     }
   }
-  // joint rows are bilateral: always free, g = lambda_bar
+  // joint rows: g = lambda_bar on the free ones (bilateral rows always,
+  // bounded limit and motor rows strictly inside their bounds)
   for (int j = tid; j < nj; j += nt) {
     const TapeJoint<T> &jt = D.jnt[j];
     T *J6 = joint_slots(j);
@@ -485,12 +572,13 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       R[RLB] = lb;
       for (int i = 0; i < 6; i++) R[RJB + i] = jt.lam[r] * u1[i];
       for (int i = 0; i < 6; i++) R[RJB + 6 + i] = u2 ? jt.lam[r] * u2[i] : T(0);
+      const T g = joint_row_free(jt, r, P.act_tol) ? lb : T(0);
       R[RY] = T(0);
-      R[RR] = lb;
-      R[RRH] = lb;
+      R[RR] = g;
+      R[RRH] = g;
       R[RP] = T(0);
       R[RV] = T(0);
-      rr_loc += lb * lb;
+      rr_loc += g * g;
     }
   }
   T gnorm2 = par.sum(rr_loc);
@@ -549,7 +637,9 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       if (jt.b2 >= 0) minv6(bsc + ADJ_BODY_FLOATS * jt.b2, D.invMass[jt.b2], bsc + ADJ_BODY_FLOATS * jt.b2 + BJY, m2);
       for (int r = 0; r < 6; r++) {
         T *R = J6 + ADJ_ROW_FLOATS * r;
-        R[dst] = r < jt.m ? jdot6(R, m1) + (jt.b2 >= 0 ? jdot6(R + 6, m2) : T(0)) + jt.cfm * hinv * R[src] : T(0);
+        R[dst] = (r < jt.m && joint_row_free(jt, r, P.act_tol))
+                     ? jdot6(R, m1) + (jt.b2 >= 0 ? jdot6(R + 6, m2) : T(0)) + jt.rcfm[r] * R[src]
+                     : T(0);
       }
     }
     par.sync();
@@ -561,7 +651,7 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     T scfm;
     if (!slot_rows(s, sb1, sb2, scfm)) return T(0);
     const T *R = rsc + ADJ_ROW_FLOATS * (long long)s;
-    T m1[6], d = scfm * hinv;
+    T m1[6], d = scfm;
     minv6(bsc + ADJ_BODY_FLOATS * sb1, D.invMass[sb1], R, m1);
     d += jdot6(R, m1);
     if (sb2 >= 0) {
@@ -886,7 +976,7 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
       pose_dual(x1, R1, side == 0 ? kind : -1, px1, pR1);
       pose_dual(x2, R2, side == 1 ? kind : -1, px2, pR2);
       S Jd[6 * 12], cd[6];
-      joint_rows<T, S>(jt, px1, pR1, px2, pR2, jt.erp * hinv, Jd, 12, cd);
+      joint_rows<T, S>(jt, px1, pR1, px2, pR2, hinv, Jd, 12, cd);
       T acc[3] = {T(0), T(0), T(0)};
       for (int r = 0; r < jt.m; r++) {
         const T *R = J6 + ADJ_ROW_FLOATS * r;
@@ -925,6 +1015,22 @@ OD_HD void adjoint_step(Par &par, const StepData<T> &D, const StepParams<T> &P, 
     th = th - cross(L, a) + cross(w, Iw.mul(a));
     for (int i = 0; i < 3; i++) par.add(G.gw + 4 * b + i, comp(wb, i));
     st3(bs + BTH, th);
+    st3(bs + BTAUB, taub);
+  }
+  par.sync();
+  // hinge2 torques: tau_1 = t, tau_2 = -t with t = t1 R1 axis1 + t2 R2 axis2;
+  // d(R a) = dtheta x (R a) gives theta_bar += t_k (R a_k) x (tau_bar_1 - tau_bar_2)
+  for (int j = tid; j < nj; j += nt) {
+    const TapeJoint<T> &jt = D.jnt[j];
+    if (jt.type != JT_HINGE2 || jt.b2 < 0 || (jt.torque[0] == T(0) && jt.torque[1] == T(0))) continue;
+    const T *b1s = bsc + ADJ_BODY_FLOATS * jt.b1, *b2s = bsc + ADJ_BODY_FLOATS * jt.b2;
+    const V3<T> tb = ld3(b1s + BTAUB) - ld3(b2s + BTAUB);
+    const V3<T> g1 = jt.torque[0] * cross(ldm(b1s + BR).mul(ld3(jt.ax1)), tb);
+    const V3<T> g2 = jt.torque[1] * cross(ldm(b2s + BR).mul(ld3(jt.ax2)), tb);
+    for (int i = 0; i < 3; i++) {
+      par.add(bsc + ADJ_BODY_FLOATS * jt.b1 + BTH + i, comp(g1, i));
+      par.add(bsc + ADJ_BODY_FLOATS * jt.b2 + BTH + i, comp(g2, i));
+    }
   }
   par.sync();
 

@@ -17,29 +17,21 @@ hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], con
 
 namespace {
 
-// A world the adjoint supports: contact joints, ball joints and hinges
-// without limits or a motor (their rows are bilateral), dContactApprox1
-// friction (or mu = 0), optional bounce, no other surface modes.
+// A world the adjoint supports: contact joints, ball joints, hinges without
+// limits, hinge2 joints, dContactApprox1 friction (or mu = 0), optional
+// bounce, no other surface modes.  Rows a recorded step has that the adjoint
+// does not cover (a hinge limit set by the controls, a motor against a
+// one-sided limit, a bouncy limit) are reported by dBatchStepBackward.
 bool world_supported(const BatchImpl *b, int w, char *why, size_t n) {
   const WorldInfo &W = b->wi[w];
   const WorldParams &P = b->wp[w];
   const float PI = 3.14159265358979323846f;
   for (int j = 0; j < W.nj; j++) {
     const JointDev &J = b->h_joints[W.j0 + j];
-    if (J.type != JT_BALL && J.type != JT_HINGE) {
-      snprintf(why, n, "world %d has a hinge2 joint (the adjoint covers ball joints and hinges)", w);
+    if (J.type == JT_HINGE && (J.lm1.lostop >= -PI || J.lm1.histop <= PI) && J.lm1.lostop <= J.lm1.histop) {
+      snprintf(why, n, "world %d has a hinge with a limit (the adjoint covers hinge motors, not hinge limits)", w);
       return false;
     }
-    if (J.type == JT_HINGE &&
-        (J.lm1.fmax > 0.f || ((J.lm1.lostop >= -PI || J.lm1.histop <= PI) && J.lm1.lostop <= J.lm1.histop))) {
-      snprintf(why, n, "world %d has a hinge with a limit or a motor (the adjoint covers its bilateral rows only)",
-               w);
-      return false;
-    }
-  }
-  if (W.nj > 0 && b->ctrl_set) {
-    snprintf(why, n, "world %d: per-step joint controls are set", w);
-    return false;
   }
   for (int g = 0; g < W.ng; g++)
     if (b->h_gtype[W.g0 + g] != GEOM_SPHERE && b->h_gtype[W.g0 + g] != GEOM_BOX &&
@@ -98,6 +90,11 @@ int tape_begin(BatchImpl *b, int max_steps) {
     OD_HIP(hipMemcpy(t.kcon_off.p, con_off.data(), sizeof(long long) * nw, hipMemcpyHostToDevice));
     OD_HIP(hipMemset(t.kovf.p, 0, sizeof(int)));
   }
+  if (b->nj > 0 && !t.kctrl.alloc(sizeof(float4) * (size_t)b->nj * max_steps)) {
+    report_error("dBatchTapeBegin: device allocation for the controls of %d steps failed", max_steps);
+    return 0;
+  }
+  t.ctrl_on.assign(max_steps, 0);
   t.ckpt.emplace_back(new Snapshot());
   if (!snapshot(b, *t.ckpt.back(), true)) return 0;
   // the mode is latched only once every buffer it needs exists; the backward
@@ -148,6 +145,13 @@ int tape_step(BatchImpl *b, bool sync) {
     report_error("dBatchStep: the tape holds at most %d steps", t.max_steps);
     return 0;
   }
+  if (b->nj > 0) {
+    // the controls this step runs with, for the replays
+    t.ctrl_on[t.nsteps] = b->ctrl_set ? 1 : 0;
+    if (b->ctrl_set)
+      OD_HIP(hipMemcpyAsync(t.kctrl.as<float4>() + (size_t)t.nsteps * b->nj, b->d.ctrl.p, sizeof(float4) * b->nj,
+                            hipMemcpyDeviceToDevice, b->stream));
+  }
   if (t.kept) {
     // this step's pre-state, then (after the step) its contacts
     const int nb = b->nb, nw = b->nworlds;
@@ -197,14 +201,16 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
     nscr += adj::adj_scratch_elems<float>(b->wi[w].nb, b->wi[w].kcap, b->wi[w].nj);
     nscr = (nscr + 63) & ~63LL;
   }
-  DevBuf d_con_off, d_scr_off, seg_st, seg_con, seg_nk, seg_jnt, gbuf, scr, gmu, gbn, stats;
+  DevBuf d_con_off, d_scr_off, seg_st, seg_con, seg_nk, seg_jnt, gbuf, scr, gmu, gbn, stats, ctrl_save;
   const size_t stbytes = sizeof(float4) * (size_t)nb;
   bool ok = d_con_off.alloc(sizeof(long long) * nw) && d_scr_off.alloc(sizeof(long long) * nw) &&
             seg_st.alloc(4 * stbytes * K) && seg_con.alloc(sizeof(adj::TapeContact<float>) * (size_t)ncon * K) &&
             seg_nk.alloc(sizeof(int) * (size_t)nw * K) && gbuf.alloc(8 * stbytes) &&
             scr.alloc(sizeof(float) * (size_t)nscr) && gmu.alloc(sizeof(float) * nw) &&
             gbn.alloc(sizeof(float) * nw) && stats.alloc(sizeof(int) * (2 * (size_t)nw + 1)) &&
-            (b->nj == 0 || t.kept || seg_jnt.alloc(sizeof(adj::TapeJoint<float>) * (size_t)b->nj * K));
+            (b->nj == 0 || t.kept ||
+             (seg_jnt.alloc(sizeof(adj::TapeJoint<float>) * (size_t)b->nj * K) &&
+              ctrl_save.alloc(sizeof(float4) * (size_t)b->nj)));
   if (!ok) {
     report_error("dBatchStepBackward: device allocation failed (%lld contacts, %lld scratch floats per step)", ncon,
                  nscr);
@@ -260,6 +266,10 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       cur = 1 - cur;
     }
   }
+  // the replays re-apply each step's controls; the current ones are restored
+  const bool ctrl_set0 = b->ctrl_set;
+  if (!t.kept && b->nj > 0)
+    OD_HIP(hipMemcpyAsync(ctrl_save.p, b->d.ctrl.p, sizeof(float4) * b->nj, hipMemcpyDeviceToDevice, b->stream));
   int seg0 = ((T - 1) / K) * K;
   const int seg_first = (first / K) * K;
   for (; !t.kept && seg0 >= seg_first && !fail; seg0 -= K) {
@@ -273,6 +283,12 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       OD_HIP(hipMemcpyAsync(slot + nb, b->d.q.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
       OD_HIP(hipMemcpyAsync(slot + 2 * (size_t)nb, b->d.lvel.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
       OD_HIP(hipMemcpyAsync(slot + 3 * (size_t)nb, b->d.avel.p, stbytes, hipMemcpyDeviceToDevice, b->stream));
+      if (b->nj > 0) {
+        b->ctrl_set = t.ctrl_on[s] != 0;
+        if (b->ctrl_set)
+          OD_HIP(hipMemcpyAsync(b->d.ctrl.p, t.kctrl.as<float4>() + (size_t)s * b->nj, sizeof(float4) * b->nj,
+                                hipMemcpyDeviceToDevice, b->stream));
+      }
       if (!batch_launch_once(b)) return 0;
       OD_HIP(launch_tape_capture(b->dev, seg_con.as<adj::TapeContact<float>>() + (size_t)j * ncon,
                                  d_con_off.as<long long>(), seg_nk.as<int>() + (size_t)j * nw,
@@ -305,6 +321,10 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
                                  b->stream));
       cur = 1 - cur;
     }
+  }
+  if (!t.kept && b->nj > 0) {
+    b->ctrl_set = ctrl_set0;
+    OD_HIP(hipMemcpyAsync(b->d.ctrl.p, ctrl_save.p, sizeof(float4) * b->nj, hipMemcpyDeviceToDevice, b->stream));
   }
   if (!snapshot(b, endsnap, false)) return 0;
   if (fail) return 0;

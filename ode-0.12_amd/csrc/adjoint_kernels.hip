@@ -44,6 +44,78 @@ struct AdjSettings {
   int lin_maxit;
 };
 
+// Every permanent joint of world w after its step (a joint's constraint
+// follows the step's nk contacts): frame data, Info2 erp, each row's bounds
+// and cfm / h and impulse, the limit / motor rows and the hinge2 torques of
+// the step's controls.  ovf |= 2 when a joint has rows the adjoint does not
+// cover.
+__device__ void capture_joints(const DevBatch &B, int w, const WorldInfo &W, const char *S, const ScratchLayout &L,
+                               const Cons *cons, const Row *rows, int nk, TapeJoint<float> *jt, int *ovf) {
+  const float *erp_of = B.order_mode == ORDER_REFERENCE ? (const float *)((const int *)(S + L.color) + W.nb) : nullptr;
+  for (int j = threadIdx.x; j < W.nj; j += blockDim.x) {
+    const JointDev &J = B.joints[W.j0 + j];
+    const Cons &q = cons[nk + j];
+    TapeJoint<float> t;
+    t.type = J.type; t.b1 = J.b1; t.b2 = J.b2;
+    t.m = q.m;
+    bool covered = q.m <= 6 && q.row0 + q.m <= W.rcap;
+    for (int r = 0; r < 6; r++) {
+      const bool in = covered && r < q.m;
+      const Row *R = in ? rows + q.row0 + r : nullptr;
+      t.lam[r] = in ? R->lambda : 0.f;
+      t.lo[r] = in ? R->lo : 0.f;
+      t.hi[r] = in ? R->hi : 0.f;
+      t.rcfm[r] = in ? R->cfm : 0.f;   // divided by h in the step's rhs pass
+    }
+    for (int k = 0; k < 3; k++) {
+      t.anc1[k] = J.anchor1[k]; t.anc2[k] = J.anchor2[k];
+      t.ax1[k] = J.axis1[k]; t.ax2[k] = J.axis2[k];
+      t.v1[k] = J.v1[k]; t.v2[k] = J.v2[k];
+    }
+    // Info2 erp: a ball joint's own; otherwise the world's, or in the
+    // reference order the running value the step used
+    t.erp = J.type == JT_BALL ? J.erp : (erp_of ? erp_of[nk + j] : B.wp[w].erp);
+    t.susp_erp = J.susp_erp;
+    t.c0 = J.c0;
+    t.s0 = J.s0;
+    // limit / motor rows after the joint's own rows (joint.cpp:556-733)
+    int row = J.type == JT_HINGE2 ? 4 : 5;
+    for (int l = 0; l < 2; l++) {
+      t.lrow[l] = -1;
+      t.lkind[l] = 0;
+      t.lerp[l] = t.lstop[l] = t.lc[l] = 0.f;
+      const Limot &lm = l == 0 ? J.lm1 : J.lm2;
+      if (J.type == JT_BALL || (l == 1 && J.type != JT_HINGE2)) continue;
+      bool powered = lm.fmax > 0.f;
+      if (!(powered || lm.limit)) continue;
+      t.lrow[l] = row;
+      if (lm.limit && lm.lostop == lm.histop) powered = false;
+      if (powered && !lm.limit) {
+        t.lkind[l] = 2;
+        t.lc[l] = lm.vel;
+      } else {
+        // a motor pushing against a one-sided limit, a bouncy limit and the
+        // hinge's angle are not covered
+        if (powered || (lm.bounce > 0.f && lm.lostop != lm.histop) || J.type == JT_HINGE) covered = false;
+        t.lkind[l] = 1;
+        t.lerp[l] = lm.stop_erp;
+        t.lstop[l] = lm.limit == 1 ? lm.lostop : lm.histop;
+      }
+      row++;
+    }
+    if (row != q.m && J.type != JT_BALL) covered = false;
+    if (J.type == JT_BALL && q.m != 3) covered = false;
+    t.torque[0] = t.torque[1] = 0.f;
+    if (J.type == JT_HINGE2 && B.ctrl && J.b1 >= 0 && J.b2 >= 0) {
+      const float4 u = B.ctrl[W.j0 + j];
+      t.torque[0] = u.z;
+      t.torque[1] = u.w;
+    }
+    if (!covered) atomicOr(ovf, 2);
+    jt[W.j0 + j] = t;
+  }
+}
+
 // After a replayed forward step: the step's contacts (in creation order)
 // with the impulses of their rows, from the world's scratch; and (jt not
 // null) every permanent joint's frame data, Info2 erp / cfm and row impulses
@@ -78,32 +150,7 @@ __global__ void tape_capture_kernel(DevBatch B, TapeContact<float> *con, const l
     }
     dst[i] = t;
   }
-  if (jt && !(flags & OVF_ROWS)) {
-    const WorldParams &P = B.wp[w];
-    const float *erp_of = B.order_mode == ORDER_REFERENCE ? (const float *)((const int *)(S + L.color) + W.nb) : nullptr;
-    for (int j = threadIdx.x; j < W.nj; j += blockDim.x) {
-      const JointDev &J = B.joints[W.j0 + j];
-      const Cons &q = cons[nk + j];
-      TapeJoint<float> t;
-      t.type = J.type; t.b1 = J.b1; t.b2 = J.b2;
-      t.m = q.m;
-      for (int k = 0; k < 3; k++) {
-        t.anc1[k] = J.anchor1[k]; t.anc2[k] = J.anchor2[k];
-        t.ax1[k] = J.axis1[k]; t.ax2[k] = J.axis2[k];
-      }
-      // Info2 erp: a ball joint's own; otherwise the world's, or in the
-      // reference order the running value the step used
-      t.erp = J.type == JT_BALL ? J.erp : (erp_of ? erp_of[nk + j] : P.erp);
-      t.cfm = J.type == JT_BALL ? J.cfm : P.cfm;
-      for (int r = 0; r < 6; r++) {
-        const int ri = q.row0 + r;
-        t.lam[r] = (r < q.m && ri < W.rcap) ? rows[ri].lambda : 0.f;
-      }
-      const bool covered = (J.type == JT_BALL && q.m == 3) || (J.type == JT_HINGE && q.m == 5);
-      if (!covered) atomicOr(ovf, 2);
-      jt[W.j0 + j] = t;
-    }
-  }
+  if (jt && !(flags & OVF_ROWS)) capture_joints(B, w, W, S, L, cons, rows, nk, jt, ovf);
   if (threadIdx.x == 0) {
     nk_out[w] = nk;
     if (flags & (OVF_PAIRS | OVF_CONTACTS | OVF_ROWS)) atomicOr(ovf, 1);
@@ -174,7 +221,7 @@ __global__ __launch_bounds__(256) void adjoint_step_kernel(
 // host's contact joints): the step's contacts in creation order with the
 // impulses of their rows, for dWorldQuickStepBackward.
 __global__ void api_tape_capture_kernel(DevBatch B, const ContactIn *cin, int nk, TapeContact<float> *con,
-                                        int *nk_out) {
+                                        int *nk_out, TapeJoint<float> *jt) {
   const WorldInfo W = B.wi[0];
   const ScratchLayout L = scratch_layout(W.nb, W.ng, W.nj, W.pcap, W.kcap, W.rcap);
   const char *S = B.scratch + W.scratch;
@@ -197,12 +244,16 @@ __global__ void api_tape_capture_kernel(DevBatch B, const ContactIn *cin, int nk
     }
     con[i] = t;
   }
+  // nk_out[1]: 2 when a joint has rows the adjoint does not cover
+  if (threadIdx.x == 0) nk_out[1] = 0;
+  __syncthreads();
+  if (jt && !(flags & OVF_ROWS)) capture_joints(B, 0, W, S, L, cons, rows, nk, jt, nk_out + 1);
   if (threadIdx.x == 0) *nk_out = nk;
 }
 
 hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int nk, TapeContact<float> *con,
-                                   int *nk_out, hipStream_t st) {
-  api_tape_capture_kernel<<<1, 128, 0, st>>>(B, cin, nk, con, nk_out);
+                                   int *nk_out, TapeJoint<float> *jt, hipStream_t st) {
+  api_tape_capture_kernel<<<1, 128, 0, st>>>(B, cin, nk, con, nk_out, jt);
   return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ void *noalloc_scratch(hipError_t *err);
 hipError_t launch_collide_list(const GeomView *a, const GeomView *b, int npairs, int maxc, CPoint *out, int *n,
                                hipStream_t st);
 hipError_t launch_api_tape_capture(const DevBatch &B, const ContactIn *cin, int nk, adj::TapeContact<float> *con,
-                                   int *nk_out, hipStream_t st);
+                                   int *nk_out, adj::TapeJoint<float> *jt, hipStream_t st);
 hipError_t launch_adjoint_step(const DevBatch &B, const float4 *const st[4], const adj::TapeContact<float> *con,
                                const long long *con_off, const int *nk, const float4 *const gin[4],
                                float4 *const gout[4], float *gmu, float *gbounce, int *stats, float *scr,
@@ -65,7 +65,7 @@ struct dxWorld {
   // the last dWorldQuickStep recorded for dWorldQuickStepBackward: pre-step
   // state (pos, q, lvel, avel; nb float4 each), contacts with impulses, and
   // the contact policy the adjoint differentiates
-  DevBuf tape_st, tape_con, tape_nk;
+  DevBuf tape_st, tape_con, tape_nk, tape_jnt;
   DevBuf lcp_ws;                           // dWorldStep's exact-LCP workspace (grown on demand)
   bool tape_ok = false;
   char tape_why[200] = "no step recorded (dWorldSetTape(w, 1) before dWorldQuickStep)";
@@ -438,17 +438,18 @@ void write_contact(dContactGeom *c, const float *pos, const float *n, float dept
 }
 
 // Checks that the coming step is one dWorldQuickStepBackward differentiates
-// (the conditions of adj_step.hpp: contact joints only, one contact policy
-// -- dContactApprox1 friction or none, optional dContactBounce -- for every
-// contact, no forces but gravity, every contact's geoms in the world's
-// space) and sizes the tape; records the reason when it is not.
+// (the conditions of adj_step.hpp: contact joints, ball joints, hinges
+// without limits and hinge2 joints; one contact policy -- dContactApprox1
+// friction or none, optional dContactBounce -- for every contact, no forces
+// but gravity, every contact's geoms in the world's space) and sizes the
+// tape; records the reason when it is not.  Joint rows the adjoint does not
+// cover are found by the capture after the step.
 bool tape_prepare(dxWorld *w, const std::vector<ContactIn> &cin) {
   auto refuse = [&](const char *fmt, auto... a) {
     std::snprintf(w->tape_why, sizeof(w->tape_why), fmt, a...);
     w->tape_ok = false;
     return false;
   };
-  if (!w->mjoints.empty()) return refuse("the world has %zu ball/hinge/hinge2 joints", w->mjoints.size());
   for (dxGeom *g : w->mgeoms)
     if (g->cls != dSphereClass && g->cls != dBoxClass && g->cls != dPlaneClass)
       return refuse("the world has a capsule, cylinder or ray geom (the adjoint colliders are sphere, box and plane)");
@@ -470,8 +471,10 @@ bool tape_prepare(dxWorld *w, const std::vector<ContactIn> &cin) {
   }
   w->tape_maxc = w->mir_space ? w->mir_space->cached_maxc : 4;
   const size_t nb = w->mir->nb, kcap = w->mir->wi[0].kcap;
+  const size_t nj = w->mir->nj;
   if (!w->tape_st.alloc(4 * sizeof(float4) * (nb + 1)) ||
-      !w->tape_con.alloc(sizeof(adj::TapeContact<float>) * (kcap + 1)) || !w->tape_nk.alloc(16))
+      !w->tape_con.alloc(sizeof(adj::TapeContact<float>) * (kcap + 1)) || !w->tape_nk.alloc(16) ||
+      !w->tape_jnt.alloc(sizeof(adj::TapeJoint<float>) * (nj + 1)))
     return refuse("device allocation of the tape failed");
   return true;
 }
@@ -637,7 +640,8 @@ static int world_step_impl(dWorldID w, dReal stepsize, bool exact, const char *w
   OD_HIP(launch_world_kernel(KM_API_STEP, D, 0, 1, 0, m->wg, 0, m->stream));
   if (record)
     OD_HIP(launch_api_tape_capture(D, w->cin.as<ContactIn>(), nk, w->tape_con.as<adj::TapeContact<float>>(),
-                                   w->tape_nk.as<int>(), m->stream));
+                                   w->tape_nk.as<int>(), m->nj ? w->tape_jnt.as<adj::TapeJoint<float>>() : nullptr,
+                                   m->stream));
   OD_HIP(hipStreamSynchronize(m->stream));
   long long cnt[5];
   batch_counters(m, cnt);
@@ -647,6 +651,15 @@ static int world_step_impl(dWorldID w, dReal stepsize, bool exact, const char *w
     return 0;
   }
   if (!exact) w->tape_ok = record;
+  if (record && m->nj) {
+    int cov[2];
+    OD_HIP(hipMemcpy(cov, w->tape_nk.p, sizeof(cov), hipMemcpyDeviceToHost));
+    if (cov[1]) {
+      std::snprintf(w->tape_why, sizeof(w->tape_why),
+                    "a joint had rows the adjoint does not cover (a hinge limit, a bouncy or motor-loaded limit)");
+      w->tape_ok = false;
+    }
+  }
   // RNG state, feedback, joint limit state, space order
   WorldParams P;
   OD_HIP(hipMemcpy(&P, m->d.wp.p, sizeof(P), hipMemcpyDeviceToHost));
@@ -1731,7 +1744,7 @@ int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in) {
   P.max_contacts = w->tape_maxc;
   OD_HIP(hipMemcpy(m->d.wp.p, &P, sizeof(P), hipMemcpyHostToDevice));
   DevBuf gbuf, gsc, stats, scr, off;
-  const long long nscr = adj::adj_scratch_elems<float>(nb, kcap);
+  const long long nscr = adj::adj_scratch_elems<float>(nb, kcap, m->nj);
   bool ok = gbuf.alloc(8 * sizeof(float4) * (size_t)(nb + 1)) && gsc.alloc(2 * sizeof(float)) &&
             stats.alloc(2 * sizeof(int)) && scr.alloc(sizeof(float) * (size_t)nscr) &&
             off.alloc(sizeof(long long));
@@ -1768,7 +1781,7 @@ int dWorldQuickStepBackward(dWorldID w, const dReal *grad_out, dReal *grad_in) {
   hipError_t e = launch_adjoint_step(m->dev, st, w->tape_con.as<adj::TapeContact<float>>(), off.as<long long>(),
                                      w->tape_nk.as<int>(), gin, gout, gsc.as<float>(), gsc.as<float>() + 1,
                                      stats.as<int>(), scr.as<float>(), off.as<long long>(), 1e-6f, 1e-6f, 500, 256,
-                                     nullptr, m->stream);
+                                     m->nj ? w->tape_jnt.as<adj::TapeJoint<float>>() : nullptr, m->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   OD_HIP(hipMemcpy(m->d.wp.p, &P0, sizeof(P0), hipMemcpyHostToDevice));
   if (e != hipSuccess) {

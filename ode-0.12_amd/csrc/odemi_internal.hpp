@@ -34,6 +34,9 @@ struct Tape {
   bool kept = false;                             // mode the current tape was recorded in
   DevBuf kst, kcon, knk, kovf, kcon_off, kjnt;   // kjnt: every permanent joint's record per kept step
   long long kncon = 0;
+  // the joint controls each recorded step ran with (replays re-apply them)
+  DevBuf kctrl;
+  std::vector<char> ctrl_on;
 };
 
 // device arrays of a batch

@@ -378,8 +378,90 @@ extern "C" void orc_plane_space(const double *n, double *p, double *q) {
 // b2 (after attach), flags, rows.  states holds (nsteps+1) records (the
 // last is the final state).  Returns the total contact count, or -1 when
 // max_total is exceeded.
+// The adjoint's record of permanent joint j after a step (JREC doubles):
+// impulses[6], bounds lo[6] and hi[6], cfm/h[6], anchor1[3], anchor2[3],
+// axis1[3], axis2[3] (body frames), Info2 erp, suspension erp, c0, s0, v1[3],
+// v2[3], per limit/motor k: stop erp[2], stop[2], motor velocity[2]; the
+// hinge2 torques of the step[2]; type, body 1, body 2, rows, row of limot
+// k[2], its kind[2] (1 limit, 2 motor), 1 when the adjoint covers every row.
+// Row bounds and cfm follow the joints' getInfo2 (hinge.cpp:76-149,
+// hinge2.cpp:127-180, joint.cpp:556-733).
+static const int JREC = 64;
+static void joint_record(World<R> &w, int ji, double h, const double *ctl, double *o) {
+  const Joint<R> &j = w.joints[ji];
+  for (int k = 0; k < JREC; k++) o[k] = 0;
+  const int m = j.nrows;
+  for (int r = 0; r < 6; r++) {
+    o[r] = r < m ? (double)j.lam[r] : 0.0;
+    o[6 + r] = -INFINITY;
+    o[12 + r] = INFINITY;
+    o[18 + r] = (double)w.prm.cfm / h;
+  }
+  for (int k = 0; k < 3; k++) {
+    o[24 + k] = (double)j.anchor1[k];
+    o[27 + k] = (double)j.anchor2[k];
+    o[30 + k] = (double)j.axis1[k];
+    o[33 + k] = (double)j.axis2[k];
+    o[40 + k] = (double)j.v1[k];
+    o[43 + k] = (double)j.v2[k];
+  }
+  o[36] = (double)j.used_erp;
+  o[37] = (double)j.susp_erp;
+  o[38] = (double)j.c0;
+  o[39] = (double)j.s0;
+  o[54] = j.type;
+  o[55] = j.node0;
+  o[56] = j.node1;
+  o[57] = m;
+  o[58] = o[59] = -1;
+  bool covered = true;
+  int row = 3;
+  if (j.type == SC_BALL) {
+    for (int r = 0; r < 3; r++) o[18 + r] = (double)j.cfm / h;
+  } else if (j.type == SC_HINGE) {
+    row = 5;
+  } else {
+    o[18] = (double)j.susp_cfm / h;
+    row = 4;
+  }
+  for (int l = 0; l < 2 && j.type != SC_BALL; l++) {
+    const LimMot<R> &lm = l == 0 ? j.limot : j.limot2;
+    if (l == 1 && j.type != SC_HINGE2) break;
+    bool powered = lm.fmax > 0;
+    if (!(powered || lm.limit)) continue;
+    if (row >= m) { covered = false; break; }
+    o[58 + l] = row;
+    if (lm.limit && lm.lostop == lm.histop) powered = false;
+    if (powered && !lm.limit) {
+      o[60 + l] = 2;
+      o[50 + l] = (double)lm.vel;
+      o[6 + row] = -(double)lm.fmax;
+      o[12 + row] = (double)lm.fmax;
+      o[18 + row] = (double)lm.normal_cfm / h;
+    } else {
+      // a limit row; a motor pushing against a one-sided limit, a bouncy
+      // limit and the hinge's angle are not covered
+      if (powered || (lm.bounce > 0 && lm.lostop != lm.histop) || j.type == SC_HINGE) covered = false;
+      o[60 + l] = 1;
+      o[46 + l] = (double)lm.stop_erp;
+      o[48 + l] = (double)(lm.limit == 1 ? lm.lostop : lm.histop);
+      o[18 + row] = (double)lm.stop_cfm / h;
+      if (lm.lostop != lm.histop) {
+        if (lm.limit == 1) o[6 + row] = 0;
+        else o[12 + row] = 0;
+      }
+    }
+    row++;
+  }
+  if (j.type == SC_HINGE2 && ctl) {
+    o[52] = ctl[2];
+    o[53] = ctl[3];
+  }
+  o[62] = covered ? 1 : 0;
+}
+
 extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int *nk, double *contacts,
-                             long max_total, double *jlam) {
+                             long max_total, double *jrec) {
   World<R> w;
   build_world(s, w);
   Surface<R> surf;
@@ -397,6 +479,19 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
   long total = 0;
   for (int step = 0; step < nsteps; step++) {
     record(w, states + (size_t)step * s->nb * 13);
+    if (s->step_ctrl) {
+      // per-step joint controls, as scene_run applies them
+      const double *ctl = s->step_ctrl + (size_t)step * s->nj * 4;
+      for (int i = 0; i < s->nj; i++) {
+        const double *u = ctl + 4 * (size_t)i;
+        Joint<R> &j = w.joints[i];
+        if (j.type == SC_HINGE2 || j.type == SC_HINGE) {
+          if (has(u[0])) j.limot.lostop = (R)u[0];
+          if (has(u[1])) j.limot.histop = (R)u[1];
+        }
+        if (j.type == SC_HINGE2 && (u[2] != 0 || u[3] != 0)) w.hinge2_add_torques(i, (R)u[2], (R)u[3]);
+      }
+    }
     w.hash_pairs(pairs);
     if (g_colored) colored_pair_order(pairs);
     std::vector<int> cj;
@@ -413,12 +508,9 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
     }
     w.quickstep((R)s->dt);
     nk[step] = (int)cj.size();
-    // permanent joints: their rows' impulses, then the row count
-    for (int j = 0; jlam && j < w.n_perm; j++) {
-      double *o = jlam + 7 * ((size_t)step * w.n_perm + j);
-      for (int r = 0; r < 6; r++) o[r] = r < w.joints[j].nrows ? (double)w.joints[j].lam[r] : 0.0;
-      o[6] = w.joints[j].nrows;
-    }
+    for (int j = 0; jrec && j < w.n_perm; j++)
+      joint_record(w, j, s->dt, s->step_ctrl ? s->step_ctrl + ((size_t)step * s->nj + j) * 4 : nullptr,
+                   jrec + JREC * ((size_t)step * w.n_perm + j));
     if (total + (long)cj.size() > max_total) return -1;
     for (size_t i = 0; i < cj.size(); i++) {
       const Joint<R> &j = w.joints[cj[i]];
@@ -439,32 +531,6 @@ extern "C" long orc_tape_run(const SceneDesc *s, int nsteps, double *states, int
   }
   record(w, states + (size_t)nsteps * s->nb * 13);
   return total;
-}
-
-// Frame data of the scene's permanent joints as the restatement holds them
-// after set-up, 17 doubles each: type, first and second body, anchor1[3] and
-// anchor2[3] (body frames; anchor2 a world point without a second body),
-// axis1[3] and axis2[3] (likewise), and the erp and cfm of the joint's rows
-// in the coloured order (a ball joint's own, the world's for a hinge).
-extern "C" int orc_joint_frames(const SceneDesc *s, double *out) {
-  World<R> w;
-  build_world(s, w);
-  for (int j = 0; j < w.n_perm; j++) {
-    const Joint<R> &jt = w.joints[j];
-    double *o = out + 17 * (size_t)j;
-    o[0] = jt.type;
-    o[1] = jt.node0;
-    o[2] = jt.node1;
-    for (int k = 0; k < 3; k++) {
-      o[3 + k] = (double)jt.anchor1[k];
-      o[6 + k] = (double)jt.anchor2[k];
-      o[9 + k] = (double)jt.axis1[k];
-      o[12 + k] = (double)jt.axis2[k];
-    }
-    o[15] = (double)(jt.type == SC_BALL ? jt.erp : w.prm.erp);
-    o[16] = (double)(jt.type == SC_BALL ? jt.cfm : w.prm.cfm);
-  }
-  return w.n_perm;
 }
 
 // The exact box-constrained LCP of the dWorldStep restatement on one dense

@@ -118,6 +118,7 @@ template <class R> struct Joint {
   R c0 = 0, s0 = 0;
   R lam[6] = {0, 0, 0, 0, 0, 0};   // impulses of the rows after the last step
   int nrows = 0;
+  R used_erp = 0;               // the Info2 erp of the joint's rows in the last step
   long long wkey = -1;          // contact: (g1, g2, index within the pair), the warm-start key
 };
 
@@ -823,6 +824,7 @@ template <class R> struct World {
         in.J = &J[12 * (size_t)ofs];
         in.c = &c[ofs]; in.cfm = &cfm[ofs]; in.lo = &lo[ofs]; in.hi = &hi[ofs]; in.findex = &findex[ofs];
         info2(jt, in);
+        jt.used_erp = in.erp;
         if (jt.fb >= 0) Jcopy.insert(Jcopy.end(), J.begin() + 12 * (size_t)ofs, J.begin() + 12 * (size_t)(ofs + jm[a]));
         for (int r = 0; r < jm[a]; r++)
           if (findex[ofs + r] != -1) findex[ofs + r] += ofs;

@@ -176,7 +176,9 @@ def run(lib, sc, nsteps: int, record_every: int = 1, diag_steps: int = 0, max_pa
 
 def tape_run(lib, sc, nsteps: int, max_total: int = 200000) -> dict:
     """Step-by-step record of a run (orc_tape_run): states before every step
-    (and the final one) and every step's contacts with their row impulses."""
+    (and the final one), every step's contacts with their row impulses, and
+    every permanent joint's record after each step (64 values: impulses,
+    row bounds and cfm, frames, limit/motor rows, torques; orc_tape_run)."""
     fn = lib.orc_tape_run
     fn.restype = C.c_long
     fn.argtypes = [C.POINTER(SceneDesc), C.c_int, _dp, _ip, _dp, C.c_long, _dp]
@@ -184,23 +186,10 @@ def tape_run(lib, sc, nsteps: int, max_total: int = 200000) -> dict:
     states = np.zeros((nsteps + 1, sc.nb, 13))
     nk = np.zeros(nsteps, np.int32)
     con = np.zeros((max_total, 16))
-    jlam = np.zeros((nsteps, max(sc.nj, 1), 7))
+    jrec = np.zeros((nsteps, max(sc.nj, 1), 64))
     total = fn(C.byref(desc), nsteps, states.ctypes.data_as(_dp), nk.ctypes.data_as(_ip),
-               con.ctypes.data_as(_dp), max_total, jlam.ctypes.data_as(_dp))
+               con.ctypes.data_as(_dp), max_total, jrec.ctypes.data_as(_dp))
     del keep
     if total < 0:
         raise RuntimeError("tape_run: contact buffer too small")
-    return dict(states=states, nk=nk, contacts=con[:total].copy(), jlam=jlam[:, :sc.nj].copy())
-
-
-def joint_frames(lib, sc) -> np.ndarray:
-    """(nj, 17) frame data of the scene's permanent joints as the restatement
-    holds them after set-up (orc_joint_frames)."""
-    fn = lib.orc_joint_frames
-    fn.restype = C.c_int
-    fn.argtypes = [C.POINTER(SceneDesc), _dp]
-    desc, keep = make_desc(sc)
-    out = np.zeros((max(sc.nj, 1), 17))
-    n = fn(C.byref(desc), out.ctypes.data_as(_dp))
-    del keep
-    return out[:n].copy()
+    return dict(states=states, nk=nk, contacts=con[:total].copy(), joints=jrec[:, :sc.nj].copy())

@@ -27,16 +27,14 @@ extern "C" {
 // prm[ng*4], pos[ng*3], R9[ng*9] (static geoms); params[16]: h, erp, cfm,
 // min_depth, max_vel, gx, gy, gz, mu, bounce, bounce_vel, surf_bounce, maxc,
 // act_tol, lin_tol, lin_maxit.  states: (nsteps+1)*nb*13; nk[nsteps];
-// contacts: 16 doubles each (see orc_tape_run).  Permanent joints: nj,
-// jdef[nj*17] (type, b1, b2, anchor1[3], anchor2[3], axis1[3], axis2[3] in
-// the body frames, erp, cfm) and jlam[nsteps*nj*7] (the rows' impulses after
-// each step, then the row count).  grad_out: nb*13 (dL/d final state);
-// grad_in: nb*13 out; gmu, gbounce out; stats[2] out.
+// contacts: 16 doubles each (see orc_tape_run).  Permanent joints: nj and
+// jrec[nsteps*nj*64], each step's joint records of orc_tape_run.  grad_out:
+// nb*13 (dL/d final state); grad_in: nb*13 out; gmu, gbounce out; stats[2]
+// out.  Returns 0 when a recorded joint has rows the adjoint does not cover.
 int adj_host_backward(int nb, int ng, const double *mass, const double *I9, const int *gtype, const int *gbody,
                       const double *gprm, const double *gpos, const double *gR9, const double *params, int nsteps,
-                      const double *states, const int *nk, const double *contacts, int nj, const double *jdef,
-                      const double *jlam, const double *grad_out, double *grad_in, double *gmu, double *gbounce,
-                      int *stats) {
+                      const double *states, const int *nk, const double *contacts, int nj, const double *jrec,
+                      const double *grad_out, double *grad_in, double *gmu, double *gbounce, int *stats) {
   StepParams<double> P;
   P.h = params[0]; P.erp = params[1]; P.cfm = params[2]; P.min_depth = params[3]; P.max_vel = params[4];
   P.gravity[0] = params[5]; P.gravity[1] = params[6]; P.gravity[2] = params[7];
@@ -103,16 +101,22 @@ int adj_host_backward(int nb, int ng, const double *mass, const double *I9, cons
     }
     std::vector<TapeJoint<double>> jt(nj);
     for (int j = 0; j < nj; j++) {
-      const double *d = jdef + 17 * (size_t)j;
-      const double *l = jlam + 7 * ((size_t)s * nj + j);
+      const double *o = jrec + 64 * ((size_t)s * nj + j);
       TapeJoint<double> &t = jt[j];
-      t.type = (int)d[0]; t.b1 = (int)d[1]; t.b2 = (int)d[2];
-      for (int k = 0; k < 3; k++) {
-        t.anc1[k] = d[3 + k]; t.anc2[k] = d[6 + k]; t.ax1[k] = d[9 + k]; t.ax2[k] = d[12 + k];
+      if (o[62] == 0) return 0;
+      for (int k = 0; k < 6; k++) {
+        t.lam[k] = o[k]; t.lo[k] = o[6 + k]; t.hi[k] = o[12 + k]; t.rcfm[k] = o[18 + k];
       }
-      t.erp = d[15]; t.cfm = d[16];
-      for (int k = 0; k < 6; k++) t.lam[k] = l[k];
-      t.m = (int)l[6];
+      for (int k = 0; k < 3; k++) {
+        t.anc1[k] = o[24 + k]; t.anc2[k] = o[27 + k]; t.ax1[k] = o[30 + k]; t.ax2[k] = o[33 + k];
+        t.v1[k] = o[40 + k]; t.v2[k] = o[43 + k];
+      }
+      t.erp = o[36]; t.susp_erp = o[37]; t.c0 = o[38]; t.s0 = o[39];
+      for (int l = 0; l < 2; l++) {
+        t.lerp[l] = o[46 + l]; t.lstop[l] = o[48 + l]; t.lc[l] = o[50 + l]; t.torque[l] = o[52 + l];
+        t.lrow[l] = (int)o[58 + l]; t.lkind[l] = (int)o[60 + l];
+      }
+      t.type = (int)o[54]; t.b1 = (int)o[55]; t.b2 = (int)o[56]; t.m = (int)o[57];
     }
     StepData<double> D;
     D.nb = nb; D.nk = nk[s]; D.ng = ng;

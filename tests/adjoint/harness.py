@@ -4,6 +4,7 @@
   plane, optional restitution) of the differentiable-pile kind.
 * ``jointed_scene``: a ball/hinge chain on a plane with a hinge and a ball
   joint to the static world.
+* ``vehicle_scene``: one car on hinge2 wheels with per-step controls.
 * ``converged_oracle``: the FP64 restatement solving to a tight residual.
 * ``com_loss``: sum over worlds of the squared centre of mass of the final
   positions, and its gradient with respect to the final state.
@@ -76,6 +77,19 @@ def jointed_scene(seed: int, n_links: int = 4, iterations: int = 20000, dt: floa
     sc = b.scene(dt=dt, mu=mu, seed=seed, iterations=iterations, surf_mode=scenes.APPROX1)
     sc.body_lvel = rng.uniform(-0.8, 0.8, (sc.nb, 3))
     sc.body_avel = rng.uniform(-1.0, 1.0, (sc.nb, 3))
+    return sc
+
+
+def vehicle_scene(seed: int, nsteps: int) -> scenes.Scene:
+    """One car of the vehicle workload (scenes.vehicle_world: chassis on four
+    hinge2 wheels with suspension, per-step steering stops on the front
+    wheels and wheel torques) without ramps, with seeded small initial
+    velocities; it lands on the plane within the first 40 steps."""
+    rng = np.random.default_rng(seed)
+    sc = scenes.vehicle_world(rng, nsteps, n_ramps=0, seed=seed)
+    sc.body_lvel = rng.uniform(-0.5, 0.5, (sc.nb, 3))
+    sc.body_avel = rng.uniform(-0.5, 0.5, (sc.nb, 3))
+    sc.iterations = 20000
     return sc
 
 
@@ -157,18 +171,18 @@ def host_adjoint(sc, nsteps, lib_path=None):
                        1.0 if sc.surf_mode & scenes.BOUNCE else 0.0, sc.max_contacts, 1e-9, 1e-13, 4000],
                       np.float64)
     arr = lambda a, t=np.float64: np.ascontiguousarray(a, t)
-    jdef = po.joint_frames(po.oracle_lib("f64"), sc) if sc.nj else np.zeros((0, 17))
     keep = [arr(sc.body_mass), arr(sc.body_I), arr(sc.geom_type, np.int32), arr(sc.geom_body, np.int32),
             arr(sc.geom_param), arr(sc.geom_pos), arr(_geom_rot(sc)), params, arr(t["states"]),
-            arr(t["nk"], np.int32), arr(t["contacts"]), arr(jdef.reshape(-1, 17)),
-            arr(t["jlam"].reshape(nsteps, -1, 7)), arr(gout)]
+            arr(t["nk"], np.int32), arr(t["contacts"]), arr(t["joints"]), arr(gout)]
     ptr = lambda a: a.ctypes.data_as(_ip if a.dtype == np.int32 else _dp)
     gin = np.zeros((sc.nb, 13))
     gmu, gb = C.c_double(), C.c_double()
     stats = np.zeros(2, np.int32)
-    lib.adj_host_backward(sc.nb, sc.ng, *[ptr(a) for a in keep[:8]], nsteps, ptr(keep[8]), ptr(keep[9]),
-                          ptr(keep[10]), sc.nj, ptr(keep[11]), ptr(keep[12]), ptr(keep[13]), ptr(gin),
-                          C.byref(gmu), C.byref(gb), ptr(stats))
+    ok = lib.adj_host_backward(sc.nb, sc.ng, *[ptr(a) for a in keep[:8]], nsteps, ptr(keep[8]), ptr(keep[9]),
+                               ptr(keep[10]), sc.nj, ptr(keep[11]), ptr(keep[12]), ptr(gin), C.byref(gmu),
+                               C.byref(gb), ptr(stats))
+    if not ok:
+        raise ValueError("a recorded joint has rows the adjoint does not cover")
     return L, gin, gmu.value, gb.value, stats
 
 

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from adjoint.harness import (Rollout, com_loss, contact_scene, converged_oracle, fd_grad, forward_f64, host_adjoint,
-                             jointed_scene)
+                             jointed_scene, vehicle_scene)
 from conftest import say
 from odemi import scenes
 
@@ -240,14 +240,18 @@ def test_world_quickstep_backward_matches_central_differences(odemi_mod):
     dWorldSetTape and differentiates it): dL/d(state before the last step)
     for L = squared centre of mass after it, against FP64 central
     differences (h = 1e-6) of that one step through the restatement from the
-    same pre-step state, for every position and velocity component; the
-    reference-order solve runs 20000 iterations, so it converges."""
+    same pre-step state, for every position and velocity component, on two
+    contact scenes, a ball/hinge chain with joints to the static world and a
+    car on hinge2 wheels with its steering stops; the reference-order solve
+    runs 20000 iterations, so it converges."""
     import pyoracle as po
     from conftest import ROOT
     lib = po.load(os.path.join(ROOT, "tests", "_build", "libscene_odemi.so"))
     orc = po.oracle_lib("f64")
     n = 8
-    for sc in (contact_scene(4, 4, (0,), bounce=0.5), contact_scene(5, 4, (1,))):
+    car = vehicle_scene(5, n)
+    car.step_ctrl = None   # steering locked by its stops at 0; no wheel torques (body torques are not taped)
+    for sc in (contact_scene(4, 4, (0,), bounce=0.5), contact_scene(5, 4, (1,)), jointed_scene(3, 4), car):
         say("dWorldQuickStepBackward:", sc.nb, "bodies, step", n)
         pre = po.run(lib, sc, n - 1, n - 1)["traj"][-1]
         fin = po.run(lib, sc, n, n)["traj"][-1]
@@ -382,14 +386,73 @@ def test_gpu_adjoint_on_uneven_worlds_with_chains(odemi_mod):
     assert np.isfinite(gi).all() and np.isfinite(gmu).all()
 
 
+def _vehicle_gpu(odemi_mod, scs, n, interval, keep=False):
+    b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
+    b.set_tolerance(1e-6, 20000, 10, accel=True)
+    b.tape_options(interval)
+    b.tape_keep_steps(keep)
+    b.tape_begin(n)
+    ctrl = np.concatenate([sc.step_ctrl[:n] for sc in scs], axis=1)
+    for k in range(n):
+        b.set_controls(ctrl[k])
+        b.step(1)
+    st = b.state().astype(np.float64)
+    finals, off = [], 0
+    for sc in scs:
+        finals.append(st[off:off + sc.nb])
+        off += sc.nb
+    _, gl = com_loss(scs, finals)
+    gi, gmu, _ = b.step_backward(n, np.concatenate(gl))
+    stats = b.tape_stats()
+    b.close()
+    return finals, gi.astype(np.float64), gmu, stats
+
+
+def test_gpu_adjoint_of_vehicles(odemi_mod):
+    """Cars of the vehicle workload (hinge2 suspension, steering stops and
+    wheel torques set per step by dBatchSetControls) over 50 steps in which
+    they land: the GPU gradients against the host double adjoint and FP64
+    central differences, every velocity component of every body, within
+    1e-3 of the largest gradient; the replays re-apply each step's controls,
+    so checkpoint intervals 7 and 50 and the kept-steps pass agree."""
+    n = 50
+    scs = [vehicle_scene(5, n), vehicle_scene(6, n)]
+    say("adjoint of vehicles:", len(scs), "cars,", n, "steps")
+    finals, gi, gmu, stats = _vehicle_gpu(odemi_mod, scs, n, 7)
+    say("adjoint stats", stats)
+    assert stats["collider_mismatches"] == 0 and np.isfinite(gi).all()
+    off = 0
+    for w, sc in enumerate(scs):
+        g = gi[off:off + sc.nb]
+        div = float(np.abs(finals[w][:, :3] - forward_f64(sc, n)[:, :3]).max())
+        _, hg, _, _, hst = host_adjoint(sc, n)
+        assert hst[1] == 0
+        gmax = float(np.abs(hg[:, 7:13]).max())
+        err_host = float(np.abs(g[:, 7:13] - hg[:, 7:13]).max()) / gmax
+        worst = 0.0
+        for bb in range(sc.nb):
+            for field, col in (("body_lvel", 7), ("body_avel", 10)):
+                for k in range(3):
+                    fd = fd_grad(sc, n, field, (bb, k))
+                    worst = max(worst, abs(fd - g[bb, col + k]) / gmax)
+        say(f"car {w}: GPU vs FP64 final positions {div:.2e} m, max rel err vs FD {worst:.2e}, "
+            f"vs host adjoint {err_host:.2e}")
+        assert div <= 1e-4 and worst <= TOL and err_host <= TOL
+        off += sc.nb
+    scale = float(np.abs(gi).max())
+    for interval, keep in ((50, False), (10, True)):
+        _, g2, _, _ = _vehicle_gpu(odemi_mod, scs, n, interval, keep)
+        say(f"interval {interval} keep {keep}: max |d grad| / max |grad| {float(np.abs(g2 - gi).max()) / scale:.2e}")
+        assert np.abs(g2 - gi).max() <= 1e-5 * scale
+
+
 def test_tape_refuses_joints_it_does_not_cover(odemi_mod):
-    """hinge2 joints (vehicles) and hinges with limits or a motor (the joint
-    zoo) are refused by dBatchTapeBegin with an error, not differentiated."""
-    for name, sc in (("vehicle", scenes.vehicle_worlds(1, seed=11)[0]), ("joint zoo", scenes.joint_zoo())):
-        say("tape refusal:", name)
-        b = odemi_mod.Batch([sc], odemi_mod.ORDER_COLORED)
-        try:
-            with pytest.raises(RuntimeError):
-                b.tape_begin(5)
-        finally:
-            b.close()
+    """Hinges with limits (the joint zoo) are refused by dBatchTapeBegin with
+    an error, not differentiated."""
+    say("tape refusal: joint zoo")
+    b = odemi_mod.Batch([scenes.joint_zoo()], odemi_mod.ORDER_COLORED)
+    try:
+        with pytest.raises(RuntimeError):
+            b.tape_begin(5)
+    finally:
+        b.close()
