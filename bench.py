@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--accel", type=int, default=1,
                     help="nonsmooth nonlinear CG steps between the sweeps of the iterate-to-tolerance solver (0: off)")
+    ap.add_argument("--queue-order", choices=("cost", "size"), default="size",
+                    help="persistent queue: worlds by size (bodies + geoms) or by estimated solver work (cost)")
     ap.add_argument("--warm", type=float, default=1.0,
                     help="warm start of the iterate-to-tolerance solver (factor on the previous impulses; 0: off)")
     ap.add_argument("--adjoint-steps", type=int, default=100, help="steps of the adjoint run (0: skip)")
@@ -447,6 +449,7 @@ def main():
     # (1) the quickstep's fixed iteration count (20, as dWorldQuickStep):
     # steps 0..t_from untimed, then the window
     b = odemi.Batch(worlds, odemi.ORDER_COLORED)
+    b.set_queue_order(args.queue_order == "cost")
     if args.launch:
         wg, cs, lds, split, grid = args.launch.split(",")
         b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
@@ -473,6 +476,7 @@ def main():
     b.set_launch_config(cfg["wg"], cfg["cell_scale"], cfg["lds"], cfg["split"], cfg["grid"])
     b.set_tolerance(args.tol, args.max_iters, args.check_every, accel=bool(args.accel))
     b.set_warm_start(args.warm)
+    b.set_queue_order(args.queue_order == "cost")
     if args.launch_tol:
         wg, cs, lds, split, grid = args.launch_tol.split(",")
         b.set_launch_config(int(wg), float(cs), int(lds), int(split), int(grid))
@@ -565,7 +569,9 @@ def main():
                              f"until LCP residual "
                              f"<= {args.tol:g} (checked every {args.check_every}, cap {args.max_iters}), w=1.0, "
                              f"warm start {args.warm:g} x the previous step's impulses, every step of the rollout",
-                   "launch": cfg_tol, "launch_fixed_iterations": cfg},
+                   "launch": cfg_tol, "launch_fixed_iterations": cfg,
+                   "queue_order": "by estimated solver work (rows x sweeps)" if args.queue_order == "cost"
+                                  else "by size (bodies + geoms), largest first"},
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
         "lcp_residual": {"tol": args.tol, "converged_fraction": conv_solves,

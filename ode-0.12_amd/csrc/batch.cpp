@@ -318,7 +318,8 @@ BatchImpl *batch_create_arrays(BatchArrays &A, int order_mode) {
             upload(d.gpos, A.gpos) && upload(d.gR, A.gR) && upload(d.grank, A.grank) && upload(d.gcat, A.gcat) &&
             upload(d.gcol, A.gcol) && upload(d.joints, A.joints) && upload(d.adj_start, A.adj_start) &&
             upload(d.adj_other, A.adj_other) && upload(d.bgeom_start, A.bgeom_start) && upload(d.bgeom, A.bgeom) &&
-            upload(d.world_order, order) && d.R.alloc(sizeof(float4) * 3 * (size_t)(nb ? nb : 1)) &&
+            upload(d.world_order, order) && upload(d.world_rank, order) &&
+            upload(d.cost_est, std::vector<float>(b->nworlds, 0.f)) && d.R.alloc(sizeof(float4) * 3 * (size_t)(nb ? nb : 1)) &&
             d.counters.alloc(sizeof(int) * CNT_N * (size_t)b->nworlds) &&
             d.residual.alloc(sizeof(float) * (size_t)b->nworlds) && d.work_counter.alloc(64 * WORK_QUEUES) &&
             d.scratch.alloc((size_t)scratch) && d.warm.alloc((size_t)(warm > 0 ? warm : 256)) &&
@@ -485,6 +486,13 @@ static int persistent_grid(BatchImpl *b) {
 // one step of every world: one fused kernel, or collide and step kernels, or
 // (coloured order at a tolerance) collide + set-up, the sweeps, integration
 static hipError_t launch_step(BatchImpl *b, int first_world, int nworlds, int grid) {
+  if (b->cost_order && grid > 0 && first_world == 0 && nworlds == b->nworlds && b->nworlds >= 2 &&
+      b->nworlds <= 8192) {
+    // the queue order by estimated solver work, rewritten every step
+    hipError_t e = launch_order_worlds(b->dev, b->d.world_rank.as<int>(), b->d.cost_est.as<float>(), b->stream);
+    if (e != hipSuccess) return e;
+    b->order_rewritten = true;
+  }
   if (b->tol_on && b->sor_split && b->order_mode == ORDER_COLORED) {
     hipError_t e = launch_world_kernel(KM_FUSED_PRE, b->dev, first_world, nworlds, grid, b->wg, 0, b->stream, 0);
     if (e != hipSuccess) return e;
@@ -1017,6 +1025,21 @@ int dBatchSetPersistentGrid(dBatchID b, int grid) {
   return 1;
 }
 int dBatchGetSplit(dBatchID b) { return b ? (b->impl->split ? 1 : 0) : -1; }
+int dBatchSetQueueOrder(dBatchID b, int by_cost) {
+  if (!b) return 0;
+  BatchImpl *m = b->impl;
+  m->cost_order = by_cost != 0;
+  destroy_graph(m);
+  if (!m->cost_order && m->order_rewritten) {
+    // back to the static (largest first) order
+    OD_HIP(hipMemcpyAsync(m->d.world_order.p, m->d.world_rank.p, sizeof(int) * m->nworlds, hipMemcpyDeviceToDevice,
+                          m->stream));
+    OD_HIP(hipStreamSynchronize(m->stream));
+    m->order_rewritten = false;
+  }
+  return 1;
+}
+int dBatchGetQueueOrder(dBatchID b) { return b ? (b->impl->cost_order ? 1 : 0) : -1; }
 int dBatchSetSorLaunch(dBatchID b, int on, int wg, int lds, int grid) {
   return b ? batch_set_sor_launch(b->impl, on, wg, lds, grid) : 0;
 }

@@ -276,6 +276,10 @@ enum {
 hipError_t launch_world_kernel(int mode, const DevBatch &B, int first_world, int nworlds_launch, int grid, int wg,
                                int api_maxc, hipStream_t st, int queue = 0);
 hipError_t launch_init_rot(const DevBatch &B, int nbodies, hipStream_t st);
+// the persistent queue's order of B.world_order by expected cost (rank: the
+// static order; est: the per-world cost estimates it keeps); at most 8192
+// worlds
+hipError_t launch_order_worlds(const DevBatch &B, const int *rank, float *est, hipStream_t st);
 int world_kernel_static_lds();
 hipError_t allow_world_kernel_lds(int bytes);
 int world_kernel_occupancy(int mode, int wg, int lds_bytes);

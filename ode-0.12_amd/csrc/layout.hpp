@@ -216,21 +216,27 @@ OD_HD inline long long warm_bytes(int kcap, int nj) { return od_align(24LL * kca
 // rows: the matrix and its factor (m*m each), then seven vectors of m
 OD_HD inline long long lcp_ws_floats(int m) { return 2LL * m * m + 7LL * m + 64; }
 
-// bytes of the coloured solver's working set: rows, bodies, constraints in
-// colour order, colour starts, row impulses and (accelerated mode) the
-// conjugate-gradient state; copied to the LDS pool for the sweeps when it fits
-OD_HD inline long long sor_lds_bytes(int nb, int nrows, int ncons, bool accel = false) {
-  long long b = (long long)sizeof(SRow) * nrows + (long long)sizeof(SBody) * nb + (long long)sizeof(SCons) * ncons +
+// bytes of the coloured solver's working set held in LDS when its rows and
+// the bodies' conjugate-gradient state stay in global memory: bodies,
+// constraints in colour order, colour starts, row impulses and (accelerated
+// mode) the rows' conjugate-gradient state
+OD_HD inline long long sor_lds_bytes_rows_acc_global(int nb, int nrows, int ncons, bool accel = false) {
+  long long b = (long long)sizeof(SBody) * nb + (long long)sizeof(SCons) * ncons +
                 ((4LL * (2LL * ncons + 8) + 15) & ~15LL) + ((4LL * nrows + 15) & ~15LL);
-  if (accel) b += (long long)sizeof(SAccBody) * nb + 8LL * nrows;   // conjugate-gradient state
+  if (accel) b += (8LL * nrows + 15) & ~15LL;
   return b;
 }
 
-// the part of it held in LDS when the rows stay in global memory (bodies,
-// constraints, colour starts, impulses, conjugate-gradient state): the
-// placement of a world whose whole working set exceeds the pool
+// ... with the bodies' conjugate-gradient state in LDS too (the rows in
+// global memory)
 OD_HD inline long long sor_lds_bytes_rows_global(int nb, int nrows, int ncons, bool accel = false) {
-  return sor_lds_bytes(nb, nrows, ncons, accel) - (long long)sizeof(SRow) * nrows;
+  return sor_lds_bytes_rows_acc_global(nb, nrows, ncons, accel) + (accel ? (long long)sizeof(SAccBody) * nb : 0LL);
+}
+
+// the whole working set (rows first); copied to the LDS pool for the sweeps
+// when it fits
+OD_HD inline long long sor_lds_bytes(int nb, int nrows, int ncons, bool accel = false) {
+  return (long long)sizeof(SRow) * nrows + sor_lds_bytes_rows_global(nb, nrows, ncons, accel);
 }
 
 // the same working set in the world's scratch

@@ -44,7 +44,8 @@ struct BatchBufs {
   DevBuf wi, wp, pos, q, lvel, avel, facc, tacc, R, I, invI, mass, invMass;
   DevBuf gtype, gbody, gparam, gpos, gR, grank, gcat, gcol;
   DevBuf joints, adj_start, adj_other, bgeom_start, bgeom;
-  DevBuf ctrl, counters, residual, work_counter, world_order, scratch, feedback, prof, warm, warm_n;
+  DevBuf ctrl, counters, residual, work_counter, world_order, world_rank, cost_est, scratch, feedback, prof, warm,
+      warm_n;
 };
 
 struct BatchImpl {
@@ -70,6 +71,10 @@ struct BatchImpl {
   int lds_max = 65536;    // largest pool one workgroup may request
   bool prof_on = false;   // per-phase wall-clock counters
   bool split = false;     // collide and quickstep as two kernels per step
+  // persistent launches pull the worlds in order of their estimated solver
+  // work (rows x sweeps) instead of the static largest-first order
+  bool cost_order = false;
+  bool order_rewritten = false;   // world_order differs from the static order
   bool tol_on = false;    // some world iterates to a residual tolerance
   // coloured order at a tolerance: the sweeps in a kernel of their own
   // (collide + set-up | sweeps | integration), with their own launch shape

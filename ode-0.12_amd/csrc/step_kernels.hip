@@ -1092,10 +1092,11 @@ __device__ inline void warm_lookup(const WorldParams &P, const Cons &q, const Co
 
 // The coloured solver's working set: the rows (read-only during the sweeps),
 // bodies, constraints in colour order, colour starts, the row impulses and
-// (accelerated mode) the conjugate-gradient state: per body fc before the
-// sweep and the direction's fc, per row lambda before the sweep and the
-// direction.  Built in the world's scratch; copied to the LDS pool for the
-// sweeps when it fits there (same layout).
+// (accelerated mode) the conjugate-gradient state: per row lambda before the
+// sweep and the direction, per body fc before the sweep and the direction's
+// fc (last, so that a pool can hold all but it).  Built in the world's
+// scratch; copied to the LDS pool for the sweeps when it fits there (same
+// layout).
 struct SorSet {
   SRow *sr;
   SBody *sb;
@@ -1113,8 +1114,8 @@ __device__ inline SorSet sor_set(char *base, SRow *sr, int nb, int nrows, int na
   S.sc = (SCons *)(S.sb + nb);
   S.sstart = (int *)(S.sc + nact);
   S.slam = (float *)((char *)S.sstart + ((4LL * (2LL * nact + 8) + 15) & ~15LL));
-  S.sa = (SAccBody *)((char *)S.slam + ((4LL * nrows + 15) & ~15LL));
-  S.ld = (float2 *)(S.sa + nb);
+  S.ld = (float2 *)((char *)S.slam + ((4LL * nrows + 15) & ~15LL));
+  S.sa = (SAccBody *)((char *)S.ld + ((8LL * nrows + 15) & ~15LL));
   return S;
 }
 
@@ -1184,9 +1185,10 @@ __device__ void sor_prepare(const DevBatch &B, const Ctx &c, const WorldParams &
 }
 
 // Placement of the coloured solver's working set during the sweeps: all in
-// global scratch, all in the LDS pool, or everything but the (read-only)
-// rows in the pool.
-enum { SOR_IN_GLOBAL = 0, SOR_IN_LDS = 1, SOR_ROWS_GLOBAL = 2 };
+// global scratch, all in the LDS pool, everything but the (read-only) rows
+// in the pool, or everything but the rows and the bodies' conjugate-gradient
+// state (read and written once per sweep, by one coalesced pass) in the pool.
+enum { SOR_IN_GLOBAL = 0, SOR_IN_LDS = 1, SOR_ROWS_GLOBAL = 2, SOR_ROWS_ACC_GLOBAL = 3 };
 
 // The sweeps.  The working set (rows included, or all but the rows) is
 // copied to the LDS pool first and back at the end; the placement is a
@@ -1213,7 +1215,7 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   SCons *__restrict__ sc = S.sc;
   int *__restrict__ sstart = S.sstart;
   float *__restrict__ slam = S.slam;
-  SAccBody *__restrict__ sa = S.sa;
+  SAccBody *__restrict__ sa = PLACE == SOR_ROWS_ACC_GLOBAL ? G.sa : S.sa;
   float2 *__restrict__ ld = S.ld;
   if (LDS) {
     if (ROWS_LDS)
@@ -1389,6 +1391,13 @@ __device__ __attribute__((noinline)) void sor_solve_rows_global(const DevBatch &
 }
 
 template <bool TOL>
+__device__ __attribute__((noinline)) void sor_solve_rows_acc_global(const DevBatch &B, const Ctx &c,
+                                                                    const WorldParams &P, int nrows, int ngroups,
+                                                                    int nact, bool host_contacts, Shared &sh) {
+  sor_solve<TOL, SOR_ROWS_ACC_GLOBAL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+}
+
+template <bool TOL>
 __device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const WorldParams &P, int nrows, int ngroups,
                                      int nact, bool host_contacts, Shared &sh) {
   const bool accel = TOL && P.tol > 0.f && P.accel != 0;
@@ -1396,6 +1405,8 @@ __device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const Worl
     sor_solve<TOL, SOR_IN_LDS>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
   else if (sor_lds_bytes_rows_global(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
     sor_solve_rows_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  else if (accel && sor_lds_bytes_rows_acc_global(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
+    sor_solve_rows_acc_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
   else
     sor_solve_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
 }
@@ -2353,6 +2364,63 @@ __global__ __launch_bounds__(WG) void sor_kernel(DevBatch B, int first_world, in
 
 __global__ void reset_counter_kernel(int *ctr) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0;
+}
+
+// Processing order of the persistent queue by expected cost, most expensive
+// first, ties in the static order (largest first, `rank`).  The estimate of
+// a world (est, kept per world) is the larger of its last step's rows times
+// solver sweeps and 3/4 of its previous estimate: a tolerance solve that
+// ended at the sweep cap far from the tolerance starts the next one cold,
+// which may then converge quickly, so a world's cost alternates and the
+// last step alone predicts the next one badly; plus bodies and geoms,
+// which is all the first step has.  One workgroup; a bitonic sort of
+// n <= 8192 64-bit keys (cost << 32 | position from the end of the static
+// order) in the dynamic LDS (8 bytes per key, n rounded up to a power of two).
+__global__ __launch_bounds__(1024) void order_worlds_kernel(DevBatch B, const int *rank, float *est, int n) {
+  extern __shared__ unsigned long long okey[];
+  int N = 1;
+  while (N < n) N <<= 1;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    unsigned long long k = 0;   // padding sorts last
+    if (i < n) {
+      const int w = rank[i];
+      const int *cnt = B.counters + (size_t)w * CNT_N;
+      const float work = (float)max(cnt[CNT_ROWS], 0) * (float)max(cnt[CNT_ITERS], 1);
+      const float e = fmaxf(work, 0.75f * est[w]);
+      est[w] = e;
+      const float c = e + (float)(B.wi[w].nb + B.wi[w].ng);
+      const unsigned long long cost = c < 4.0e9f ? (unsigned long long)c : 0xffffffffull;
+      k = (cost << 32) | (unsigned long long)(unsigned)(N - 1 - i);
+    }
+    okey[i] = k;
+  }
+  // descending bitonic sort
+  for (int size = 2; size <= N; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < (N >> 1); i += blockDim.x) {
+        const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+        const bool desc = (lo & size) == 0;
+        const unsigned long long a = okey[lo], b = okey[hi];
+        if ((a < b) == desc) {
+          okey[lo] = b;
+          okey[hi] = a;
+        }
+      }
+    }
+  __syncthreads();
+  int *order = const_cast<int *>(B.world_order);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) order[k] = rank[N - 1 - (int)(okey[k] & 0xffffffffull)];
+}
+
+hipError_t launch_order_worlds(const DevBatch &B, const int *rank, float *est, hipStream_t st) {
+  const int n = B.nworlds;
+  if (n < 2) return hipSuccess;
+  if (n > 8192) return hipErrorInvalidValue;
+  int N = 1;
+  while (N < n) N <<= 1;
+  order_worlds_kernel<<<1, 1024, sizeof(unsigned long long) * (size_t)N, st>>>(B, rank, est, n);
+  return hipGetLastError();
 }
 
 // one dCollide call on the device (ODE API path outside a cached batch)

@@ -227,6 +227,15 @@ class Batch:
         if grid is not None and not self.L.dBatchSetPersistentGrid(self.h, int(grid)):
             raise RuntimeError(f"bad grid {grid}")
 
+    def set_queue_order(self, by_cost: bool):
+        """Persistent launches pull worlds by estimated solver work (True) or
+        by size (False, the default)."""
+        if not self.L.dBatchSetQueueOrder(self.h, 1 if by_cost else 0):
+            raise RuntimeError("dBatchSetQueueOrder failed")
+
+    def queue_order(self) -> bool:
+        return self.L.dBatchGetQueueOrder(self.h) == 1
+
     PHASES = ("aabb", "broadphase", "narrowphase", "bodies_info1", "info2_rhs", "order", "sor", "integrate")
 
     def profile(self, on: bool = True):

@@ -71,6 +71,8 @@ def load_library(path: str = LIB_PATH):
         "dBatchGetSplit": (C.c_int, [V]),
         "dBatchAutotuneEx": (C.c_int, [V, C.c_int, C.c_int]),
         "dBatchSetPersistentGrid": (C.c_int, [V, C.c_int]),
+        "dBatchSetQueueOrder": (C.c_int, [V, C.c_int]),
+        "dBatchGetQueueOrder": (C.c_int, [V]),
         "dBatchSetSplit": (C.c_int, [V, C.c_int]),
         "dBatchSetLds": (C.c_int, [V, C.c_int]),
         "dBatchSetProfiling": (C.c_int, [V, C.c_int]),

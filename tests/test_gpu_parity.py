@@ -1042,12 +1042,13 @@ def test_coloured_order_is_reproducible(odemi_mod):
 
 def test_coloured_solver_placements_agree(odemi_mod):
     """The coloured solver's working set placed in the LDS pool, with only its
-    rows in global scratch, or all in global scratch (the pool size decides
+    rows in global scratch, with its rows and the bodies' conjugate-gradient
+    state in global scratch, or all in global scratch (the pool size decides
     per world): the same bits at one workgroup size, at a tolerance with CG
     steps and warm start, over 15 steps of uneven worlds."""
     ws = scenes.uneven_worlds(12, seed=11)
     outs = []
-    for lds in (None, 32768, 16384, 6144, 0):
+    for lds in (None, 32768, 16384, 12288, 9216, 6144, 0):
         say("coloured placement, pool", lds)
         b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
         b.set_launch_config(128, 1.5, lds, 0, 256)
@@ -1059,6 +1060,29 @@ def test_coloured_solver_placements_agree(odemi_mod):
     for st, wc in outs[1:]:
         assert np.array_equal(outs[0][0], st, equal_nan=True)
         assert np.array_equal(outs[0][1], wc)
+
+
+def test_queue_order_by_cost_keeps_every_world(odemi_mod):
+    """The persistent queue ordered by estimated solver work and by size (the
+    default): every world is stepped exactly once per step either
+    way, so the states and counters are the same bits, over 20 steps of 48
+    uneven worlds at a tolerance with warm start, on a grid smaller than the
+    batch (workgroups pull several worlds each)."""
+    ws = scenes.uneven_worlds(48, seed=13)
+    outs = []
+    for by_cost in (True, False):
+        say("queue order by cost:", by_cost)
+        b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+        b.set_launch_config(64, 1.5, 38400, 0, 16)
+        b.set_queue_order(by_cost)
+        assert b.queue_order() == by_cost
+        b.set_tolerance(1e-4, 400, 10, accel=True)
+        b.set_warm_start(1.0)
+        b.step(20)
+        outs.append((b.state(), b.world_counters()))
+        b.close()
+    assert np.array_equal(outs[0][0], outs[1][0], equal_nan=True)
+    assert np.array_equal(outs[0][1], outs[1][1])
 
 
 def test_uneven_worlds_rollout_at_tolerance(odemi_mod):
