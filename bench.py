@@ -197,6 +197,7 @@ def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
     the headline path (coloured, iterated to the tolerance) and the FP64
     build."""
     import multiprocessing as mp
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
     every = 10
     has_ctrl = worlds[0].step_ctrl is not None
     ctrl = np.concatenate([w.step_ctrl[:nsteps] for w in worlds], axis=1) if has_ctrl else None
@@ -213,10 +214,12 @@ def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
             b.step(1)
             if (k + 1) % every == 0:
                 out.append(b.state()[:, :3].astype(np.float64))
+            if (k + 1) % 50 == 0:
+                log(f"divergence ({label}): GPU {'reference' if order == odemi.ORDER_REFERENCE else 'coloured'} "
+                    f"order{' at the tolerance' if tolerance else ''}, step {k + 1} of {nsteps}")
         b.close()
         return np.stack(out)
 
-    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ref = gpu(odemi.ORDER_REFERENCE, False)
     col = gpu(odemi.ORDER_COLORED, False)

@@ -217,14 +217,17 @@ OD_HD inline long long warm_bytes(int kcap, int nj) { return od_align(24LL * kca
 OD_HD inline long long lcp_ws_floats(int m) { return 2LL * m * m + 7LL * m + 64; }
 
 // bytes of the coloured solver's working set held in LDS when its rows and
-// the bodies' conjugate-gradient state stay in global memory: bodies,
-// constraints in colour order, colour starts, row impulses and (accelerated
-// mode) the rows' conjugate-gradient state
+// all of the conjugate-gradient state stay in global memory: bodies,
+// constraints in colour order, colour starts and row impulses
+OD_HD inline long long sor_lds_bytes_rows_cg_global(int nb, int nrows, int ncons) {
+  return (long long)sizeof(SBody) * nb + (long long)sizeof(SCons) * ncons +
+         ((4LL * (2LL * ncons + 8) + 15) & ~15LL) + ((4LL * nrows + 15) & ~15LL);
+}
+
+// ... with the rows' conjugate-gradient state in LDS too (the rows and the
+// bodies' conjugate-gradient state in global memory)
 OD_HD inline long long sor_lds_bytes_rows_acc_global(int nb, int nrows, int ncons, bool accel = false) {
-  long long b = (long long)sizeof(SBody) * nb + (long long)sizeof(SCons) * ncons +
-                ((4LL * (2LL * ncons + 8) + 15) & ~15LL) + ((4LL * nrows + 15) & ~15LL);
-  if (accel) b += (8LL * nrows + 15) & ~15LL;
-  return b;
+  return sor_lds_bytes_rows_cg_global(nb, nrows, ncons) + (accel ? (8LL * nrows + 15) & ~15LL : 0LL);
 }
 
 // ... with the bodies' conjugate-gradient state in LDS too (the rows in

@@ -1186,9 +1186,11 @@ __device__ void sor_prepare(const DevBatch &B, const Ctx &c, const WorldParams &
 
 // Placement of the coloured solver's working set during the sweeps: all in
 // global scratch, all in the LDS pool, everything but the (read-only) rows
-// in the pool, or everything but the rows and the bodies' conjugate-gradient
-// state (read and written once per sweep, by one coalesced pass) in the pool.
-enum { SOR_IN_GLOBAL = 0, SOR_IN_LDS = 1, SOR_ROWS_GLOBAL = 2, SOR_ROWS_ACC_GLOBAL = 3 };
+// in the pool, everything but the rows and the bodies' conjugate-gradient
+// state (read and written once per sweep, by one coalesced pass) in the
+// pool, or also the rows' conjugate-gradient state in global scratch (inside
+// the colour chain a relaxation only stores to it).
+enum { SOR_IN_GLOBAL = 0, SOR_IN_LDS = 1, SOR_ROWS_GLOBAL = 2, SOR_ROWS_ACC_GLOBAL = 3, SOR_ROWS_CG_GLOBAL = 4 };
 
 // The sweeps.  The working set (rows included, or all but the rows) is
 // copied to the LDS pool first and back at the end; the placement is a
@@ -1215,8 +1217,8 @@ __device__ __attribute__((always_inline)) void sor_solve(const DevBatch &B, cons
   SCons *__restrict__ sc = S.sc;
   int *__restrict__ sstart = S.sstart;
   float *__restrict__ slam = S.slam;
-  SAccBody *__restrict__ sa = PLACE == SOR_ROWS_ACC_GLOBAL ? G.sa : S.sa;
-  float2 *__restrict__ ld = S.ld;
+  SAccBody *__restrict__ sa = PLACE >= SOR_ROWS_ACC_GLOBAL ? G.sa : S.sa;
+  float2 *__restrict__ ld = PLACE == SOR_ROWS_CG_GLOBAL ? G.ld : S.ld;
   if (LDS) {
     if (ROWS_LDS)
       for (int r = threadIdx.x; r < nrows; r += nt) sr[r] = G.sr[r];
@@ -1398,6 +1400,13 @@ __device__ __attribute__((noinline)) void sor_solve_rows_acc_global(const DevBat
 }
 
 template <bool TOL>
+__device__ __attribute__((noinline)) void sor_solve_rows_cg_global(const DevBatch &B, const Ctx &c,
+                                                                   const WorldParams &P, int nrows, int ngroups,
+                                                                   int nact, bool host_contacts, Shared &sh) {
+  sor_solve<TOL, SOR_ROWS_CG_GLOBAL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+}
+
+template <bool TOL>
 __device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const WorldParams &P, int nrows, int ngroups,
                                      int nact, bool host_contacts, Shared &sh) {
   const bool accel = TOL && P.tol > 0.f && P.accel != 0;
@@ -1407,6 +1416,8 @@ __device__ inline void sor_solve_any(const DevBatch &B, const Ctx &c, const Worl
     sor_solve_rows_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
   else if (accel && sor_lds_bytes_rows_acc_global(c.W.nb, nrows, nact, accel) <= B.lds_bytes)
     sor_solve_rows_acc_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
+  else if (accel && sor_lds_bytes_rows_cg_global(c.W.nb, nrows, nact) <= B.lds_bytes)
+    sor_solve_rows_cg_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
   else
     sor_solve_global<TOL>(B, c, P, nrows, ngroups, nact, host_contacts, sh);
 }

@@ -1043,12 +1043,13 @@ def test_coloured_order_is_reproducible(odemi_mod):
 def test_coloured_solver_placements_agree(odemi_mod):
     """The coloured solver's working set placed in the LDS pool, with only its
     rows in global scratch, with its rows and the bodies' conjugate-gradient
+    state in global scratch, with its rows and all of the conjugate-gradient
     state in global scratch, or all in global scratch (the pool size decides
     per world): the same bits at one workgroup size, at a tolerance with CG
     steps and warm start, over 15 steps of uneven worlds."""
     ws = scenes.uneven_worlds(12, seed=11)
     outs = []
-    for lds in (None, 32768, 16384, 12288, 9216, 6144, 0):
+    for lds in (None, 32768, 16384, 12288, 9216, 6144, 4096, 3072, 0):
         say("coloured placement, pool", lds)
         b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
         b.set_launch_config(128, 1.5, lds, 0, 256)
