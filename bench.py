@@ -77,8 +77,14 @@ def cpu_baseline(worlds, skip, steps, procs):
         return None
     jobs = [(worlds[k::procs], skip, steps) for k in range(procs)]
     t0 = time.perf_counter()
-    with mp.get_context("spawn").Pool(procs) as pool:
+    # the workers are closed and joined (they exit on their own) rather than
+    # terminated by the context manager
+    pool = mp.get_context("spawn").Pool(procs)
+    try:
         res = pool.map(_cpu_worker, jobs)
+    finally:
+        pool.close()
+        pool.join()
     work = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     return work / wall, wall, time.perf_counter() - t0
@@ -229,12 +235,16 @@ def divergence_run(odemi, np, worlds, nsteps, tol, max_iters, procs, label):
     # both restatements in one pool of host processes, in order
     jobs = [(w, nsteps, every, prec) for prec in ("f64", "f32") for w in worlds]
     out = []
-    with mp.get_context("spawn").Pool(procs) as pool:
+    pool = mp.get_context("spawn").Pool(procs)
+    try:
         log(f"divergence ({label}): {len(jobs)} restatement runs on {procs} host processes")
         for k, tr in enumerate(pool.imap(_oracle_traj, jobs)):
             out.append(tr)
             if (k + 1) % 8 == 0:
                 log(f"divergence ({label}): {k + 1} of {len(jobs)} restatement runs")
+    finally:
+        pool.close()
+        pool.join()
     f64 = np.concatenate(out[:len(worlds)], axis=1)
     f32 = np.concatenate(out[len(worlds):], axis=1)
     d = lambda a, b: float(np.abs(a - b).max())
