@@ -267,9 +267,21 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
     }
   }
   // the replays re-apply each step's controls; the current ones are restored
-  const bool ctrl_set0 = b->ctrl_set;
-  if (!t.kept && b->nj > 0)
+  // when this function returns, on every path
+  struct CtrlRestore {
+    BatchImpl *b;
+    DevBuf *save;
+    bool set0, armed;
+    ~CtrlRestore() {
+      if (!armed) return;
+      b->ctrl_set = set0;
+      (void)hipMemcpyAsync(b->d.ctrl.p, save->p, sizeof(float4) * b->nj, hipMemcpyDeviceToDevice, b->stream);
+    }
+  } ctrl_restore{b, &ctrl_save, b->ctrl_set, false};
+  if (!t.kept && b->nj > 0) {
     OD_HIP(hipMemcpyAsync(ctrl_save.p, b->d.ctrl.p, sizeof(float4) * b->nj, hipMemcpyDeviceToDevice, b->stream));
+    ctrl_restore.armed = true;
+  }
   int seg0 = ((T - 1) / K) * K;
   const int seg_first = (first / K) * K;
   for (; !t.kept && seg0 >= seg_first && !fail; seg0 -= K) {
@@ -322,9 +334,10 @@ int tape_backward(BatchImpl *b, int nsteps, const float *grad_out, float *grad_i
       cur = 1 - cur;
     }
   }
-  if (!t.kept && b->nj > 0) {
-    b->ctrl_set = ctrl_set0;
+  if (ctrl_restore.armed) {
+    b->ctrl_set = ctrl_restore.set0;
     OD_HIP(hipMemcpyAsync(b->d.ctrl.p, ctrl_save.p, sizeof(float4) * b->nj, hipMemcpyDeviceToDevice, b->stream));
+    ctrl_restore.armed = false;
   }
   if (!snapshot(b, endsnap, false)) return 0;
   if (fail) return 0;
