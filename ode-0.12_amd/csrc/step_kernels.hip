@@ -1873,6 +1873,9 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
       cb1[i] = cons[i].b1;
       cb2[i] = cons[i].b2;
     }
+    // per body: the colours taken by its coloured constraints (64-bit mask
+    // in the body's unused temp words 24-25)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) *(unsigned long long *)(bt + 32 * b + 24) = 0ull;
     __syncthreads();
     int round = 0;
     while (true) {
@@ -1892,12 +1895,29 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
         if (col[i] != -1) continue;
         int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);
         bool win = bmax[cb1[i]] == pr && (cb2[i] < 0 || bmax[cb2[i]] == pr);
-        if (win) col[i] = round;
+        if (win) {
+          // first fit: the lowest colour neither body has taken (a round's
+          // winners share no body, so each mask has one writer per round);
+          // past 64 colours, one colour of its own per round
+          unsigned long long *m1 = (unsigned long long *)(bt + 32 * cb1[i] + 24);
+          unsigned long long *m2 = cb2[i] >= 0 ? (unsigned long long *)(bt + 32 * cb2[i] + 24) : nullptr;
+          const unsigned long long used = *m1 | (m2 ? *m2 : 0ull);
+          const int cc = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
+          col[i] = cc;
+          if (cc < 64) {
+            *m1 |= 1ull << cc;
+            if (m2) *m2 |= 1ull << cc;
+          }
+        }
       }
       __syncthreads();
       round++;
     }
-    ngroups = round;
+    {
+      int mc = -1;
+      for (int i = threadIdx.x; i < ncons; i += blockDim.x) mc = max(mc, col[i]);
+      ngroups = (int)block_max((float)mc, sh) + 1;
+    }
     // bucket constraints by colour, stably (index order inside a colour):
     // the sweep result does not depend on the order inside a colour, but
     // the block sums of the stopping test and the CG step do, so the

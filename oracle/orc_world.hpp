@@ -1226,10 +1226,13 @@ template <class R> struct World {
       if (jb[2 * i + 1] != -1) for (int k = 6; k < 12; k++) s += iMJ[12 * i + k] * J[12 * i + k];
       Ad[i] = prm.w / (s + cfm[i]);
     }
-    // colouring
+    // colouring: Jones-Plassmann rounds, each round's winners taking the
+    // lowest colour neither of their bodies has (first fit; past 64 colours
+    // a colour of their own per round)
     std::vector<int> col(nc, -1);
     std::vector<long long> bmax(nb);
-    int ncol = 0;
+    std::vector<unsigned long long> used_mask(nb, 0ull);
+    int ncol = 0, round = 0;
     auto prio = [](int i) { return (long long)(((unsigned)i * 2654435761u) & 0x7fffffffu); };
     while (true) {
       std::fill(bmax.begin(), bmax.end(), -1LL);
@@ -1245,10 +1248,23 @@ template <class R> struct World {
       for (int i = 0; i < nc; i++) {
         if (col[i] != -1 || jm[i] <= 0) continue;
         int r = row0[i];
-        if (bmax[jb[2 * r]] == prio(cid[i]) && (jb[2 * r + 1] == -1 || bmax[jb[2 * r + 1]] == prio(cid[i])))
-          col[i] = ncol;
+        const int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+        if (bmax[b1] == prio(cid[i]) && (b2 == -1 || bmax[b2] == prio(cid[i]))) {
+          const unsigned long long used = used_mask[b1] | (b2 != -1 ? used_mask[b2] : 0ull);
+          int cc = 64 + round;
+          if (used != ~0ull) {
+            cc = 0;
+            while (used & (1ull << cc)) cc++;
+          }
+          col[i] = cc;
+          if (cc < 64) {
+            used_mask[b1] |= 1ull << cc;
+            if (b2 != -1) used_mask[b2] |= 1ull << cc;
+          }
+          ncol = std::max(ncol, cc + 1);
+        }
       }
-      ncol++;
+      round++;
     }
     auto sweep = [&]() {
       for (int g = 0; g < ncol; g++)
