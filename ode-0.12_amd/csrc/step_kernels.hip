@@ -83,6 +83,20 @@ __device__ inline int block_excl_scan(int v, int &total, Shared &sh) {
   return off + x - v;
 }
 
+// colouring: the lowest colour below 64 that neither body's mask (body temp
+// words 24-25) holds and that is not full; 64 if none
+__device__ inline int first_free_colour(const float *bt, int b1, int b2, unsigned long long full) {
+  const unsigned long long used = *(const unsigned long long *)(bt + 32 * b1 + 24) |
+                                  (b2 >= 0 ? *(const unsigned long long *)(bt + 32 * b2 + 24) : 0ull) | full;
+  return used == ~0ull ? 64 : __ffsll((long long)~used) - 1;
+}
+
+__device__ inline void take_colour(float *bt, int b1, int b2, int *col, int c) {
+  *col = c;
+  *(unsigned long long *)(bt + 32 * b1 + 24) |= 1ull << c;
+  if (b2 >= 0) *(unsigned long long *)(bt + 32 * b2 + 24) |= 1ull << c;
+}
+
 __device__ inline float block_max(float v, Shared &sh) {
   int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -1876,6 +1890,9 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
     // per body: the colours taken by its coloured constraints (64-bit mask
     // in the body's unused temp words 24-25)
     for (int b = threadIdx.x; b < nb; b += blockDim.x) *(unsigned long long *)(bt + 32 * b + 24) = 0ull;
+    // members of each of the first 64 colours, and the mask of full ones
+    for (int k = threadIdx.x; k < 64; k += blockDim.x) sh.ccount[k] = 0;
+    if (threadIdx.x == 0) sh.ival[9] = sh.ival[10] = 0;
     __syncthreads();
     int round = 0;
     while (true) {
@@ -1894,21 +1911,62 @@ __device__ __attribute__((always_inline)) void phase_step(const DevBatch &B, con
       for (int i = threadIdx.x; i < ncons; i += blockDim.x) {
         if (col[i] != -1) continue;
         int pr = (int)(((unsigned)i * 2654435761u) & 0x7fffffffu);
-        bool win = bmax[cb1[i]] == pr && (cb2[i] < 0 || bmax[cb2[i]] == pr);
-        if (win) {
-          // first fit: the lowest colour neither body has taken (a round's
-          // winners share no body, so each mask has one writer per round);
-          // past 64 colours, one colour of its own per round
-          unsigned long long *m1 = (unsigned long long *)(bt + 32 * cb1[i] + 24);
-          unsigned long long *m2 = cb2[i] >= 0 ? (unsigned long long *)(bt + 32 * cb2[i] + 24) : nullptr;
-          const unsigned long long used = *m1 | (m2 ? *m2 : 0ull);
-          const int cc = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
-          col[i] = cc;
-          if (cc < 64) {
-            *m1 |= 1ull << cc;
-            if (m2) *m2 |= 1ull << cc;
+        if (bmax[cb1[i]] == pr && (cb2[i] < 0 || bmax[cb2[i]] == pr)) col[i] = -3;   // a winner, colour pending
+      }
+      __syncthreads();
+      // first fit, capped: the winners, in constraint order, each take the
+      // lowest colour that neither body has taken and that has fewer than
+      // blockDim.x members (a colour is then one pass of the block); past 64
+      // colours, one colour of their own per round.  Colours are settled in
+      // increasing order: below the lowest colour the winners would overfill
+      // every proposal stands; that colour goes to its first proposers in
+      // constraint order and is full, and the rest propose again.  A
+      // winner's proposal only rises, and a round's winners share no body
+      // (each mask has one writer per round), so this equals the sequential
+      // assignment.
+      while (true) {
+        const unsigned long long full = ((unsigned long long)(unsigned)sh.ival[10] << 32) | (unsigned)sh.ival[9];
+        for (int k = threadIdx.x; k < 64; k += blockDim.x) sh.ccur[k] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < ncons; i += blockDim.x)
+          if (col[i] == -3) {
+            const int p = first_free_colour(bt, cb1[i], cb2[i], full);
+            if (p < 64) atomicAdd(&sh.ccur[p], 1);
           }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          int cs = 64;
+          for (int k = 0; k < 64; k++)
+            if (sh.ccount[k] + sh.ccur[k] > (int)blockDim.x) { cs = k; break; }
+          sh.ival[8] = cs;
         }
+        __syncthreads();
+        const int cs = sh.ival[8];
+        for (int i = threadIdx.x; i < ncons; i += blockDim.x)
+          if (col[i] == -3) {
+            const int p = first_free_colour(bt, cb1[i], cb2[i], full);
+            if (p < cs) take_colour(bt, cb1[i], cb2[i], col + i, p);
+            else if (p >= 64) col[i] = 64 + round;
+          }
+        for (int k = threadIdx.x; k < cs; k += blockDim.x) sh.ccount[k] += sh.ccur[k];
+        if (cs == 64) break;
+        const int room = (int)blockDim.x - sh.ccount[cs];
+        int base = 0;
+        for (int start = 0; start < ncons && base < room; start += blockDim.x) {
+          const int i = start + threadIdx.x;
+          const bool want = i < ncons && col[i] == -3 && first_free_colour(bt, cb1[i], cb2[i], full) == cs;
+          int tot;
+          const int r = base + block_excl_scan(want ? 1 : 0, tot, sh);
+          if (want && r < room) take_colour(bt, cb1[i], cb2[i], col + i, cs);
+          base += tot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          sh.ccount[cs] = (int)blockDim.x;
+          if (cs < 32) sh.ival[9] |= (int)(1u << cs);
+          else sh.ival[10] |= (int)(1u << (cs - 32));
+        }
+        __syncthreads();
       }
       __syncthreads();
       round++;

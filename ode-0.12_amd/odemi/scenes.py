@@ -312,6 +312,18 @@ def cylinder_pile(n: int = 32, seed: int = 31) -> Scene:
     return b.scene(dt=0.01, mu=0.6, seed=seed)
 
 
+def box_floor(n: int = 12, seed: int = 1) -> Scene:
+    """n x n boxes 0.5 m on a side, density 1, resting 1 mm deep on a z=0
+    plane with 0.1 m between them; h = 0.01, mu = 0.5.  Every box has four
+    contacts with the plane from the first step, so a graph colouring has
+    four colours of n*n members each unless their size is capped."""
+    b = _Builder()
+    b.geom(PLANE, (0.0, 0.0, 1.0, 0.0))
+    for k in range(n * n):
+        b.box_body((0.6 * (k % n), 0.6 * (k // n), 0.249), (0.5, 0.5, 0.5), 1.0)
+    return b.scene(dt=0.01, mu=0.5, seed=seed)
+
+
 def box_stack(n: int = 10, gap: float = 1e-3, seed: int = 0) -> Scene:
     """10 boxes 1x1x1 m, density 1, on a z=0 plane with 1 mm gaps; h=0.01,
     20 iterations, ERP 0.2, CFM 1e-5, mu=0.5 (Coulomb friction through findex)."""

@@ -69,6 +69,22 @@ void record(const World<R> &w, double *dst) {
 // mode instead of the reference order (not part of the reference).
 static int g_colored = 0;
 extern "C" void orc_set_colored(int on) { g_colored = on ? 1 : 0; }
+// coloured mode: most members of a colour, the workgroup width of the MI355X
+// launch being restated (default 256, the library's default width)
+static int g_colour_width = 256;
+extern "C" void orc_set_colour_width(int w) { g_colour_width = w > 0 ? w : 256; }
+// coloured mode: when on, the colouring of the last coloured solve is kept,
+// five ints per constraint (cid, b1, b2, colour, round) in the order the
+// colours were assigned; orc_colour_log copies up to cap entries and returns
+// the number kept
+static int g_colour_log_on = 0;
+static std::vector<int> g_colour_log;
+extern "C" void orc_set_colour_log(int on) { g_colour_log_on = on; g_colour_log.clear(); }
+extern "C" long long orc_colour_log(int *out, long long cap) {
+  const long long n = (long long)g_colour_log.size() / 5;
+  for (long long k = 0; k < 5 * std::min(n, cap); k++) out[k] = g_colour_log[k];
+  return n;
+}
 // coloured mode: iterate to a residual tolerance, optionally with the
 // conjugate-gradient acceleration; the sweep count of the runs since the
 // last call is returned by orc_colored_sweeps
@@ -103,6 +119,8 @@ extern "C" void orc_set_state_override(const double *st, int nb) {
 // Builds the world exactly as the scene driver does through the ODE API.
 static void build_world(const SceneDesc *s, World<R> &w) {
   w.prm.colored = g_colored;
+  w.prm.colour_width = g_colour_width;
+  if (g_colour_log_on) w.colour_log = &g_colour_log;
   w.prm.exact = s->world_step ? 1 : 0;
   w.prm.tol = (R)g_tol;
   w.prm.max_iterations = g_maxit;

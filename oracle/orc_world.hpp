@@ -130,6 +130,7 @@ template <class R> struct WorldParams {
   R max_vel = INFINITY, min_depth = 0;
   int hash_min = -3, hash_max = 10;
   int colored = 0;   // 1: graph-coloured Gauss-Seidel order (no shuffle), as the MI355X coloured mode
+  int colour_width = 256;   // coloured mode: most members of a colour (the MI355X workgroup width)
   int exact = 0;     // 1: dWorldStep -- the island's LCP solved exactly (step.cpp, lcp.cpp)
   // coloured mode only: iterate until the LCP residual <= tol (checked every
   // check_every sweeps, at most max_iterations); accel = 1 adds the
@@ -1226,10 +1227,13 @@ template <class R> struct World {
       if (jb[2 * i + 1] != -1) for (int k = 6; k < 12; k++) s += iMJ[12 * i + k] * J[12 * i + k];
       Ad[i] = prm.w / (s + cfm[i]);
     }
-    // colouring: Jones-Plassmann rounds, each round's winners taking the
-    // lowest colour neither of their bodies has (first fit; past 64 colours
-    // a colour of their own per round)
-    std::vector<int> col(nc, -1);
+    // colouring: Jones-Plassmann rounds; each round's winners, in
+    // constraint-id order, take the lowest colour below 64 that neither of
+    // their bodies has and that has fewer than colour_width members (first
+    // fit, capped at the MI355X workgroup width); past that, a colour of
+    // their own per round
+    std::vector<int> col(nc, -1), ccnt(64, 0), win;
+    if (colour_log) colour_log->clear();
     std::vector<long long> bmax(nb);
     std::vector<unsigned long long> used_mask(nb, 0ull);
     int ncol = 0, round = 0;
@@ -1245,24 +1249,30 @@ template <class R> struct World {
         if (jb[2 * r + 1] != -1) bmax[jb[2 * r + 1]] = std::max(bmax[jb[2 * r + 1]], prio(cid[i]));
       }
       if (!left) break;
+      win.clear();
       for (int i = 0; i < nc; i++) {
         if (col[i] != -1 || jm[i] <= 0) continue;
         int r = row0[i];
         const int b1 = jb[2 * r], b2 = jb[2 * r + 1];
-        if (bmax[b1] == prio(cid[i]) && (b2 == -1 || bmax[b2] == prio(cid[i]))) {
-          const unsigned long long used = used_mask[b1] | (b2 != -1 ? used_mask[b2] : 0ull);
-          int cc = 64 + round;
-          if (used != ~0ull) {
-            cc = 0;
-            while (used & (1ull << cc)) cc++;
-          }
-          col[i] = cc;
-          if (cc < 64) {
-            used_mask[b1] |= 1ull << cc;
-            if (b2 != -1) used_mask[b2] |= 1ull << cc;
-          }
-          ncol = std::max(ncol, cc + 1);
+        if (bmax[b1] == prio(cid[i]) && (b2 == -1 || bmax[b2] == prio(cid[i]))) win.push_back(i);
+      }
+      std::sort(win.begin(), win.end(), [&](int x, int y) { return cid[x] < cid[y]; });
+      for (int i : win) {
+        int r = row0[i];
+        const int b1 = jb[2 * r], b2 = jb[2 * r + 1];
+        const unsigned long long used = used_mask[b1] | (b2 != -1 ? used_mask[b2] : 0ull);
+        int cc = 0;
+        while (cc < 64 && ((used >> cc) & 1ull || ccnt[cc] >= prm.colour_width)) cc++;
+        if (cc == 64) {
+          cc = 64 + round;
+        } else {
+          ccnt[cc]++;
+          used_mask[b1] |= 1ull << cc;
+          if (b2 != -1) used_mask[b2] |= 1ull << cc;
         }
+        col[i] = cc;
+        ncol = std::max(ncol, cc + 1);
+        if (colour_log) colour_log->insert(colour_log->end(), {cid[i], b1, b2, cc, round});
       }
       round++;
     }
@@ -1472,6 +1482,9 @@ template <class R> struct World {
   std::vector<std::vector<int>> islands_b, islands_j;
   std::vector<std::pair<long long, std::array<R, 3>>> warm_prev;   // sorted by key
   R last_res = 0;   // residual at the last check of the last coloured solve (0: none)
+  // when set, the last coloured solve's colouring: (cid, b1, b2, colour,
+  // round) per constraint, in the order the colours were assigned
+  std::vector<int> *colour_log = nullptr;
   bool warm_ok = false;   // warm_prev and the joints' impulses hold a usable start
 };
 

@@ -783,20 +783,32 @@ def test_autotune_keeps_state(odemi_mod):
     assert a.launch_config()["split"] == 0
 
 
-def test_coloured_order_matches_coloured_restatement(odemi_mod):
+@pytest.mark.parametrize("wg", [256, 64], ids=["wg256", "wg64-capped"])
+def test_coloured_order_matches_coloured_restatement(odemi_mod, wg):
     """Coloured order against its CPU restatement (oracle coloured mode, the
-    same pair order and colouring): first-step pairs and contacts exact,
-    trajectories over 40 steps within 4x the restatement's own FP32-FP64
-    distance + 1e-3 m."""
+    same pair order and colouring, colours capped at the workgroup width):
+    first-step pairs and contacts exact, trajectories over 40 steps within 4x
+    the restatement's own FP32-FP64 distance + 1e-3 m.  At width 64 the
+    worlds include 144 boxes on a plane, whose colours the cap splits."""
     ws = scenes.uneven_worlds(6, seed=11)
+    if wg == 64:
+        ws = [scenes.box_floor(12)] + ws[:3]
+
+    def batch():
+        b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+        if wg != 256:
+            b.set_launch_config(wg, 1.0)
+        return b
+
     lib32, lib64 = po.oracle_lib("f32"), po.oracle_lib("f64")
-    b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    b = batch()
     b.step(1)
     n = 40
     refs = []
     try:
-        lib32.orc_set_colored(1)
-        lib64.orc_set_colored(1)
+        for lib in (lib32, lib64):
+            lib.orc_set_colored(1)
+            lib.orc_set_colour_width(wg)
         for w, sc in enumerate(ws):
             say("coloured parity world", w, "nb", sc.nb, "nj", sc.nj)
             r = po.run(lib64, sc, 1, 1, diag_steps=1, max_pairs=100000, max_contacts=100000)
@@ -812,10 +824,11 @@ def test_coloured_order_matches_coloured_restatement(odemi_mod):
             t64 = po.run(lib64, sc, n, 10)["traj"]
             refs.append((t32, t64))
     finally:
-        lib32.orc_set_colored(0)
-        lib64.orc_set_colored(0)
+        for lib in (lib32, lib64):
+            lib.orc_set_colored(0)
+            lib.orc_set_colour_width(256)
     b.close()
-    b = _batch(odemi_mod, ws, odemi_mod.ORDER_COLORED)
+    b = batch()
     offs = np.concatenate([[0], np.cumsum([w.nb for w in ws])])
     worst = 0.0
     for k in range(n // 10):

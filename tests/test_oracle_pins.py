@@ -550,3 +550,59 @@ def test_exact_lcp_against_enumeration():
         for s in sols:
             np.testing.assert_allclose(s, sols[0], atol=1e-7)
         np.testing.assert_allclose(x, sols[0], atol=1e-7)
+
+
+def test_coloured_order_caps_colours_at_the_workgroup_width():
+    """The coloured mode's colouring (Jones-Plassmann rounds, first fit)
+    caps a colour at the workgroup width, so a colour is one pass of the
+    block.  On 144 boxes resting on a plane (four plane contacts each):
+    every colour is a proper colouring (no body twice); width 256 gives four
+    colours of 144, width 64 none above 64 and fewer 64-lane passes; rounds
+    rise and, inside a round, constraint ids rise (the assignment order);
+    and each constraint took the lowest colour its bodies had not taken and
+    that was not full at its turn (first fit)."""
+    from odemi import scenes
+    L = po.oracle_lib("f64")
+    L.orc_colour_log.restype = C.c_longlong
+    L.orc_colour_log.argtypes = [C.c_void_p, C.c_longlong]
+    sc = scenes.box_floor(12)
+    logs = {}
+    try:
+        L.orc_set_colored(1)
+        L.orc_set_colour_log(1)
+        for width in (256, 64):
+            L.orc_set_colour_width(width)
+            po.run(L, sc, 1, 20)
+            n = L.orc_colour_log(None, 0)
+            buf = np.zeros(5 * n, np.int32)
+            L.orc_colour_log(buf.ctypes.data, n)
+            logs[width] = buf.reshape(n, 5)
+    finally:
+        L.orc_set_colour_log(0)
+        L.orc_set_colour_width(256)
+        L.orc_set_colored(0)
+    passes = {}
+    for width, log in logs.items():
+        cid, b1, b2, col, rnd = (log[:, k] for k in range(5))
+        assert len(log) == 4 * 144
+        for c in np.unique(col):
+            bodies = np.concatenate([b1[col == c], b2[(col == c) & (b2 >= 0)]])
+            assert len(bodies) == len(np.unique(bodies))
+        sizes = np.bincount(col)
+        assert sizes.max() <= width
+        passes[width] = int(sum((s + 63) // 64 for s in sizes))
+        order = rnd.astype(np.int64) * (1 << 32) + cid
+        assert np.all(np.diff(order) > 0)
+        used = {}
+        cnt = np.zeros(64, int)
+        for i in range(len(log)):
+            taken = used.get(b1[i], set()) | (used.get(b2[i], set()) if b2[i] >= 0 else set())
+            c = int(col[i])
+            assert c not in taken and cnt[c] < width
+            assert all(k in taken or cnt[k] >= width for k in range(c))
+            cnt[c] += 1
+            for b in (b1[i], b2[i]):
+                if b >= 0:
+                    used.setdefault(b, set()).add(c)
+    assert sorted(np.bincount(logs[256][:, 3])) == [144] * 4
+    assert passes[64] < passes[256]
