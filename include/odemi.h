@@ -351,7 +351,9 @@ ODE_API void dGeomPlaneGetParams(dGeomID g, dVector4 result);                 /*
  *    shuffle (quickstep.cpp:406-482, misc.cpp:33-117), pairs in the hash
  *    space's callback order; results track the reference build.
  *  ODEMI_ORDER_COLORED: graph-coloured projected Gauss-Seidel (constraints of
- *    one colour share no body and are relaxed in parallel). */
+ *    one colour share no body and are relaxed in parallel; a colour holds at
+ *    most one workgroup width of constraints, so results are reproducible at
+ *    one launch shape and differ in rounding between shapes). */
 enum { ODEMI_ORDER_REFERENCE = 0, ODEMI_ORDER_COLORED = 1 };
 
 /* one world of a batch; meaning as the corresponding ODE setters */
