@@ -386,9 +386,17 @@ def test_gpu_adjoint_on_uneven_worlds_with_chains(odemi_mod):
     assert np.isfinite(gi).all() and np.isfinite(gmu).all()
 
 
-def _vehicle_gpu(odemi_mod, scs, n, interval, keep=False):
+def _vehicle_gpu(odemi_mod, scs, n, interval, keep=False, settled=False):
+    # the FP32 solve of a landed car stays above a 1e-6 residual, so every
+    # solve runs to the cap.  settled: 100 000 sweeps let the impulses
+    # settle, as the adjoint of the converged LCP assumes; at 20 000 the
+    # gradients depend on where the sweep order leaves them (up to 3e-2 off
+    # FD), which is enough to compare replays with the kept steps
     b = odemi_mod.Batch(scs, odemi_mod.ORDER_COLORED)
-    b.set_tolerance(1e-6, 20000, 10, accel=True)
+    if settled:
+        b.set_tolerance(3e-7, 100000, 10, accel=True)
+    else:
+        b.set_tolerance(1e-6, 20000, 10, accel=True)
     b.tape_options(interval)
     b.tape_keep_steps(keep)
     b.tape_begin(n)
@@ -413,12 +421,13 @@ def test_gpu_adjoint_of_vehicles(odemi_mod):
     wheel torques set per step by dBatchSetControls) over 50 steps in which
     they land: the GPU gradients against the host double adjoint and FP64
     central differences, every velocity component of every body, within
-    1e-3 of the largest gradient; the replays re-apply each step's controls,
-    so checkpoint intervals 7 and 50 and the kept-steps pass agree."""
+    1e-3 of the largest gradient (forward solves of 100 000 sweeps, steps
+    kept); the replays re-apply each step's controls, so checkpoint
+    intervals 7 and 50 and the kept-steps pass agree (20 000 sweeps)."""
     n = 50
     scs = [vehicle_scene(5, n), vehicle_scene(6, n)]
     say("adjoint of vehicles:", len(scs), "cars,", n, "steps")
-    finals, gi, gmu, stats = _vehicle_gpu(odemi_mod, scs, n, 7)
+    finals, gi, gmu, stats = _vehicle_gpu(odemi_mod, scs, n, 10, True, settled=True)
     say("adjoint stats", stats)
     assert stats["collider_mismatches"] == 0 and np.isfinite(gi).all()
     off = 0
@@ -439,11 +448,12 @@ def test_gpu_adjoint_of_vehicles(odemi_mod):
             f"vs host adjoint {err_host:.2e}")
         assert div <= 1e-4 and worst <= TOL and err_host <= TOL
         off += sc.nb
-    scale = float(np.abs(gi).max())
-    for interval, keep in ((50, False), (10, True)):
-        _, g2, _, _ = _vehicle_gpu(odemi_mod, scs, n, interval, keep)
-        say(f"interval {interval} keep {keep}: max |d grad| / max |grad| {float(np.abs(g2 - gi).max()) / scale:.2e}")
-        assert np.abs(g2 - gi).max() <= 1e-5 * scale
+    _, gk, _, _ = _vehicle_gpu(odemi_mod, scs, n, 10, True)
+    scale = float(np.abs(gk).max())
+    for interval in (7, 50):
+        _, g2, _, _ = _vehicle_gpu(odemi_mod, scs, n, interval)
+        say(f"interval {interval}: max |d grad| / max |grad| vs kept steps {float(np.abs(g2 - gk).max()) / scale:.2e}")
+        assert np.abs(g2 - gk).max() <= 1e-5 * scale
 
 
 def test_tape_refuses_joints_it_does_not_cover(odemi_mod):
